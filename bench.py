@@ -1,0 +1,208 @@
+"""Benchmark: time to relative residual 1e-10 of the 256^3 vec3 projection solve.
+
+Workload (one "step" = one full solve from x = 0): the transformed-coordinate
+projection operator of trans_sim_3d for simple shear T = [[1,0,0.5],[0,1,0],
+[0,0,1]], 256^3 vertices, fully periodic, mu = 1, lamb = 1.5, h = 1/256,
+dt = 1; RHS = Gaussian random field (seed 2, correlation length 8 cells, unit
+variance per component, mean removed).  V(2,2) cycles with the reference's
+level schedule and Galerkin coarse operators (6 levels, 256 -> 8).
+
+The timed solve is FP32 multigrid inside FP64 iterative refinement; the plain
+FP64 GPU multigrid solve of the same system is timed beside it, and the error
+of the FP32+IR solution against it is reported.  Inputs are resident in HBM
+when the timed region starts.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_problem(n):
+    from stzpp_amd import fields
+    from stzpp_amd import multigrid as MG
+
+    T = np.array([[1.0, 0.0, 0.5], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+    h = 1.0 / n
+    st = MG.trans_stencil(T, 1.0, 1.5, h, h, h, 1.0)
+    mg = MG.Multigrid(n, n, n, (1, 1, 1))
+    mg.setup_matrices_const(st, None)
+    b = fields.grf_vec3((n, n, n), 8.0, 2)
+    b -= b.mean(axis=0)
+    return mg, b, st
+
+
+def cpu_baseline(n, st, b, budget_s=30.0):
+    """Reference algorithm (lexicographic block GS, reference schedule) in the
+    C oracle on one host core: one V-cycle at the bench size, scaled by the
+    number of FP64 cycles the GPU solve needs."""
+    import oracle as O
+
+    orc = O.OracleMG(n, n, n, (1, 1, 1), mode=O.PLANE)
+    orc.set_ops(0, O.plane_operator(n, 1, st, None))
+    orc.rat()
+    orc.params(smoother=0)
+    N = n ** 3
+    orc.set_fields(0, np.zeros((N, 3)), b)
+    t0 = time.perf_counter()
+    orc.vcycle(1)
+    dt = time.perf_counter() - t0
+    orc.close()
+    return dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--inner", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from stzpp_amd import multigrid as MG
+
+    n = args.size
+    log(f"[bench] building {n}^3 problem")
+    mg, b, st = make_problem(n)
+    N = n ** 3
+    zeros = np.zeros((N, 3))
+    log(f"[bench] levels {[mg.dims(l) for l in range(mg.nlevels)]}")
+
+    # Host->device upload once; each solve restarts from x = 0 with a
+    # device-to-device copy of the resident RHS/zero fields.
+    import ctypes
+
+    mg.setup_fields(zeros, b)
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    hip.hipDeviceSynchronize.argtypes = []
+    bytes3 = 8 * 3 * N
+    dx0 = ctypes.c_void_p()
+    db = ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(dx0), bytes3) == 0
+    assert hip.hipMalloc(ctypes.byref(db), bytes3) == 0
+    assert hip.hipMemset(dx0, 0, bytes3) == 0
+    # resident SoA copy of the rhs (component-major), uploaded once
+    bsoa = np.ascontiguousarray(b.T)
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    assert hip.hipMemcpy(db, bsoa.ctypes.data, bytes3, 1) == 0  # hipMemcpyHostToDevice
+
+    def solve(mode, inner=args.inner):
+        mg.set_fields_device(dx0, db)
+        hip.hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        s = mg.solve(mode=mode, rtol=1e-10, inner_cycles=inner, max_cycles=400)
+        hip.hipDeviceSynchronize()
+        return time.perf_counter() - t0, s
+
+    for w in range(args.warmup):
+        log(f"[bench] warmup {w}")
+        solve(MG.SOLVE_FP32_IR)
+        solve(MG.SOLVE_RELRES)
+
+    t64s, s64 = [], None
+    for k in range(args.steps):
+        t, s64 = solve(MG.SOLVE_RELRES)
+        t64s.append(t)
+    x64 = mg.x(0)
+    log(f"[bench] fp64: {np.mean(t64s)*1e3:.1f} ms, {s64}")
+
+    t32s, s32 = [], None
+    for k in range(args.steps):
+        t, s32 = solve(MG.SOLVE_FP32_IR)
+        t32s.append(t)
+    x32 = mg.x(0)
+    log(f"[bench] fp32+ir: {np.mean(t32s)*1e3:.1f} ms, {s32}")
+
+    # error of the FP32+IR solution against the FP64 one (periodic system:
+    # compare after removing the constant null-space component)
+    d32 = x32 - x32.mean(axis=0)
+    d64 = x64 - x64.mean(axis=0)
+    err = float(np.linalg.norm(d32 - d64) / np.linalg.norm(d64))
+
+    # Roofline of the dominant kernel: one FP32 Gauss-Seidel sweep of the
+    # fine level (8 colour launches), algorithmic bytes 36 per point.
+    ms_launch = mg.time_kernel(2, 0, 20)
+    sweep_s = 8 * ms_launch * 1e-3
+    achieved = 36.0 * N / sweep_s / 1e9
+    ms_launch64 = mg.time_kernel(0, 0, 10)
+    achieved64 = 72.0 * N / (8 * ms_launch64 * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline (oracle, 1 core, one V-cycle)")
+        cyc_s = cpu_baseline(n, st, b)
+        cpu = {
+            "value": cyc_s * s64["cycles"],
+            "unit": "s (estimated time to 1e-10: one measured V-cycle x GPU FP64 cycle count)",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"one lexicographic V-cycle of the C oracle at {n}^3 took {cyc_s:.2f} s; "
+                      f"x {s64['cycles']} cycles",
+        }
+
+    t32 = float(np.mean(t32s))
+    t64 = float(np.mean(t64s))
+    out = {
+        "metric": "time-to-1e-10 residual, 256^3 vec3 MG solve: FP32+IR vs FP64; achieved HBM GB/s",
+        "value": t32,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t32 * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"trans-operator projection solve {n}^3 vec3, simple shear T(gamma=0.5), "
+                        f"fully periodic, GRF rhs seed 2 l=8, V(2,2)+reference growth, rtol 1e-10",
+            "fp64_s": t64,
+            "fp64_cycles": s64["cycles"],
+            "fp32_ir_cycles": s32["cycles"],
+            "fp32_ir_outer": s32["outer"],
+            "speedup_vs_fp64": t64 / t32,
+            "rel_err_vs_fp64": err,
+            "levels": mg.nlevels,
+            "fp64_gs_sweep_GBs": achieved64,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,168 @@
+/* stzmg.h -- C ABI of the MI355X (gfx950) multigrid projection solver.
+ *
+ * Drop-in for the reference's duck-typed multigrid<vec3,M>/region<vec3,M>
+ * template surface (mg_3d_template/multigrid.hh:19-113, region.hh:17-656) on a
+ * single GPU.  Plain pointers and sizes only; no C++ or torch types.
+ *
+ * Conventions
+ *   point index   p = i + m*(j + n*k), x fastest (as region::x0 / r0)
+ *   vec3 arrays   host arrays are AoS, 3 doubles per point (reference vec3)
+ *   stencil       27 blocks of 3x3, block b = (di+1) + 3*(dj+1) + 9*(dk+1)
+ *                 (reference index 13+i+3j+9k, src/shear_sim_3d.cpp:222),
+ *                 each block row-major a11 a12 a13 a21 ... a33 (src/mat3.hh)
+ *   status        0 = OK, negative = error; stzmg_last_error() gives the text.
+ *                 Nothing aborts the process (the reference calls MPI_Abort,
+ *                 mg_3d_template/common.cc:52-55).
+ *   threading     one HIP stream per handle; calls are synchronous unless
+ *                 documented otherwise; a handle is not thread safe, distinct
+ *                 handles are independent (no function-static state, unlike
+ *                 the reference's std_iters, src/shear_sim_3d.cpp:561).
+ */
+#ifndef STZMG_H
+#define STZMG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct stzmg_s *stzmg_t;
+
+enum {
+    STZMG_OK = 0,
+    STZMG_ERR_ARG = -1,
+    STZMG_ERR_UNSUPPORTED = -2,
+    STZMG_ERR_HIP = -3,
+    STZMG_ERR_NOMEM = -4,
+    STZMG_ERR_STATE = -5,
+    STZMG_ERR_NOT_CONVERGED = -6,
+    STZMG_ERR_NONFINITE = -7,
+    STZMG_ERR_SINGULAR = -8
+};
+
+/* How the operator of a level varies in space. */
+enum {
+    STZMG_OP_PLANE = 0, /* one 27-block stencil per z plane (x, y periodic) */
+    STZMG_OP_POINT = 1  /* one 27-block stencil per point (general) */
+};
+
+enum { STZMG_NORM_QUARTIC = 0, STZMG_NORM_L2SQ = 1 };
+
+enum {
+    STZMG_SOLVE_REF = 0,     /* reference projection loop (sum |r|^4 <= tol, + extra cycles) */
+    STZMG_SOLVE_RELRES = 1,  /* FP64 V-cycles to ||r||_2 <= rtol ||b - A x0||_2 */
+    STZMG_SOLVE_FP32_IR = 2  /* FP32 V-cycles on the defect inside FP64 refinement */
+};
+
+typedef struct {
+    int m, n, o;          /* fine-level vertices per dimension */
+    int prd[3];           /* periodicity flags */
+    long coarsest_total;  /* coarsen while m*n*o > this (reference: 1024, multigrid.cc:157) */
+    int v_down, v_up;     /* pre/post sweeps on level 0 (reference: 2, 2, multigrid.hh:160) */
+    int bottom_sweeps;    /* sweeps on the coarsest level (reference: 24, common.hh:57) */
+    int sweep_growth;     /* level l gets v_down+growth*l sweeps (reference: 1, multigrid.cc:265) */
+    int op_mode;          /* STZMG_OP_PLANE or STZMG_OP_POINT */
+} stzmg_desc;
+
+typedef struct {
+    int mode;             /* STZMG_SOLVE_* */
+    double tol;           /* REF: threshold on sum |r|^4 (region.cc:3031-3057) */
+    int extra_cycles;     /* REF: cycles after convergence (reference: 2, src/shear_sim_3d.cpp:599) */
+    int ref_norm_schedule;/* REF: 0 = norm after every cycle (shear_sim_3d.cpp:567-582),
+                             1 = only once num_iters >= std_iters/16 (trans_sim_3d.cpp:1020-1040) */
+    double rtol;          /* RELRES / FP32_IR: relative residual target */
+    int max_cycles;       /* cap on V-cycles (the reference loop has none) */
+    int inner_cycles;     /* FP32_IR: FP32 V-cycles per refinement step */
+    int use_graph;        /* replay V-cycles from a captured hipGraph */
+} stzmg_solve_opts;
+
+typedef struct {
+    int cycles;           /* V-cycles run (FP64 or FP32) */
+    int outer;            /* FP64 refinement steps (FP32_IR) */
+    int norms;            /* residual-norm evaluations */
+    double res0;          /* initial ||r||_2 (RELRES/FP32_IR) or sum |r|^4 (REF) */
+    double res;           /* final value of the same quantity */
+    double ms;            /* wall time of the solve in milliseconds */
+} stzmg_solve_stats;
+
+/* Fill *d with the reference's defaults for an m x n x o grid. */
+int stzmg_desc_default(stzmg_desc *d, int m, int n, int o, int px, int py, int pz);
+
+/* multigrid<V,M>::multigrid + setup_hierarchy (multigrid.hh:157-189,
+ * multigrid.cc:136-249): allocate all levels on the current HIP device. */
+int stzmg_create(const stzmg_desc *d, stzmg_t *h);
+int stzmg_destroy(stzmg_t h);
+const char *stzmg_last_error(stzmg_t h);
+int stzmg_nlevels(stzmg_t h);
+int stzmg_level_dims(stzmg_t h, int level, int dims[3]);
+
+/* Fine operator, then the Galerkin hierarchy on the GPU in FP64
+ * (multigrid::setup_matrices -> compute_rats, multigrid.hh:195-200,
+ * multigrid.cc:330-344; region::compute_rats, region.cc:1912-1966).
+ *   _const : interior stencil everywhere; every point on a non-periodic face
+ *            carries the one-block fix, box [0,1)^3 (the shear_sim problem
+ *            callbacks, src/shear_sim_3d.hh:388-402); fix may be NULL if all
+ *            dimensions are periodic.
+ *   _planes: one stencil per z plane (STZMG_OP_PLANE handles).
+ *   _points: one stencil per point (STZMG_OP_POINT handles; the reference's
+ *            internal()/fill_entries path, region.hh:663-790). */
+int stzmg_set_operator_const(stzmg_t h, const double *blocks27x9, const double *fix9);
+int stzmg_set_operator_planes(stzmg_t h, const double *planes);
+int stzmg_set_operator_points(stzmg_t h, const double *points);
+/* Copy out the operator of a level: ncls x 243 doubles, ncls = o (PLANE) or
+ * m*n*o (POINT) of that level. */
+int stzmg_get_level_operator(stzmg_t h, int level, double *out, long *ncls);
+
+/* Fields (region::setup_fields / x0 / r0, region.hh:209-241). */
+int stzmg_set_level_fields(stzmg_t h, int level, const double *x, const double *r);
+int stzmg_get_level_fields(stzmg_t h, int level, double *x, double *r);
+/* Device-resident SoA variants on level 0: x_dev/r_dev hold 3 planes of
+ * m*n*o doubles (component-major).  The copy runs on the handle's stream. */
+int stzmg_set_fields_device(stzmg_t h, const double *x_dev, const double *r_dev);
+int stzmg_get_x_device(stzmg_t h, double *x_dev);
+
+/* Level operations (multigrid::gauss_seidel / restriction / interpolation,
+ * multigrid.cc:412-450).  restriction(l) fills level l from level l-1 and
+ * clears x on level l; interpolation(l) adds P x_l into x_{l-1}.  The
+ * smoother is 8-colour block Gauss-Seidel (a lexicographic sweep is not
+ * parallel). */
+int stzmg_gauss_seidel(stzmg_t h, int level, int nsweeps);
+int stzmg_restriction(stzmg_t h, int level);
+int stzmg_interpolation(stzmg_t h, int level);
+/* r - A x on a level, copied to host AoS (region::residual, region.cc:1009-1055). */
+int stzmg_residual(stzmg_t h, int level, double *out);
+/* STZMG_NORM_QUARTIC = sum |r|^4 (l2_error_all, region.cc:3031-3057),
+ * STZMG_NORM_L2SQ = sum |r|^2. */
+int stzmg_norm(stzmg_t h, int level, int kind, double *out);
+/* multigrid::v_cycle (multigrid.cc:253-297), FP64. */
+int stzmg_vcycle(stzmg_t h, int n);
+/* One FP32 V-cycle on level-0 FP32 fields (x32, r32), for measurement. */
+int stzmg_vcycle_fp32(stzmg_t h, int n);
+/* The projection solve loop (src/shear_sim_3d.cpp:567-600) and the added
+ * FP32 + FP64 iterative-refinement mode. */
+int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *stats);
+void stzmg_solve_opts_default(stzmg_solve_opts *o);
+
+/* The HIP stream (hipStream_t) the handle launches on. */
+void *stzmg_stream(stzmg_t h);
+/* Per-kernel timing of the last n launches of one kernel class, for roofline
+ * accounting: kind 0 = fine-level GS colour pass, 1 = fine residual. */
+int stzmg_time_kernel(stzmg_t h, int kind, int level, int reps, double *ms_per_launch);
+
+/* Operator builders (host side, FP64).
+ * stzmg_shear_stencil: stencil = -dt*stencil_base and fix of
+ *   set_up_stencil_base / set_up_stencil_and_matrices
+ *   (src/shear_sim_3d.cpp:216-331, src/shear_sim_3d.hh:45-55).
+ * stzmg_trans_stencil: the transformed-coordinate blocks of
+ *   trans_sim_3d::set_up_stencil_and_matrices (src/trans_sim_3d.cpp:1187-1489,
+ *   src/stenc_funcs.hh), scaled by -dt. */
+int stzmg_shear_stencil(double mu, double lamb, double dx, double dy, double dz, double dt,
+                        double kappa, double *blocks27x9, double *fix9);
+int stzmg_trans_stencil(const double *T9, double mu, double lamb, double dx, double dy,
+                        double dz, double dt, double *blocks27x9);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

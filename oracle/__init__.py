@@ -1,0 +1,442 @@
+"""CPU oracle for the STZ++ hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this package, and only as the checker.  It holds:
+
+* ``liboracle.so`` (mg_oracle.c): a plain-C FP64 restatement of the reference
+  multigrid (mg_3d_template/region.cc, multigrid.cc) -- residual, lexicographic
+  and 8-colour block Gauss-Seidel, restriction, interpolation, Galerkin RAT,
+  V-cycle, the sum |r|^4 stopping quantity;
+* numpy restatements of the operator builders: the shear stencil
+  (src/shear_sim_3d.cpp:216-331, src/shear_sim_3d.hh:45-55) and the
+  transformed-coordinate stencil (src/trans_sim_3d.cpp:1187-1489 with
+  src/stenc_funcs.hh);
+* ``RefMG`` / ``RefShear``: ctypes views of oracle/_ref/libstzref.so, the
+  reference sources compiled in place (built only where the reference tree
+  exists).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip) if a is not None else None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (run make -C oracle)")
+        L = ctypes.CDLL(path)
+        L.mgo_new.restype = ctypes.c_void_p
+        L.mgo_new.argtypes = [ctypes.c_int] * 3 + [_ip, ctypes.c_int, ctypes.c_long]
+        vp = ctypes.c_void_p
+        for name, args, res in [
+            ("mgo_delete", [vp], None),
+            ("mgo_nlev", [vp], ctypes.c_int),
+            ("mgo_dims", [vp, ctypes.c_int, _ip], None),
+            ("mgo_set_level_ops", [vp, ctypes.c_int, _dp], None),
+            ("mgo_get_level_ops", [vp, ctypes.c_int, _dp], None),
+            ("mgo_set_fields", [vp, ctypes.c_int, _dp, _dp], None),
+            ("mgo_get_fields", [vp, ctypes.c_int, _dp, _dp], None),
+            ("mgo_params", [vp] + [ctypes.c_int] * 5, None),
+            ("mgo_do_rat", [vp], None),
+            ("mgo_do_residual", [vp, ctypes.c_int, _dp], None),
+            ("mgo_do_apply", [vp, ctypes.c_int, _dp, _dp], None),
+            ("mgo_do_gs", [vp, ctypes.c_int, ctypes.c_int], None),
+            ("mgo_do_restrict", [vp, ctypes.c_int], None),
+            ("mgo_do_interp", [vp, ctypes.c_int], None),
+            ("mgo_do_vcycle", [vp, ctypes.c_int], None),
+            ("mgo_do_norm_quartic", [vp, ctypes.c_int], ctypes.c_double),
+            ("mgo_do_norm2", [vp, ctypes.c_int], ctypes.c_double),
+        ]:
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _LIB = L
+    return _LIB
+
+
+PLANE, POINT = 0, 1
+
+
+class OracleMG:
+    """Multigrid hierarchy restated from the reference (FP64, CPU)."""
+
+    def __init__(self, m, n, o, prd, mode=PLANE, coarsest_total=1024):
+        self.L = lib()
+        p = np.ascontiguousarray(prd, dtype=np.int32)
+        self.h = self.L.mgo_new(m, n, o, _i(p), mode, coarsest_total)
+        if not self.h:
+            raise ValueError("oracle hierarchy construction failed")
+        self.mode = mode
+
+    def close(self):
+        if self.h:
+            self.L.mgo_delete(self.h)
+            self.h = None
+
+    __del__ = close
+
+    @property
+    def nlevels(self):
+        return self.L.mgo_nlev(self.h)
+
+    def dims(self, l):
+        d = np.zeros(3, np.int32)
+        self.L.mgo_dims(self.h, l, _i(d))
+        return tuple(int(v) for v in d)
+
+    def npts(self, l):
+        m, n, o = self.dims(l)
+        return m * n * o
+
+    def ncls(self, l):
+        m, n, o = self.dims(l)
+        return o if self.mode == PLANE else m * n * o
+
+    def set_ops(self, l, ops):
+        a = np.ascontiguousarray(ops, dtype=np.float64).reshape(self.ncls(l), 243)
+        self.L.mgo_set_level_ops(self.h, l, _d(a))
+
+    def ops(self, l):
+        a = np.zeros((self.ncls(l), 27, 3, 3))
+        self.L.mgo_get_level_ops(self.h, l, _d(a))
+        return a
+
+    def set_fields(self, l, x=None, r=None):
+        N = self.npts(l)
+        xa = None if x is None else np.ascontiguousarray(x, dtype=np.float64).reshape(N, 3)
+        ra = None if r is None else np.ascontiguousarray(r, dtype=np.float64).reshape(N, 3)
+        self.L.mgo_set_fields(self.h, l, _d(xa), _d(ra))
+
+    def fields(self, l):
+        N = self.npts(l)
+        x = np.zeros((N, 3))
+        r = np.zeros((N, 3))
+        self.L.mgo_get_fields(self.h, l, _d(x), _d(r))
+        return x, r
+
+    def params(self, v_down=2, v_up=2, bottom=24, growth=1, smoother=0):
+        self.L.mgo_params(self.h, v_down, v_up, bottom, growth, smoother)
+
+    def rat(self):
+        self.L.mgo_do_rat(self.h)
+
+    def residual(self, l=0):
+        out = np.zeros((self.npts(l), 3))
+        self.L.mgo_do_residual(self.h, l, _d(out))
+        return out
+
+    def apply(self, l, x):
+        xa = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros_like(xa)
+        self.L.mgo_do_apply(self.h, l, _d(xa), _d(out))
+        return out
+
+    def gs(self, l=0, n=1):
+        self.L.mgo_do_gs(self.h, l, n)
+
+    def restrict(self, l):
+        self.L.mgo_do_restrict(self.h, l)
+
+    def interp(self, l):
+        self.L.mgo_do_interp(self.h, l)
+
+    def vcycle(self, n=1):
+        self.L.mgo_do_vcycle(self.h, n)
+
+    def norm_quartic(self, l=0):
+        return self.L.mgo_do_norm_quartic(self.h, l)
+
+    def norm2(self, l=0):
+        return self.L.mgo_do_norm2(self.h, l)
+
+
+# ---------------------------------------------------------------------------
+# Operator builders (numpy restatements).
+
+def block_index(di, dj, dk):
+    """Stencil slot of offset (di, dj, dk): 13 + di + 3 dj + 9 dk
+    (src/shear_sim_3d.cpp:222)."""
+    return 13 + di + 3 * dj + 9 * dk
+
+
+def shear_stencil_base(mu, lamb, dx, dy, dz, kappa=0.0, dt=1.0):
+    """stencil_base of shear_sim_3d (src/shear_sim_3d.cpp:216-331): the 27
+    3x3 blocks of the staggered elastic operator, plus the kappa/dt viscous
+    terms (zero in QS runs).  The centre block keeps the reference's a33
+    parenthesisation, in which kappa/dt*(...) is added rather than subtracted
+    (src/shear_sim_3d.cpp:276-278); it is inert for kappa = 0."""
+    S = np.zeros((27, 3, 3))
+    L2 = lamb + 2 * mu
+    kd = kappa / dt
+    ax = np.array([L2, mu, mu]) / dx**2 + kd / dx**2
+    ay = np.array([mu, L2, mu]) / dy**2 + kd / dy**2
+    az = np.array([mu, mu, L2]) / dz**2 + kd / dz**2
+    for s in (-1, 1):
+        S[block_index(s, 0, 0)] = np.diag(ax)
+        S[block_index(0, s, 0)] = np.diag(ay)
+        S[block_index(0, 0, s)] = np.diag(az)
+    c = (lamb + mu) / 4
+    # mixed x-y blocks: +c/(dx dy) at (+,+) and (-,-), -c at (+,-), (-,+)
+    for si in (-1, 1):
+        for sj in (-1, 1):
+            v = si * sj * c / (dx * dy)
+            B = np.zeros((3, 3)); B[0, 1] = B[1, 0] = v
+            S[block_index(si, sj, 0)] = B
+            v = si * sj * c / (dx * dz)
+            B = np.zeros((3, 3)); B[0, 2] = B[2, 0] = v
+            S[block_index(si, 0, sj)] = B
+            v = si * sj * c / (dy * dz)
+            B = np.zeros((3, 3)); B[1, 2] = B[2, 1] = v
+            S[block_index(0, si, sj)] = B
+    ksum = kd * (1 / dx**2 + 1 / dy**2 + 1 / dz**2)
+    S[13] = np.diag([
+        -2 * (L2 / dx**2 + mu / dy**2 + mu / dz**2 + ksum),
+        -2 * (mu / dx**2 + L2 / dy**2 + mu / dz**2 + ksum),
+        -2 * (mu / dx**2 + mu / dy**2 + L2 / dz**2) + ksum,
+    ])
+    return S
+
+
+def shear_operator(mu, lamb, dx, dy, dz, dt, kappa=0.0):
+    """(stencil, fix) of set_up_stencil_and_matrices (src/shear_sim_3d.hh:45-55):
+    stencil = -dt * base; fix = -dt*2*((lamb+2mu)/dx^2 + mu/dy^2 + mu/dz^2) I."""
+    st = -dt * shear_stencil_base(mu, lamb, dx, dy, dz, kappa, dt)
+    fix = -dt * 2 * ((lamb + 2 * mu) / dx**2 + mu / dy**2 + mu / dz**2) * np.eye(3)
+    return st, fix
+
+
+def trans_stencil_blocks(T, mu, lamb, dx, dy, dz):
+    """The 19 non-zero blocks of src/stenc_funcs.hh as one closed form.
+
+    With S the symmetric matrix built from the upper triangle of T (the
+    generated functions never read t21, t31, t32) and W = adj(S)/det(T), the
+    coefficient of d_a d_b U_m in row i of the transformed elastic operator is
+        K[i,m,a,b] = mu S_im sum_j W_aj W_bj + (lamb+mu) W_ai delta_bm
+    (symmetrised over a<->b).  Axis blocks take K/h_a^2, edge blocks
+    +-K/(4 h_a h_b), the centre block -2 sum_a K[.,.,a,a]/h_a^2."""
+    T = np.asarray(T, dtype=np.float64).reshape(3, 3)
+    S = np.triu(T) + np.triu(T, 1).T
+    D = np.linalg.det(T)
+    adj = np.zeros((3, 3))
+    for a in range(3):
+        for b in range(3):
+            minor = np.delete(np.delete(S, b, 0), a, 1)
+            adj[a, b] = (-1) ** (a + b) * np.linalg.det(minor)
+    W = adj / D
+    h = np.array([dx, dy, dz])
+    WW = W @ W.T
+    K = np.zeros((3, 3, 3, 3))
+    for i in range(3):
+        for m in range(3):
+            for a in range(3):
+                for b in range(3):
+                    K[i, m, a, b] = mu * S[i, m] * WW[a, b] + (lamb + mu) * W[a, i] * (b == m)
+    out = np.zeros((27, 3, 3))
+    for a in range(3):
+        off = [0, 0, 0]
+        off[a] = 1
+        blk = K[:, :, a, a] / h[a] ** 2
+        out[block_index(*off)] += blk
+        off[a] = -1
+        out[block_index(*off)] += blk
+        out[13] -= 2 * blk
+    for a in range(3):
+        for b in range(a + 1, 3):
+            Ks = (K[:, :, a, b] + K[:, :, b, a]) / (4 * h[a] * h[b])
+            for sa in (-1, 1):
+                for sb in (-1, 1):
+                    off = [0, 0, 0]
+                    off[a] = sa
+                    off[b] = sb
+                    out[block_index(*off)] += sa * sb * Ks
+    return out
+
+
+def point_operator(dims, prd, stencil, fix=None):
+    """Per-point boxes S[6] and 27-block stencils for a problem whose interior
+    points carry ``stencil`` and whose points on a non-periodic face carry the
+    one-block ``fix`` with box [0,1)^3 (src/shear_sim_3d.hh:388-402)."""
+    m, n, o = dims
+    N = m * n * o
+    S = np.zeros((N, 6), np.int32)
+    B = np.zeros((N, 27, 3, 3))
+    k, j, i = np.meshgrid(np.arange(o), np.arange(n), np.arange(m), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    bnd = np.zeros(N, bool)
+    if not prd[0]:
+        bnd |= (i == 0) | (i == m - 1)
+    if not prd[1]:
+        bnd |= (j == 0) | (j == n - 1)
+    if not prd[2]:
+        bnd |= (k == 0) | (k == o - 1)
+    S[:] = [-1, 2, -1, 2, -1, 2]
+    S[bnd] = [0, 1, 0, 1, 0, 1]
+    B[~bnd] = stencil
+    if fix is not None:
+        B[bnd, 13] = fix
+    return S, B
+
+
+def plane_operator(o, prd_z, stencil, fix):
+    """Per-plane table for an operator with Dirichlet planes k=0, o-1 (z walled)."""
+    ops = np.repeat(np.asarray(stencil)[None], o, axis=0).copy()
+    if not prd_z:
+        for k in (0, o - 1):
+            ops[k] = 0
+            ops[k, 13] = fix
+    return ops
+
+
+# ---------------------------------------------------------------------------
+# Reference library (compiled from the reference tree where it exists).
+
+def ref_lib():
+    global _REF
+    if _REF is None:
+        path = os.path.join(HERE, "_ref", "libstzref.so")
+        if not os.path.exists(path):
+            return None
+        L = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+        vp = ctypes.c_void_p
+        L.ref_mg_create.restype = vp
+        L.ref_mg_create.argtypes = [ctypes.c_int] * 3 + [_ip, _ip, _dp]
+        for name, args, res in [
+            ("ref_mg_destroy", [vp], None),
+            ("ref_mg_nlevels", [vp], ctypes.c_int),
+            ("ref_mg_dims", [vp, ctypes.c_int, _ip], None),
+            ("ref_mg_set_sweeps", [vp, ctypes.c_int, ctypes.c_int], None),
+            ("ref_mg_get_operator", [vp, ctypes.c_int, _dp], ctypes.c_int),
+            ("ref_mg_get_boxes", [vp, ctypes.c_int, _ip], None),
+            ("ref_mg_set_fields", [vp, ctypes.c_int, _dp, _dp], None),
+            ("ref_mg_get_fields", [vp, ctypes.c_int, _dp, _dp], None),
+            ("ref_mg_residual", [vp, ctypes.c_int, _dp], None),
+            ("ref_mg_gs", [vp, ctypes.c_int, ctypes.c_int], None),
+            ("ref_mg_restrict", [vp, ctypes.c_int], None),
+            ("ref_mg_interp", [vp, ctypes.c_int], None),
+            ("ref_mg_vcycle", [vp, ctypes.c_int], None),
+            ("ref_mg_l2_error_all", [vp, ctypes.c_int], ctypes.c_double),
+            ("ref_trans_stencil", [_dp] + [ctypes.c_double] * 5 + [_dp], None),
+            ("ref_shear_create", [ctypes.c_int] * 3 + [_dp, _dp, _ip, ctypes.c_char_p], vp),
+            ("ref_shear_setup", [vp, ctypes.c_double], None),
+            ("ref_shear_get_stencil", [vp, _dp, _dp], None),
+            ("ref_shear_set_state", [vp, ctypes.c_double, _dp, _dp, _dp], None),
+            ("ref_shear_step", [vp, ctypes.c_double, ctypes.c_int], None),
+            ("ref_shear_get_state", [vp, _dp, _dp, _dp, _dp, _dp], None),
+            ("ref_shear_vcycles", [vp], ctypes.c_int),
+            ("ref_shear_destroy", [vp], None),
+        ]:
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _REF = L
+    return _REF
+
+
+class RefMG:
+    """The reference multigrid<vec3,mat3> on one rank, fed per-point stencils."""
+
+    def __init__(self, dims, prd, S, B):
+        self.L = ref_lib()
+        if self.L is None:
+            raise RuntimeError("reference library not built")
+        self._S = np.ascontiguousarray(S, dtype=np.int32)
+        self._B = np.ascontiguousarray(B, dtype=np.float64)
+        p = np.ascontiguousarray(prd, dtype=np.int32)
+        self.h = self.L.ref_mg_create(dims[0], dims[1], dims[2], _i(p), _i(self._S), _d(self._B))
+
+    def close(self):
+        if self.h:
+            self.L.ref_mg_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    @property
+    def nlevels(self):
+        return self.L.ref_mg_nlevels(self.h)
+
+    def dims(self, l):
+        d = np.zeros(3, np.int32)
+        self.L.ref_mg_dims(self.h, l, _i(d))
+        return tuple(int(v) for v in d)
+
+    def npts(self, l):
+        m, n, o = self.dims(l)
+        return m * n * o
+
+    def operator(self, l):
+        out = np.zeros((self.npts(l), 27, 3, 3))
+        if self.L.ref_mg_get_operator(self.h, l, _d(out)) != 0:
+            raise ValueError("reference box wider than 3x3x3")
+        return out
+
+    def set_fields(self, l, x=None, r=None):
+        N = self.npts(l)
+        xa = None if x is None else np.ascontiguousarray(x, dtype=np.float64).reshape(N, 3)
+        ra = None if r is None else np.ascontiguousarray(r, dtype=np.float64).reshape(N, 3)
+        self.L.ref_mg_set_fields(self.h, l, _d(xa), _d(ra))
+
+    def fields(self, l):
+        N = self.npts(l)
+        x = np.zeros((N, 3))
+        r = np.zeros((N, 3))
+        self.L.ref_mg_get_fields(self.h, l, _d(x), _d(r))
+        return x, r
+
+    def residual(self, l=0):
+        out = np.zeros((self.npts(l), 3))
+        self.L.ref_mg_residual(self.h, l, _d(out))
+        return out
+
+    def gs(self, l=0, n=1):
+        self.L.ref_mg_gs(self.h, l, n)
+
+    def restrict(self, l):
+        self.L.ref_mg_restrict(self.h, l)
+
+    def interp(self, l):
+        self.L.ref_mg_interp(self.h, l)
+
+    def vcycle(self, n=1):
+        self.L.ref_mg_vcycle(self.h, n)
+
+    def l2_error_all(self, l=0):
+        return self.L.ref_mg_l2_error_all(self.h, l)
+
+
+def ref_trans_stencil(T, mu, lamb, dx, dy, dz):
+    L = ref_lib()
+    Ta = np.ascontiguousarray(T, dtype=np.float64).reshape(9)
+    out = np.zeros((27, 3, 3))
+    L.ref_trans_stencil(_d(Ta), mu, lamb, dx, dy, dz, _d(out))
+    return out
+
+
+def det_field(N, salt, comps=3):
+    """Deterministic pseudo-random field in [-1, 1) built from integer
+    arithmetic only, so a fixture can be regenerated bit for bit anywhere."""
+    p = np.arange(N, dtype=np.int64)[:, None]
+    c = np.arange(comps, dtype=np.int64)[None, :]
+    v = (p * 7919 + c * 104729 + salt * 15485863) % 2000003
+    v = (v * 48271) % 2000003
+    return v.astype(np.float64) / 1000001.5 - 1.0

@@ -1,0 +1,869 @@
+// Host side of the multigrid projection solver and its C ABI (include/stzmg.h).
+//
+// The hierarchy, cycle schedule and solve loop follow the reference
+// multigrid<vec3,M> (mg_3d_template/multigrid.cc:136-297) and the projection
+// loop of shear_sim_3d::projection_step (src/shear_sim_3d.cpp:529-600); all
+// arithmetic runs in the kernels of mg_kernels.hpp on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/stzmg.h"
+#include "mg_kernels.hpp"
+
+using namespace stzmg;
+
+namespace {
+
+struct Level {
+    Dims d{};
+    int mode = 0;
+    long ncls = 0;
+    double *ops64 = nullptr, *inv64 = nullptr;
+    float *ops32 = nullptr, *inv32 = nullptr;
+    double *x64 = nullptr, *r64 = nullptr, *t64 = nullptr;
+    float *x32 = nullptr, *r32 = nullptr, *t32 = nullptr;
+};
+
+inline unsigned nblk(long n, int t = kThreads) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+struct stzmg_s {
+    stzmg_desc desc{};
+    std::vector<Level> lv;
+    hipStream_t stream = nullptr;
+    std::string err;
+    double *part = nullptr;   // per-block norm partials
+    long part_cap = 0;
+    double *dres = nullptr;   // reduced norms (2 doubles)
+    int *dflag = nullptr;     // singular-block flag
+    bool ops_set = false;
+    int std_iters = 16;       // per-handle copy of the reference's static heuristic
+    hipGraphExec_t g64 = nullptr, g32 = nullptr;
+};
+
+namespace {
+
+int fail(stzmg_t h, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (h) h->err = buf;
+    return code;
+}
+
+#define HIPCHK(h, call)                                                                \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail((h), STZMG_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCHCHK(h)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = hipGetLastError();                                             \
+        if (e_ != hipSuccess)                                                          \
+            return fail((h), STZMG_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e_)); \
+    } while (0)
+
+int coarse_size(int nf, int prd) { return (nf + (prd ? 1 : 2)) >> 1; }
+
+void free_level(Level &L) {
+    hipFree(L.ops64); hipFree(L.inv64); hipFree(L.ops32); hipFree(L.inv32);
+    hipFree(L.x64); hipFree(L.r64); hipFree(L.t64);
+    hipFree(L.x32); hipFree(L.r32); hipFree(L.t32);
+    L = Level{};
+}
+
+template <typename T>
+hipError_t alloc(T **p, long n) {
+    return hipMalloc((void **)p, sizeof(T) * (size_t)(n > 0 ? n : 1));
+}
+
+// ---- kernel launch wrappers ------------------------------------------------
+
+template <typename T>
+const T *ops_of(const Level &L);
+template <>
+const double *ops_of<double>(const Level &L) { return L.ops64; }
+template <>
+const float *ops_of<float>(const Level &L) { return L.ops32; }
+template <typename T>
+const T *inv_of(const Level &L);
+template <>
+const double *inv_of<double>(const Level &L) { return L.inv64; }
+template <>
+const float *inv_of<float>(const Level &L) { return L.inv32; }
+template <typename T>
+T *x_of(Level &L);
+template <>
+double *x_of<double>(Level &L) { return L.x64; }
+template <>
+float *x_of<float>(Level &L) { return L.x32; }
+template <typename T>
+T *r_of(Level &L);
+template <>
+double *r_of<double>(Level &L) { return L.r64; }
+template <>
+float *r_of<float>(Level &L) { return L.r32; }
+template <typename T>
+T *t_of(Level &L);
+template <>
+double *t_of<double>(Level &L) { return L.t64; }
+template <>
+float *t_of<float>(Level &L) { return L.t32; }
+
+// out = r - A x on level L (optionally with norm partials into h->part).
+template <typename T, bool NORM>
+void launch_resid(stzmg_t h, Level &L, const T *x, const T *r, T *out) {
+    dim3 g(nblk((long)L.d.m * L.d.n), L.d.o);
+    if (L.mode == STZMG_OP_PLANE)
+        hipLaunchKernelGGL((k_resid<T, 0, NORM, false>), g, dim3(kThreads), 0, h->stream, L.d, x, r,
+                           ops_of<T>(L), out, h->part);
+    else
+        hipLaunchKernelGGL((k_resid<T, 1, NORM, false>), g, dim3(kThreads), 0, h->stream, L.d, x, r,
+                           ops_of<T>(L), out, h->part);
+}
+
+template <typename T>
+void launch_apply(stzmg_t h, Level &L, const T *x, T *out) {
+    dim3 g(nblk((long)L.d.m * L.d.n), L.d.o);
+    if (L.mode == STZMG_OP_PLANE)
+        hipLaunchKernelGGL((k_resid<T, 0, false, true>), g, dim3(kThreads), 0, h->stream, L.d, x,
+                           (const T *)nullptr, ops_of<T>(L), out, h->part);
+    else
+        hipLaunchKernelGGL((k_resid<T, 1, false, true>), g, dim3(kThreads), 0, h->stream, L.d, x,
+                           (const T *)nullptr, ops_of<T>(L), out, h->part);
+}
+
+int ncol(int n, int prd) { return (prd && (n & 1)) ? 3 : 2; }
+
+template <typename T>
+void launch_gs_sweep(stzmg_t h, Level &L) {
+    const Dims &d = L.d;
+    const int ncx = ncol(d.m, d.px), ncy = ncol(d.n, d.py), ncz = ncol(d.o, d.pz);
+    for (int cz = 0; cz < ncz; cz++)
+        for (int cy = 0; cy < ncy; cy++)
+            for (int cx = 0; cx < ncx; cx++) {
+                const int cntx = col_count(d.m, cx, d.px), cnty = col_count(d.n, cy, d.py),
+                          cntz = col_count(d.o, cz, d.pz);
+                if (cntx == 0 || cnty == 0 || cntz == 0) continue;
+                dim3 g(nblk((long)cntx * cnty), cntz);
+                if (L.mode == STZMG_OP_PLANE)
+                    hipLaunchKernelGGL((k_gs<T, 0>), g, dim3(kThreads), 0, h->stream, d, x_of<T>(L),
+                                       (const T *)r_of<T>(L), ops_of<T>(L), inv_of<T>(L), cx, cy, cz,
+                                       cntx, cnty);
+                else
+                    hipLaunchKernelGGL((k_gs<T, 1>), g, dim3(kThreads), 0, h->stream, d, x_of<T>(L),
+                                       (const T *)r_of<T>(L), ops_of<T>(L), inv_of<T>(L), cx, cy, cz,
+                                       cntx, cnty);
+            }
+}
+
+// Level l (coarse) receives the restricted residual of level l-1.
+template <typename T>
+void launch_restrict(stzmg_t h, int l) {
+    Level &F = h->lv[l - 1], &C = h->lv[l];
+    launch_resid<T, false>(h, F, x_of<T>(F), r_of<T>(F), t_of<T>(F));
+    dim3 g(nblk((long)C.d.m * C.d.n), C.d.o);
+    hipLaunchKernelGGL((k_restrict<T>), g, dim3(kThreads), 0, h->stream, F.d, C.d,
+                       (const T *)t_of<T>(F), r_of<T>(C), x_of<T>(C));
+}
+
+template <typename T>
+void launch_interp(stzmg_t h, int l) {
+    Level &F = h->lv[l - 1], &C = h->lv[l];
+    dim3 g(nblk((long)F.d.m * F.d.n), F.d.o);
+    hipLaunchKernelGGL((k_interp<T>), g, dim3(kThreads), 0, h->stream, F.d, C.d,
+                       (const T *)x_of<T>(C), x_of<T>(F));
+}
+
+int sweeps_down(stzmg_t h, int l) { return h->desc.v_down + h->desc.sweep_growth * l; }
+int sweeps_up(stzmg_t h, int l) { return h->desc.v_up + h->desc.sweep_growth * l; }
+
+// One V-cycle (multigrid::v_cycle, multigrid.cc:253-297) in type T.  In FP32
+// the coarsest level is solved in FP64: r32 -> r64, bottom sweeps, x64 -> x32.
+template <typename T>
+void enqueue_vcycle(stzmg_t h) {
+    const int L = (int)h->lv.size();
+    for (int i = 0; i < L - 1; i++) {
+        for (int q = 0; q < sweeps_down(h, i); q++) launch_gs_sweep<T>(h, h->lv[i]);
+        launch_restrict<T>(h, i + 1);
+    }
+    Level &B = h->lv[L - 1];
+    const long n3 = 3 * B.d.N;
+    if (sizeof(T) == 4 && L > 1) {
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream,
+                           (const float *)B.r32, B.r64, n3);
+        hipLaunchKernelGGL((k_fill<double>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream, B.x64, 0.0, n3);
+        for (int q = 0; q < h->desc.bottom_sweeps; q++) launch_gs_sweep<double>(h, B);
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream,
+                           (const double *)B.x64, B.x32, n3);
+    } else {
+        for (int q = 0; q < h->desc.bottom_sweeps; q++) launch_gs_sweep<T>(h, B);
+    }
+    for (int i = L - 1; i > 0; i--) {
+        launch_interp<T>(h, i);
+        for (int q = 0; q < sweeps_up(h, i - 1); q++) launch_gs_sweep<T>(h, h->lv[i - 1]);
+    }
+}
+
+int ensure_part(stzmg_t h, long nb) {
+    if (nb <= h->part_cap) return 0;
+    hipFree(h->part);
+    h->part = nullptr;
+    HIPCHK(h, alloc(&h->part, 2 * nb));
+    h->part_cap = nb;
+    return 0;
+}
+
+// Residual norms of level l in FP64: out[0] = sum |r|^4, out[1] = sum |r|^2.
+// The residual is left in t64 of that level.
+int norms64(stzmg_t h, int l, double out[2]) {
+    Level &L = h->lv[l];
+    const long nb = (long)nblk((long)L.d.m * L.d.n) * L.d.o;
+    if (ensure_part(h, nb)) return STZMG_ERR_NOMEM;
+    launch_resid<double, true>(h, L, L.x64, L.r64, L.t64);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, h->stream, (const double *)h->part, nb,
+                       h->dres);
+    LAUNCHCHK(h);
+    HIPCHK(h, hipMemcpyAsync(out, h->dres, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int check_level(stzmg_t h, int l) {
+    if (!h) return STZMG_ERR_ARG;
+    if (l < 0 || l >= (int)h->lv.size()) return fail(h, STZMG_ERR_ARG, "level %d out of range", l);
+    return 0;
+}
+
+int require_ops(stzmg_t h) {
+    if (!h->ops_set) return fail(h, STZMG_ERR_STATE, "operator not set");
+    return 0;
+}
+
+// Galerkin hierarchy and inverse tables after the fine operator changed.
+int build_hierarchy_ops(stzmg_t h) {
+    HIPCHK(h, hipMemsetAsync(h->dflag, 0, sizeof(int), h->stream));
+    for (size_t l = 0; l < h->lv.size(); l++) {
+        Level &L = h->lv[l];
+        if (l > 0) {
+            Level &F = h->lv[l - 1];
+            const long nthr = L.ncls * 27;
+            if (F.mode == STZMG_OP_PLANE && L.mode == STZMG_OP_PLANE)
+                hipLaunchKernelGGL((k_rat<0, 0>), dim3(nblk(nthr, 64)), dim3(64), 0, h->stream, F.d, L.d,
+                                   (const double *)F.ops64, L.ops64, L.ncls);
+            else if (F.mode == STZMG_OP_POINT && L.mode == STZMG_OP_POINT)
+                hipLaunchKernelGGL((k_rat<1, 1>), dim3(nblk(nthr, 64)), dim3(64), 0, h->stream, F.d, L.d,
+                                   (const double *)F.ops64, L.ops64, L.ncls);
+            else
+                return fail(h, STZMG_ERR_STATE, "mixed operator modes");
+            LAUNCHCHK(h);
+        }
+        const long ne = L.ncls * kBlk;
+        hipLaunchKernelGGL(k_cast_d2f, dim3(nblk(ne)), dim3(kThreads), 0, h->stream, (const double *)L.ops64,
+                           L.ops32, ne);
+        hipLaunchKernelGGL(k_inverse, dim3(nblk(L.ncls, 64)), dim3(64), 0, h->stream, (const double *)L.ops64,
+                           L.ncls, L.inv64, L.inv32, h->dflag);
+        LAUNCHCHK(h);
+    }
+    int flag = 0;
+    HIPCHK(h, hipMemcpyAsync(&flag, h->dflag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (h->g64) { hipGraphExecDestroy(h->g64); h->g64 = nullptr; }
+    if (h->g32) { hipGraphExecDestroy(h->g32); h->g32 = nullptr; }
+    h->ops_set = true;
+    if (flag) return fail(h, STZMG_ERR_SINGULAR, "singular or non-finite centre block on some level");
+    return 0;
+}
+
+template <typename T>
+int capture_vcycle(stzmg_t h, hipGraphExec_t *exec) {
+    if (*exec) return 0;
+    hipGraph_t g = nullptr;
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_vcycle<T>(h);
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (e != hipSuccess) return fail(h, STZMG_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(h, STZMG_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+    return 0;
+}
+
+template <typename T>
+int run_vcycles(stzmg_t h, int n, bool graph) {
+    if (graph) {
+        hipGraphExec_t *ex = sizeof(T) == 8 ? &h->g64 : &h->g32;
+        int rc = capture_vcycle<T>(h, ex);
+        if (rc) return rc;
+        for (int q = 0; q < n; q++) HIPCHK(h, hipGraphLaunch(*ex, h->stream));
+    } else {
+        for (int q = 0; q < n; q++) enqueue_vcycle<T>(h);
+        LAUNCHCHK(h);
+    }
+    return 0;
+}
+
+bool size_supported(int n, int prd) { return prd ? (n % 2 == 0) : (n % 2 == 1); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int stzmg_desc_default(stzmg_desc *d, int m, int n, int o, int px, int py, int pz) {
+    if (!d) return STZMG_ERR_ARG;
+    d->m = m; d->n = n; d->o = o;
+    d->prd[0] = px; d->prd[1] = py; d->prd[2] = pz;
+    d->coarsest_total = 1024;
+    d->v_down = 2; d->v_up = 2;
+    d->bottom_sweeps = 24;
+    d->sweep_growth = 1;
+    d->op_mode = (px && py) ? STZMG_OP_PLANE : STZMG_OP_POINT;
+    return 0;
+}
+
+static thread_local std::string g_create_err;
+
+int stzmg_create(const stzmg_desc *d, stzmg_t *out) {
+    if (!d || !out) return STZMG_ERR_ARG;
+    *out = nullptr;
+    stzmg_t h = new stzmg_s;
+    h->desc = *d;
+    auto bail = [&](int code) {
+        g_create_err = h->err;
+        for (auto &L : h->lv) free_level(L);
+        if (h->stream) hipStreamDestroy(h->stream);
+        delete h;
+        return code;
+    };
+    if (d->m < 3 || d->n < 3 || d->o < 3 || d->coarsest_total < 1 || d->v_down < 0 || d->v_up < 0 ||
+        d->bottom_sweeps < 0 || d->sweep_growth < 0)
+        return bail(fail(h, STZMG_ERR_ARG, "invalid descriptor"));
+    if (d->op_mode == STZMG_OP_PLANE && !(d->prd[0] && d->prd[1]))
+        return bail(fail(h, STZMG_ERR_UNSUPPORTED, "PLANE operators need periodic x and y"));
+    // Level sizes: m' = (m + (prd?1:2)) >> 1 while m*n*o > coarsest_total
+    // (region.cc:856, multigrid.cc:157).
+    std::vector<Dims> dims;
+    Dims cur{d->m, d->n, d->o, d->prd[0] != 0, d->prd[1] != 0, d->prd[2] != 0, 0};
+    for (;;) {
+        cur.N = (long)cur.m * cur.n * cur.o;
+        dims.push_back(cur);
+        if (cur.N <= d->coarsest_total) break;
+        if ((int)dims.size() == 16) return bail(fail(h, STZMG_ERR_UNSUPPORTED, "more than 16 levels"));
+        // Coarsened dimensions must be periodic-even or non-periodic-odd: for
+        // other sizes the reference's transfer routines deviate from P^T A P
+        // and from R = P^T/8 (odd periodic wrap in the RAT; the even
+        // non-periodic restriction of region.cc:2230-2240).
+        if (!size_supported(cur.m, cur.px) || !size_supported(cur.n, cur.py) ||
+            !size_supported(cur.o, cur.pz))
+            return bail(fail(h, STZMG_ERR_UNSUPPORTED,
+                             "level %zu size %dx%dx%d: coarsened dimensions must be even when periodic "
+                             "and odd when walled",
+                             dims.size() - 1, cur.m, cur.n, cur.o));
+        Dims nx = cur;
+        nx.m = coarse_size(cur.m, cur.px);
+        nx.n = coarse_size(cur.n, cur.py);
+        nx.o = coarse_size(cur.o, cur.pz);
+        if ((nx.px && nx.m < 3) || (nx.py && nx.n < 3) || (nx.pz && nx.o < 3) || nx.m < 2 || nx.n < 2 ||
+            nx.o < 2)
+            return bail(fail(h, STZMG_ERR_UNSUPPORTED, "grid too small to coarsen to %d points",
+                             (int)d->coarsest_total));
+        cur = nx;
+    }
+    if (hipStreamCreate(&h->stream) != hipSuccess) return bail(fail(h, STZMG_ERR_HIP, "hipStreamCreate failed"));
+    h->lv.resize(dims.size());
+    for (size_t l = 0; l < dims.size(); l++) {
+        Level &L = h->lv[l];
+        L.d = dims[l];
+        L.mode = d->op_mode;
+        L.ncls = L.mode == STZMG_OP_PLANE ? L.d.o : L.d.N;
+        const long n3 = 3 * L.d.N;
+        if (alloc(&L.ops64, L.ncls * kBlk) || alloc(&L.ops32, L.ncls * kBlk) || alloc(&L.inv64, L.ncls * 9) ||
+            alloc(&L.inv32, L.ncls * 9) || alloc(&L.x64, n3) || alloc(&L.r64, n3) || alloc(&L.t64, n3) ||
+            alloc(&L.x32, n3) || alloc(&L.r32, n3) || alloc(&L.t32, n3))
+            return bail(fail(h, STZMG_ERR_NOMEM, "device allocation failed at level %zu", l));
+        hipMemsetAsync(L.ops64, 0, sizeof(double) * L.ncls * kBlk, h->stream);
+        hipMemsetAsync(L.x64, 0, sizeof(double) * n3, h->stream);
+        hipMemsetAsync(L.r64, 0, sizeof(double) * n3, h->stream);
+        hipMemsetAsync(L.x32, 0, sizeof(float) * n3, h->stream);
+        hipMemsetAsync(L.r32, 0, sizeof(float) * n3, h->stream);
+    }
+    if (alloc(&h->dres, 2) || alloc(&h->dflag, 1)) return bail(fail(h, STZMG_ERR_NOMEM, "allocation failed"));
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(fail(h, STZMG_ERR_HIP, "init sync failed"));
+    *out = h;
+    return 0;
+}
+
+int stzmg_destroy(stzmg_t h) {
+    if (!h) return STZMG_ERR_ARG;
+    hipStreamSynchronize(h->stream);
+    if (h->g64) hipGraphExecDestroy(h->g64);
+    if (h->g32) hipGraphExecDestroy(h->g32);
+    for (auto &L : h->lv) free_level(L);
+    hipFree(h->part);
+    hipFree(h->dres);
+    hipFree(h->dflag);
+    hipStreamDestroy(h->stream);
+    delete h;
+    return 0;
+}
+
+const char *stzmg_last_error(stzmg_t h) { return h ? h->err.c_str() : g_create_err.c_str(); }
+
+int stzmg_nlevels(stzmg_t h) { return h ? (int)h->lv.size() : STZMG_ERR_ARG; }
+
+int stzmg_level_dims(stzmg_t h, int level, int dims[3]) {
+    if (int rc = check_level(h, level)) return rc;
+    dims[0] = h->lv[level].d.m;
+    dims[1] = h->lv[level].d.n;
+    dims[2] = h->lv[level].d.o;
+    return 0;
+}
+
+void *stzmg_stream(stzmg_t h) { return h ? (void *)h->stream : nullptr; }
+
+int stzmg_set_operator_planes(stzmg_t h, const double *planes) {
+    if (!h || !planes) return STZMG_ERR_ARG;
+    Level &L = h->lv[0];
+    if (L.mode != STZMG_OP_PLANE) return fail(h, STZMG_ERR_STATE, "handle holds per-point operators");
+    HIPCHK(h, hipMemcpyAsync(L.ops64, planes, sizeof(double) * L.ncls * kBlk, hipMemcpyHostToDevice, h->stream));
+    return build_hierarchy_ops(h);
+}
+
+int stzmg_set_operator_points(stzmg_t h, const double *points) {
+    if (!h || !points) return STZMG_ERR_ARG;
+    Level &L = h->lv[0];
+    if (L.mode != STZMG_OP_POINT) return fail(h, STZMG_ERR_STATE, "handle holds per-plane operators");
+    HIPCHK(h, hipMemcpyAsync(L.ops64, points, sizeof(double) * L.ncls * kBlk, hipMemcpyHostToDevice, h->stream));
+    return build_hierarchy_ops(h);
+}
+
+int stzmg_set_operator_const(stzmg_t h, const double *blocks, const double *fix) {
+    if (!h || !blocks) return STZMG_ERR_ARG;
+    Level &L = h->lv[0];
+    const Dims &d = L.d;
+    const bool walls = !d.px || !d.py || !d.pz;
+    if (walls && !fix) return fail(h, STZMG_ERR_ARG, "fix block required with non-periodic faces");
+    std::vector<double> tab((size_t)L.ncls * kBlk, 0.0);
+    auto put = [&](long c, bool bnd) {
+        double *o = tab.data() + c * kBlk;
+        if (bnd) {
+            for (int e = 0; e < 9; e++) o[9 * 13 + e] = fix[e];
+        } else {
+            memcpy(o, blocks, sizeof(double) * kBlk);
+        }
+    };
+    if (L.mode == STZMG_OP_PLANE) {
+        for (int k = 0; k < d.o; k++) put(k, !d.pz && (k == 0 || k == d.o - 1));
+    } else {
+        for (int k = 0; k < d.o; k++)
+            for (int j = 0; j < d.n; j++)
+                for (int i = 0; i < d.m; i++) {
+                    const bool bnd = (!d.px && (i == 0 || i == d.m - 1)) || (!d.py && (j == 0 || j == d.n - 1)) ||
+                                     (!d.pz && (k == 0 || k == d.o - 1));
+                    put(i + (long)d.m * (j + (long)d.n * k), bnd);
+                }
+    }
+    HIPCHK(h, hipMemcpyAsync(L.ops64, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return build_hierarchy_ops(h);
+}
+
+int stzmg_get_level_operator(stzmg_t h, int level, double *out, long *ncls) {
+    if (int rc = check_level(h, level)) return rc;
+    Level &L = h->lv[level];
+    if (ncls) *ncls = L.ncls;
+    if (out) {
+        HIPCHK(h, hipMemcpyAsync(out, L.ops64, sizeof(double) * L.ncls * kBlk, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+    }
+    return 0;
+}
+
+int stzmg_set_level_fields(stzmg_t h, int level, const double *x, const double *r) {
+    if (int rc = check_level(h, level)) return rc;
+    Level &L = h->lv[level];
+    const long N = L.d.N;
+    // Stage through the temporary field (AoS upload, SoA conversion).
+    for (int q = 0; q < 2; q++) {
+        const double *src = q == 0 ? x : r;
+        double *dst = q == 0 ? L.x64 : L.r64;
+        if (!src) continue;
+        HIPCHK(h, hipMemcpyAsync(L.t64, src, sizeof(double) * 3 * N, hipMemcpyHostToDevice, h->stream));
+        hipLaunchKernelGGL(k_aos_to_soa, dim3(nblk(N)), dim3(kThreads), 0, h->stream, (const double *)L.t64, dst, N);
+        LAUNCHCHK(h);
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int stzmg_get_level_fields(stzmg_t h, int level, double *x, double *r) {
+    if (int rc = check_level(h, level)) return rc;
+    Level &L = h->lv[level];
+    const long N = L.d.N;
+    for (int q = 0; q < 2; q++) {
+        double *dst = q == 0 ? x : r;
+        const double *src = q == 0 ? L.x64 : L.r64;
+        if (!dst) continue;
+        hipLaunchKernelGGL(k_soa_to_aos, dim3(nblk(N)), dim3(kThreads), 0, h->stream, src, L.t64, N);
+        LAUNCHCHK(h);
+        HIPCHK(h, hipMemcpyAsync(dst, L.t64, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, h->stream));
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int stzmg_set_fields_device(stzmg_t h, const double *x_dev, const double *r_dev) {
+    if (!h) return STZMG_ERR_ARG;
+    Level &L = h->lv[0];
+    const size_t bytes = sizeof(double) * 3 * L.d.N;
+    if (x_dev) HIPCHK(h, hipMemcpyAsync(L.x64, x_dev, bytes, hipMemcpyDeviceToDevice, h->stream));
+    if (r_dev) HIPCHK(h, hipMemcpyAsync(L.r64, r_dev, bytes, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int stzmg_get_x_device(stzmg_t h, double *x_dev) {
+    if (!h || !x_dev) return STZMG_ERR_ARG;
+    Level &L = h->lv[0];
+    HIPCHK(h, hipMemcpyAsync(x_dev, L.x64, sizeof(double) * 3 * L.d.N, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int stzmg_gauss_seidel(stzmg_t h, int level, int nsweeps) {
+    if (int rc = check_level(h, level)) return rc;
+    if (int rc = require_ops(h)) return rc;
+    for (int q = 0; q < nsweeps; q++) launch_gs_sweep<double>(h, h->lv[level]);
+    LAUNCHCHK(h);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int stzmg_restriction(stzmg_t h, int level) {
+    if (int rc = check_level(h, level)) return rc;
+    if (level == 0) return fail(h, STZMG_ERR_ARG, "restriction targets a coarse level (>= 1)");
+    if (int rc = require_ops(h)) return rc;
+    launch_restrict<double>(h, level);
+    LAUNCHCHK(h);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int stzmg_interpolation(stzmg_t h, int level) {
+    if (int rc = check_level(h, level)) return rc;
+    if (level == 0) return fail(h, STZMG_ERR_ARG, "interpolation starts from a coarse level (>= 1)");
+    launch_interp<double>(h, level);
+    LAUNCHCHK(h);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int stzmg_residual(stzmg_t h, int level, double *out) {
+    if (int rc = check_level(h, level)) return rc;
+    if (int rc = require_ops(h)) return rc;
+    Level &L = h->lv[level];
+    launch_resid<double, false>(h, L, L.x64, L.r64, L.t64);
+    LAUNCHCHK(h);
+    if (out) {
+        // t64 holds the residual; convert through x32/r32-free staging: use
+        // the FP32 temporary's storage is too small, so convert in place on
+        // the host side of a scratch copy.
+        std::vector<double> soa((size_t)3 * L.d.N);
+        HIPCHK(h, hipMemcpyAsync(soa.data(), L.t64, sizeof(double) * soa.size(), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        const long N = L.d.N;
+        for (long p = 0; p < N; p++) {
+            out[3 * p] = soa[p];
+            out[3 * p + 1] = soa[p + N];
+            out[3 * p + 2] = soa[p + 2 * N];
+        }
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int stzmg_norm(stzmg_t h, int level, int kind, double *out) {
+    if (int rc = check_level(h, level)) return rc;
+    if (int rc = require_ops(h)) return rc;
+    if (!out || (kind != STZMG_NORM_QUARTIC && kind != STZMG_NORM_L2SQ)) return STZMG_ERR_ARG;
+    double v[2];
+    if (int rc = norms64(h, level, v)) return rc;
+    *out = kind == STZMG_NORM_QUARTIC ? v[0] : v[1];
+    return 0;
+}
+
+int stzmg_vcycle(stzmg_t h, int n) {
+    if (!h) return STZMG_ERR_ARG;
+    if (int rc = require_ops(h)) return rc;
+    if (int rc = run_vcycles<double>(h, n, false)) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int stzmg_vcycle_fp32(stzmg_t h, int n) {
+    if (!h) return STZMG_ERR_ARG;
+    if (int rc = require_ops(h)) return rc;
+    if (int rc = run_vcycles<float>(h, n, false)) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+void stzmg_solve_opts_default(stzmg_solve_opts *o) {
+    if (!o) return;
+    o->mode = STZMG_SOLVE_RELRES;
+    o->tol = 0;
+    o->extra_cycles = 2;
+    o->ref_norm_schedule = 0;
+    o->rtol = 1e-10;
+    o->max_cycles = 1000;
+    o->inner_cycles = 3;
+    o->use_graph = 1;
+}
+
+int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) {
+    if (!h || !opts) return STZMG_ERR_ARG;
+    if (int rc = require_ops(h)) return rc;
+    stzmg_solve_stats s{};
+    const bool graph = opts->use_graph != 0;
+    auto t0 = std::chrono::steady_clock::now();
+    double v[2];
+    int rc = 0;
+    if (opts->mode == STZMG_SOLVE_REF) {
+        // shear_sim_3d::projection_step, src/shear_sim_3d.cpp:555-600.
+        int num_iters = 1;
+        double err = 1e16;
+        if ((rc = norms64(h, 0, v))) return rc;
+        s.res0 = v[0];
+        s.norms++;
+        while (err > opts->tol) {
+            if (s.cycles >= opts->max_cycles) { rc = STZMG_ERR_NOT_CONVERGED; break; }
+            if ((rc = run_vcycles<double>(h, 1, graph))) return rc;
+            s.cycles++;
+            const bool eval = opts->ref_norm_schedule == 0 || num_iters >= h->std_iters / 16;
+            if (eval) {
+                if ((rc = norms64(h, 0, v))) return rc;
+                s.norms++;
+                err = v[0];
+                if (!std::isfinite(err)) { rc = STZMG_ERR_NONFINITE; break; }
+            }
+            num_iters++;
+        }
+        if (num_iters >= h->std_iters / 16) h->std_iters = num_iters * 16;
+        h->std_iters--;
+        if (rc == 0) {
+            if ((rc = run_vcycles<double>(h, opts->extra_cycles, graph))) return rc;
+            s.cycles += opts->extra_cycles;
+        }
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        if (rc == 0 || rc == STZMG_ERR_NOT_CONVERGED) {
+            int r2 = norms64(h, 0, v);
+            if (r2) return r2;
+            s.res = v[0];
+        }
+    } else if (opts->mode == STZMG_SOLVE_RELRES) {
+        if ((rc = norms64(h, 0, v))) return rc;
+        s.norms++;
+        const double r0 = std::sqrt(v[1]);
+        s.res0 = s.res = r0;
+        const double target = opts->rtol * r0;
+        while (s.res > target) {
+            if (s.cycles >= opts->max_cycles) { rc = STZMG_ERR_NOT_CONVERGED; break; }
+            if ((rc = run_vcycles<double>(h, 1, graph))) return rc;
+            s.cycles++;
+            if ((rc = norms64(h, 0, v))) return rc;
+            s.norms++;
+            s.res = std::sqrt(v[1]);
+            if (!std::isfinite(s.res)) { rc = STZMG_ERR_NONFINITE; break; }
+        }
+    } else if (opts->mode == STZMG_SOLVE_FP32_IR) {
+        // FP64 defect -> scaled FP32 defect -> FP32 V-cycles from zero ->
+        // FP64 update, repeated until the FP64 residual meets rtol.
+        Level &L = h->lv[0];
+        const long n3 = 3 * L.d.N;
+        if ((rc = norms64(h, 0, v))) return rc;  // leaves r64 - A x64 in t64
+        s.norms++;
+        const double r0 = std::sqrt(v[1]);
+        s.res0 = s.res = r0;
+        const double target = opts->rtol * r0;
+        const int inner = opts->inner_cycles > 0 ? opts->inner_cycles : 1;
+        while (s.res > target) {
+            if (s.cycles >= opts->max_cycles) { rc = STZMG_ERR_NOT_CONVERGED; break; }
+            const double scale = s.res / std::sqrt((double)n3);  // RMS of the defect
+            hipLaunchKernelGGL(k_defect_f32, dim3(nblk(n3)), dim3(kThreads), 0, h->stream, (const double *)L.t64,
+                               1.0 / scale, L.r32, L.x32, n3);
+            if ((rc = run_vcycles<float>(h, inner, graph))) return rc;
+            s.cycles += inner;
+            hipLaunchKernelGGL(k_update_f64, dim3(nblk(n3)), dim3(kThreads), 0, h->stream, L.x64,
+                               (const float *)L.x32, scale, n3);
+            LAUNCHCHK(h);
+            s.outer++;
+            if ((rc = norms64(h, 0, v))) return rc;
+            s.norms++;
+            s.res = std::sqrt(v[1]);
+            if (!std::isfinite(s.res)) { rc = STZMG_ERR_NONFINITE; break; }
+        }
+    } else {
+        return fail(h, STZMG_ERR_ARG, "unknown solve mode %d", opts->mode);
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    s.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (st) *st = s;
+    if (rc == STZMG_ERR_NOT_CONVERGED) return fail(h, rc, "not converged after %d cycles (res %.3e)", s.cycles, s.res);
+    if (rc == STZMG_ERR_NONFINITE) return fail(h, rc, "non-finite residual after %d cycles", s.cycles);
+    return rc;
+}
+
+int stzmg_time_kernel(stzmg_t h, int kind, int level, int reps, double *ms_per_launch) {
+    if (int rc = check_level(h, level)) return rc;
+    if (int rc = require_ops(h)) return rc;
+    if (reps < 1 || !ms_per_launch) return STZMG_ERR_ARG;
+    Level &L = h->lv[level];
+    hipEvent_t a, b;
+    HIPCHK(h, hipEventCreate(&a));
+    HIPCHK(h, hipEventCreate(&b));
+    int launches = 0;
+    auto body = [&]() {
+        if (kind == 0) {
+            launch_gs_sweep<double>(h, L);
+            launches += 8;
+        } else if (kind == 1) {
+            launch_resid<double, false>(h, L, L.x64, L.r64, L.t64);
+            launches += 1;
+        } else if (kind == 2) {
+            launch_gs_sweep<float>(h, L);
+            launches += 8;
+        }
+    };
+    body();
+    launches = 0;
+    HIPCHK(h, hipEventRecord(a, h->stream));
+    for (int q = 0; q < reps; q++) body();
+    HIPCHK(h, hipEventRecord(b, h->stream));
+    HIPCHK(h, hipEventSynchronize(b));
+    float ms = 0;
+    HIPCHK(h, hipEventElapsedTime(&ms, a, b));
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    *ms_per_launch = launches ? ms / launches : 0;
+    return 0;
+}
+
+// ---- operator builders -------------------------------------------------------
+
+int stzmg_shear_stencil(double mu, double lamb, double dx, double dy, double dz, double dt, double kappa,
+                        double *B, double *fix) {
+    if (!B) return STZMG_ERR_ARG;
+    // stencil_base (src/shear_sim_3d.cpp:216-331), then -dt scaling and the
+    // Dirichlet fix block (src/shear_sim_3d.hh:45-55).
+    double base[27][9];
+    memset(base, 0, sizeof(base));
+    const double L2 = lamb + 2 * mu, kd = kappa / dt, c = (lamb + mu) / 4;
+    const double ax[3] = {L2 / (dx * dx) + kd / (dx * dx), mu / (dx * dx) + kd / (dx * dx), mu / (dx * dx) + kd / (dx * dx)};
+    const double ay[3] = {mu / (dy * dy) + kd / (dy * dy), L2 / (dy * dy) + kd / (dy * dy), mu / (dy * dy) + kd / (dy * dy)};
+    const double az[3] = {mu / (dz * dz) + kd / (dz * dz), mu / (dz * dz) + kd / (dz * dz), L2 / (dz * dz) + kd / (dz * dz)};
+    auto bi = [](int di, int dj, int dk) { return 13 + di + 3 * dj + 9 * dk; };
+    for (int s = -1; s <= 1; s += 2) {
+        for (int e = 0; e < 3; e++) {
+            base[bi(s, 0, 0)][4 * e] = ax[e];
+            base[bi(0, s, 0)][4 * e] = ay[e];
+            base[bi(0, 0, s)][4 * e] = az[e];
+        }
+    }
+    for (int si = -1; si <= 1; si += 2)
+        for (int sj = -1; sj <= 1; sj += 2) {
+            const double sg = si * sj;
+            double *b = base[bi(si, sj, 0)];
+            b[1] = b[3] = sg * c / (dx * dy);
+            b = base[bi(si, 0, sj)];
+            b[2] = b[6] = sg * c / (dx * dz);
+            b = base[bi(0, si, sj)];
+            b[5] = b[7] = sg * c / (dy * dz);
+        }
+    const double ks = kd * (1 / (dx * dx) + 1 / (dy * dy) + 1 / (dz * dz));
+    base[13][0] = -2 * (L2 / (dx * dx) + mu / (dy * dy) + mu / (dz * dz) + ks);
+    base[13][4] = -2 * (mu / (dx * dx) + L2 / (dy * dy) + mu / (dz * dz) + ks);
+    // The reference adds kappa/dt*(...) to a33 outside the -2*(...) factor
+    // (src/shear_sim_3d.cpp:278); inert for kappa = 0 as in every QS run.
+    base[13][8] = -2 * (mu / (dx * dx) + mu / (dy * dy) + L2 / (dz * dz)) + ks;
+    for (int b = 0; b < 27; b++)
+        for (int e = 0; e < 9; e++) B[9 * b + e] = -dt * base[b][e];
+    if (fix) {
+        const double f = -dt * 2 * (L2 / (dx * dx) + mu / (dy * dy) + mu / (dz * dz));
+        for (int e = 0; e < 9; e++) fix[e] = (e % 4 == 0) ? f : 0.0;
+    }
+    return 0;
+}
+
+int stzmg_trans_stencil(const double *T9, double mu, double lamb, double dx, double dy, double dz, double dt,
+                        double *B) {
+    if (!T9 || !B) return STZMG_ERR_ARG;
+    // Closed form of the stenc_funcs blocks: S = symmetric matrix from the
+    // upper triangle of T, W = adj(S)/det(T);
+    // K[i][m][a][b] = mu S_im (W W^T)_ab + (lamb+mu) W_ai delta_bm.
+    double S[3][3];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) S[a][b] = a <= b ? T9[3 * a + b] : T9[3 * b + a];
+    const double D = T9[0] * (T9[4] * T9[8] - T9[5] * T9[7]) - T9[1] * (T9[3] * T9[8] - T9[5] * T9[6]) +
+                     T9[2] * (T9[3] * T9[7] - T9[4] * T9[6]);
+    if (!(D != 0.0)) return STZMG_ERR_SINGULAR;
+    double W[3][3];
+    W[0][0] = (S[1][1] * S[2][2] - S[1][2] * S[2][1]) / D;
+    W[0][1] = -(S[0][1] * S[2][2] - S[0][2] * S[2][1]) / D;
+    W[0][2] = (S[0][1] * S[1][2] - S[0][2] * S[1][1]) / D;
+    W[1][0] = -(S[1][0] * S[2][2] - S[1][2] * S[2][0]) / D;
+    W[1][1] = (S[0][0] * S[2][2] - S[0][2] * S[2][0]) / D;
+    W[1][2] = -(S[0][0] * S[1][2] - S[0][2] * S[1][0]) / D;
+    W[2][0] = (S[1][0] * S[2][1] - S[1][1] * S[2][0]) / D;
+    W[2][1] = -(S[0][0] * S[2][1] - S[0][1] * S[2][0]) / D;
+    W[2][2] = (S[0][0] * S[1][1] - S[0][1] * S[1][0]) / D;
+    double WW[3][3];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) WW[a][b] = W[a][0] * W[b][0] + W[a][1] * W[b][1] + W[a][2] * W[b][2];
+    auto K = [&](int i, int m, int a, int b) {
+        return mu * S[i][m] * WW[a][b] + (lamb + mu) * W[a][i] * (b == m ? 1.0 : 0.0);
+    };
+    const double h[3] = {dx, dy, dz};
+    double out[27][9];
+    memset(out, 0, sizeof(out));
+    for (int a = 0; a < 3; a++) {
+        int off[3] = {0, 0, 0};
+        for (int s = -1; s <= 1; s += 2) {
+            off[a] = s;
+            const int b = 13 + off[0] + 3 * off[1] + 9 * off[2];
+            for (int i = 0; i < 3; i++)
+                for (int m = 0; m < 3; m++) out[b][3 * i + m] += K(i, m, a, a) / (h[a] * h[a]);
+        }
+        for (int i = 0; i < 3; i++)
+            for (int m = 0; m < 3; m++) out[13][3 * i + m] -= 2 * K(i, m, a, a) / (h[a] * h[a]);
+    }
+    for (int a = 0; a < 3; a++)
+        for (int c = a + 1; c < 3; c++)
+            for (int sa = -1; sa <= 1; sa += 2)
+                for (int sc = -1; sc <= 1; sc += 2) {
+                    int off[3] = {0, 0, 0};
+                    off[a] = sa;
+                    off[c] = sc;
+                    const int b = 13 + off[0] + 3 * off[1] + 9 * off[2];
+                    for (int i = 0; i < 3; i++)
+                        for (int m = 0; m < 3; m++)
+                            out[b][3 * i + m] += sa * sc * (K(i, m, a, c) + K(i, m, c, a)) / (4 * h[a] * h[c]);
+                }
+    for (int b = 0; b < 27; b++)
+        for (int e = 0; e < 9; e++) B[9 * b + e] = -dt * out[b][e];
+    return 0;
+}
+
+}  // extern "C"

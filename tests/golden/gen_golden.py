@@ -1,0 +1,121 @@
+"""Write golden vectors for the multigrid path from the compiled reference.
+
+Run in a container that holds the reference tree (oracle/_ref/libstzref.so
+built by ``make -C oracle/ref``):
+
+    python tests/golden/gen_golden.py
+
+Each case runs in its own process so that the reference's function-static
+state (std_iters, src/shear_sim_3d.cpp:561) cannot leak between cases.
+Outputs are small ``.npz`` files of inputs and reference outputs.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle as O  # noqa: E402
+
+MU, LAMB = 1.0, 1.5
+
+
+def mg_case(name, dims, prd, kind, seed):
+    m, n, o = dims
+    h = 1.0 / m
+    if kind == "shear":
+        st, fix = O.shear_operator(MU, LAMB, h, h, h, 1.0)
+    else:
+        T = np.array([[1.0, 0.0, 0.5], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+        st = -O.trans_stencil_blocks(T, MU, LAMB, h, h, h)
+        fix = None
+    S, B = O.point_operator(dims, prd, st, fix)
+    ref = O.RefMG(dims, prd, S, B)
+    nl = ref.nlevels
+    N = m * n * o
+    x = O.det_field(N, seed)
+    r = O.det_field(N, seed + 1)
+    small = N <= 8192
+    out = dict(dims=np.array(dims), prd=np.array(prd), stencil=st,
+               fix=np.zeros((3, 3)) if fix is None else fix, seed=np.array(seed),
+               nlevels=np.array(nl))
+    for l in range(nl):
+        out[f"dims{l}"] = np.array(ref.dims(l))
+    # Coarse operators, compressed to one stencil per z plane after checking
+    # that they do not vary in x and y.
+    for l in range(1, nl):
+        A = ref.operator(l)
+        ml, nl_, ol = ref.dims(l)
+        A = A.reshape(ol, nl_ * ml, 27, 3, 3)
+        assert np.all(A == A[:, :1]), "coarse operator varies in x/y"
+        out[f"rat{l}"] = A[:, 0]
+    ref.set_fields(0, x, r)
+    out["l2_error_all0"] = np.array(ref.l2_error_all(0))
+    if small:
+        out["residual"] = ref.residual(0)
+        ref.gs(0, 1)
+        out["gs1"] = ref.fields(0)[0]
+        ref.restrict(1)
+        out["restrict_r1"] = ref.fields(1)[1]
+        xc = O.det_field(ref.npts(1), seed + 100)
+        ref.set_fields(1, xc, None)
+        ref.interp(1)
+        out["interp_x0"] = ref.fields(0)[0]
+    # V-cycles from x = 0: the stopping quantity after each cycle.
+    ref.set_fields(0, np.zeros((N, 3)), r)
+    norms = []
+    for _ in range(4):
+        ref.vcycle(1)
+        norms.append(ref.l2_error_all(0))
+    out["vcycle_norms"] = np.array(norms)
+    if small:
+        out["vcycle_x"] = ref.fields(0)[0]
+    ref.close()
+    np.savez_compressed(os.path.join(HERE, f"mg_{name}.npz"), **out)
+
+
+def trans_case():
+    rng = np.random.default_rng(7)
+    Ts, outs = [], []
+    for q in range(6):
+        if q == 0:
+            T = np.eye(3)
+        elif q == 1:
+            T = np.array([[1.0, 0.0, 0.5], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])  # simple shear
+        elif q == 2:
+            e = np.exp(0.3)
+            T = np.diag([e, 1.0, 1.0 / e])  # pure shear (trans_sim_3d.hh:76-94)
+        else:
+            T = np.eye(3) + 0.3 * rng.uniform(-1, 1, (3, 3))
+        Ts.append(T)
+        outs.append(O.ref_trans_stencil(T, 1.3, 1.7, 0.11, 0.13, 0.17))
+    np.savez_compressed(os.path.join(HERE, "trans_stencil.npz"), T=np.array(Ts),
+                        blocks=np.array(outs), mu=1.3, lamb=1.7, h=np.array([0.11, 0.13, 0.17]))
+
+
+CASES = {
+    "shear_16x16x17": (("shear_16x16x17", (16, 16, 17), (1, 1, 0), "shear", 11),),
+    "trans_16x16x16": (("trans_16x16x16", (16, 16, 16), (1, 1, 1), "trans", 12),),
+    "shear_32x32x33": (("shear_32x32x33", (32, 32, 33), (1, 1, 0), "shear", 13),),
+}
+
+if __name__ == "__main__":
+    if O.ref_lib() is None:
+        sys.exit("oracle/_ref/libstzref.so is missing; run make -C oracle/ref")
+    if len(sys.argv) > 1:
+        which = sys.argv[1]
+        if which == "trans":
+            trans_case()
+        else:
+            mg_case(*CASES[which][0])
+        sys.exit(0)
+    for key in ["trans", *CASES]:
+        subprocess.run([sys.executable, __file__, key], check=True, cwd=ROOT,
+                       stdout=subprocess.DEVNULL)
+        print("wrote", key)

@@ -1,0 +1,251 @@
+"""Pins the CPU oracle of the multigrid path (no GPU).
+
+* against golden vectors written by the compiled reference
+  (tests/golden/gen_golden.py): coarse operators, residual, one
+  lexicographic Gauss-Seidel sweep, restriction, interpolation and the
+  V-cycle stopping quantity;
+* against what the mathematics fixes: the Galerkin product checked as a
+  dense P^T A P / 8, P 1 = 1, R 1 = 1, symmetry of the coarse operators of
+  a symmetric periodic operator, constants in the null space, Gauss-Seidel
+  converging to a dense direct solve.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+REL = 1e-13
+
+
+def relerr(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def build_oracle(g, mode=O.PLANE):
+    m, n, o = (int(v) for v in g["dims"])
+    prd = tuple(int(v) for v in g["prd"])
+    orc = O.OracleMG(m, n, o, prd, mode=mode)
+    if mode == O.PLANE:
+        orc.set_ops(0, O.plane_operator(o, prd[2], g["stencil"], g["fix"]))
+    else:
+        _, B = O.point_operator((m, n, o), prd, g["stencil"], None if prd == (1, 1, 1) else g["fix"])
+        orc.set_ops(0, B)
+    orc.rat()
+    return orc
+
+
+CASES = ["mg_shear_16x16x17.npz", "mg_trans_16x16x16.npz", "mg_shear_32x32x33.npz"]
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("mode", [O.PLANE, O.POINT])
+def test_oracle_rat_matches_reference(golden, case, mode):
+    g = golden(case)
+    if mode == O.POINT and case.endswith("32x32x33.npz"):
+        pytest.skip("per-point table of the large case is covered by PLANE")
+    orc = build_oracle(g, mode)
+    assert orc.nlevels == int(g["nlevels"])
+    for l in range(orc.nlevels):
+        assert orc.dims(l) == tuple(int(v) for v in g[f"dims{l}"])
+    for l in range(1, orc.nlevels):
+        ops = orc.ops(l)
+        if mode == O.POINT:
+            m, n, o = orc.dims(l)
+            ops = ops.reshape(o, m * n, 27, 3, 3)
+            assert np.all(ops == ops[:, :1])
+            ops = ops[:, 0]
+        assert relerr(ops, g[f"rat{l}"]) < REL
+
+
+@pytest.mark.parametrize("case", CASES[:2])
+def test_oracle_level_ops_match_reference(golden, case):
+    g = golden(case)
+    orc = build_oracle(g)
+    N = orc.npts(0)
+    seed = int(g["seed"])
+    x, r = O.det_field(N, seed), O.det_field(N, seed + 1)
+    orc.set_fields(0, x, r)
+    assert relerr(orc.residual(0), g["residual"]) < REL
+    assert abs(orc.norm_quartic(0) / float(g["l2_error_all0"]) - 1) < 1e-12
+    orc.gs(0, 1)  # lexicographic, ghost snapshot per sweep
+    assert relerr(orc.fields(0)[0], g["gs1"]) < REL
+    orc.restrict(1)
+    assert relerr(orc.fields(1)[1], g["restrict_r1"]) < REL
+    orc.set_fields(1, O.det_field(orc.npts(1), seed + 100), None)
+    orc.interp(1)
+    assert relerr(orc.fields(0)[0], g["interp_x0"]) < REL
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_oracle_vcycle_matches_reference(golden, case):
+    g = golden(case)
+    orc = build_oracle(g)
+    N = orc.npts(0)
+    r = O.det_field(N, int(g["seed"]) + 1)
+    orc.set_fields(0, np.zeros((N, 3)), r)
+    for q, ref in enumerate(g["vcycle_norms"]):
+        orc.vcycle(1)
+        assert abs(orc.norm_quartic(0) / ref - 1) < 1e-9, q
+    if "vcycle_x" in g:
+        assert relerr(orc.fields(0)[0], g["vcycle_x"]) < 1e-11
+
+
+def test_trans_stencil_matches_stenc_funcs(golden):
+    g = golden("trans_stencil.npz")
+    h = g["h"]
+    for T, ref in zip(g["T"], g["blocks"]):
+        got = O.trans_stencil_blocks(T, float(g["mu"]), float(g["lamb"]), *h)
+        assert relerr(got, ref) < REL
+        # the corner blocks are structurally zero (src/stencil_writer.py:7)
+        for di in (-1, 1):
+            for dj in (-1, 1):
+                for dk in (-1, 1):
+                    assert np.all(got[O.block_index(di, dj, dk)] == 0)
+
+
+def test_trans_stencil_identity_is_shear_stencil():
+    # With T = I the transformed operator is the untransformed elastic
+    # operator of shear_sim_3d.
+    h = (0.11, 0.13, 0.17)
+    a = O.trans_stencil_blocks(np.eye(3), 1.3, 1.7, *h)
+    b = O.shear_stencil_base(1.3, 1.7, *h)
+    assert relerr(a, b) < REL
+
+
+def test_shear_stencil_closed_form():
+    mu, lamb, h = 1.0, 1.5, 1.0 / 64
+    S = O.shear_stencil_base(mu, lamb, h, h, h)
+    # constants are in the null space of the elastic operator
+    assert np.abs(S.sum(axis=0)).max() < 1e-9 * np.abs(S).max()
+    # the operator is symmetric: block(d) = block(-d)^T
+    for b in range(27):
+        di, dj, dk = b % 3 - 1, (b // 3) % 3 - 1, b // 9 - 1
+        assert np.array_equal(S[b], S[O.block_index(-di, -dj, -dk)].T)
+    assert S[O.block_index(1, 0, 0)][0, 0] == (lamb + 2 * mu) / h**2
+    assert S[O.block_index(1, 1, 0)][0, 1] == (lamb + mu) / 4 / h**2
+    assert S[O.block_index(1, -1, 0)][0, 1] == -(lamb + mu) / 4 / h**2
+    st, fix = O.shear_operator(mu, lamb, h, h, h, 0.01)
+    assert np.allclose(fix, -0.01 * 2 * ((lamb + 2 * mu) + 2 * mu) / h**2 * np.eye(3))
+
+
+def dense(orc, l):
+    """Dense matrix of the level-l operator built column by column."""
+    N = orc.npts(l)
+    A = np.zeros((3 * N, 3 * N))
+    e = np.zeros((N, 3))
+    for q in range(3 * N):
+        e.flat[q] = 1.0
+        A[:, q] = orc.apply(l, e).ravel()
+        e.flat[q] = 0.0
+    return A
+
+
+def dense_P(orc):
+    """Dense interpolation from level 1 to level 0 via the oracle's interp."""
+    N0, N1 = orc.npts(0), orc.npts(1)
+    P = np.zeros((3 * N0, 3 * N1))
+    e = np.zeros((N1, 3))
+    for q in range(3 * N1):
+        e.flat[q] = 1.0
+        orc.set_fields(0, np.zeros((N0, 3)), None)
+        orc.set_fields(1, e, None)
+        orc.interp(1)
+        P[:, q] = orc.fields(0)[0].ravel()
+        e.flat[q] = 0.0
+    return P
+
+
+@pytest.mark.parametrize("prd", [(1, 1, 1), (1, 1, 0), (0, 0, 0)])
+def test_galerkin_is_dense_PtAP(prd):
+    dims = tuple(12 if p else 13 for p in prd)
+    mu, lamb, h = 1.0, 1.5, 1.0 / 12
+    st, fix = O.shear_operator(mu, lamb, h, h, h, 1.0)
+    T = np.array([[1.0, 0.2, 0.5], [0.0, 1.0, 0.1], [0.0, 0.0, 1.0]])
+    st = -O.trans_stencil_blocks(T, mu, lamb, h, h, h)  # non-symmetric blocks
+    _, B = O.point_operator(dims, prd, st, fix)
+    orc = O.OracleMG(*dims, prd, mode=O.POINT, coarsest_total=300)
+    orc.set_ops(0, B)
+    orc.rat()
+    A0, A1, P = dense(orc, 0), dense(orc, 1), dense_P(orc)
+    assert relerr(A1, P.T @ A0 @ P / 8) < 1e-13
+    # P reproduces constants, R = P^T/8 reproduces them in the interior
+    ones = np.ones(P.shape[1])
+    assert np.allclose(P @ ones, 1.0)
+    if prd == (1, 1, 1):
+        assert np.allclose(P.T @ np.ones(P.shape[0]) / 8, 1.0)
+    # restriction of the oracle equals P^T/8 applied to the residual
+    N0 = orc.npts(0)
+    x, r = O.det_field(N0, 5), O.det_field(N0, 6)
+    orc.set_fields(0, x, r)
+    res = orc.residual(0).ravel()
+    orc.restrict(1)
+    assert relerr(orc.fields(1)[1].ravel(), P.T @ res / 8) < 1e-13
+
+
+def test_symmetric_operator_has_symmetric_coarse_levels():
+    dims, prd = (16, 16, 16), (1, 1, 1)
+    h = 1.0 / 16
+    st = O.shear_stencil_base(1.0, 1.5, h, h, h)
+    orc = O.OracleMG(*dims, prd, mode=O.PLANE, coarsest_total=64)
+    orc.set_ops(0, O.plane_operator(16, 1, -st, None))
+    orc.rat()
+    for l in range(1, orc.nlevels):
+        S = orc.ops(l)[0]
+        for b in range(27):
+            di, dj, dk = b % 3 - 1, (b // 3) % 3 - 1, b // 9 - 1
+            assert np.allclose(S[b], S[O.block_index(-di, -dj, -dk)].T, rtol=0, atol=1e-12 * np.abs(S).max())
+        assert np.abs(S.sum(axis=0)).max() < 1e-9 * np.abs(S).max()
+
+
+@pytest.mark.parametrize("smoother", [0, 1])
+def test_gauss_seidel_converges_to_dense_solve(smoother):
+    dims, prd = (8, 8, 9), (1, 1, 0)
+    h = 1.0 / 8
+    st, fix = O.shear_operator(1.0, 1.5, h, h, h, 1.0)
+    orc = O.OracleMG(*dims, prd, mode=O.PLANE)
+    orc.set_ops(0, O.plane_operator(9, 0, st, fix))
+    orc.params(smoother=smoother)
+    N = orc.npts(0)
+    r = O.det_field(N, 3)
+    A = dense(orc, 0)
+    xs = np.linalg.solve(A, r.ravel())
+    orc.set_fields(0, np.zeros((N, 3)), r)
+    orc.gs(0, 3000)
+    assert relerr(orc.fields(0)[0].ravel(), xs) < 1e-9
+
+
+def test_colour_order_sweep_is_order_independent_within_colour():
+    # one 8-colour sweep equals the lexicographic sweep restricted to one
+    # colour at a time (a planted reordering of points inside a colour would
+    # not change it, a planted change of colour order would).
+    dims, prd = (8, 8, 9), (1, 1, 0)
+    h = 1.0 / 8
+    st, fix = O.shear_operator(1.0, 1.5, h, h, h, 1.0)
+    a = O.OracleMG(*dims, prd, mode=O.PLANE)
+    a.set_ops(0, O.plane_operator(9, 0, st, fix))
+    a.params(smoother=1)
+    N = a.npts(0)
+    x, r = O.det_field(N, 8), O.det_field(N, 9)
+    a.set_fields(0, x, r)
+    a.gs(0, 1)
+    xa = a.fields(0)[0]
+    # reference implementation of the colour sweep in numpy on the dense matrix
+    A = dense(a, 0)
+    xv = x.ravel().copy()
+    m, n, o = dims
+    for cz in range(2):
+        for cy in range(2):
+            for cx in range(2):
+                for k in range(o):
+                    for j in range(n):
+                        for i in range(m):
+                            if (i % 2, j % 2, k % 2) != (cx, cy, cz):
+                                continue
+                            p = i + m * (j + n * k)
+                            rows = slice(3 * p, 3 * p + 3)
+                            Ad = A[rows, rows]
+                            rhs = r.ravel()[rows] - A[rows] @ xv + Ad @ xv[rows]
+                            xv[rows] = np.linalg.solve(Ad, rhs)
+    assert relerr(xa.ravel(), xv) < 1e-12
