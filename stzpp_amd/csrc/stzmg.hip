@@ -648,7 +648,8 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
         if ((rc = norms64(h, 0, v))) return rc;
         s.res0 = v[0];
         s.norms++;
-        while (err > opts->tol) {
+        if (!std::isfinite(v[0])) rc = STZMG_ERR_NONFINITE;
+        while (rc == 0 && err > opts->tol) {
             if (s.cycles >= opts->max_cycles) { rc = STZMG_ERR_NOT_CONVERGED; break; }
             if ((rc = run_vcycles<double>(h, 1, graph))) return rc;
             s.cycles++;
@@ -679,7 +680,8 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
         const double r0 = std::sqrt(v[1]);
         s.res0 = s.res = r0;
         const double target = opts->rtol * r0;
-        while (s.res > target) {
+        if (!std::isfinite(r0)) rc = STZMG_ERR_NONFINITE;
+        while (rc == 0 && s.res > target) {
             if (s.cycles >= opts->max_cycles) { rc = STZMG_ERR_NOT_CONVERGED; break; }
             if ((rc = run_vcycles<double>(h, 1, graph))) return rc;
             s.cycles++;
@@ -699,7 +701,8 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
         s.res0 = s.res = r0;
         const double target = opts->rtol * r0;
         const int inner = opts->inner_cycles > 0 ? opts->inner_cycles : 1;
-        while (s.res > target) {
+        if (!std::isfinite(r0)) rc = STZMG_ERR_NONFINITE;
+        while (rc == 0 && s.res > target) {
             if (s.cycles >= opts->max_cycles) { rc = STZMG_ERR_NOT_CONVERGED; break; }
             const double scale = s.res / std::sqrt((double)n3);  // RMS of the defect
             hipLaunchKernelGGL(k_defect_f32, dim3(nblk(n3)), dim3(kThreads), 0, h->stream, (const double *)L.t64,

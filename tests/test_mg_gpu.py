@@ -58,12 +58,16 @@ def test_gpu_level_ops_match_reference(golden, case, mode):
     mg.setup_fields(x, r)
     assert relerr(mg.residual(0), g["residual"]) < REL
     assert abs(mg.l2_error_all(0) / float(g["l2_error_all0"]) - 1) < 1e-12
+    # the golden restriction/interpolation start from x after one
+    # lexicographic sweep of the reference
+    x1 = g["gs1"]
+    mg.setup_fields(x1, r)
     mg.restriction(1)
     assert relerr(mg.fields(1)[1], g["restrict_r1"]) < REL
-    mg.setup_fields(x, r)
+    mg.setup_fields(x1, r)
     mg.setup_fields(O.det_field(mg.npts(1), seed + 100), None, level=1)
     mg.interpolation(1)
-    assert relerr(mg.x(0) - x, g["interp_x0"] - x) < 1e-12
+    assert relerr(mg.x(0) - x1, g["interp_x0"] - x1) < 1e-12
 
 
 @pytest.mark.parametrize("case", CASES)
