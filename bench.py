@@ -75,6 +75,8 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--inner", type=int, default=3)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch V-cycles kernel by kernel instead of replaying a hipGraph")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -114,7 +116,8 @@ def main():
         mg.set_fields_device(dx0, db)
         hip.hipDeviceSynchronize()
         t0 = time.perf_counter()
-        s = mg.solve(mode=mode, rtol=1e-10, inner_cycles=inner, max_cycles=400)
+        s = mg.solve(mode=mode, rtol=1e-10, inner_cycles=inner, max_cycles=400,
+                     use_graph=not args.no_graph)
         hip.hipDeviceSynchronize()
         return time.perf_counter() - t0, s
 

@@ -1,0 +1,70 @@
+/* stz.h -- C ABI of the MI355X quasi-static STZ step.
+ *
+ * Drop-in for the simulation step API of shear_sim_3d<sym_mat3>
+ * (src/shear_sim_3d.hh:35-82): state in, N quasi-static steps
+ * (step_forward_qs, src/shear_sim_3d.cpp:381-389), state out.  One
+ * subdomain on one GPU; periodic in x and y, walls (plates) or periodic in z.
+ *
+ * State arrays on the host (AoS, x fastest):
+ *   u    vertex velocity, 3 doubles per vertex, Nx*Ny*Nzv vertices with
+ *        Nzv = Nz+1 when z is walled (the top plate plane included), Nz when
+ *        periodic -- the multigrid grid of region.hh:805
+ *   sig  cell stress s11 s12 s13 s22 s23 s33, 6 doubles per cell, Nx*Ny*Nz
+ *   chi  cell effective temperature, Nx*Ny*Nz
+ *   dpl  cell ad_Dpl (2 mu dt Dpl / sbar scaled as in the reference), Nx*Ny*Nz
+ * Status codes are those of stzmg.h.
+ */
+#ifndef STZ_H
+#define STZ_H
+
+#include "stzmg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct stz_s *stz_t;
+
+typedef struct {
+    int Nx, Ny, Nz;        /* cells per dimension (gN_x, gN_y, gN_z) */
+    int prd[3];            /* periodicity; x and y must be periodic */
+    double ax, bx, ay, by, az, bz; /* domain */
+    double mu, lamb;       /* elastic moduli */
+    double sy, tau0, c0, eps0, delta, omega, bath_temp, chi_inf, ez; /* STZ */
+    double diff_l;         /* plasticity-mediated diffusion length */
+    double U;              /* plate speed; the plates ramp as U*t for t < 1 */
+    double eta;            /* adaptive plastic-step tolerance (5e-4, src/shear_sim_3d.cpp:111) */
+} stz_params;
+
+typedef struct {
+    int steps;             /* steps taken */
+    int vcycles;           /* V-cycles over all projection solves */
+    int yielded_cells;     /* cells with sbar > sy in the last step */
+    double solve_ms;       /* wall time in the projection solves */
+    double step_ms;        /* wall time of the whole call */
+    double last_res;       /* final solve quantity of the last step */
+} stz_step_stats;
+
+/* Reference defaults: domain [-1,1)^2 x [-0.5,0.5], x,y periodic, z walled,
+ * parameters of the shear_sim driver after nondimensionalisation
+ * (src/shear_sim.cpp:18-86). */
+int stz_params_default(stz_params *p, int Nx, int Ny, int Nz);
+int stz_create(const stz_params *p, stz_t *s);
+int stz_destroy(stz_t s);
+const char *stz_last_error(stz_t s);
+/* Host state in/out; set_state applies the reference's boundary conditions
+ * (plate velocities at time t, set_boundary_conditions,
+ * src/shear_sim_3d.cpp:2362-2658). */
+int stz_set_state(stz_t s, double t, const double *u, const double *sig, const double *chi);
+int stz_get_state(stz_t s, double *t, double *u, double *sig, double *chi, double *dpl);
+/* nsteps quasi-static steps of size dt.  opts == NULL selects the reference
+ * projection loop (sum |r|^4 <= (eps*stencil[13].a11*1e3)^2, then two more
+ * cycles, src/shear_sim_3d.cpp:551-600); otherwise the given solve mode. */
+int stz_step_qs(stz_t s, double dt, int nsteps, const stzmg_solve_opts *opts, stz_step_stats *st);
+/* The projection solver of the simulation (valid after the first step). */
+stzmg_t stz_multigrid(stz_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -440,3 +440,121 @@ def det_field(N, salt, comps=3):
     v = (p * 7919 + c * 104729 + salt * 15485863) % 2000003
     v = (v * 48271) % 2000003
     return v.astype(np.float64) / 1000001.5 - 1.0
+
+
+# ---------------------------------------------------------------------------
+# Quasi-static step oracle (stz_oracle.c) and the reference step harness.
+
+def _step_lib():
+    L = lib()
+    if not hasattr(L, "_sto_bound"):
+        vp = ctypes.c_void_p
+        L.sto_new.restype = vp
+        L.sto_new.argtypes = [ctypes.c_int] * 3 + [_ip, _dp, _dp]
+        L.sto_delete.argtypes = [vp]
+        L.sto_set_state.argtypes = [vp, ctypes.c_double, _dp, _dp, _dp]
+        L.sto_get_state.argtypes = [vp, _dp, _dp, _dp, _dp, _dp]
+        L.sto_step.argtypes = [vp, ctypes.c_double, ctypes.c_int]
+        L.sto_vcycles.argtypes = [vp]
+        L.sto_vcycles.restype = ctypes.c_int
+        L._sto_bound = True
+    return L
+
+
+STEP_PARAM_ORDER = ("mu", "lamb", "sy", "tau0", "c0", "eps0", "delta", "omega", "bath_temp",
+                    "chi_inf", "ez", "diff_l", "U")
+
+
+def _nzv(N, prd):
+    return N[2] + (0 if prd[2] else 1)
+
+
+class OracleShear:
+    """shear_sim_3d QS step restated in C (padded ghost grid, in-place order)."""
+
+    def __init__(self, N, params, box=(-1, 1, -1, 1, -0.5, 0.5), prd=(1, 1, 0)):
+        self.L = _step_lib()
+        self.N, self.prd = tuple(N), tuple(prd)
+        p = np.array([params[k] for k in STEP_PARAM_ORDER], dtype=np.float64)
+        b = np.array(box, dtype=np.float64)
+        pr = np.array(prd, dtype=np.int32)
+        self.h = self.L.sto_new(N[0], N[1], N[2], _i(pr), _d(b), _d(p))
+
+    def close(self):
+        if self.h:
+            self.L.sto_delete(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def set_state(self, t, u, sig, chi):
+        u, sig, chi = (np.ascontiguousarray(a, dtype=np.float64) for a in (u, sig, chi))
+        self.L.sto_set_state(self.h, t, _d(u), _d(sig), _d(chi))
+
+    def state(self):
+        Nx, Ny, Nz = self.N
+        nv = Nx * Ny * _nzv(self.N, self.prd)
+        nc = Nx * Ny * Nz
+        t = np.zeros(1)
+        u, sig, chi, dpl = np.zeros((nv, 3)), np.zeros((nc, 6)), np.zeros(nc), np.zeros(nc)
+        self.L.sto_get_state(self.h, _d(t), _d(u), _d(sig), _d(chi), _d(dpl))
+        return dict(t=float(t[0]), u=u, sig=sig, chi=chi, dpl=dpl)
+
+    def step(self, dt, n=1):
+        self.L.sto_step(self.h, dt, n)
+
+    @property
+    def vcycles(self):
+        return self.L.sto_vcycles(self.h)
+
+
+class RefShear:
+    """The reference shear_sim_3d<sym_mat3> driven through oracle/_ref."""
+
+    def __init__(self, N, params, outdir, box=(-1, 1, -1, 1, -0.5, 0.5), prd=(1, 1, 0)):
+        self.L = ref_lib()
+        if self.L is None:
+            raise RuntimeError("reference library not built")
+        self.N, self.prd = tuple(N), tuple(prd)
+        p = np.array([params[k] for k in ("mu", "lamb", "sy", "tau0", "c0", "eps0", "delta", "omega",
+                                          "bath_temp", "chi_inf", "ez")] + [21000.0, params["diff_l"],
+                                                                               params["U"]], dtype=np.float64)
+        self._p = p
+        b = np.array([box[0], box[1], box[2], box[3], box[4], box[5]], dtype=np.float64)
+        pr = np.array(prd, dtype=np.int32)
+        self.h = self.L.ref_shear_create(N[0], N[1], N[2], _d(b), _d(p), _i(pr), outdir.encode())
+
+    def close(self):
+        if self.h:
+            self.L.ref_shear_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def setup(self, dt):
+        self.L.ref_shear_setup(self.h, dt)
+
+    def stencil(self):
+        b, f = np.zeros((27, 6)), np.zeros(6)
+        self.L.ref_shear_get_stencil(self.h, _d(b), _d(f))
+        return b, f
+
+    def set_state(self, t, u, sig, chi):
+        u, sig, chi = (np.ascontiguousarray(a, dtype=np.float64) for a in (u, sig, chi))
+        self.L.ref_shear_set_state(self.h, t, _d(u), _d(sig), _d(chi))
+
+    def step(self, dt, n=1):
+        self.L.ref_shear_step(self.h, dt, n)
+
+    def state(self):
+        Nx, Ny, Nz = self.N
+        nv = Nx * Ny * _nzv(self.N, self.prd)
+        nc = Nx * Ny * Nz
+        t = np.zeros(1)
+        u, sig, chi, dpl = np.zeros((nv, 3)), np.zeros((nc, 6)), np.zeros(nc), np.zeros(nc)
+        self.L.ref_shear_get_state(self.h, _d(t), _d(u), _d(sig), _d(chi), _d(dpl))
+        return dict(t=float(t[0]), u=u, sig=sig, chi=chi, dpl=dpl)
+
+    @property
+    def vcycles(self):
+        return self.L.ref_shear_vcycles(self.h)

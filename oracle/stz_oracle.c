@@ -1,0 +1,464 @@
+/* CPU oracle of the quasi-static STZ step -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Restates shear_sim_3d<sym_mat3>::step_forward_qs (src/shear_sim_3d.cpp:
+ * 381-389) on one subdomain with the reference's data structure: a field
+ * grid padded by two ghost layers (src/shear_sim_3d.cpp:49-79), ghosts
+ * refreshed the reference's way, and the advection loop run in place in
+ * lexicographic order, so that chi diffusion sees this step's ad_Dpl at the
+ * -x/-y/-z neighbours and last step's elsewhere (src/shear_sim_3d.cpp:
+ * 393-405, :1217-1237).
+ *
+ *   advection_grid_step      src/shear_sim_3d.cpp:409-525
+ *   adaptive_plastic_term    src/shear_sim_3d.cpp:109-149
+ *   calc_Dpl / curly_c / Lambda / chi_F   :1173-1188, :1241-1243
+ *   eno2                     :1257-1266
+ *   merge_updates            :154-162, src/field.hh:31-47
+ *   set_up_ghost_regions     :1270-1294 (planes/edges/corners of the
+ *                            periodic dimensions, real extent otherwise,
+ *                            :1296-1380)
+ *   set_boundary_conditions  :2362-2658 with set_proper_ghost_for_boundaries
+ *                            :2662-2789 (linear extrapolation of the ten
+ *                            primary fields, other members zeroed by
+ *                            Field's operators, src/field.hh:49-112)
+ *   projection_step          :529-725 with r_field src/shear_sim_3d.hh:283-385
+ *   stress update            :684-712
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "mg_oracle.h"
+
+enum { F_U, F_V, F_W, F_S11, F_S12, F_S13, F_S22, F_S23, F_S33, F_CHI,
+       F_CS11, F_CS12, F_CS13, F_CS22, F_CS23, F_CS33, F_CCHI, F_SBAR, F_DPL, F_NF };
+
+typedef struct {
+    int Nx, Ny, Nz, prd[3];
+    double dx, dy, dz;
+    double mu, lamb, sy, tau0, c0, eps0, delta, omega, bath_temp, chi_inf, ez, diff_l, U;
+    double eta;            /* adaptive tolerance: 5e-4 (shear) */
+    double t;
+    int nxg, nyg, nzg;
+    long nxyg;
+    double *f[F_NF];       /* padded grid, one array per Field member */
+    mgo_hier mg;
+    double stencil[243], fix[9];
+    int std_iters;         /* the reference's static counter, per instance here */
+    int vcycles;           /* V-cycles of the last step */
+    int mg_ready;
+} sto_sim;
+
+static long gi(const sto_sim *S, int i, int j, int k) {
+    return (long)(i + 2) + S->nxg * ((long)(j + 2) + (long)S->nyg * (k + 2));
+}
+
+#define FV(S, fld, i, j, k) ((S)->f[fld][gi(S, i, j, k)])
+
+void *sto_new(int Nx, int Ny, int Nz, const int *prd, const double *box, const double *p) {
+    sto_sim *S = (sto_sim *)calloc(1, sizeof(sto_sim));
+    S->Nx = Nx; S->Ny = Ny; S->Nz = Nz;
+    for (int a = 0; a < 3; a++) S->prd[a] = prd[a];
+    S->dx = (box[1] - box[0]) / Nx;
+    S->dy = (box[3] - box[2]) / Ny;
+    S->dz = (box[5] - box[4]) / Nz;
+    S->mu = p[0]; S->lamb = p[1]; S->sy = p[2]; S->tau0 = p[3]; S->c0 = p[4]; S->eps0 = p[5];
+    S->delta = p[6]; S->omega = p[7]; S->bath_temp = p[8]; S->chi_inf = p[9]; S->ez = p[10];
+    S->diff_l = p[11]; S->U = p[12];
+    S->eta = 0.0005;
+    S->nxg = Nx + 4; S->nyg = Ny + 4; S->nzg = Nz + 4;
+    S->nxyg = (long)S->nxg * S->nyg;
+    long n = S->nxyg * S->nzg;
+    for (int q = 0; q < F_NF; q++) S->f[q] = (double *)calloc((size_t)n, sizeof(double));
+    S->std_iters = 16;
+    return S;
+}
+
+void sto_delete(void *hp) {
+    sto_sim *S = (sto_sim *)hp;
+    for (int q = 0; q < F_NF; q++) free(S->f[q]);
+    if (S->mg_ready) mgo_free(&S->mg);
+    free(S);
+}
+
+/* Periodic images of every member in the periodic dimensions; a ghost cell
+ * is refreshed only when all of its ghost coordinates are periodic. */
+static void ghosts(sto_sim *S) {
+    int N[3] = {S->Nx, S->Ny, S->Nz};
+    for (int k = -2; k < S->Nz + 2; k++)
+        for (int j = -2; j < S->Ny + 2; j++)
+            for (int i = -2; i < S->Nx + 2; i++) {
+                int c[3] = {i, j, k}, src[3], ghost = 0, ok = 1;
+                for (int a = 0; a < 3; a++) {
+                    src[a] = c[a];
+                    if (c[a] < 0 || c[a] >= N[a]) {
+                        ghost = 1;
+                        if (!S->prd[a]) { ok = 0; break; }
+                        src[a] = (c[a] + N[a]) % N[a];
+                    }
+                }
+                if (!ghost || !ok) continue;
+                long d = gi(S, i, j, k), s = gi(S, src[0], src[1], src[2]);
+                for (int q = 0; q < F_NF; q++) S->f[q][d] = S->f[q][s];
+            }
+}
+
+/* *dst = 2*(*a) - (*b) on the ten primary members, the rest zero
+ * (Field's operator+ / operator- build a zero Field, src/field.hh:49-112). */
+static void extrap(sto_sim *S, long d, long a, long b) {
+    for (int q = 0; q <= F_CHI; q++) S->f[q][d] = (S->f[q][a] + S->f[q][a]) - S->f[q][b];
+    for (int q = F_CHI + 1; q < F_NF; q++) S->f[q][d] = 0.0;
+}
+
+static double top_u(const sto_sim *S) { return S->t < 1 ? S->U * S->t : S->U; }
+
+static void boundary(sto_sim *S) {
+    if (S->prd[2]) return;
+    int Nz = S->Nz;
+    for (int j = -2; j < S->Ny + 2; j++)
+        for (int i = -2; i < S->Nx + 2; i++) {
+            extrap(S, gi(S, i, j, Nz), gi(S, i, j, Nz - 1), gi(S, i, j, Nz - 2));
+            extrap(S, gi(S, i, j, Nz + 1), gi(S, i, j, Nz), gi(S, i, j, Nz - 1));
+            extrap(S, gi(S, i, j, -1), gi(S, i, j, 0), gi(S, i, j, 1));
+            extrap(S, gi(S, i, j, -2), gi(S, i, j, -1), gi(S, i, j, 0));
+        }
+    double ut = top_u(S);
+    for (int j = -2; j < S->Ny + 2; j++)
+        for (int i = -2; i < S->Nx + 2; i++) {
+            long t0 = gi(S, i, j, Nz), t1 = gi(S, i, j, Nz + 1), tm = gi(S, i, j, Nz - 1);
+            S->f[F_U][t0] = ut; S->f[F_V][t0] = 0; S->f[F_W][t0] = 0;
+            for (int q = F_U; q <= F_W; q++) S->f[q][t1] = 2 * S->f[q][t0] - S->f[q][tm];
+            long b0 = gi(S, i, j, 0), b1 = gi(S, i, j, 1), bm1 = gi(S, i, j, -1), bm2 = gi(S, i, j, -2);
+            S->f[F_U][b0] = -ut; S->f[F_V][b0] = 0; S->f[F_W][b0] = 0;
+            for (int q = F_U; q <= F_W; q++) {
+                S->f[q][bm1] = 2 * S->f[q][b0] - S->f[q][b1];
+                S->f[q][bm2] = 2 * S->f[q][bm1] - S->f[q][b0];
+            }
+        }
+}
+
+static void refresh(sto_sim *S) {
+    ghosts(S);
+    boundary(S);
+}
+
+/* Field::dev, src/field.hh:149-184 */
+static double dev(const sto_sim *S, long c) {
+    double s11 = S->f[F_S11][c], s22 = S->f[F_S22][c], s33 = S->f[F_S33][c];
+    double p = -(s11 + s22 + s33) * (1 / 3.);
+    double t11 = s11 + p, t22 = s22 + p, t33 = s33 + p;
+    double s12 = S->f[F_S12][c], s13 = S->f[F_S13][c], s23 = S->f[F_S23][c];
+    return sqrt(0.5 * (t11 * t11 + t22 * t22 + t33 * t33) + s12 * s12 + s13 * s13 + s23 * s23);
+}
+
+static double curly_c(const sto_sim *S, double sb) {
+    double a = exp((S->omega * S->eps0 * sb - S->delta) / S->bath_temp);
+    double b = exp(-(S->omega * S->eps0 * sb + S->delta) / S->bath_temp);
+    return .5 * (a + b);
+}
+
+static double calc_dpl(const sto_sim *S, double sb, double chi) {
+    return sb >= S->sy ? exp(-S->ez / chi) * curly_c(S, sb) * (1 - S->sy / sb) / S->tau0 : 0;
+}
+
+static double chi_F(const sto_sim *S, double dpl, double sb, double chi) {
+    return 2 * dpl * sb * (S->chi_inf - chi) / (S->sy * S->c0);
+}
+
+/* Returns 1 - sbar_end/sbar_0 and adds the plastic chi change into cchi. */
+static double adaptive(sto_sim *S, long c, double dt) {
+    double tr = dt, ts;
+    int keep = 1;
+    double sb0 = dev(S, c), chi0 = S->f[F_CHI][c];
+    double sb = sb0, chi = chi0;
+    while (keep) {
+        double dpl = calc_dpl(S, sb, chi);
+        double dprime = 2 * S->mu * dpl / S->sy;
+        double F = chi_F(S, dpl, sb, chi);
+        if (dprime * tr > S->eta) {
+            ts = S->eta / fabs(dprime);
+            tr = tr - ts;
+        } else {
+            ts = tr;
+            keep = 0;
+        }
+        sb = sb - ts * dprime * S->sy;
+        chi = chi + ts * F;
+    }
+    S->f[F_CCHI][c] += chi - chi0;
+    return 1 - sb / sb0;
+}
+
+static double eno2(double h, double vel, double f2p, double fp, double f, double fm, double f2m) {
+    double c2 = fabs(fp - 2 * f + fm);
+    double fw = fabs(f2p - 2 * fp + f);
+    double bw = fabs(f - 2 * fm + f2m);
+    return .5 / h * (vel < 0 ? (c2 > fw ? -f2p + 4 * fp - 3 * f : fp - fm)
+                             : (c2 > bw ? 3 * f - 4 * fm + f2m : fp - fm));
+}
+
+/* Cell-centred velocity gradient from the 8 vertices of cell c
+ * (calculate_first_order_velocities, src/shear_sim_3d.cpp:901-916). */
+static void vel_grad(const sto_sim *S, long c, double g[3][3]) {
+    long X = 1, Y = S->nxg, Z = S->nxyg;
+    double ix = .25 * (1 / S->dx), iy = .25 * (1 / S->dy), iz = .25 * (1 / S->dz);
+    for (int q = 0; q < 3; q++) {
+        const double *u = S->f[F_U + q] + c;
+        g[q][0] = ix * (u[X] - u[0] + u[X + Y] - u[Y] + u[X + Z] - u[Z] + u[X + Y + Z] - u[Y + Z]);
+        g[q][1] = iy * (u[Y] - u[0] + u[X + Y] - u[X] + u[Y + Z] - u[Z] + u[X + Y + Z] - u[X + Z]);
+        g[q][2] = iz * (u[Z] - u[0] + u[X + Z] - u[X] + u[Y + Z] - u[Y] + u[X + Y + Z] - u[X + Y]);
+    }
+}
+
+static double chi_diffusion(const sto_sim *S, long c) {
+    long X = 1, Y = S->nxg, Z = S->nxyg;
+    const double *D = S->f[F_DPL] + c, *x = S->f[F_CHI] + c;
+    double ix = 1 / S->dx, iy = 1 / S->dy, iz = 1 / S->dz;
+    double dDx = .5 * ix * (D[X] - D[-X]), dDy = .5 * iy * (D[Y] - D[-Y]), dDz = .5 * iz * (D[Z] - D[-Z]);
+    double dcx = .5 * ix * (x[X] - x[-X]), dcy = .5 * iy * (x[Y] - x[-Y]), dcz = .5 * iz * (x[Z] - x[-Z]);
+    double ddx = ix * ix * (x[X] - 2 * x[0] + x[-X]);
+    double ddy = iy * iy * (x[Y] - 2 * x[0] + x[-Y]);
+    double ddz = iz * iz * (x[Z] - 2 * x[0] + x[-Z]);
+    double a = dDx * dcx + D[0] * ddx, b = dDy * dcy + D[0] * ddy, cc = dDz * dcz + D[0] * ddz;
+    return S->diff_l * S->diff_l * (a + b + cc);
+}
+
+static void advection_cell(sto_sim *S, long c, double dt) {
+    long X = 1, Y = S->nxg, Z = S->nxyg;
+    double hx = S->dx / dt, hy = S->dy / dt, hz = S->dz / dt;
+    double g[3][3];
+    vel_grad(S, c, g);
+    double du_dx = g[0][0], du_dy = g[0][1], du_dz = g[0][2];
+    double dv_dx = g[1][0], dv_dy = g[1][1], dv_dz = g[1][2];
+    double dw_dx = g[2][0], dw_dy = g[2][1], dw_dz = g[2][2];
+    double sb = dev(S, c);
+    S->f[F_SBAR][c] = sb;
+    double adapt = sb > S->sy ? adaptive(S, c, dt) : 0;
+    S->f[F_DPL][c] = adapt / 2. / S->mu * sb;
+    double vel[3];
+    for (int q = 0; q < 3; q++) {
+        const double *u = S->f[F_U + q] + c;
+        vel[q] = 0.125 * (u[0] + u[X] + u[Z] + u[X + Z] + u[Y] + u[X + Y] + u[Y + Z] + u[X + Y + Z]);
+    }
+    double ug[7];
+    for (int q = 0; q < 7; q++) {
+        const double *s = S->f[F_S11 + q == F_S11 + 6 ? F_CHI : F_S11 + q] + c;
+        double dx_ = eno2(hx, vel[0], s[2 * X], s[X], s[0], s[-X], s[-2 * X]);
+        double dy_ = eno2(hy, vel[1], s[2 * Y], s[Y], s[0], s[-Y], s[-2 * Y]);
+        double dz_ = eno2(hz, vel[2], s[2 * Z], s[Z], s[0], s[-Z], s[-2 * Z]);
+        ug[q] = vel[0] * dx_ + vel[1] * dy_ + vel[2] * dz_;
+    }
+    double s11 = S->f[F_S11][c], s12 = S->f[F_S12][c], s13 = S->f[F_S13][c];
+    double s22 = S->f[F_S22][c], s23 = S->f[F_S23][c], s33 = S->f[F_S33][c];
+    double third = 1. / 3.;
+    double t11 = s11 * (du_dx - dv_dy - dw_dz) + 2 * s13 * du_dz + 2 * s12 * du_dy;
+    double t12 = s23 * du_dz + s13 * dv_dz - s12 * dw_dz + s22 * du_dy + s11 * dv_dx;
+    double t13 = s33 * du_dz + s11 * dw_dx + s23 * du_dy + s12 * dw_dy - s13 * dv_dy;
+    double t22 = 2 * s23 * dv_dz + s22 * (dv_dy - du_dx - dw_dz) + 2 * s12 * dv_dx;
+    double t23 = s33 * dv_dz + s22 * dw_dy + s13 * dv_dx - s23 * du_dx + s12 * dw_dx;
+    double t33 = s33 * (dw_dz - dv_dy - du_dx) + 2 * s23 * dw_dy + 2 * s13 * dw_dx;
+    S->f[F_CS11][c] = adapt * third * (s33 + s22 - 2 * s11) - ug[0] + dt * t11;
+    S->f[F_CS12][c] = -adapt * s12 - ug[1] + dt * t12;
+    S->f[F_CS13][c] = -adapt * s13 - ug[2] + dt * t13;
+    S->f[F_CS22][c] = adapt * third * (s11 - 2 * s22 + s33) - ug[3] + dt * t22;
+    S->f[F_CS23][c] = -adapt * s23 - ug[4] + dt * t23;
+    S->f[F_CS33][c] = adapt * third * (s11 + s22 - 2 * s33) - ug[5] + dt * t33;
+    S->f[F_CCHI][c] -= ug[6];
+    S->f[F_CCHI][c] += chi_diffusion(S, c) / S->c0;
+}
+
+static void merge(sto_sim *S) {
+    for (int k = 0; k < S->Nz; k++)
+        for (int j = 0; j < S->Ny; j++)
+            for (int i = 0; i < S->Nx; i++) {
+                long c = gi(S, i, j, k);
+                for (int q = 0; q < 6; q++) {
+                    S->f[F_S11 + q][c] += S->f[F_CS11 + q][c];
+                    S->f[F_CS11 + q][c] = 0;
+                }
+                S->f[F_CHI][c] += S->f[F_CCHI][c];
+                S->f[F_CCHI][c] = 0;
+            }
+}
+
+static int mg_o(const sto_sim *S) { return S->Nz + (S->prd[2] ? 0 : 1); }
+
+static void setup_mg(sto_sim *S, double dt) {
+    if (S->mg_ready) mgo_free(&S->mg);
+    int o = mg_o(S);
+    mgo_build(&S->mg, S->Nx, S->Ny, o, S->prd, MGO_PLANE, 1024);
+    /* stencil = -dt*stencil_base, fix (src/shear_sim_3d.hh:45-55) */
+    double L2 = S->lamb + 2 * S->mu, c = (S->lamb + S->mu) / 4, mu = S->mu;
+    double dx = S->dx, dy = S->dy, dz = S->dz;
+    double base[27][9];
+    memset(base, 0, sizeof(base));
+    for (int s = -1; s <= 1; s += 2) {
+        double *bx = base[13 + s], *by = base[13 + 3 * s], *bz = base[13 + 9 * s];
+        bx[0] = L2 / dx / dx; bx[4] = mu / dx / dx; bx[8] = mu / dx / dx;
+        by[0] = mu / dy / dy; by[4] = L2 / dy / dy; by[8] = mu / dy / dy;
+        bz[0] = mu / dz / dz; bz[4] = mu / dz / dz; bz[8] = L2 / dz / dz;
+    }
+    for (int si = -1; si <= 1; si += 2)
+        for (int sj = -1; sj <= 1; sj += 2) {
+            double sg = si * sj;
+            double *b = base[13 + si + 3 * sj];
+            b[1] = b[3] = sg * c / dx / dy;
+            b = base[13 + si + 9 * sj];
+            b[2] = b[6] = sg * c / dx / dz;
+            b = base[13 + 3 * si + 9 * sj];
+            b[5] = b[7] = sg * c / dz / dy;
+        }
+    base[13][0] = -2 * (L2 / dx / dx + mu / dy / dy + mu / dz / dz);
+    base[13][4] = -2 * (mu / dx / dx + L2 / dy / dy + mu / dz / dz);
+    base[13][8] = -2 * (mu / dx / dx + mu / dy / dy + L2 / dz / dz);
+    for (int b = 0; b < 27; b++)
+        for (int e = 0; e < 9; e++) S->stencil[9 * b + e] = -dt * base[b][e];
+    memset(S->fix, 0, sizeof(S->fix));
+    double fv = -dt * 2 * (L2 / dx / dx + mu / dy / dy + mu / dz / dz);
+    S->fix[0] = S->fix[4] = S->fix[8] = fv;
+    double *ops = (double *)calloc((size_t)o * 243, sizeof(double));
+    for (int k = 0; k < o; k++) {
+        if (!S->prd[2] && (k == 0 || k == o - 1)) memcpy(ops + 243L * k + 9 * 13, S->fix, sizeof(S->fix));
+        else memcpy(ops + 243L * k, S->stencil, sizeof(S->stencil));
+    }
+    mgo_set_ops(&S->mg, ops);
+    free(ops);
+    mgo_rat(&S->mg);
+    S->mg_ready = 1;
+}
+
+static void projection(sto_sim *S, double dt) {
+    mgo_level *L = &S->mg.lv[0];
+    int o = mg_o(S);
+    long X = 1, Y = S->nxg, Z = S->nxyg;
+    double ix = .25 * (1 / S->dx), iy = .25 * (1 / S->dy), iz = .25 * (1 / S->dz);
+    double ut = top_u(S);
+    for (int k = 0; k < o; k++)
+        for (int j = 0; j < S->Ny; j++)
+            for (int i = 0; i < S->Nx; i++) {
+                long p = i + (long)S->Nx * (j + (long)S->Ny * k), c = gi(S, i, j, k);
+                L->x[3 * p] = S->f[F_U][c];
+                L->x[3 * p + 1] = S->f[F_V][c];
+                L->x[3 * p + 2] = S->f[F_W][c];
+                double r0, r1, r2;
+                if (!S->prd[2] && k == 0) { r0 = -S->fix[0] * ut; r1 = 0; r2 = 0; }
+                else if (!S->prd[2] && k == S->Nz) { r0 = S->fix[0] * ut; r1 = 0; r2 = 0; }
+                else {
+                    /* calculate_first_order_stresses, src/shear_sim_3d.cpp:980-996 */
+                    const double *a;
+#define DX(fld) (a = S->f[fld] + c, ix * (a[0] - a[-X] + a[-Y] - a[-X - Y] + a[-Z] - a[-X - Z] + a[-Y - Z] - a[-X - Y - Z]))
+#define DY(fld) (a = S->f[fld] + c, iy * (a[0] - a[-Y] + a[-X] - a[-X - Y] + a[-Z] - a[-Y - Z] + a[-X - Z] - a[-X - Y - Z]))
+#define DZ(fld) (a = S->f[fld] + c, iz * (a[0] - a[-Z] + a[-X] - a[-X - Z] + a[-Y] - a[-Y - Z] + a[-X - Y] - a[-X - Y - Z]))
+                    double d11x = DX(F_S11), d12x = DX(F_S12), d13x = DX(F_S13);
+                    double d12y = DY(F_S12), d22y = DY(F_S22), d23y = DY(F_S23);
+                    double d13z = DZ(F_S13), d23z = DZ(F_S23), d33z = DZ(F_S33);
+#undef DX
+#undef DY
+#undef DZ
+                    r0 = d11x + d12y + d13z;
+                    r1 = d12x + d22y + d23z;
+                    r2 = d13x + d23y + d33z;
+                }
+                L->r[3 * p] = r0;
+                L->r[3 * p + 1] = r1;
+                L->r[3 * p + 2] = r2;
+            }
+    /* tolerance loop (src/shear_sim_3d.cpp:551-600) */
+    double tol = 2.220446049250313e-16 * S->stencil[9 * 13] * 1e3;
+    tol *= tol;
+    int num_iters = 1;
+    double err = 1e16;
+    S->vcycles = 0;
+    while (err > tol) {
+        mgo_vcycle(&S->mg);
+        S->vcycles++;
+        err = mgo_norm_quartic(L);
+        if (num_iters >= S->std_iters / 16) err = mgo_norm_quartic(L);
+        num_iters++;
+        if (S->vcycles > 10000) break;
+    }
+    if (num_iters >= S->std_iters / 16) S->std_iters = num_iters * 16;
+    S->std_iters--;
+    mgo_vcycle(&S->mg);
+    mgo_vcycle(&S->mg);
+    S->vcycles += 2;
+    /* copy-out including the top vertex plane */
+    for (int k = 0; k < o; k++)
+        for (int j = 0; j < S->Ny; j++)
+            for (int i = 0; i < S->Nx; i++) {
+                long p = i + (long)S->Nx * (j + (long)S->Ny * k), c = gi(S, i, j, k);
+                S->f[F_U][c] = L->x[3 * p];
+                S->f[F_V][c] = L->x[3 * p + 1];
+                S->f[F_W][c] = L->x[3 * p + 2];
+            }
+    refresh(S);
+    for (int k = 0; k < S->Nz; k++)
+        for (int j = 0; j < S->Ny; j++)
+            for (int i = 0; i < S->Nx; i++) {
+                long c = gi(S, i, j, k);
+                double g[3][3];
+                vel_grad(S, c, g);
+                double lam = S->lamb, mu = S->mu, L2 = lam + 2 * mu;
+                S->f[F_S11][c] += dt * (lam * (g[2][2] + g[1][1]) + L2 * g[0][0]);
+                S->f[F_S12][c] += dt * mu * (g[0][1] + g[1][0]);
+                S->f[F_S13][c] += dt * mu * (g[0][2] + g[2][0]);
+                S->f[F_S22][c] += dt * (lam * (g[2][2] + g[0][0]) + L2 * g[1][1]);
+                S->f[F_S23][c] += dt * mu * (g[1][2] + g[2][1]);
+                S->f[F_S33][c] += dt * (lam * (g[1][1] + g[0][0]) + L2 * g[2][2]);
+            }
+}
+
+void sto_step(void *hp, double dt, int nsteps) {
+    sto_sim *S = (sto_sim *)hp;
+    if (!S->mg_ready) setup_mg(S, dt);
+    for (int s = 0; s < nsteps; s++) {
+        for (int k = 0; k < S->Nz; k++)
+            for (int j = 0; j < S->Ny; j++)
+                for (int i = 0; i < S->Nx; i++) advection_cell(S, gi(S, i, j, k), dt);
+        S->t += dt;
+        merge(S);
+        ghosts(S);
+        projection(S, dt);
+        refresh(S);
+    }
+}
+
+void sto_set_state(void *hp, double t, const double *u, const double *sig, const double *chi) {
+    sto_sim *S = (sto_sim *)hp;
+    S->t = t;
+    int o = mg_o(S);
+    for (int k = 0; k < o; k++)
+        for (int j = 0; j < S->Ny; j++)
+            for (int i = 0; i < S->Nx; i++) {
+                long q = i + (long)S->Nx * (j + (long)S->Ny * k), c = gi(S, i, j, k);
+                S->f[F_U][c] = u[3 * q];
+                S->f[F_V][c] = u[3 * q + 1];
+                S->f[F_W][c] = u[3 * q + 2];
+                if (k < S->Nz) {
+                    for (int a = 0; a < 6; a++) S->f[F_S11 + a][c] = sig[6 * q + a];
+                    S->f[F_CHI][c] = chi[q];
+                    for (int a = F_CS11; a < F_NF; a++) S->f[a][c] = 0;
+                }
+            }
+    refresh(S);
+}
+
+void sto_get_state(void *hp, double *t, double *u, double *sig, double *chi, double *dpl) {
+    sto_sim *S = (sto_sim *)hp;
+    *t = S->t;
+    int o = mg_o(S);
+    for (int k = 0; k < o; k++)
+        for (int j = 0; j < S->Ny; j++)
+            for (int i = 0; i < S->Nx; i++) {
+                long q = i + (long)S->Nx * (j + (long)S->Ny * k), c = gi(S, i, j, k);
+                u[3 * q] = S->f[F_U][c];
+                u[3 * q + 1] = S->f[F_V][c];
+                u[3 * q + 2] = S->f[F_W][c];
+                if (k < S->Nz) {
+                    for (int a = 0; a < 6; a++) sig[6 * q + a] = S->f[F_S11 + a][c];
+                    chi[q] = S->f[F_CHI][c];
+                    dpl[q] = S->f[F_DPL][c];
+                }
+            }
+}
+
+int sto_vcycles(void *hp) { return ((sto_sim *)hp)->vcycles; }

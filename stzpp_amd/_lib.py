@@ -48,8 +48,30 @@ class StzmgSolveStats(ctypes.Structure):
     ]
 
 
-# Every symbol declared in include/stzmg.h with (argtypes, restype).
+class StzParams(ctypes.Structure):
+    _fields_ = [("Nx", ctypes.c_int), ("Ny", ctypes.c_int), ("Nz", ctypes.c_int),
+                ("prd", ctypes.c_int * 3)] + [
+        (k, ctypes.c_double) for k in (
+            "ax", "bx", "ay", "by", "az", "bz", "mu", "lamb", "sy", "tau0", "c0", "eps0",
+            "delta", "omega", "bath_temp", "chi_inf", "ez", "diff_l", "U", "eta")]
+
+
+class StzStepStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int), ("vcycles", ctypes.c_int), ("yielded_cells", ctypes.c_int),
+                ("solve_ms", ctypes.c_double), ("step_ms", ctypes.c_double),
+                ("last_res", ctypes.c_double)]
+
+
+# Every symbol declared in include/*.h with (argtypes, restype).
 SIGNATURES = {
+    "stz_params_default": ([ctypes.POINTER(StzParams)] + [ctypes.c_int] * 3, ctypes.c_int),
+    "stz_create": ([ctypes.POINTER(StzParams), ctypes.POINTER(_vp)], ctypes.c_int),
+    "stz_destroy": ([_vp], ctypes.c_int),
+    "stz_last_error": ([_vp], ctypes.c_char_p),
+    "stz_set_state": ([_vp, ctypes.c_double, _dp, _dp, _dp], ctypes.c_int),
+    "stz_get_state": ([_vp, _dp, _dp, _dp, _dp, _dp], ctypes.c_int),
+    "stz_step_qs": ([_vp, ctypes.c_double, ctypes.c_int, _vp, ctypes.POINTER(StzStepStats)], ctypes.c_int),
+    "stz_multigrid": ([_vp], _vp),
     "stzmg_desc_default": ([ctypes.POINTER(StzmgDesc)] + [ctypes.c_int] * 6, ctypes.c_int),
     "stzmg_create": ([ctypes.POINTER(StzmgDesc), ctypes.POINTER(_vp)], ctypes.c_int),
     "stzmg_destroy": ([_vp], ctypes.c_int),

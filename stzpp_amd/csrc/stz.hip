@@ -1,0 +1,288 @@
+// Host side of the quasi-static STZ step and its C ABI (include/stz.h).
+//
+// One step follows shear_sim_3d::step_forward_qs (src/shear_sim_3d.cpp:
+// 381-389): plasticity + advection (two passes, k_plastic then k_advect, so
+// that the reference's in-place ad_Dpl ordering is reproduced without a
+// sequential loop), time advance, projection (right-hand side, multigrid
+// solve, copy-out with plate values, stress update).  Everything runs on the
+// multigrid handle's HIP stream; the state never leaves HBM between steps.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <utility>
+
+#include "../../include/stz.h"
+#include "mg_internal.hpp"
+#include "stz_kernels.hpp"
+
+using namespace stz;
+
+struct stz_s {
+    stz_params p{};
+    Grid g{};
+    Mat M{};
+    double t = 0;
+    stzmg_t mg = nullptr;
+    double mg_dt = -1;          // dt the operator was built for
+    double a11 = 0;             // stencil[13].a11 of the current operator
+    double fix_a11 = 0;
+    hipStream_t stream = nullptr;
+    double *vel = nullptr, *sig = nullptr, *sig2 = nullptr, *chi = nullptr, *chi2 = nullptr;
+    double *dpl = nullptr, *dpl2 = nullptr, *adapt = nullptr, *dchi = nullptr, *stage = nullptr;
+    int *counters = nullptr;
+    std::string err;
+};
+
+namespace {
+
+int sfail(stz_t s, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (s) s->err = buf;
+    return code;
+}
+
+#define SHIP(s, call)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) return sfail((s), STZMG_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+inline unsigned nb(long n) { return (unsigned)((n + kT - 1) / kT); }
+
+double top_u(const stz_t s, double t) { return t < 1 ? s->p.U * t : s->p.U; }
+
+// (Re)build the projection operator for step size dt
+// (set_up_stencil_and_matrices, src/shear_sim_3d.hh:45-55).
+int ensure_operator(stz_t s, double dt) {
+    if (s->mg && s->mg_dt == dt) return 0;
+    const Grid &g = s->g;
+    if (!s->mg) {
+        stzmg_desc d;
+        stzmg_desc_default(&d, g.Nx, g.Ny, g.Nzv, 1, 1, g.pz);
+        int rc = stzmg_create(&d, &s->mg);
+        if (rc) return sfail(s, rc, "projection solver: %s", stzmg_last_error(nullptr));
+        s->stream = (hipStream_t)stzmg_stream(s->mg);
+    }
+    double B[243], F[9];
+    stzmg_shear_stencil(s->p.mu, s->p.lamb, g.dx, g.dy, g.dz, dt, 0.0, B, F);
+    int rc = stzmg_set_operator_const(s->mg, B, g.pz ? nullptr : F);
+    if (rc) return sfail(s, rc, "projection operator: %s", stzmg_last_error(s->mg));
+    s->a11 = B[9 * 13];
+    s->fix_a11 = F[0];
+    s->mg_dt = dt;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int stz_params_default(stz_params *p, int Nx, int Ny, int Nz) {
+    if (!p) return STZMG_ERR_ARG;
+    memset(p, 0, sizeof(*p));
+    p->Nx = Nx; p->Ny = Ny; p->Nz = Nz;
+    p->prd[0] = 1; p->prd[1] = 1; p->prd[2] = 0;
+    p->ax = -1; p->bx = 1; p->ay = -1; p->by = 1; p->az = -0.5; p->bz = 0.5;
+    // Vitreloy 1 parameters of the shear_sim driver, nondimensionalised by
+    // the yield stress, the length 1 cm and the shear-wave time
+    // (src/shear_sim.cpp:30-86), with TZ = 21000 K.
+    const double TZ = 21000, kb = 1.3806503e-23;
+    const double E = 101e9, nu = .35, sy = .85e9;
+    const double mu = E / (2 * (1 + nu)), K = E / (3 * (1 - 2 * nu)), rho = 6125;
+    const double ts = 1e-2 / std::sqrt(mu / rho);
+    p->mu = mu / sy;
+    p->lamb = K / sy - (2. / 3.) * p->mu;
+    p->sy = 1;
+    p->tau0 = 1e-13 / ts;
+    p->c0 = .4;
+    p->eps0 = .3;
+    p->delta = 8000 / TZ;
+    p->omega = 300 * 26. / 17. * 1e-30 * sy / (TZ * kb);
+    p->bath_temp = 400 / TZ;
+    p->chi_inf = 900 / TZ;
+    p->ez = 1;
+    p->diff_l = 1.5 * (p->bx - p->ax) / Nx;
+    p->U = 1e-7;
+    p->eta = 0.0005;
+    return 0;
+}
+
+static thread_local std::string g_stz_err;
+
+int stz_create(const stz_params *p, stz_t *out) {
+    if (!p || !out) return STZMG_ERR_ARG;
+    *out = nullptr;
+    if (!p->prd[0] || !p->prd[1]) {
+        g_stz_err = "x and y must be periodic";
+        return STZMG_ERR_UNSUPPORTED;
+    }
+    if (p->Nx < 4 || p->Ny < 4 || p->Nz < 4) {
+        g_stz_err = "grid too small";
+        return STZMG_ERR_ARG;
+    }
+    stz_t s = new stz_s;
+    s->p = *p;
+    Grid &g = s->g;
+    g.Nx = p->Nx; g.Ny = p->Ny; g.Nz = p->Nz;
+    g.pz = p->prd[2] != 0;
+    g.Nzv = g.Nz + (g.pz ? 0 : 1);
+    g.Nc = (long)g.Nx * g.Ny * g.Nz;
+    g.Nv = (long)g.Nx * g.Ny * g.Nzv;
+    g.dx = (p->bx - p->ax) / g.Nx;
+    g.dy = (p->by - p->ay) / g.Ny;
+    g.dz = (p->bz - p->az) / g.Nz;
+    s->M = Mat{p->mu, p->lamb, p->sy, p->tau0, p->c0, p->eps0, p->delta, p->omega, p->bath_temp,
+               p->chi_inf, p->ez, p->diff_l, p->eta};
+    // The solver owns the stream; create it now with a placeholder dt.
+    if (int rc = ensure_operator(s, 1.0)) {
+        g_stz_err = s->err;
+        if (s->mg) stzmg_destroy(s->mg);
+        delete s;
+        return rc;
+    }
+    const long Nc = g.Nc, Nv = g.Nv;
+    hipError_t e = hipSuccess;
+    auto A = [&](double **ptr, long n) {
+        if (e == hipSuccess) e = hipMalloc((void **)ptr, sizeof(double) * n);
+    };
+    A(&s->vel, 3 * Nv); A(&s->sig, 6 * Nc); A(&s->sig2, 6 * Nc); A(&s->chi, Nc); A(&s->chi2, Nc);
+    A(&s->dpl, Nc); A(&s->dpl2, Nc); A(&s->adapt, Nc); A(&s->dchi, Nc);
+    A(&s->stage, 6 * (Nc > Nv ? Nc : Nv));
+    if (e == hipSuccess) e = hipMalloc((void **)&s->counters, sizeof(int) * 4);
+    if (e != hipSuccess) {
+        g_stz_err = "device allocation failed";
+        stz_destroy(s);
+        return STZMG_ERR_NOMEM;
+    }
+    hipMemsetAsync(s->dpl, 0, sizeof(double) * Nc, s->stream);
+    hipMemsetAsync(s->vel, 0, sizeof(double) * 3 * Nv, s->stream);
+    hipMemsetAsync(s->sig, 0, sizeof(double) * 6 * Nc, s->stream);
+    hipMemsetAsync(s->chi, 0, sizeof(double) * Nc, s->stream);
+    hipStreamSynchronize(s->stream);
+    *out = s;
+    return 0;
+}
+
+int stz_destroy(stz_t s) {
+    if (!s) return STZMG_ERR_ARG;
+    if (s->stream) hipStreamSynchronize(s->stream);
+    for (double *q : {s->vel, s->sig, s->sig2, s->chi, s->chi2, s->dpl, s->dpl2, s->adapt, s->dchi, s->stage})
+        hipFree(q);
+    hipFree(s->counters);
+    if (s->mg) stzmg_destroy(s->mg);
+    delete s;
+    return 0;
+}
+
+const char *stz_last_error(stz_t s) { return s ? s->err.c_str() : g_stz_err.c_str(); }
+
+stzmg_t stz_multigrid(stz_t s) { return s ? s->mg : nullptr; }
+
+int stz_set_state(stz_t s, double t, const double *u, const double *sig, const double *chi) {
+    if (!s || !u || !sig || !chi) return STZMG_ERR_ARG;
+    const Grid &g = s->g;
+    s->t = t;
+    SHIP(s, hipMemcpyAsync(s->stage, u, sizeof(double) * 3 * g.Nv, hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_aos2soa, dim3(nb(g.Nv)), dim3(kT), 0, s->stream, (const double *)s->stage, s->sig2, g.Nv, 3);
+    // plates at time t (set_boundary_conditions)
+    hipLaunchKernelGGL(k_copy_vel, dim3(nb(g.Nv)), dim3(kT), 0, s->stream, g, (const double *)s->sig2, top_u(s, t),
+                       s->vel);
+    SHIP(s, hipMemcpyAsync(s->stage, sig, sizeof(double) * 6 * g.Nc, hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_aos2soa, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, (const double *)s->stage, s->sig, g.Nc, 6);
+    SHIP(s, hipMemcpyAsync(s->chi, chi, sizeof(double) * g.Nc, hipMemcpyHostToDevice, s->stream));
+    SHIP(s, hipMemsetAsync(s->dpl, 0, sizeof(double) * g.Nc, s->stream));
+    SHIP(s, hipGetLastError());
+    SHIP(s, hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int stz_get_state(stz_t s, double *t, double *u, double *sig, double *chi, double *dpl) {
+    if (!s) return STZMG_ERR_ARG;
+    const Grid &g = s->g;
+    if (t) *t = s->t;
+    if (u) {
+        hipLaunchKernelGGL(k_soa2aos, dim3(nb(g.Nv)), dim3(kT), 0, s->stream, (const double *)s->vel, s->stage, g.Nv, 3);
+        SHIP(s, hipMemcpyAsync(u, s->stage, sizeof(double) * 3 * g.Nv, hipMemcpyDeviceToHost, s->stream));
+        SHIP(s, hipStreamSynchronize(s->stream));
+    }
+    if (sig) {
+        hipLaunchKernelGGL(k_soa2aos, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, (const double *)s->sig, s->stage, g.Nc, 6);
+        SHIP(s, hipMemcpyAsync(sig, s->stage, sizeof(double) * 6 * g.Nc, hipMemcpyDeviceToHost, s->stream));
+        SHIP(s, hipStreamSynchronize(s->stream));
+    }
+    if (chi) SHIP(s, hipMemcpyAsync(chi, s->chi, sizeof(double) * g.Nc, hipMemcpyDeviceToHost, s->stream));
+    if (dpl) SHIP(s, hipMemcpyAsync(dpl, s->dpl, sizeof(double) * g.Nc, hipMemcpyDeviceToHost, s->stream));
+    SHIP(s, hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int stz_step_qs(stz_t s, double dt, int nsteps, const stzmg_solve_opts *opts, stz_step_stats *st) {
+    if (!s || !(dt > 0) || nsteps < 0) return STZMG_ERR_ARG;
+    if (int rc = ensure_operator(s, dt)) return rc;
+    const Grid &g = s->g;
+    stz_step_stats S{};
+    auto t0 = std::chrono::steady_clock::now();
+    stzmg_solve_opts o;
+    if (opts) {
+        o = *opts;
+    } else {
+        stzmg_solve_opts_default(&o);
+        o.mode = STZMG_SOLVE_REF;
+        // tol = (eps * stencil[13].a11 * 1e3)^2, src/shear_sim_3d.cpp:551-555
+        const double tol = std::numeric_limits<double>::epsilon() * s->a11 * 1e3;
+        o.tol = tol * tol;
+        o.extra_cycles = 2;
+        o.ref_norm_schedule = 0;
+        o.max_cycles = 100000;
+    }
+    for (int q = 0; q < nsteps; q++) {
+        SHIP(s, hipMemsetAsync(s->counters, 0, sizeof(int) * 4, s->stream));
+        hipLaunchKernelGGL(k_plastic, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt, (const double *)s->sig,
+                           (const double *)s->chi, s->adapt, s->dchi, s->dpl2, s->counters);
+        hipLaunchKernelGGL(k_advect, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt, (const double *)s->vel,
+                           (const double *)s->sig, (const double *)s->chi, (const double *)s->adapt,
+                           (const double *)s->dchi, (const double *)s->dpl2, (const double *)s->dpl, s->sig2,
+                           s->chi2);
+        SHIP(s, hipGetLastError());
+        std::swap(s->sig, s->sig2);
+        std::swap(s->chi, s->chi2);
+        std::swap(s->dpl, s->dpl2);
+        s->t += dt;
+        const double ut = top_u(s, s->t);
+        hipLaunchKernelGGL(k_rhs, dim3(nb(g.Nv)), dim3(kT), 0, s->stream, g, (const double *)s->sig,
+                           (const double *)s->vel, s->fix_a11, ut, stzmg_internal::x0(s->mg),
+                           stzmg_internal::r0(s->mg));
+        SHIP(s, hipGetLastError());
+        stzmg_solve_stats ms{};
+        int rc = stzmg_solve(s->mg, &o, &ms);
+        S.vcycles += ms.cycles;
+        S.solve_ms += ms.ms;
+        S.last_res = ms.res;
+        if (rc) return sfail(s, rc, "projection solve at t=%g: %s", s->t, stzmg_last_error(s->mg));
+        hipLaunchKernelGGL(k_copy_vel, dim3(nb(g.Nv)), dim3(kT), 0, s->stream, g,
+                           (const double *)stzmg_internal::x0(s->mg), ut, s->vel);
+        hipLaunchKernelGGL(k_stress_update, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt,
+                           (const double *)s->vel, s->sig);
+        SHIP(s, hipGetLastError());
+        int cnt[4];
+        SHIP(s, hipMemcpyAsync(cnt, s->counters, sizeof(cnt), hipMemcpyDeviceToHost, s->stream));
+        SHIP(s, hipStreamSynchronize(s->stream));
+        S.yielded_cells = cnt[0];
+        S.steps++;
+    }
+    S.step_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (st) *st = S;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,332 @@
+// Device kernels of the quasi-static STZ step (gfx950, FP64).
+//
+// Layout in HBM (SoA, x fastest, no ghost padding):
+//   vel   3 x Nv   vertex velocity, Nv = Nx*Ny*Nzv (Nzv = Nz+1 walled, Nz periodic);
+//                  this is also the multigrid's level-0 layout
+//   sig   6 x Nc   cell stress (s11 s12 s13 s22 s23 s33), Nc = Nx*Ny*Nz
+//   chi   Nc       effective temperature
+//   dpl   Nc       ad_Dpl
+// Ghost values are produced on the fly: periodic wrap in x, y (and z when
+// periodic); across a wall the reference's linear extrapolation
+// f(-1) = 2f(0) - f(1), f(-2) = 2f(-1) - f(0) of the primary fields, with
+// ad_Dpl zero there (set_proper_ghost_for_boundaries,
+// src/shear_sim_3d.cpp:2662-2789).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace stz {
+
+constexpr int kT = 256;
+
+struct Grid {
+    int Nx, Ny, Nz, Nzv;
+    int pz;
+    long Nc, Nv;
+    double dx, dy, dz;
+};
+
+struct Mat {
+    double mu, lamb, sy, tau0, c0, eps0, delta, omega, bath_temp, chi_inf, ez, diff_l, eta;
+};
+
+__device__ __forceinline__ int wrapc(int i, int n) {
+    while (i < 0) i += n;
+    while (i >= n) i -= n;
+    return i;
+}
+
+__device__ __forceinline__ long cidx(const Grid &g, int i, int j, int k) {
+    return wrapc(i, g.Nx) + (long)g.Nx * (wrapc(j, g.Ny) + (long)g.Ny * k);
+}
+
+// Cell value with ghosts: k in [-2, Nz+2).
+__device__ __forceinline__ double cellv(const double *__restrict__ f, const Grid &g, int i, int j, int k) {
+    if (g.pz) return f[cidx(g, i, j, wrapc(k, g.Nz))];
+    if (k >= 0 && k < g.Nz) return f[cidx(g, i, j, k)];
+    if (k < 0) {
+        const double f0 = f[cidx(g, i, j, 0)], f1 = f[cidx(g, i, j, 1)];
+        const double m1 = (f0 + f0) - f1;
+        return k == -1 ? m1 : (m1 + m1) - f0;
+    }
+    const double a = f[cidx(g, i, j, g.Nz - 1)], b = f[cidx(g, i, j, g.Nz - 2)];
+    const double p0 = (a + a) - b;
+    return k == g.Nz ? p0 : (p0 + p0) - a;
+}
+
+// Vertex velocity component at (i,j,k), k in [0, Nz] (wrapped when periodic).
+__device__ __forceinline__ double vertv(const double *__restrict__ u, const Grid &g, int i, int j, int k) {
+    if (g.pz) k = wrapc(k, g.Nz);
+    return u[wrapc(i, g.Nx) + (long)g.Nx * (wrapc(j, g.Ny) + (long)g.Ny * k)];
+}
+
+// Velocity gradient G[q][a] = d u_q / d x_a and cell-centred velocity of
+// cell (i,j,k) from its 8 vertices (calculate_first_order_velocities,
+// src/shear_sim_3d.cpp:901-916; staggered average, :429-431).
+__device__ __forceinline__ void vel_grad(const double *__restrict__ vel, const Grid &g, int i, int j, int k,
+                                         double G[3][3], double vc[3]) {
+    const double ix = .25 * (1.0 / g.dx), iy = .25 * (1.0 / g.dy), iz = .25 * (1.0 / g.dz);
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const double *u = vel + q * g.Nv;
+        const double u000 = vertv(u, g, i, j, k), u100 = vertv(u, g, i + 1, j, k);
+        const double u010 = vertv(u, g, i, j + 1, k), u110 = vertv(u, g, i + 1, j + 1, k);
+        const double u001 = vertv(u, g, i, j, k + 1), u101 = vertv(u, g, i + 1, j, k + 1);
+        const double u011 = vertv(u, g, i, j + 1, k + 1), u111 = vertv(u, g, i + 1, j + 1, k + 1);
+        G[q][0] = ix * (u100 - u000 + u110 - u010 + u101 - u001 + u111 - u011);
+        G[q][1] = iy * (u010 - u000 + u110 - u100 + u011 - u001 + u111 - u101);
+        G[q][2] = iz * (u001 - u000 + u101 - u100 + u011 - u010 + u111 - u110);
+        vc[q] = 0.125 * (u000 + u100 + u001 + u101 + u010 + u110 + u011 + u111);
+    }
+}
+
+// Field::dev (src/field.hh:149-184).
+__device__ __forceinline__ double sbar_of(const double s[6]) {
+    const double p = -(s[0] + s[3] + s[5]) * (1 / 3.);
+    const double t11 = s[0] + p, t22 = s[3] + p, t33 = s[5] + p;
+    return sqrt(0.5 * (t11 * t11 + t22 * t22 + t33 * t33) + s[1] * s[1] + s[2] * s[2] + s[4] * s[4]);
+}
+
+__device__ __forceinline__ double curly_c(const Mat &M, double sb) {
+    const double a = exp((M.omega * M.eps0 * sb - M.delta) / M.bath_temp);
+    const double b = exp(-(M.omega * M.eps0 * sb + M.delta) / M.bath_temp);
+    return .5 * (a + b);
+}
+
+__device__ __forceinline__ double calc_dpl(const Mat &M, double sb, double chi) {
+    return sb >= M.sy ? exp(-M.ez / chi) * curly_c(M, sb) * (1 - M.sy / sb) / M.tau0 : 0.0;
+}
+
+// adaptive_plastic_term (src/shear_sim_3d.cpp:109-149): returns
+// 1 - sbar_end/sbar_0, the plastic chi change in dchi, the substep count in n.
+__device__ __forceinline__ double adaptive(const Mat &M, double sb0, double chi0, double dt, double &dchi,
+                                           int &n) {
+    double tr = dt, ts, sb = sb0, chi = chi0;
+    bool keep = true;
+    n = 0;
+    while (keep) {
+        const double dpl = calc_dpl(M, sb, chi);
+        const double dprime = 2 * M.mu * dpl / M.sy;
+        const double F = 2 * dpl * sb * (M.chi_inf - chi) / (M.sy * M.c0);
+        if (dprime * tr > M.eta) {
+            ts = M.eta / fabs(dprime);
+            tr = tr - ts;
+        } else {
+            ts = tr;
+            keep = false;
+        }
+        sb = sb - ts * dprime * M.sy;
+        chi = chi + ts * F;
+        n++;
+    }
+    dchi = chi - chi0;
+    return 1 - sb / sb0;
+}
+
+// eno2 (src/shear_sim_3d.cpp:1257-1266), h already divided by dt.
+__device__ __forceinline__ double eno2(double h, double v, double f2p, double fp, double f, double fm, double f2m) {
+    const double c2 = fabs(fp - 2 * f + fm);
+    const double fw = fabs(f2p - 2 * fp + f);
+    const double bw = fabs(f - 2 * fm + f2m);
+    return .5 / h * (v < 0 ? (c2 > fw ? -f2p + 4 * fp - 3 * f : fp - fm)
+                           : (c2 > bw ? 3 * f - 4 * fm + f2m : fp - fm));
+}
+
+// Pass 1: pointwise STZ plasticity of every cell (the part of
+// advection_grid_step before the advective terms, src/shear_sim_3d.cpp:
+// 424-434).  Writes adapt = 2 mu dt Dpl/sbar, the plastic chi change and
+// this step's ad_Dpl.
+__global__ __launch_bounds__(kT) void k_plastic(Grid g, Mat M, double dt, const double *__restrict__ sig,
+                                                const double *__restrict__ chi, double *__restrict__ adapt,
+                                                double *__restrict__ dchi, double *__restrict__ dpl_new,
+                                                int *__restrict__ counters) {
+    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    if (c >= g.Nc) return;
+    double s[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) s[q] = sig[q * g.Nc + c];
+    const double sb = sbar_of(s);
+    double a = 0, dc = 0;
+    if (sb > M.sy) {
+        int n;
+        a = adaptive(M, sb, chi[c], dt, dc, n);
+        atomicAdd(&counters[0], 1);
+    }
+    adapt[c] = a;
+    dchi[c] = dc;
+    dpl_new[c] = a / 2. / M.mu * sb;
+}
+
+// Pass 2: advection of stress and chi with ENO2, Truesdell terms and chi
+// diffusion (advection_grid_step, src/shear_sim_3d.cpp:436-525), merged
+// into new arrays (merge_updates, :154-162).  ad_Dpl follows the reference's
+// in-place lexicographic loop: this step's value at the cell and at its
+// -x/-y/-z neighbours inside the domain, last step's value at +x/+y/+z and
+// at periodic images, zero across a wall.
+__global__ __launch_bounds__(kT) void k_advect(Grid g, Mat M, double dt, const double *__restrict__ vel,
+                                               const double *__restrict__ sig, const double *__restrict__ chi,
+                                               const double *__restrict__ adapt, const double *__restrict__ dchi,
+                                               const double *__restrict__ dnew, const double *__restrict__ dold,
+                                               double *__restrict__ sig_out, double *__restrict__ chi_out) {
+    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    if (c >= g.Nc) return;
+    const int i = (int)(c % g.Nx);
+    const int j = (int)((c / g.Nx) % g.Ny);
+    const int k = (int)(c / ((long)g.Nx * g.Ny));
+    double G[3][3], v[3];
+    vel_grad(vel, g, i, j, k, G, v);
+    const double hx = g.dx / dt, hy = g.dy / dt, hz = g.dz / dt;
+    double ug[7];
+#pragma unroll
+    for (int q = 0; q < 7; q++) {
+        const double *f = q < 6 ? sig + q * g.Nc : chi;
+        const double f0 = f[c];
+        const double ddx = eno2(hx, v[0], cellv(f, g, i + 2, j, k), cellv(f, g, i + 1, j, k), f0,
+                                cellv(f, g, i - 1, j, k), cellv(f, g, i - 2, j, k));
+        const double ddy = eno2(hy, v[1], cellv(f, g, i, j + 2, k), cellv(f, g, i, j + 1, k), f0,
+                                cellv(f, g, i, j - 1, k), cellv(f, g, i, j - 2, k));
+        const double ddz = eno2(hz, v[2], cellv(f, g, i, j, k + 2), cellv(f, g, i, j, k + 1), f0,
+                                cellv(f, g, i, j, k - 1), cellv(f, g, i, j, k - 2));
+        ug[q] = v[0] * ddx + v[1] * ddy + v[2] * ddz;
+    }
+    const double s11 = sig[c], s12 = sig[g.Nc + c], s13 = sig[2 * g.Nc + c];
+    const double s22 = sig[3 * g.Nc + c], s23 = sig[4 * g.Nc + c], s33 = sig[5 * g.Nc + c];
+    const double du_dx = G[0][0], du_dy = G[0][1], du_dz = G[0][2];
+    const double dv_dx = G[1][0], dv_dy = G[1][1], dv_dz = G[1][2];
+    const double dw_dx = G[2][0], dw_dy = G[2][1], dw_dz = G[2][2];
+    const double t11 = s11 * (du_dx - dv_dy - dw_dz) + 2 * s13 * du_dz + 2 * s12 * du_dy;
+    const double t12 = s23 * du_dz + s13 * dv_dz - s12 * dw_dz + s22 * du_dy + s11 * dv_dx;
+    const double t13 = s33 * du_dz + s11 * dw_dx + s23 * du_dy + s12 * dw_dy - s13 * dv_dy;
+    const double t22 = 2 * s23 * dv_dz + s22 * (dv_dy - du_dx - dw_dz) + 2 * s12 * dv_dx;
+    const double t23 = s33 * dv_dz + s22 * dw_dy + s13 * dv_dx - s23 * du_dx + s12 * dw_dx;
+    const double t33 = s33 * (dw_dz - dv_dy - du_dx) + 2 * s23 * dw_dy + 2 * s13 * dw_dx;
+    const double a = adapt[c], third = 1. / 3.;
+    sig_out[c] = s11 + (a * third * (s33 + s22 - 2 * s11) - ug[0] + dt * t11);
+    sig_out[g.Nc + c] = s12 + (-a * s12 - ug[1] + dt * t12);
+    sig_out[2 * g.Nc + c] = s13 + (-a * s13 - ug[2] + dt * t13);
+    sig_out[3 * g.Nc + c] = s22 + (a * third * (s11 - 2 * s22 + s33) - ug[3] + dt * t22);
+    sig_out[4 * g.Nc + c] = s23 + (-a * s23 - ug[4] + dt * t23);
+    sig_out[5 * g.Nc + c] = s33 + (a * third * (s11 + s22 - 2 * s33) - ug[5] + dt * t33);
+    // chi diffusion (chi_diffusion, src/shear_sim_3d.cpp:1217-1237)
+    const double Dc = dnew[c];
+    const double Dxm = i > 0 ? dnew[c - 1] : dold[cidx(g, i - 1, j, k)];
+    const double Dxp = dold[cidx(g, i + 1, j, k)];
+    const double Dym = j > 0 ? dnew[c - g.Nx] : dold[cidx(g, i, j - 1, k)];
+    const double Dyp = dold[cidx(g, i, j + 1, k)];
+    double Dzm, Dzp;
+    if (g.pz) {
+        Dzm = k > 0 ? dnew[c - (long)g.Nx * g.Ny] : dold[cidx(g, i, j, g.Nz - 1)];
+        Dzp = dold[cidx(g, i, j, wrapc(k + 1, g.Nz))];
+    } else {
+        Dzm = k > 0 ? dnew[c - (long)g.Nx * g.Ny] : 0.0;
+        Dzp = k < g.Nz - 1 ? dold[c + (long)g.Nx * g.Ny] : 0.0;
+    }
+    const double x0 = chi[c];
+    const double xxm = cellv(chi, g, i - 1, j, k), xxp = cellv(chi, g, i + 1, j, k);
+    const double xym = cellv(chi, g, i, j - 1, k), xyp = cellv(chi, g, i, j + 1, k);
+    const double xzm = cellv(chi, g, i, j, k - 1), xzp = cellv(chi, g, i, j, k + 1);
+    const double ix = 1 / g.dx, iy = 1 / g.dy, iz = 1 / g.dz;
+    const double ddx = .5 * ix * (Dxp - Dxm) * (.5 * ix * (xxp - xxm)) + Dc * (ix * ix * (xxp - 2 * x0 + xxm));
+    const double ddy = .5 * iy * (Dyp - Dym) * (.5 * iy * (xyp - xym)) + Dc * (iy * iy * (xyp - 2 * x0 + xym));
+    const double ddz = .5 * iz * (Dzp - Dzm) * (.5 * iz * (xzp - xzm)) + Dc * (iz * iz * (xzp - 2 * x0 + xzm));
+    const double diff = M.diff_l * M.diff_l * (ddx + ddy + ddz);
+    double cchi = dchi[c];
+    cchi -= ug[6];
+    cchi += diff / M.c0;
+    chi_out[c] = x0 + cchi;
+}
+
+// Right-hand side and warm start of the projection solve (setup_fields with
+// x_field / r_field, src/shear_sim_3d.hh:276-385): x = current velocity;
+// r = div sigma* at interior vertices (backward 8-cell differences,
+// calculate_first_order_stresses, src/shear_sim_3d.cpp:980-996), and
+// (-+ fix_a11 * top_u, 0, 0) on the bottom/top plates.
+__global__ __launch_bounds__(kT) void k_rhs(Grid g, const double *__restrict__ sig, const double *__restrict__ vel,
+                                            double fix_a11, double top_u, double *__restrict__ x,
+                                            double *__restrict__ r) {
+    const long p = (long)blockIdx.x * kT + threadIdx.x;
+    if (p >= g.Nv) return;
+    const int i = (int)(p % g.Nx);
+    const int j = (int)((p / g.Nx) % g.Ny);
+    const int k = (int)(p / ((long)g.Nx * g.Ny));
+#pragma unroll
+    for (int q = 0; q < 3; q++) x[q * g.Nv + p] = vel[q * g.Nv + p];
+    double r0, r1, r2;
+    if (!g.pz && k == 0) {
+        r0 = -fix_a11 * top_u; r1 = 0; r2 = 0;
+    } else if (!g.pz && k == g.Nz) {
+        r0 = fix_a11 * top_u; r1 = 0; r2 = 0;
+    } else {
+        const double ix = .25 * (1.0 / g.dx), iy = .25 * (1.0 / g.dy), iz = .25 * (1.0 / g.dz);
+        const int km = g.pz ? wrapc(k - 1, g.Nz) : k - 1;
+        const long c000 = cidx(g, i, j, k), c100 = cidx(g, i - 1, j, k), c010 = cidx(g, i, j - 1, k),
+                   c110 = cidx(g, i - 1, j - 1, k), c001 = cidx(g, i, j, km), c101 = cidx(g, i - 1, j, km),
+                   c011 = cidx(g, i, j - 1, km), c111 = cidx(g, i - 1, j - 1, km);
+        auto DX = [&](const double *a) {
+            return ix * (a[c000] - a[c100] + a[c010] - a[c110] + a[c001] - a[c101] + a[c011] - a[c111]);
+        };
+        auto DY = [&](const double *a) {
+            return iy * (a[c000] - a[c010] + a[c100] - a[c110] + a[c001] - a[c011] + a[c101] - a[c111]);
+        };
+        auto DZ = [&](const double *a) {
+            return iz * (a[c000] - a[c001] + a[c100] - a[c101] + a[c010] - a[c011] + a[c110] - a[c111]);
+        };
+        const double *S11 = sig, *S12 = sig + g.Nc, *S13 = sig + 2 * g.Nc, *S22 = sig + 3 * g.Nc,
+                     *S23 = sig + 4 * g.Nc, *S33 = sig + 5 * g.Nc;
+        r0 = DX(S11) + DY(S12) + DZ(S13);
+        r1 = DX(S12) + DY(S22) + DZ(S23);
+        r2 = DX(S13) + DY(S23) + DZ(S33);
+    }
+    r[p] = r0;
+    r[g.Nv + p] = r1;
+    r[2 * g.Nv + p] = r2;
+}
+
+// Copy the solution back into the velocity field with the plates set
+// exactly (projection copy-out and set_boundary_conditions,
+// src/shear_sim_3d.cpp:615-681, :2362-2470).
+__global__ __launch_bounds__(kT) void k_copy_vel(Grid g, const double *__restrict__ x, double top_u,
+                                                 double *__restrict__ vel) {
+    const long p = (long)blockIdx.x * kT + threadIdx.x;
+    if (p >= g.Nv) return;
+    const int k = (int)(p / ((long)g.Nx * g.Ny));
+    double v0 = x[p], v1 = x[g.Nv + p], v2 = x[2 * g.Nv + p];
+    if (!g.pz && k == 0) { v0 = -top_u; v1 = 0; v2 = 0; }
+    if (!g.pz && k == g.Nz) { v0 = top_u; v1 = 0; v2 = 0; }
+    vel[p] = v0;
+    vel[g.Nv + p] = v1;
+    vel[2 * g.Nv + p] = v2;
+}
+
+// Stress update sigma += dt C:D(u) (src/shear_sim_3d.cpp:684-712).
+__global__ __launch_bounds__(kT) void k_stress_update(Grid g, Mat M, double dt, const double *__restrict__ vel,
+                                                      double *__restrict__ sig) {
+    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    if (c >= g.Nc) return;
+    const int i = (int)(c % g.Nx);
+    const int j = (int)((c / g.Nx) % g.Ny);
+    const int k = (int)(c / ((long)g.Nx * g.Ny));
+    double G[3][3], v[3];
+    vel_grad(vel, g, i, j, k, G, v);
+    const double lam = M.lamb, mu = M.mu, L2 = lam + 2 * mu;
+    sig[c] += dt * (lam * (G[2][2] + G[1][1]) + L2 * G[0][0]);
+    sig[g.Nc + c] += dt * mu * (G[0][1] + G[1][0]);
+    sig[2 * g.Nc + c] += dt * mu * (G[0][2] + G[2][0]);
+    sig[3 * g.Nc + c] += dt * (lam * (G[2][2] + G[0][0]) + L2 * G[1][1]);
+    sig[4 * g.Nc + c] += dt * mu * (G[1][2] + G[2][1]);
+    sig[5 * g.Nc + c] += dt * (lam * (G[1][1] + G[0][0]) + L2 * G[2][2]);
+}
+
+// Host AoS <-> device SoA for arbitrary component counts.
+__global__ void k_aos2soa(const double *__restrict__ a, double *__restrict__ s, long n, int nc) {
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    for (int q = 0; q < nc; q++) s[q * n + p] = a[(long)nc * p + q];
+}
+__global__ void k_soa2aos(const double *__restrict__ s, double *__restrict__ a, long n, int nc) {
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    for (int q = 0; q < nc; q++) a[(long)nc * p + q] = s[q * n + p];
+}
+
+}  // namespace stz

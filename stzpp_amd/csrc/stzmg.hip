@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/stzmg.h"
+#include "mg_internal.hpp"
 #include "mg_kernels.hpp"
 
 using namespace stzmg;
@@ -318,6 +319,11 @@ int run_vcycles(stzmg_t h, int n, bool graph) {
 bool size_supported(int n, int prd) { return prd ? (n % 2 == 0) : (n % 2 == 1); }
 
 }  // namespace
+
+namespace stzmg_internal {
+double *x0(stzmg_t h) { return h->lv[0].x64; }
+double *r0(stzmg_t h) { return h->lv[0].r64; }
+}  // namespace stzmg_internal
 
 // ---------------------------------------------------------------------------
 extern "C" {

@@ -1,0 +1,145 @@
+"""Host-side mirror of the reference ``shear_sim_3d<sym_mat3>`` QS API.
+
+``ShearSim`` keeps the reference's names (set_initial_conditions,
+step_forward_qs, solve_qs, sim_time, n_vcycles, vcycle_time;
+src/shear_sim_3d.hh:35-82) and runs every step on the GPU through the C ABI
+of include/stz.h.  Fields stay in HBM between steps; ``state()`` copies them
+out.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .multigrid import MultigridError, SOLVE_FP32_IR, SOLVE_REF, SOLVE_RELRES
+
+
+def reference_params(Nx=32, Ny=None, Nz=None, **overrides):
+    """Parameters of the reference shear_sim driver after nondimensionalisation
+    (src/shear_sim.cpp:18-86), as a dict; keyword arguments override them."""
+    Ny = Nx if Ny is None else Ny
+    Nz = Nx if Nz is None else Nz
+    L = _lib.load()
+    p = _lib.StzParams()
+    L.stz_params_default(ctypes.byref(p), Nx, Ny, Nz)
+    d = {k: getattr(p, k) for k, _ in _lib.StzParams._fields_ if k != "prd"}
+    d["prd"] = tuple(p.prd)
+    d.update(overrides)
+    return d
+
+
+def config0_params(N=32):
+    """Config 0: 32^3 cells, mu = 1, K = 2.2, s_y = 1, eps0 = 10, chi_inf = 0.15,
+    c0 = 0.4, plates +-0.01; the remaining STZ constants keep the driver's values."""
+    return reference_params(N, N, N, mu=1.0, lamb=2.2 - 2.0 / 3.0, sy=1.0, eps0=10.0,
+                            chi_inf=0.15, c0=0.4, U=0.01)
+
+
+class StepError(MultigridError):
+    pass
+
+
+class ShearSim:
+    def __init__(self, N_x, N_y, N_z, params=None, **kw):
+        self.L = _lib.load()
+        prm = dict(reference_params(N_x, N_y, N_z))
+        if params:
+            prm.update(params)
+        prm.update(kw)
+        p = _lib.StzParams()
+        self.L.stz_params_default(ctypes.byref(p), N_x, N_y, N_z)
+        for k, v in prm.items():
+            if k == "prd":
+                for a in range(3):
+                    p.prd[a] = int(v[a])
+            elif k in ("Nx", "Ny", "Nz"):
+                continue
+            else:
+                setattr(p, k, float(v))
+        self.params = prm
+        self.N = (N_x, N_y, N_z)
+        self.prd = tuple(p.prd)
+        h = ctypes.c_void_p()
+        rc = self.L.stz_create(ctypes.byref(p), ctypes.byref(h))
+        if rc:
+            raise StepError(rc, self.L.stz_last_error(None).decode())
+        self.h = h
+        self.n_vcycles = 0
+        self.vcycle_time = 0.0  # hours, as the reference accumulates it
+        self.last_stats = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.stz_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc):
+        if rc:
+            raise StepError(rc, self.L.stz_last_error(self.h).decode())
+
+    @property
+    def nverts(self):
+        Nx, Ny, Nz = self.N
+        return Nx * Ny * (Nz + (0 if self.prd[2] else 1))
+
+    @property
+    def ncells(self):
+        return self.N[0] * self.N[1] * self.N[2]
+
+    def set_initial_conditions(self, chi, u=None, sig=None, t=0.0):
+        chi = np.ascontiguousarray(chi, dtype=np.float64).reshape(self.ncells)
+        u = np.zeros((self.nverts, 3)) if u is None else np.ascontiguousarray(u, dtype=np.float64)
+        sig = np.zeros((self.ncells, 6)) if sig is None else np.ascontiguousarray(sig, dtype=np.float64)
+        self._chk(self.L.stz_set_state(self.h, t, _lib.dptr(u), _lib.dptr(sig), _lib.dptr(chi)))
+
+    set_state = set_initial_conditions
+
+    def state(self):
+        t = np.zeros(1)
+        u = np.zeros((self.nverts, 3))
+        sig = np.zeros((self.ncells, 6))
+        chi = np.zeros(self.ncells)
+        dpl = np.zeros(self.ncells)
+        self._chk(self.L.stz_get_state(self.h, _lib.dptr(t), _lib.dptr(u), _lib.dptr(sig),
+                                       _lib.dptr(chi), _lib.dptr(dpl)))
+        return dict(t=float(t[0]), u=u, sig=sig, chi=chi, dpl=dpl)
+
+    @property
+    def sim_time(self):
+        return self.state_time()
+
+    def state_time(self):
+        t = np.zeros(1)
+        self._chk(self.L.stz_get_state(self.h, _lib.dptr(t), None, None, None, None))
+        return float(t[0])
+
+    def step_forward_qs(self, dt, nsteps=1, solve=None, rtol=1e-12, inner_cycles=3):
+        """nsteps QS steps.  solve=None runs the reference projection loop;
+        SOLVE_RELRES / SOLVE_FP32_IR solve each projection to rtol."""
+        st = _lib.StzStepStats()
+        if solve is None or solve == SOLVE_REF:
+            opts = None
+        else:
+            o = _lib.StzmgSolveOpts()
+            self.L.stzmg_solve_opts_default(ctypes.byref(o))
+            o.mode, o.rtol, o.inner_cycles, o.max_cycles = solve, rtol, inner_cycles, 10000
+            opts = ctypes.byref(o)
+        self._chk(self.L.stz_step_qs(self.h, dt, nsteps, opts, ctypes.byref(st)))
+        self.n_vcycles += st.vcycles
+        self.vcycle_time += st.solve_ms / 1e3 / 3600.0
+        self.last_stats = {k: getattr(st, k) for k, _ in _lib.StzStepStats._fields_}
+        return self.last_stats
+
+    def solve_qs(self, steps, dt, **kw):
+        """The QS time loop (solve_qs, src/shear_sim_3d.cpp:336-377) without
+        file output: ``steps`` steps of size dt."""
+        return self.step_forward_qs(dt, steps, **kw)
+
+
+__all__ = ["ShearSim", "reference_params", "config0_params", "SOLVE_REF", "SOLVE_RELRES",
+           "SOLVE_FP32_IR"]

@@ -99,6 +99,69 @@ def trans_case():
                         blocks=np.array(outs), mu=1.3, lamb=1.7, h=np.array([0.11, 0.13, 0.17]))
 
 
+def step_case(name):
+    """QS steps of the reference shear_sim_3d<sym_mat3>.
+
+    step_plastic: 16x16x8 cells, config-0 material with eps0 = 0.3 and a
+      pre-stress s12 above yield, 3 steps of dt = 0.01 (adaptive plasticity,
+      chi diffusion and the in-place ad_Dpl ordering all active).
+    step_c0: config 0, 32^3 cells, 10 steps of dt = 0.01 from rest with
+      chi0 = 0.09 + 0.01 GRF(seed 0, l = 3); stored as per-plane sums and two
+      full z planes of every field.
+    """
+    import tempfile
+
+    from stzpp_amd import fields
+
+    # config-0 parameters (see stzpp_amd.shear_sim.config0_params); the
+    # remaining constants are the reference driver's, src/shear_sim.cpp:18-86
+    TZ, kb = 21000.0, 1.3806503e-23
+    E, nu, sy = 101e9, 0.35, 0.85e9
+    mu_r, rho = E / (2 * (1 + nu)), 6125.0
+    ts = 1e-2 / np.sqrt(mu_r / rho)
+    N = (16, 16, 8) if name == "step_plastic" else (32, 32, 32)
+    p = dict(mu=1.0, lamb=2.2 - 2.0 / 3.0, sy=1.0, tau0=1e-13 / ts, c0=0.4, eps0=10.0,
+             delta=8000 / TZ, omega=300 * 26.0 / 17.0 * 1e-30 * sy / (TZ * kb), bath_temp=400 / TZ,
+             chi_inf=0.15, ez=1.0, diff_l=1.5 * 2.0 / N[0], U=0.01)
+    Nx, Ny, Nz = N
+    nc, nv = Nx * Ny * Nz, Nx * Ny * (Nz + 1)
+    chi0 = 0.09 + 0.01 * fields.grf((Nz, Ny, Nx), 3.0, 0).ravel()
+    sig0 = np.zeros((nc, 6))
+    nsteps = 10
+    if name == "step_plastic":
+        p["eps0"] = 0.3
+        sig0[:, 1] = 1.1 + 0.1 * np.cos(np.arange(nc) * 0.37)
+        nsteps = 3
+    u0 = np.zeros((nv, 3))
+    ref = O.RefShear(N, p, tempfile.mkdtemp())
+    dt = 0.01
+    ref.setup(dt)
+    st, fx = ref.stencil()
+    ref.set_state(0.0, u0, sig0, chi0)
+    cyc = []
+    for _ in range(nsteps):
+        before = ref.vcycles
+        ref.step(dt, 1)
+        cyc.append(ref.vcycles - before)
+    s = ref.state()
+    out = dict(N=np.array(N), dt=np.array(dt), nsteps=np.array(nsteps), chi0=chi0, sig0=sig0,
+               vcycles=np.array(cyc), t=np.array(s["t"]), stencil_sym=st, fix_sym=fx,
+               **{f"p_{k}": np.array(v) for k, v in p.items()})
+    if name == "step_plastic":
+        out.update(u=s["u"], sig=s["sig"], chi=s["chi"], dpl=s["dpl"])
+    else:
+        def planes(a, nz):
+            return a.reshape(nz, Nx * Ny, -1)
+        for key, nz in (("u", Nz + 1), ("sig", Nz), ("chi", Nz), ("dpl", Nz)):
+            a = planes(s[key], nz)
+            out[f"{key}_plane_sum"] = a.sum(axis=1)
+            out[f"{key}_plane_sq"] = (a * a).sum(axis=1)
+            out[f"{key}_k8"] = a[8]
+            out[f"{key}_k20"] = a[20]
+    ref.close()
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+
+
 CASES = {
     "shear_16x16x17": (("shear_16x16x17", (16, 16, 17), (1, 1, 0), "shear", 11),),
     "trans_16x16x16": (("trans_16x16x16", (16, 16, 16), (1, 1, 1), "trans", 12),),
@@ -112,10 +175,12 @@ if __name__ == "__main__":
         which = sys.argv[1]
         if which == "trans":
             trans_case()
+        elif which.startswith("step_"):
+            step_case(which)
         else:
             mg_case(*CASES[which][0])
         sys.exit(0)
-    for key in ["trans", *CASES]:
+    for key in ["trans", *CASES, "step_plastic", "step_c0"]:
         subprocess.run([sys.executable, __file__, key], check=True, cwd=ROOT,
                        stdout=subprocess.DEVNULL)
         print("wrote", key)
