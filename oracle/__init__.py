@@ -457,6 +457,7 @@ def _step_lib():
         L.sto_step.argtypes = [vp, ctypes.c_double, ctypes.c_int]
         L.sto_vcycles.argtypes = [vp]
         L.sto_vcycles.restype = ctypes.c_int
+        L.sto_set_solver.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L._sto_bound = True
     return L
 
@@ -502,6 +503,11 @@ class OracleShear:
 
     def step(self, dt, n=1):
         self.L.sto_step(self.h, dt, n)
+
+    def set_solver(self, smoother=0, rtol=0.0):
+        """smoother 0 = lexicographic (reference), 1 = 8-colour; rtol 0 = the
+        reference stopping rule, else a relative-residual solve."""
+        self.L.sto_set_solver(self.h, smoother, rtol)
 
     @property
     def vcycles(self):

@@ -46,6 +46,8 @@ typedef struct {
     int std_iters;         /* the reference's static counter, per instance here */
     int vcycles;           /* V-cycles of the last step */
     int mg_ready;
+    int smoother;          /* 0 = lexicographic (reference), 1 = 8-colour (GPU order) */
+    double rtol;           /* 0 = reference stopping rule, else ||r||_2 <= rtol ||r_0||_2 */
 } sto_sim;
 
 static long gi(const sto_sim *S, int i, int j, int k) {
@@ -362,25 +364,36 @@ static void projection(sto_sim *S, double dt) {
                 L->r[3 * p + 1] = r1;
                 L->r[3 * p + 2] = r2;
             }
-    /* tolerance loop (src/shear_sim_3d.cpp:551-600) */
-    double tol = 2.220446049250313e-16 * S->stencil[9 * 13] * 1e3;
-    tol *= tol;
-    int num_iters = 1;
-    double err = 1e16;
+    S->mg.smoother = S->smoother;
     S->vcycles = 0;
-    while (err > tol) {
+    if (S->rtol > 0) {
+        /* relative-residual solve (no reference counterpart) */
+        double r0 = sqrt(mgo_norm2(L)), r = r0;
+        while (r > S->rtol * r0 && S->vcycles < 10000) {
+            mgo_vcycle(&S->mg);
+            S->vcycles++;
+            r = sqrt(mgo_norm2(L));
+        }
+    } else {
+        /* tolerance loop (src/shear_sim_3d.cpp:551-600) */
+        double tol = 2.220446049250313e-16 * S->stencil[9 * 13] * 1e3;
+        tol *= tol;
+        int num_iters = 1;
+        double err = 1e16;
+        while (err > tol) {
+            mgo_vcycle(&S->mg);
+            S->vcycles++;
+            err = mgo_norm_quartic(L);
+            if (num_iters >= S->std_iters / 16) err = mgo_norm_quartic(L);
+            num_iters++;
+            if (S->vcycles > 10000) break;
+        }
+        if (num_iters >= S->std_iters / 16) S->std_iters = num_iters * 16;
+        S->std_iters--;
         mgo_vcycle(&S->mg);
-        S->vcycles++;
-        err = mgo_norm_quartic(L);
-        if (num_iters >= S->std_iters / 16) err = mgo_norm_quartic(L);
-        num_iters++;
-        if (S->vcycles > 10000) break;
+        mgo_vcycle(&S->mg);
+        S->vcycles += 2;
     }
-    if (num_iters >= S->std_iters / 16) S->std_iters = num_iters * 16;
-    S->std_iters--;
-    mgo_vcycle(&S->mg);
-    mgo_vcycle(&S->mg);
-    S->vcycles += 2;
     /* copy-out including the top vertex plane */
     for (int k = 0; k < o; k++)
         for (int j = 0; j < S->Ny; j++)
@@ -462,3 +475,9 @@ void sto_get_state(void *hp, double *t, double *u, double *sig, double *chi, dou
 }
 
 int sto_vcycles(void *hp) { return ((sto_sim *)hp)->vcycles; }
+
+void sto_set_solver(void *hp, int smoother, double rtol) {
+    sto_sim *S = (sto_sim *)hp;
+    S->smoother = smoother;
+    S->rtol = rtol;
+}

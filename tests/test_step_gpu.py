@@ -1,5 +1,17 @@
-"""Parity of the GPU QS step (C ABI include/stz.h) with the reference's
-golden fields and with the CPU oracle."""
+"""Parity of the GPU QS step (C ABI include/stz.h) with the CPU oracle and
+the reference's golden fields.
+
+The reference stops each projection solve when sum |r|^4 <= (eps a11 1e3)^2
+(src/shear_sim_3d.cpp:551-582 with mg3d_mod_sq(vec3) = |r|^4): a per-point
+residual near (eps a11 1e3)^(1/2), which leaves the reference's own fields
+about 1e-5 (relative) from the converged projection in these cases.  Hence:
+  * GPU vs oracle with the same 8-colour smoother and the same stopping rule:
+    the same iterates, agreement to 1e-10;
+  * GPU vs oracle, both solved to ||r|| <= 1e-13 ||r0||: the converged
+    projection, agreement to 1e-10 (lexicographic oracle, 8-colour GPU);
+  * GPU vs the reference's fields: within the reference's solve error,
+    stated as 1e-4 relative (measured 9e-6 for u and 1.9e-5 for sigma).
+"""
 import numpy as np
 import pytest
 
@@ -7,6 +19,8 @@ import oracle as O
 from stzpp_amd import shear_sim as SS
 
 pytestmark = pytest.mark.gpu
+
+KEYS = ("u", "sig", "chi", "dpl")
 
 
 def relerr(a, b):
@@ -29,43 +43,59 @@ def gpu_run(g, solve=None, rtol=1e-12):
     return sim.state(), cyc
 
 
-def test_gpu_step_plastic_matches_reference(golden):
-    g = golden("step_plastic.npz")
-    s, cyc = gpu_run(g)
-    assert abs(s["t"] - float(g["t"])) < 1e-15
-    # Projection solved to the reference's own tolerance with 8-colour
-    # smoothing: fields agree to the solve tolerance (1e-10 relative).
-    assert relerr(s["u"], g["u"]) < 1e-10
-    assert relerr(s["sig"], g["sig"]) < 1e-10
-    assert relerr(s["chi"], g["chi"]) < 1e-12
-    assert relerr(s["dpl"], g["dpl"]) < 1e-10
-    assert all(c >= 3 for c in cyc)
+def oracle_run(g, smoother, rtol):
+    N = tuple(int(v) for v in g["N"])
+    orc = O.OracleShear(N, params_of(g))
+    orc.set_solver(smoother, rtol)
+    nv = N[0] * N[1] * (N[2] + 1)
+    orc.set_state(0.0, np.zeros((nv, 3)), g["sig0"], g["chi0"])
+    cyc = []
+    for _ in range(int(g["nsteps"])):
+        orc.step(float(g["dt"]), 1)
+        cyc.append(orc.vcycles)
+    return orc.state(), cyc
 
 
-def test_gpu_step_config0_matches_reference(golden):
-    g = golden("step_c0.npz")
-    s, cyc = gpu_run(g)
-    Nx, Ny, Nz = (int(v) for v in g["N"])
-    for key, nz in (("u", Nz + 1), ("sig", Nz), ("chi", Nz)):
-        a = s[key].reshape(nz, Nx * Ny, -1)
-        assert relerr(a[8], g[f"{key}_k8"]) < 1e-9, key
-        assert relerr(a[20], g[f"{key}_k20"]) < 1e-9, key
-        assert relerr((a * a).sum(axis=1), g[f"{key}_plane_sq"]) < 1e-9, key
+@pytest.mark.parametrize("case", ["step_plastic.npz", "step_c0.npz"])
+def test_gpu_step_matches_oracle_same_smoother(golden, case):
+    g = golden(case)
+    a, ca = gpu_run(g)
+    b, cb = oracle_run(g, smoother=1, rtol=0.0)
+    assert ca == cb
+    assert abs(a["t"] - b["t"]) < 1e-15
+    for key in KEYS:
+        assert relerr(a[key], b[key]) < 1e-10, key
 
 
-def test_gpu_step_fp32_ir_projection_within_tolerance(golden):
-    # FP32 smoothing inside FP64 refinement, each projection to 1e-12
-    # relative: fields within 1e-9 of the reference after 10 steps.
-    g = golden("step_c0.npz")
-    s, _ = gpu_run(g, solve=SS.SOLVE_FP32_IR, rtol=1e-12)
-    Nx, Ny, Nz = (int(v) for v in g["N"])
-    for key, nz in (("u", Nz + 1), ("sig", Nz), ("chi", Nz)):
-        a = s[key].reshape(nz, Nx * Ny, -1)
-        assert relerr(a[8], g[f"{key}_k8"]) < 1e-9, key
+@pytest.mark.parametrize("case", ["step_plastic.npz", "step_c0.npz"])
+@pytest.mark.parametrize("solve", [SS.SOLVE_RELRES, SS.SOLVE_FP32_IR])
+def test_gpu_step_converged_matches_oracle(golden, case, solve):
+    g = golden(case)
+    a, _ = gpu_run(g, solve=solve, rtol=1e-13)
+    b, _ = oracle_run(g, smoother=0, rtol=1e-13)
+    for key in KEYS:
+        assert relerr(a[key], b[key]) < 1e-10, key
+
+
+@pytest.mark.parametrize("case", ["step_plastic.npz", "step_c0.npz"])
+def test_gpu_step_within_reference_solve_error(golden, case):
+    g = golden(case)
+    a, _ = gpu_run(g)
+    if case == "step_plastic.npz":
+        assert np.abs(g["dpl"]).max() > 1e-3
+        for key in KEYS:
+            assert relerr(a[key], g[key]) < 1e-4, key
+    else:
+        Nx, Ny, Nz = (int(v) for v in g["N"])
+        for key, nz in (("u", Nz + 1), ("sig", Nz), ("chi", Nz)):
+            p = a[key].reshape(nz, Nx * Ny, -1)
+            assert relerr(p[8], g[f"{key}_k8"]) < 1e-4, key
+            assert relerr(p[20], g[f"{key}_k20"]) < 1e-4, key
+            assert relerr((p * p).sum(axis=1), g[f"{key}_plane_sq"]) < 1e-4, key
 
 
 def test_gpu_step_matches_oracle_with_plasticity_and_motion():
-    # Nonzero initial velocity, yielding band, ramped plates.
+    # Nonzero initial velocity, a yielding band, plates on the ramp at t=0.3.
     N = (16, 16, 8)
     p = SS.config0_params(16)
     p["eps0"] = 0.3
@@ -78,13 +108,15 @@ def test_gpu_step_matches_oracle_with_plasticity_and_motion():
     sig[:, 0] = 0.05 * rng.standard_normal(nc)
     chi = 0.09 + 0.005 * rng.standard_normal(nc)
     orc = O.OracleShear(N, {k_: p[k_] for k_ in O.STEP_PARAM_ORDER})
+    orc.set_solver(1, 0.0)
     orc.set_state(0.3, u, sig, chi)
     sim = SS.ShearSim(*N, params=p)
     sim.set_initial_conditions(chi, u=u, sig=sig, t=0.3)
     for _ in range(3):
         orc.step(0.01, 1)
-        sim.step_forward_qs(0.01, 1)
+        st = sim.step_forward_qs(0.01, 1)
     a, b = sim.state(), orc.state()
+    assert st["yielded_cells"] > 0
     assert np.abs(b["dpl"]).max() > 1e-4
-    for key in ("u", "sig", "chi", "dpl"):
-        assert relerr(a[key], b[key]) < 1e-9, key
+    for key in KEYS:
+        assert relerr(a[key], b[key]) < 1e-10, key
