@@ -144,6 +144,18 @@ int stzmg_vcycle_fp32(stzmg_t h, int n);
 int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *stats);
 void stzmg_solve_opts_default(stzmg_solve_opts *o);
 
+/* Execution options (no reference counterpart; results are unchanged):
+ *   STZMG_OPT_COARSE_KERNEL  levels of at most `value` points run as one
+ *                            single-workgroup kernel per V-cycle; 1 selects
+ *                            the built-in threshold, 0 launches every pass
+ *                            separately. */
+enum { STZMG_OPT_COARSE_KERNEL = 0, STZMG_OPT_FUSED_SMOOTHER = 1 };
+/*   STZMG_OPT_FUSED_SMOOTHER 1 (default): per-plane levels with periodic x, y
+ *                            (sizes multiples of 64 x 16) sweep with two fused
+ *                            launches (one per z parity); 0: eight colour
+ *                            launches per sweep. */
+int stzmg_set_option(stzmg_t h, int option, int value);
+
 /* The HIP stream (hipStream_t) the handle launches on. */
 void *stzmg_stream(stzmg_t h);
 /* Per-kernel timing of the last n launches of one kernel class, for roofline

@@ -96,6 +96,7 @@ SIGNATURES = {
     "stzmg_solve": ([_vp, ctypes.POINTER(StzmgSolveOpts), ctypes.POINTER(StzmgSolveStats)], ctypes.c_int),
     "stzmg_solve_opts_default": ([ctypes.POINTER(StzmgSolveOpts)], None),
     "stzmg_stream": ([_vp], _vp),
+    "stzmg_set_option": ([_vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "stzmg_time_kernel": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
     "stzmg_shear_stencil": ([ctypes.c_double] * 7 + [_dp, _dp], ctypes.c_int),
     "stzmg_trans_stencil": ([_dp] + [ctypes.c_double] * 6 + [_dp], ctypes.c_int),

@@ -1,0 +1,161 @@
+// Coarse end of the V-cycle in one workgroup.
+//
+// Levels of a few thousand to ~33k points give a colour pass too little work
+// to fill the GPU, and the reference schedule spends many sweeps there (24 on
+// the coarsest level, v+l on level l).  Launched kernel by kernel, each of
+// those passes costs a full dispatch.  k_coarse_cycle runs the whole
+// sub-cycle below a given level -- smoothing, residual, restriction, the
+// bottom solve, interpolation -- inside one 1024-thread workgroup, with
+// __syncthreads() between dependent passes.  The operations and their order
+// are those of the multi-launch path, so the results are the same.
+#pragma once
+
+#include "mg_kernels.hpp"
+
+namespace stzmg {
+
+constexpr int kCoarseThreads = 1024;
+constexpr int kMaxCoarse = 8;
+// Levels with at most this many points go to the single-workgroup kernel.
+// Measured at 256^3 and 128^3 (V-cycle time, MI355X): one workgroup wins on
+// the coarsest level (<= 1024 points by construction), breaks even at 16^3
+// and loses at 32^3, where one CU's memory latency dominates.
+constexpr long kCoarseMaxPoints = 1024;
+
+template <typename T>
+struct CoarseLevel {
+    Dims d;
+    T *x, *r, *t;
+    const T *ops, *inv;
+    double *x64, *r64;          // FP64 fields (bottom solve of an FP32 cycle)
+    const double *ops64, *inv64;
+    int down, up;               // sweeps before restriction / after interpolation
+};
+
+template <typename T>
+struct CoarseArgs {
+    int nl;                     // levels handled (relative 0 .. nl-1)
+    int bottom;                 // sweeps on the last one
+    CoarseLevel<T> L[kMaxCoarse];
+};
+
+// Per-plane operator tables (and centre inverses) of a level are staged in
+// LDS: points of different planes share a wavefront here, so the table
+// cannot come through the scalar cache as in the one-plane-per-block kernels.
+// Per-point tables stay in global memory.
+template <typename T, int MODE>
+__device__ void stage_tables(const Dims &d, const T *ops, const T *inv, T *lds, const T **ops_s,
+                             const T **inv_s) {
+    if (MODE != 0) {
+        *ops_s = ops;
+        *inv_s = inv;
+        return;
+    }
+    const int na = d.o * kBlk, ni = d.o * 9;
+    for (int q = threadIdx.x; q < na; q += blockDim.x) lds[q] = ops[q];
+    for (int q = threadIdx.x; q < ni; q += blockDim.x) lds[na + q] = inv[q];
+    __syncthreads();
+    *ops_s = lds;
+    *inv_s = lds + na;
+}
+
+template <typename T, int MODE>
+__device__ void wg_gs_sweep(const Dims &d, T *x, const T *r, const T *ops, const T *inv) {
+    const int pxo = d.px && (d.m & 1), pyo = d.py && (d.n & 1), pzo = d.pz && (d.o & 1);
+    for (int cz = 0; cz < 2 + pzo; cz++)
+        for (int cy = 0; cy < 2 + pyo; cy++)
+            for (int cx = 0; cx < 2 + pxo; cx++) {
+                const int cntx = col_count(d.m, cx, d.px), cnty = col_count(d.n, cy, d.py),
+                          cntz = col_count(d.o, cz, d.pz);
+                const int tot = cntx * cnty * cntz;
+                for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+                    const int ii = t % cntx, rest = t / cntx;
+                    const int jj = rest % cnty, kk = rest / cnty;
+                    gs_point<T, MODE>(d, x, r, ops, inv, col_coord(ii, cx, d.m), col_coord(jj, cy, d.n),
+                                      col_coord(kk, cz, d.o));
+                }
+                __syncthreads();
+            }
+}
+
+template <typename T, int MODE>
+__device__ void wg_resid(const Dims &d, const T *x, const T *r, const T *ops, T *out) {
+    const int mn = d.m * d.n;
+    for (int p = threadIdx.x; p < (int)d.N; p += blockDim.x) {
+        const int k = p / mn, rem = p - k * mn, j = rem / d.m, i = rem - j * d.m;
+        const T *A = ops + cls_of<MODE>(p, k) * kBlk;
+        T y0 = 0, y1 = 0, y2 = 0;
+        stencil_sum<T, false>(d, x, A, i, j, k, y0, y1, y2);
+        out[p] = r[p] - y0;
+        out[p + d.N] = r[p + d.N] - y1;
+        out[p + 2 * d.N] = r[p + 2 * d.N] - y2;
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ void wg_restrict(const Dims &F, const Dims &C, const T *res, T *rc, T *xc) {
+    const int mn = C.m * C.n;
+    for (int p = threadIdx.x; p < (int)C.N; p += blockDim.x) {
+        const int K = p / mn, rem = p - K * mn, J = rem / C.m, I = rem - J * C.m;
+        restrict_point<T>(F, C, res, rc, xc, I, J, K);
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ void wg_interp(const Dims &F, const Dims &C, const T *xc, T *xf) {
+    const int mn = F.m * F.n;
+    for (int p = threadIdx.x; p < (int)F.N; p += blockDim.x) {
+        const int k = p / mn, rem = p - k * mn, j = rem / F.m, i = rem - j * F.m;
+        interp_point<T>(F, C, xc, xf, i, j, k);
+    }
+    __syncthreads();
+}
+
+template <typename A, typename B>
+__device__ void wg_convert(const A *a, B *b, long n) {
+    for (long q = threadIdx.x; q < n; q += blockDim.x) b[q] = (B)a[q];
+    __syncthreads();
+}
+
+// Dynamic LDS: the largest staged table, (243 + 9) * o_max values of the
+// widest type used (double), given at launch.
+template <typename T, int MODE>
+__global__ __launch_bounds__(kCoarseThreads) void k_coarse_cycle(CoarseArgs<T> a) {
+    extern __shared__ double coarse_lds[];
+    T *lds = reinterpret_cast<T *>(coarse_lds);
+    const T *ops_s, *inv_s;
+    for (int i = 0; i < a.nl - 1; i++) {
+        CoarseLevel<T> &L = a.L[i];
+        stage_tables<T, MODE>(L.d, L.ops, L.inv, lds, &ops_s, &inv_s);
+        for (int q = 0; q < L.down; q++) wg_gs_sweep<T, MODE>(L.d, L.x, L.r, ops_s, inv_s);
+        wg_resid<T, MODE>(L.d, L.x, L.r, ops_s, L.t);
+        wg_restrict<T>(L.d, a.L[i + 1].d, L.t, a.L[i + 1].r, a.L[i + 1].x);
+    }
+    CoarseLevel<T> &B = a.L[a.nl - 1];
+    if (sizeof(T) == 4 && B.x64 != nullptr) {
+        // The coarsest solve of an FP32 cycle runs in FP64.
+        const long n3 = 3 * B.d.N;
+        wg_convert<T, double>(B.r, B.r64, n3);
+        for (long q = threadIdx.x; q < n3; q += blockDim.x) B.x64[q] = 0.0;
+        const double *ops_d, *inv_d;
+        stage_tables<double, MODE>(B.d, B.ops64, B.inv64, coarse_lds, &ops_d, &inv_d);
+        for (int q = 0; q < a.bottom; q++) wg_gs_sweep<double, MODE>(B.d, B.x64, B.r64, ops_d, inv_d);
+        wg_convert<double, T>(B.x64, B.x, n3);
+    } else {
+        stage_tables<T, MODE>(B.d, B.ops, B.inv, lds, &ops_s, &inv_s);
+        for (int q = 0; q < a.bottom; q++) wg_gs_sweep<T, MODE>(B.d, B.x, B.r, ops_s, inv_s);
+    }
+    for (int i = a.nl - 1; i > 0; i--) {
+        CoarseLevel<T> &F = a.L[i - 1];
+        wg_interp<T>(F.d, a.L[i].d, a.L[i].x, F.x);
+        stage_tables<T, MODE>(F.d, F.ops, F.inv, lds, &ops_s, &inv_s);
+        for (int q = 0; q < F.up; q++) wg_gs_sweep<T, MODE>(F.d, F.x, F.r, ops_s, inv_s);
+    }
+}
+
+// LDS bytes k_coarse_cycle needs for levels with at most omax planes.
+inline size_t coarse_lds_bytes(int omax) { return sizeof(double) * (size_t)(kBlk + 9) * omax; }
+
+}  // namespace stzmg

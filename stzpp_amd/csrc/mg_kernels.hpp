@@ -150,21 +150,12 @@ __host__ __device__ __forceinline__ int col_count(int n, int c, int prd) {
 
 __device__ __forceinline__ int col_coord(int t, int c, int n) { return c == 2 ? n - 1 : c + 2 * t; }
 
-// One colour pass of block Gauss-Seidel: for every point of colour (cx,cy,cz)
+// Block Gauss-Seidel update of one point:
 //   x_p = inv(A_pp) (r_p - sum_{b != centre} A_b x_{p+b})
-// Points of one colour are never stencil neighbours, so the pass is
-// order-independent.  Grid: (ceil(cntx*cnty/256), cntz).
 template <typename T, int MODE>
-__global__ __launch_bounds__(kThreads) void k_gs(Dims d, T *__restrict__ x,
-                                                 const T *__restrict__ r,
-                                                 const T *__restrict__ ops,
-                                                 const T *__restrict__ inv, int cx, int cy,
-                                                 int cz, int cntx, int cnty) {
-    const int k = col_coord(blockIdx.y, cz, d.o);
-    const long t = (long)blockIdx.x * kThreads + threadIdx.x;
-    if (t >= (long)cntx * cnty) return;
-    const int jj = (int)(t / cntx), ii = (int)(t - (long)jj * cntx);
-    const int i = col_coord(ii, cx, d.m), j = col_coord(jj, cy, d.n);
+__device__ __forceinline__ void gs_point(const Dims &d, T *__restrict__ x, const T *__restrict__ r,
+                                         const T *__restrict__ ops, const T *__restrict__ inv, int i, int j,
+                                         int k) {
     const long p = i + (long)d.m * (j + (long)d.n * k);
     const long c = cls_of<MODE>(p, k);
     const T *A = ops + c * kBlk;
@@ -177,6 +168,22 @@ __global__ __launch_bounds__(kThreads) void k_gs(Dims d, T *__restrict__ x,
     x[p + 2 * d.N] = B[6] * e0 + B[7] * e1 + B[8] * e2;
 }
 
+// One colour pass of block Gauss-Seidel over the points of colour
+// (cx,cy,cz).  Points of one colour are never stencil neighbours, so the pass
+// is order-independent.  Grid: (ceil(cntx*cnty/256), cntz).
+template <typename T, int MODE>
+__global__ __launch_bounds__(kThreads) void k_gs(Dims d, T *__restrict__ x,
+                                                 const T *__restrict__ r,
+                                                 const T *__restrict__ ops,
+                                                 const T *__restrict__ inv, int cx, int cy,
+                                                 int cz, int cntx, int cnty) {
+    const int k = col_coord(blockIdx.y, cz, d.o);
+    const long t = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (t >= (long)cntx * cnty) return;
+    const int jj = (int)(t / cntx), ii = (int)(t - (long)jj * cntx);
+    gs_point<T, MODE>(d, x, r, ops, inv, col_coord(ii, cx, d.m), col_coord(jj, cy, d.n), k);
+}
+
 // 1-D restriction weight of fine vertex 2I+e for coarse vertex I (vertex
 // coarsening, periodic-even or non-periodic-odd sizes): 1 on the coarse
 // vertex, 1/2 on the two midpoints; -1 marks an index outside the domain.
@@ -187,16 +194,11 @@ __device__ __forceinline__ int fine_of(int I, int e, int nf, int prd) {
     return f;
 }
 
-// r_c = (1/8) P^T res_f and x_c = 0 (region::restriction, region.cc:1767-1812
-// and multigrid.cc:444).  Grid: (ceil(mc*nc/256), oc).
+// r_c = (1/8) P^T res_f at coarse vertex (I,J,K) and x_c = 0
+// (region::restriction, region.cc:1767-1812 and multigrid.cc:444).
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_restrict(Dims F, Dims C, const T *__restrict__ res,
-                                                       T *__restrict__ rc, T *__restrict__ xc) {
-    const int K = blockIdx.y;
-    const long mn = (long)C.m * C.n;
-    const long pl = (long)blockIdx.x * kThreads + threadIdx.x;
-    if (pl >= mn) return;
-    const int J = (int)(pl / C.m), I = (int)(pl - (long)J * C.m);
+__device__ __forceinline__ void restrict_point(const Dims &F, const Dims &C, const T *__restrict__ res,
+                                               T *__restrict__ rc, T *__restrict__ xc, int I, int J, int K) {
     T s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
     for (int ek = -1; ek <= 1; ek++) {
@@ -220,7 +222,7 @@ __global__ __launch_bounds__(kThreads) void k_restrict(Dims F, Dims C, const T *
             }
         }
     }
-    const long p = pl + mn * K;
+    const long p = I + (long)C.m * (J + (long)C.n * K);
     rc[p] = s0;
     rc[p + C.N] = s1;
     rc[p + 2 * C.N] = s2;
@@ -229,34 +231,42 @@ __global__ __launch_bounds__(kThreads) void k_restrict(Dims F, Dims C, const T *
     xc[p + 2 * C.N] = 0;
 }
 
-// x_f += P x_c, trilinear vertex interpolation (region::interpolation,
-// region.cc:1817-1843, :2062-2121).  Grid: (ceil(mf*nf/256), of).
+// Grid: (ceil(mc*nc/256), oc).
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_interp(Dims F, Dims C, const T *__restrict__ xc,
-                                                     T *__restrict__ xf) {
-    const int k = blockIdx.y;
-    const long mn = (long)F.m * F.n;
+__global__ __launch_bounds__(kThreads) void k_restrict(Dims F, Dims C, const T *__restrict__ res,
+                                                       T *__restrict__ rc, T *__restrict__ xc) {
     const long pl = (long)blockIdx.x * kThreads + threadIdx.x;
-    if (pl >= mn) return;
-    const int j = (int)(pl / F.m), i = (int)(pl - (long)j * F.m);
+    if (pl >= (long)C.m * C.n) return;
+    const int J = (int)(pl / C.m), I = (int)(pl - (long)J * C.m);
+    restrict_point<T>(F, C, res, rc, xc, I, J, blockIdx.y);
+}
+
+// Coarse parents of fine vertex f along one dimension: f even sits on coarse
+// f/2; f odd is the midpoint of f/2 and f/2+1 (wrapped when periodic).
+template <typename T>
+__device__ __forceinline__ int parents1(int f, int nc, int prd, int *P, T *w) {
+    P[0] = f >> 1;
+    if (!(f & 1)) {
+        w[0] = T(1);
+        return 1;
+    }
+    int h = (f >> 1) + 1;
+    if (prd && h >= nc) h -= nc;
+    P[1] = h;
+    w[0] = w[1] = T(0.5);
+    return 2;
+}
+
+// x_f += P x_c at fine vertex (i,j,k), trilinear vertex interpolation
+// (region::interpolation, region.cc:1817-1843, :2062-2121).
+template <typename T>
+__device__ __forceinline__ void interp_point(const Dims &F, const Dims &C, const T *__restrict__ xc,
+                                             T *__restrict__ xf, int i, int j, int k) {
     int Ki[2], Ji[2], Ii[2];
     T wk[2], wj[2], wi[2];
-    int nk, nj, ni;
-    auto par = [](int f, int nc, int prd, int *P, T *w) -> int {
-        P[0] = f >> 1;
-        if (!(f & 1)) {
-            w[0] = T(1);
-            return 1;
-        }
-        int h = (f >> 1) + 1;
-        if (prd && h >= nc) h -= nc;
-        P[1] = h;
-        w[0] = w[1] = T(0.5);
-        return 2;
-    };
-    nk = par(k, C.o, C.pz, Ki, wk);
-    nj = par(j, C.n, C.py, Ji, wj);
-    ni = par(i, C.m, C.px, Ii, wi);
+    const int nk = parents1<T>(k, C.o, C.pz, Ki, wk);
+    const int nj = parents1<T>(j, C.n, C.py, Ji, wj);
+    const int ni = parents1<T>(i, C.m, C.px, Ii, wi);
     T s0 = 0, s1 = 0, s2 = 0;
     for (int c = 0; c < nk; c++)
         for (int b = 0; b < nj; b++)
@@ -267,10 +277,20 @@ __global__ __launch_bounds__(kThreads) void k_interp(Dims F, Dims C, const T *__
                 s1 += w * xc[q + C.N];
                 s2 += w * xc[q + 2 * C.N];
             }
-    const long p = pl + mn * k;
+    const long p = i + (long)F.m * (j + (long)F.n * k);
     xf[p] += s0;
     xf[p + F.N] += s1;
     xf[p + 2 * F.N] += s2;
+}
+
+// Grid: (ceil(mf*nf/256), of).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_interp(Dims F, Dims C, const T *__restrict__ xc,
+                                                     T *__restrict__ xf) {
+    const long pl = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (pl >= (long)F.m * F.n) return;
+    const int j = (int)(pl / F.m), i = (int)(pl - (long)j * F.m);
+    interp_point<T>(F, C, xc, xf, i, j, blockIdx.y);
 }
 
 // Galerkin coarse operator A_c = (1/8) P^T A_f P (region::compute_rats,

@@ -15,7 +15,9 @@
 #include <vector>
 
 #include "../../include/stzmg.h"
+#include "mg_coarse.hpp"
 #include "mg_internal.hpp"
+#include "mg_fused.hpp"
 #include "mg_kernels.hpp"
 
 using namespace stzmg;
@@ -30,6 +32,12 @@ struct Level {
     float *ops32 = nullptr, *inv32 = nullptr;
     double *x64 = nullptr, *r64 = nullptr, *t64 = nullptr;
     float *x32 = nullptr, *r32 = nullptr, *t32 = nullptr;
+    // Second solution buffers of the fused smoother (mg_fused.hpp); x64/x32
+    // and these swap after every fused sweep.
+    double *x64b = nullptr;
+    float *x32b = nullptr;
+    bool fused = false;   // fused two-launch sweeps on this level
+    int pat = 0;          // operator zero pattern (PAT_*)
 };
 
 inline unsigned nblk(long n, int t = kThreads) { return (unsigned)((n + t - 1) / t); }
@@ -47,6 +55,8 @@ struct stzmg_s {
     int *dflag = nullptr;     // singular-block flag
     bool ops_set = false;
     int std_iters = 16;       // per-handle copy of the reference's static heuristic
+    long coarse_max = kCoarseMaxPoints;  // levels this small run in one workgroup (0: off)
+    int fused_smoother = 1;   // two-launch fused sweeps on eligible levels
     hipGraphExec_t g64 = nullptr, g32 = nullptr;
 };
 
@@ -82,6 +92,7 @@ void free_level(Level &L) {
     hipFree(L.ops64); hipFree(L.inv64); hipFree(L.ops32); hipFree(L.inv32);
     hipFree(L.x64); hipFree(L.r64); hipFree(L.t64);
     hipFree(L.x32); hipFree(L.r32); hipFree(L.t32);
+    hipFree(L.x64b); hipFree(L.x32b);
     L = Level{};
 }
 
@@ -148,8 +159,91 @@ void launch_apply(stzmg_t h, Level &L, const T *x, T *out) {
 
 int ncol(int n, int prd) { return (prd && (n & 1)) ? 3 : 2; }
 
+// Tile of the fused smoother: 64 x 16 in FP32 and 32 x 16 in FP64 keep the
+// three staged planes (3 comps, 4-point halo) within 70 KB of LDS.
+template <typename T>
+struct FusedTile;
+template <>
+struct FusedTile<float> {
+    static constexpr int TX = 64, TY = 16;
+};
+template <>
+struct FusedTile<double> {
+    static constexpr int TX = 32, TY = 16;
+};
+
+template <typename T>
+T *&xb_of(Level &L);
+template <>
+double *&xb_of<double>(Level &L) { return L.x64b; }
+template <>
+float *&xb_of<float>(Level &L) { return L.x32b; }
+template <typename T>
+T *&xref_of(Level &L);
+template <>
+double *&xref_of<double>(Level &L) { return L.x64; }
+template <>
+float *&xref_of<float>(Level &L) { return L.x32; }
+
+bool fused_ok(const Level &L) {
+    return L.mode == STZMG_OP_PLANE && L.d.px && L.d.py && L.d.o >= 4 && L.d.N > 1024 &&
+           L.d.m % FusedTile<float>::TX == 0 && L.d.n % FusedTile<float>::TY == 0 &&
+           L.d.m % FusedTile<double>::TX == 0 && L.d.n % FusedTile<double>::TY == 0;
+}
+
+template <typename T, int PAT>
+void launch_fused_pat(stzmg_t h, Level &L) {
+    constexpr int TX = FusedTile<T>::TX, TY = FusedTile<T>::TY;
+    const Dims &d = L.d;
+    T *x = xref_of<T>(L), *xb = xb_of<T>(L);
+    for (int P = 0; P < 2; P++) {
+        const int nP = (d.o - P + 1) / 2;
+        // planes per workgroup: enough workgroups to fill the chip
+        const long tiles = (long)(d.m / TX) * (d.n / TY);
+        int kchunk = 1;
+        while (kchunk < 8 && tiles * ((nP + 2 * kchunk - 1) / (2 * kchunk)) >= 2048) kchunk *= 2;
+        dim3 g(d.m / TX, d.n / TY, (nP + kchunk - 1) / kchunk);
+        hipLaunchKernelGGL((k_gs_planes<T, PAT, TX, TY>), g, dim3(kFusedThreads), 0, h->stream, d, (const T *)x,
+                           (const T *)(P == 0 ? x : xb), xb, (const T *)r_of<T>(L), ops_of<T>(L), inv_of<T>(L),
+                           P, kchunk);
+    }
+    std::swap(xref_of<T>(L), xb_of<T>(L));
+}
+
+template <typename T>
+void launch_gs_sweep_colours(stzmg_t h, Level &L);
+
+// One Gauss-Seidel sweep of a level: two fused launches where the level
+// allows, eight colour launches otherwise.
 template <typename T>
 void launch_gs_sweep(stzmg_t h, Level &L) {
+    if (!L.fused) {
+        launch_gs_sweep_colours<T>(h, L);
+        return;
+    }
+    switch (L.pat) {
+        case PAT_SHEAR: launch_fused_pat<T, PAT_SHEAR>(h, L); break;
+        case PAT_TRANS_XZ: launch_fused_pat<T, PAT_TRANS_XZ>(h, L); break;
+        case PAT_NOCORNER: launch_fused_pat<T, PAT_NOCORNER>(h, L); break;
+        default: launch_fused_pat<T, PAT_DENSE>(h, L); break;
+    }
+}
+
+// Put a level's solution back into the buffer it started in (a level swept
+// an odd number of times by the fused smoother ends in the other one).  A
+// captured V-cycle must leave every pointer as it found it.
+template <typename T>
+void restore_level(stzmg_t h, Level &L, T *home) {
+    if (xref_of<T>(L) == home) return;
+    const long n3 = 3 * L.d.N;
+    hipLaunchKernelGGL((k_convert<T, T>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream, (const T *)xref_of<T>(L),
+                       home, n3);
+    xb_of<T>(L) = xref_of<T>(L);
+    xref_of<T>(L) = home;
+}
+
+template <typename T>
+void launch_gs_sweep_colours(stzmg_t h, Level &L) {
     const Dims &d = L.d;
     const int ncx = ncol(d.m, d.px), ncy = ncol(d.n, d.py), ncz = ncol(d.o, d.pz);
     for (int cz = 0; cz < ncz; cz++)
@@ -191,31 +285,89 @@ void launch_interp(stzmg_t h, int l) {
 int sweeps_down(stzmg_t h, int l) { return h->desc.v_down + h->desc.sweep_growth * l; }
 int sweeps_up(stzmg_t h, int l) { return h->desc.v_up + h->desc.sweep_growth * l; }
 
+// First level handled by the single-workgroup coarse kernel (lv.size() when
+// that kernel is switched off).
+int first_coarse_level(stzmg_t h) {
+    const int L = (int)h->lv.size();
+    if (h->coarse_max <= 0) return L;
+    int l = 0;
+    while (l < L && h->lv[l].d.N > h->coarse_max) l++;
+    if (L - l > kMaxCoarse) l = L - kMaxCoarse;
+    // the staged per-plane tables must fit in LDS
+    while (l < L && h->lv[l].mode == STZMG_OP_PLANE && coarse_lds_bytes(h->lv[l].d.o) > 64 * 1024) l++;
+    return l;
+}
+
+template <typename T>
+void launch_coarse(stzmg_t h, int first) {
+    const int L = (int)h->lv.size();
+    CoarseArgs<T> a{};
+    a.nl = L - first;
+    a.bottom = h->desc.bottom_sweeps;
+    for (int i = 0; i < a.nl; i++) {
+        Level &V = h->lv[first + i];
+        CoarseLevel<T> &c = a.L[i];
+        c.d = V.d;
+        c.x = x_of<T>(V);
+        c.r = r_of<T>(V);
+        c.t = t_of<T>(V);
+        c.ops = ops_of<T>(V);
+        c.inv = inv_of<T>(V);
+        c.down = sweeps_down(h, first + i);
+        c.up = sweeps_up(h, first + i);
+        const bool fp64_bottom = sizeof(T) == 4 && L > 1 && i == a.nl - 1;
+        c.x64 = fp64_bottom ? V.x64 : nullptr;
+        c.r64 = fp64_bottom ? V.r64 : nullptr;
+        c.ops64 = V.ops64;
+        c.inv64 = V.inv64;
+    }
+    if (h->lv[first].mode == STZMG_OP_PLANE) {
+        int omax = 0;
+        for (int i = 0; i < a.nl; i++) omax = a.L[i].d.o > omax ? a.L[i].d.o : omax;
+        hipLaunchKernelGGL((k_coarse_cycle<T, 0>), dim3(1), dim3(kCoarseThreads), coarse_lds_bytes(omax),
+                           h->stream, a);
+    } else {
+        hipLaunchKernelGGL((k_coarse_cycle<T, 1>), dim3(1), dim3(kCoarseThreads), 0, h->stream, a);
+    }
+}
+
 // One V-cycle (multigrid::v_cycle, multigrid.cc:253-297) in type T.  In FP32
 // the coarsest level is solved in FP64: r32 -> r64, bottom sweeps, x64 -> x32.
+// Levels from first_coarse_level() down run in one workgroup (mg_coarse.hpp).
 template <typename T>
 void enqueue_vcycle(stzmg_t h) {
     const int L = (int)h->lv.size();
-    for (int i = 0; i < L - 1; i++) {
+    const int Lc = first_coarse_level(h);
+    const int top = Lc < L - 1 ? Lc : L - 1;
+    T *home[16];
+    for (int i = 0; i < L; i++) home[i] = xref_of<T>(h->lv[i]);
+    double *home_b64 = h->lv[L - 1].x64;
+    for (int i = 0; i < top; i++) {
         for (int q = 0; q < sweeps_down(h, i); q++) launch_gs_sweep<T>(h, h->lv[i]);
         launch_restrict<T>(h, i + 1);
     }
-    Level &B = h->lv[L - 1];
-    const long n3 = 3 * B.d.N;
-    if (sizeof(T) == 4 && L > 1) {
-        hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream,
-                           (const float *)B.r32, B.r64, n3);
-        hipLaunchKernelGGL((k_fill<double>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream, B.x64, 0.0, n3);
-        for (int q = 0; q < h->desc.bottom_sweeps; q++) launch_gs_sweep<double>(h, B);
-        hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream,
-                           (const double *)B.x64, B.x32, n3);
+    if (Lc < L) {
+        launch_coarse<T>(h, Lc);
     } else {
-        for (int q = 0; q < h->desc.bottom_sweeps; q++) launch_gs_sweep<T>(h, B);
+        Level &B = h->lv[L - 1];
+        const long n3 = 3 * B.d.N;
+        if (sizeof(T) == 4 && L > 1) {
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream,
+                               (const float *)B.r32, B.r64, n3);
+            hipLaunchKernelGGL((k_fill<double>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream, B.x64, 0.0, n3);
+            for (int q = 0; q < h->desc.bottom_sweeps; q++) launch_gs_sweep<double>(h, B);
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream,
+                               (const double *)B.x64, B.x32, n3);
+        } else {
+            for (int q = 0; q < h->desc.bottom_sweeps; q++) launch_gs_sweep<T>(h, B);
+        }
     }
-    for (int i = L - 1; i > 0; i--) {
+    for (int i = top; i > 0; i--) {
         launch_interp<T>(h, i);
         for (int q = 0; q < sweeps_up(h, i - 1); q++) launch_gs_sweep<T>(h, h->lv[i - 1]);
     }
+    for (int i = 0; i < L; i++) restore_level<T>(h, h->lv[i], home[i]);
+    if (sizeof(T) == 4) restore_level<double>(h, h->lv[L - 1], home_b64);
 }
 
 int ensure_part(stzmg_t h, long nb) {
@@ -281,6 +433,31 @@ int build_hierarchy_ops(stzmg_t h) {
     int flag = 0;
     HIPCHK(h, hipMemcpyAsync(&flag, h->dflag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    // Zero pattern of every per-plane level: the tightest compiled pattern
+    // that holds all nonzero entries selects the fused smoother instance.
+    for (auto &L : h->lv) {
+        L.fused = false;
+        if (!h->fused_smoother || !fused_ok(L) || !L.x64b) continue;
+        std::vector<double> tab((size_t)L.ncls * kBlk);
+        HIPCHK(h, hipMemcpy(tab.data(), L.ops64, sizeof(double) * tab.size(), hipMemcpyDeviceToHost));
+        unsigned used[27] = {0};
+        for (long c = 0; c < L.ncls; c++)
+            for (int b = 0; b < 27; b++)
+                for (int e = 0; e < 9; e++)
+                    if (tab[(size_t)c * kBlk + 9 * b + e] != 0.0) used[b] |= 1u << e;
+        int pat = PAT_DENSE;
+        for (int cand : {PAT_SHEAR, PAT_TRANS_XZ, PAT_NOCORNER}) {
+            bool fits = true;
+            for (int b = 0; b < 27; b++)
+                if (b != 13 && (used[b] & ~pat_mask(cand, b))) fits = false;
+            if (fits) {
+                pat = cand;
+                break;
+            }
+        }
+        L.pat = pat;
+        L.fused = true;
+    }
     if (h->g64) { hipGraphExecDestroy(h->g64); h->g64 = nullptr; }
     if (h->g32) { hipGraphExecDestroy(h->g32); h->g32 = nullptr; }
     h->ops_set = true;
@@ -400,6 +577,8 @@ int stzmg_create(const stzmg_desc *d, stzmg_t *out) {
             alloc(&L.inv32, L.ncls * 9) || alloc(&L.x64, n3) || alloc(&L.r64, n3) || alloc(&L.t64, n3) ||
             alloc(&L.x32, n3) || alloc(&L.r32, n3) || alloc(&L.t32, n3))
             return bail(fail(h, STZMG_ERR_NOMEM, "device allocation failed at level %zu", l));
+        if (fused_ok(L) && (alloc(&L.x64b, n3) || alloc(&L.x32b, n3)))
+            return bail(fail(h, STZMG_ERR_NOMEM, "device allocation failed at level %zu", l));
         hipMemsetAsync(L.ops64, 0, sizeof(double) * L.ncls * kBlk, h->stream);
         hipMemsetAsync(L.x64, 0, sizeof(double) * n3, h->stream);
         hipMemsetAsync(L.r64, 0, sizeof(double) * n3, h->stream);
@@ -439,6 +618,30 @@ int stzmg_level_dims(stzmg_t h, int level, int dims[3]) {
 }
 
 void *stzmg_stream(stzmg_t h) { return h ? (void *)h->stream : nullptr; }
+
+int stzmg_set_option(stzmg_t h, int option, int value) {
+    if (!h) return STZMG_ERR_ARG;
+    if (option == STZMG_OPT_COARSE_KERNEL) {
+        h->coarse_max = value == 1 ? kCoarseMaxPoints : (value < 0 ? 0 : value);
+    } else if (option == STZMG_OPT_FUSED_SMOOTHER) {
+        h->fused_smoother = value != 0;
+        if (h->ops_set) {
+            for (auto &L : h->lv) {
+                restore_level<double>(h, L, L.x64);  // pointers are home between calls
+                if (!h->fused_smoother) L.fused = false;
+            }
+            if (h->fused_smoother) {
+                int rc = build_hierarchy_ops(h);
+                if (rc) return rc;
+            }
+        }
+    } else {
+        return fail(h, STZMG_ERR_ARG, "unknown option %d", option);
+    }
+    if (h->g64) { hipGraphExecDestroy(h->g64); h->g64 = nullptr; }
+    if (h->g32) { hipGraphExecDestroy(h->g32); h->g32 = nullptr; }
+    return 0;
+}
 
 int stzmg_set_operator_planes(stzmg_t h, const double *planes) {
     if (!h || !planes) return STZMG_ERR_ARG;
@@ -552,7 +755,9 @@ int stzmg_get_x_device(stzmg_t h, double *x_dev) {
 int stzmg_gauss_seidel(stzmg_t h, int level, int nsweeps) {
     if (int rc = check_level(h, level)) return rc;
     if (int rc = require_ops(h)) return rc;
+    double *home = h->lv[level].x64;
     for (int q = 0; q < nsweeps; q++) launch_gs_sweep<double>(h, h->lv[level]);
+    restore_level<double>(h, h->lv[level], home);
     LAUNCHCHK(h);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return 0;
@@ -735,38 +940,34 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
     return rc;
 }
 
-int stzmg_time_kernel(stzmg_t h, int kind, int level, int reps, double *ms_per_launch) {
+int stzmg_time_kernel(stzmg_t h, int kind, int level, int reps, double *ms_per_unit) {
     if (int rc = check_level(h, level)) return rc;
     if (int rc = require_ops(h)) return rc;
-    if (reps < 1 || !ms_per_launch) return STZMG_ERR_ARG;
+    if (reps < 1 || !ms_per_unit || kind < 0 || kind > 2) return STZMG_ERR_ARG;
     Level &L = h->lv[level];
+    double *home64 = L.x64;
+    float *home32 = L.x32;
     hipEvent_t a, b;
     HIPCHK(h, hipEventCreate(&a));
     HIPCHK(h, hipEventCreate(&b));
-    int launches = 0;
     auto body = [&]() {
-        if (kind == 0) {
-            launch_gs_sweep<double>(h, L);
-            launches += 8;
-        } else if (kind == 1) {
-            launch_resid<double, false>(h, L, L.x64, L.r64, L.t64);
-            launches += 1;
-        } else if (kind == 2) {
-            launch_gs_sweep<float>(h, L);
-            launches += 8;
-        }
+        if (kind == 0) launch_gs_sweep<double>(h, L);
+        else if (kind == 1) launch_resid<double, false>(h, L, L.x64, L.r64, L.t64);
+        else launch_gs_sweep<float>(h, L);
     };
     body();
-    launches = 0;
     HIPCHK(h, hipEventRecord(a, h->stream));
     for (int q = 0; q < reps; q++) body();
     HIPCHK(h, hipEventRecord(b, h->stream));
     HIPCHK(h, hipEventSynchronize(b));
     float ms = 0;
     HIPCHK(h, hipEventElapsedTime(&ms, a, b));
-    hipEventDestroy(a);
-    hipEventDestroy(b);
-    *ms_per_launch = launches ? ms / launches : 0;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    restore_level<double>(h, L, home64);
+    restore_level<float>(h, L, home32);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    *ms_per_unit = ms / reps;
     return 0;
 }
 

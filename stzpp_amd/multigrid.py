@@ -19,6 +19,7 @@ from . import _lib
 PLANE, POINT = 0, 1
 NORM_QUARTIC, NORM_L2SQ = 0, 1
 SOLVE_REF, SOLVE_RELRES, SOLVE_FP32_IR = 0, 1, 2
+OPT_COARSE_KERNEL, OPT_FUSED_SMOOTHER = 0, 1
 
 
 class MultigridError(RuntimeError):
@@ -179,6 +180,9 @@ class Multigrid:
         v = ctypes.c_double()
         self._chk(self.L.stzmg_time_kernel(self.h, kind, level, reps, ctypes.byref(v)))
         return v.value
+
+    def set_option(self, option, value):
+        self._chk(self.L.stzmg_set_option(self.h, option, int(value)))
 
     @property
     def stream(self):

@@ -118,6 +118,76 @@ def test_gpu_vcycle_matches_oracle(golden, case):
     assert relerr(mg.x(0), orc.fields(0)[0]) < 1e-11
 
 
+@pytest.mark.parametrize("kind,prd", [("shear", (1, 1, 0)), ("trans", (1, 1, 1)), ("transT", (1, 1, 0)),
+                                      ("rat", (1, 1, 1))])
+def test_gpu_fused_smoother_matches_oracle(kind, prd):
+    # Two-launch fused sweeps (64 x 64 planes) against the oracle's 8-colour
+    # sweep and against the eight-launch path, for each zero pattern.
+    n = 64
+    h = 1.0 / n
+    o = n + (0 if prd[2] else 1)
+    T = np.array([[1.0, 0.0, 0.5], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+    st, fix = MG.shear_stencil(1.0, 1.5, h, h, h, 1.0)
+    if kind == "trans":
+        st = MG.trans_stencil(T, 1.0, 1.5, h, h, h, 1.0)
+    elif kind == "transT":
+        st = MG.trans_stencil(np.array([[1.1, 0.2, 0.3], [0.0, 0.9, 0.1], [0.0, 0.0, 1.0]]), 1.0, 1.5, h, h, h, 1.0)
+    elif kind == "rat":
+        # a dense 27-block operator: the Galerkin operator of a 128^3 problem
+        big = MG.Multigrid(2 * n, 2 * n, 2 * n, (1, 1, 1))
+        big.setup_matrices_const(MG.trans_stencil(T, 1.0, 1.5, h / 2, h / 2, h / 2, 1.0), None)
+        st = big.level_operator(1)[0]
+    planes = O.plane_operator(o, prd[2], st, fix if not prd[2] else None)
+    N = n * n * o
+    x, r = O.det_field(N, 41), O.det_field(N, 42)
+    orc = O.OracleMG(n, n, o, prd, mode=O.PLANE)
+    orc.set_ops(0, planes)
+    orc.params(smoother=1)
+    orc.set_fields(0, x, r)
+    orc.gs(0, 3)
+    out = []
+    for fused in (1, 0):
+        mg = MG.Multigrid(n, n, o, prd)
+        mg.set_option(MG.OPT_FUSED_SMOOTHER, fused)
+        mg.setup_matrices_planes(planes)
+        mg.setup_fields(x, r)
+        mg.gauss_seidel(0, 3)
+        out.append(mg.x(0))
+    assert relerr(out[0], out[1]) < 1e-12
+    assert relerr(out[0], orc.fields(0)[0]) < 1e-12
+
+
+@pytest.mark.parametrize("n,prd", [(32, (1, 1, 0)), (64, (1, 1, 1)), (48, (1, 1, 0))])
+@pytest.mark.parametrize("fp32", [False, True])
+def test_gpu_coarse_kernel_matches_launch_path(n, prd, fp32):
+    # The single-workgroup coarse kernel runs the same passes in the same
+    # order as the one-launch-per-pass path.
+    h = 1.0 / n
+    T = np.array([[1.0, 0.0, 0.5], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+    st, fix = MG.shear_stencil(1.0, 1.5, h, h, h, 1.0)
+    if prd == (1, 1, 1):
+        st, fix = MG.trans_stencil(T, 1.0, 1.5, h, h, h, 1.0), None
+    o = n + (0 if prd[2] else 1)
+    xs = []
+    for opt in (0, 1):
+        mg = MG.Multigrid(n, n, o, prd)
+        mg.set_option(MG.OPT_COARSE_KERNEL, opt)
+        mg.setup_matrices_const(st, fix)
+        N = mg.npts(0)
+        b = O.det_field(N, 31)
+        if prd == (1, 1, 1):
+            b -= b.mean(axis=0)  # consistent right-hand side of the singular periodic system
+        mg.setup_fields(np.zeros((N, 3)), b)
+        if fp32:
+            mg.v_cycle_fp32(0)
+            mg.solve(mode=MG.SOLVE_FP32_IR, rtol=1e-9)
+        else:
+            mg.v_cycle(3)
+        xs.append(mg.x(0))
+    # FP64: identical passes; FP32+IR: both reach 1e-9 relative residual
+    assert relerr(xs[1], xs[0]) < (1e-7 if fp32 else 1e-12)
+
+
 def test_gpu_vcycle_converges_like_reference(golden):
     # The 8-colour V-cycle and the reference's lexicographic one reach the
     # same solution; per-cycle contraction is of the same order.
