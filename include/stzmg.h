@@ -149,11 +149,16 @@ void stzmg_solve_opts_default(stzmg_solve_opts *o);
  *                            single-workgroup kernel per V-cycle; 1 selects
  *                            the built-in threshold, 0 launches every pass
  *                            separately. */
-enum { STZMG_OPT_COARSE_KERNEL = 0, STZMG_OPT_FUSED_SMOOTHER = 1 };
+enum { STZMG_OPT_COARSE_KERNEL = 0, STZMG_OPT_FUSED_SMOOTHER = 1, STZMG_OPT_TILED_RESIDUAL = 2 };
 /*   STZMG_OPT_FUSED_SMOOTHER 1 (default): per-plane levels with periodic x, y
  *                            (sizes multiples of 64 x 16) sweep with two fused
  *                            launches (one per z parity); 0: eight colour
- *                            launches per sweep. */
+ *                            launches per sweep.
+ *   STZMG_OPT_TILED_RESIDUAL 1 (default): on the same levels the residual
+ *                            marches tiles through z with planes staged in
+ *                            LDS; 0: one thread per point reading its 27
+ *                            neighbours through the caches.  Sums may differ
+ *                            in the last bits (another summation order). */
 int stzmg_set_option(stzmg_t h, int option, int value);
 
 /* The HIP stream (hipStream_t) the handle launches on. */

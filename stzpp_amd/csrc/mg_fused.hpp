@@ -85,27 +85,27 @@ __device__ __forceinline__ void acc_blocks(const T *__restrict__ A, const LD &ld
     (acc_block<T, PAT, Bs>(A, ld, a0, a1, a2), ...);
 }
 
-constexpr int kFusedThreads = 256;
-
-// Build-time tuning of the fused smoother (defaults measured on MI355X):
-// whether the next two planes and the right-hand side are prefetched into
-// registers, and the minimum number of workgroups per CU the register
-// allocation must allow.
+// Build-time tuning of the fused smoother: whether the next two planes and
+// the right-hand side are prefetched into registers ahead of the arithmetic,
+// and the minimum number of waves per SIMD the register allocation must
+// allow.  Measured on MI355X at 256^3: prefetching raises the FP32 kernel
+// to ~290 registers (one wave per SIMD) and makes a sweep 2.8x slower
+// (1.30 ms against 0.47 ms); FP64 gains 3%.  Both default to off.
 #ifndef STZ_FUSED_PREFETCH_X
-#define STZ_FUSED_PREFETCH_X 1
+#define STZ_FUSED_PREFETCH_X 0
 #endif
 #ifndef STZ_FUSED_PREFETCH_R
-#define STZ_FUSED_PREFETCH_R 1
+#define STZ_FUSED_PREFETCH_R 0
 #endif
-#ifndef STZ_FUSED_MIN_WG
-#define STZ_FUSED_MIN_WG 1
+#ifndef STZ_FUSED_WAVES
+#define STZ_FUSED_WAVES 1
 #endif
 
 __device__ __forceinline__ int wrap1(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 // Geometry of in-plane sub-pass s (colour (s & 1, s >> 1), halo 3 - s) on a
 // TX x TY tile: first relative x/y of the colour and the number of points.
-template <int TX, int TY>
+template <int TX, int TY, int NT>
 struct SubPass {
     static constexpr int h(int s) { return 3 - s; }
     static constexpr int xf(int s) { return -h(s) + (((s & 1) + h(s)) & 1); }
@@ -113,29 +113,26 @@ struct SubPass {
     static constexpr int nx(int s) { return (TX + h(s) - 1 - xf(s)) / 2 + 1; }
     static constexpr int ny(int s) { return (TY + h(s) - 1 - yf(s)) / 2 + 1; }
     static constexpr int npts(int s) { return nx(s) * ny(s); }
-    static constexpr int iters(int s) { return (npts(s) + kFusedThreads - 1) / kFusedThreads; }
+    static constexpr int iters(int s) { return (npts(s) + NT - 1) / NT; }
     static constexpr int max_iters = iters(0);
 };
 
 // Grid: (m/TX, n/TY, ceil(planes of parity P / kchunk)).  Requires a PLANE
 // operator, x and y periodic with m % TX == 0 and n % TY == 0.
 //
-// Per plane: the next two planes and this plane's right-hand side are
-// fetched into registers before the four sub-passes run on LDS, so global
-// latency overlaps the arithmetic of the current plane.
-template <typename T, int PAT, int TX, int TY>
-__global__ __launch_bounds__(kFusedThreads) void k_gs_planes(Dims d, const T *__restrict__ xs,
-                                                            const T *__restrict__ xo, T *__restrict__ xd,
-                                                            const T *__restrict__ r,
-                                                            const T *__restrict__ ops,
-                                                            const T *__restrict__ inv, int P, int kchunk) {
+// Per plane: four sub-passes on the three staged planes, write-back of the
+// tile, then the next two planes are staged into the slots that fell free.
+template <typename T, int PAT, int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(STZ_FUSED_WAVES))) void k_gs_planes(
+    Dims d, const T *__restrict__ xs, const T *__restrict__ xo, T *__restrict__ xd, const T *__restrict__ r,
+    const T *__restrict__ ops, const T *__restrict__ inv, int P, int kchunk) {
     constexpr int H = 4;                 // 3 recomputed halo points + the stencil reach
     constexpr int WX = TX + 2 * H, WY = TY + 2 * H;
     constexpr int HX = WX / 2;           // x is stored split by parity
     constexpr int CSZ = WX * WY;         // one component of one plane
     constexpr int PSZ = 3 * CSZ;         // one plane slot
-    constexpr int NPER = (PSZ + kFusedThreads - 1) / kFusedThreads;
-    using SP = SubPass<TX, TY>;
+    constexpr int NPER = (PSZ + NT - 1) / NT;
+    using SP = SubPass<TX, TY, NT>;
     constexpr int MI = SP::max_iters;
     __shared__ T lds[3 * PSZ];
 
@@ -158,7 +155,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_gs_planes(Dims d, const T *__
         const long base = (long)d.m * d.n * kw;
 #pragma unroll
         for (int u = 0; u < NPER; u++) {
-            const int q = tid + u * kFusedThreads;
+            const int q = tid + u * NT;
             T val = 0;
             if (valid && q < PSZ) {
                 const int c = q / CSZ, rem = q - c * CSZ, yy = rem / WX, xx = rem - yy * WX;
@@ -171,7 +168,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_gs_planes(Dims d, const T *__
     auto store = [&](int s, const T (&v)[NPER]) {
 #pragma unroll
         for (int u = 0; u < NPER; u++) {
-            const int q = tid + u * kFusedThreads;
+            const int q = tid + u * NT;
             if (q < PSZ) {
                 const int c = q / CSZ, rem = q - c * CSZ, yy = rem / WX, xx = rem - yy * WX;
                 lds[li(c, s, yy, xx)] = v[u];
@@ -195,33 +192,38 @@ __global__ __launch_bounds__(kFusedThreads) void k_gs_planes(Dims d, const T *__
         const int k = P + 2 * kk;
         const long base = (long)d.m * d.n * k;
         const bool more = kk + 1 < kk1;
+#if STZ_FUSED_PREFETCH_X
         if (more) {
             fetch(xs, k + 2, va);
             fetch(xo, k + 3, vb);
         }
+#endif
+        auto rload = [&](int s, int t, T (&v)[3]) {
+            const int ty = t / SP::nx(s), tx = t - ty * SP::nx(s);
+            const int gi = wrap1(x0 + SP::xf(s) + 2 * tx, d.m), gj = wrap1(y0 + SP::yf(s) + 2 * ty, d.n);
+            const long p = base + gi + (long)d.m * gj;
+            v[0] = r[p];
+            v[1] = r[p + d.N];
+            v[2] = r[p + 2 * d.N];
+        };
         // right-hand side of every point this thread computes in this plane
         T rr[4][MI][3];
+#if STZ_FUSED_PREFETCH_R
 #pragma unroll
         for (int s = 0; s < 4; s++)
 #pragma unroll
             for (int u = 0; u < MI; u++) {
-                const int t = tid + u * kFusedThreads;
-                if (u < SP::iters(s) && t < SP::npts(s)) {
-                    const int ty = t / SP::nx(s), tx = t - ty * SP::nx(s);
-                    const int gi = wrap1(x0 + SP::xf(s) + 2 * tx, d.m), gj = wrap1(y0 + SP::yf(s) + 2 * ty, d.n);
-                    const long p = base + gi + (long)d.m * gj;
-                    rr[s][u][0] = r[p];
-                    rr[s][u][1] = r[p + d.N];
-                    rr[s][u][2] = r[p + 2 * d.N];
-                }
+                const int t = tid + u * NT;
+                if (u < SP::iters(s) && t < SP::npts(s)) rload(s, t, rr[s][u]);
             }
+#endif
         const T *A = ops + (long)k * kBlk;
         const T *Bi = inv + (long)k * 9;
 #pragma unroll
         for (int s = 0; s < 4; s++) {
 #pragma unroll
             for (int u = 0; u < MI; u++) {
-                const int t = tid + u * kFusedThreads;
+                const int t = tid + u * NT;
                 if (u < SP::iters(s) && t < SP::npts(s)) {
                     const int ty = t / SP::nx(s), tx = t - ty * SP::nx(s);
                     const int xx = SP::xf(s) + 2 * tx + H, yy = SP::yf(s) + 2 * ty + H;
@@ -229,6 +231,9 @@ __global__ __launch_bounds__(kFusedThreads) void k_gs_planes(Dims d, const T *__
                         const int sl = dk < 0 ? sm : (dk > 0 ? sp : sc);
                         return lds[li(c, sl, yy + dj, xx + di)];
                     };
+#if !STZ_FUSED_PREFETCH_R
+                    rload(s, t, rr[s][u]);
+#endif
                     T a0 = 0, a1 = 0, a2 = 0;
                     acc_blocks<T, PAT>(A, ld, a0, a1, a2, std::make_integer_sequence<int, 27>{});
                     const T e0 = rr[s][u][0] - a0, e1 = rr[s][u][1] - a1, e2 = rr[s][u][2] - a2;
@@ -239,21 +244,128 @@ __global__ __launch_bounds__(kFusedThreads) void k_gs_planes(Dims d, const T *__
             }
             __syncthreads();
         }
-        for (int q = tid; q < 3 * TX * TY; q += kFusedThreads) {
+        for (int q = tid; q < 3 * TX * TY; q += NT) {
             const int c = q / (TX * TY), rem = q - c * (TX * TY), yy = rem / TX, xx = rem - yy * TX;
             xd[c * d.N + base + (x0 + xx) + (long)d.m * (y0 + yy)] = lds[li(c, sc, yy + H, xx + H)];
         }
         if (more) {
             __syncthreads();
             const int nsm = sp, nsc = sm, nsp = sc;
+#if !STZ_FUSED_PREFETCH_X
+            fetch(xs, k + 2, va);
+            store(nsc, va);
+            fetch(xo, k + 3, va);
+            store(nsp, va);
+#else
             store(nsc, va);
             store(nsp, vb);
+#endif
             sm = nsm;
             sc = nsc;
             sp = nsp;
             __syncthreads();
         }
     }
+}
+
+// out = r - A x (APPLY: out = A x) for a PLANE operator on a level periodic
+// in x and y, marching a TX x TY tile through kchunk planes with the planes
+// k-1, k, k+1 staged in LDS (one-point halo), so x is read from HBM about
+// once instead of through 27 cached neighbour loads per point.  Each thread
+// owns R = TX*TY/NT vertically adjacent points of a plane, so the LDS rows
+// they share are read once.  With NORM, one pair of partials (sum |r|^4,
+// sum |r|^2) per workgroup goes to part[blockIdx linear].
+// Grid: (m/TX, n/TY, ceil(o/kchunk)).
+template <typename T, int PAT, bool NORM, bool APPLY, int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_resid_planes(Dims d, const T *__restrict__ x, const T *__restrict__ r,
+                                                     const T *__restrict__ ops, T *__restrict__ out,
+                                                     double *__restrict__ part, int kchunk) {
+    static_assert(NT == kThreads, "block_sum2 assumes kThreads threads");
+    static_assert(NT % TX == 0 && (TX * TY) % NT == 0, "thread layout");
+    constexpr int WX = TX + 2, WY = TY + 2, CSZ = WX * WY, PSZ = 3 * CSZ;
+    constexpr int R = TX * TY / NT;  // rows per thread
+    __shared__ T lds[3 * PSZ];
+
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const int k0 = blockIdx.z * kchunk;
+    const int k1 = k0 + kchunk < d.o ? k0 + kchunk : d.o;
+    const int tid = threadIdx.x;
+    const int xx = tid % TX, yb = (tid / TX) * R;
+
+    auto stage = [&](int slot, int kg) {
+        bool valid = true;
+        if (kg < 0 || kg >= d.o) {
+            if (d.pz) kg = kg < 0 ? kg + d.o : kg - d.o;
+            else valid = false;
+        }
+        const long base = (long)d.m * d.n * kg;
+        for (int q = tid; q < PSZ; q += NT) {
+            const int c = q / CSZ, rem = q - c * CSZ, yy = rem / WX, xq = rem - yy * WX;
+            T v = 0;
+            if (valid) {
+                const int gi = wrap1(x0 + xq - 1, d.m), gj = wrap1(y0 + yy - 1, d.n);
+                v = x[c * d.N + base + gi + (long)d.m * gj];
+            }
+            lds[slot * PSZ + q] = v;
+        }
+    };
+
+    double q4 = 0, q2 = 0;
+    int sm = 0, sc = 1, sp = 2;
+    if (k0 < k1) {
+        stage(sm, k0 - 1);
+        stage(sc, k0);
+        stage(sp, k0 + 1);
+    }
+    __syncthreads();
+    for (int k = k0; k < k1; k++) {
+        const long base = (long)d.m * d.n * k;
+        const T *A = ops + (long)k * kBlk;
+#pragma unroll
+        for (int u = 0; u < R; u++) {
+            const int yy = yb + u;
+            auto ld = [&](int c, int di, int dj, int dk) -> T {
+                const int sl = dk < 0 ? sm : (dk > 0 ? sp : sc);
+                return lds[sl * PSZ + c * CSZ + (yy + 1 + dj) * WX + (xx + 1 + di)];
+            };
+            T a0 = 0, a1 = 0, a2 = 0;
+            acc_blocks<T, PAT>(A, ld, a0, a1, a2, std::make_integer_sequence<int, 27>{});
+            // centre block (acc_blocks leaves it out: the smoother inverts it)
+            {
+                const T *c = A + 9 * 13;
+                const T v0 = ld(0, 0, 0, 0), v1 = ld(1, 0, 0, 0), v2 = ld(2, 0, 0, 0);
+                a0 += c[0] * v0 + c[1] * v1 + c[2] * v2;
+                a1 += c[3] * v0 + c[4] * v1 + c[5] * v2;
+                a2 += c[6] * v0 + c[7] * v1 + c[8] * v2;
+            }
+            const long p = base + (x0 + xx) + (long)d.m * (y0 + yy);
+            T o0 = a0, o1 = a1, o2 = a2;
+            if constexpr (!APPLY) {
+                o0 = r[p] - a0;
+                o1 = r[p + d.N] - a1;
+                o2 = r[p + 2 * d.N] - a2;
+            }
+            out[p] = o0;
+            out[p + d.N] = o1;
+            out[p + 2 * d.N] = o2;
+            if constexpr (NORM) {
+                const double s = (double)o0 * o0 + (double)o1 * o1 + (double)o2 * o2;
+                q2 += s;
+                q4 += s * s;
+            }
+        }
+        if (k + 1 < k1) {
+            __syncthreads();
+            stage(sm, k + 2);
+            const int t = sm;
+            sm = sc;
+            sc = sp;
+            sp = t;
+            __syncthreads();
+        }
+    }
+    if constexpr (NORM)
+        block_sum2(q4, q2, part, (long)blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z));
 }
 
 }  // namespace stzmg

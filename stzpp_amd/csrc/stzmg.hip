@@ -37,6 +37,7 @@ struct Level {
     double *x64b = nullptr;
     float *x32b = nullptr;
     bool fused = false;   // fused two-launch sweeps on this level
+    bool tiled = false;   // LDS plane-marching residual on this level
     int pat = 0;          // operator zero pattern (PAT_*)
 };
 
@@ -57,6 +58,7 @@ struct stzmg_s {
     int std_iters = 16;       // per-handle copy of the reference's static heuristic
     long coarse_max = kCoarseMaxPoints;  // levels this small run in one workgroup (0: off)
     int fused_smoother = 1;   // two-launch fused sweeps on eligible levels
+    int tiled_resid = 1;      // plane-marching residual on eligible levels
     hipGraphExec_t g64 = nullptr, g32 = nullptr;
 };
 
@@ -134,42 +136,120 @@ double *t_of<double>(Level &L) { return L.t64; }
 template <>
 float *t_of<float>(Level &L) { return L.t32; }
 
-// out = r - A x on level L (optionally with norm partials into h->part).
-template <typename T, bool NORM>
-void launch_resid(stzmg_t h, Level &L, const T *x, const T *r, T *out) {
+// Tiles of the plane-marching residual (k_resid_planes): 64 x 16 in FP32,
+// 32 x 16 in FP64, 256 threads.
+template <typename T>
+struct ResidTile;
+template <>
+struct ResidTile<float> {
+    static constexpr int TX = 64, TY = 16;
+};
+template <>
+struct ResidTile<double> {
+    static constexpr int TX = 32, TY = 16;
+};
+
+bool tiled_ok(const Level &L) {
+    return L.mode == STZMG_OP_PLANE && L.d.px && L.d.py && L.d.m % ResidTile<float>::TX == 0 &&
+           L.d.n % ResidTile<float>::TY == 0 && L.d.m % ResidTile<double>::TX == 0 &&
+           L.d.n % ResidTile<double>::TY == 0;
+}
+
+// Grid of the plane-marching residual: planes per workgroup doubled (up to
+// 16, so the two extra staged planes cost at most 1/8 of the reads) while at
+// least 1024 workgroups remain.
+template <typename T>
+dim3 tiled_grid(const Level &L, int *kchunk) {
+    const long tiles = (long)(L.d.m / ResidTile<T>::TX) * (L.d.n / ResidTile<T>::TY);
+    int kc = 1;
+    while (kc < 16 && tiles * ((L.d.o + 2 * kc - 1) / (2 * kc)) >= 1024) kc *= 2;
+    *kchunk = kc;
+    return dim3(L.d.m / ResidTile<T>::TX, L.d.n / ResidTile<T>::TY, (L.d.o + kc - 1) / kc);
+}
+
+// Number of norm partials the residual of level L writes.
+template <typename T>
+long resid_parts(const Level &L) {
+    if (L.tiled) {
+        int kc;
+        const dim3 g = tiled_grid<T>(L, &kc);
+        return (long)g.x * g.y * g.z;
+    }
+    return (long)nblk((long)L.d.m * L.d.n) * L.d.o;
+}
+
+template <typename T, bool NORM, bool APPLY, int PAT>
+void launch_resid_tiled(stzmg_t h, Level &L, const T *x, const T *r, T *out) {
+    int kc;
+    const dim3 g = tiled_grid<T>(L, &kc);
+    hipLaunchKernelGGL((k_resid_planes<T, PAT, NORM, APPLY, ResidTile<T>::TX, ResidTile<T>::TY, kThreads>), g,
+                       dim3(kThreads), 0, h->stream, L.d, x, r, ops_of<T>(L), out, h->part, kc);
+}
+
+// out = r - A x (APPLY: out = A x) on level L, optionally with norm partials
+// into h->part.
+template <typename T, bool NORM, bool APPLY>
+void launch_resid_any(stzmg_t h, Level &L, const T *x, const T *r, T *out) {
+    if (L.tiled) {
+        switch (L.pat) {
+            case PAT_SHEAR: launch_resid_tiled<T, NORM, APPLY, PAT_SHEAR>(h, L, x, r, out); break;
+            case PAT_TRANS_XZ: launch_resid_tiled<T, NORM, APPLY, PAT_TRANS_XZ>(h, L, x, r, out); break;
+            case PAT_NOCORNER: launch_resid_tiled<T, NORM, APPLY, PAT_NOCORNER>(h, L, x, r, out); break;
+            default: launch_resid_tiled<T, NORM, APPLY, PAT_DENSE>(h, L, x, r, out); break;
+        }
+        return;
+    }
     dim3 g(nblk((long)L.d.m * L.d.n), L.d.o);
     if (L.mode == STZMG_OP_PLANE)
-        hipLaunchKernelGGL((k_resid<T, 0, NORM, false>), g, dim3(kThreads), 0, h->stream, L.d, x, r,
+        hipLaunchKernelGGL((k_resid<T, 0, NORM, APPLY>), g, dim3(kThreads), 0, h->stream, L.d, x, r,
                            ops_of<T>(L), out, h->part);
     else
-        hipLaunchKernelGGL((k_resid<T, 1, NORM, false>), g, dim3(kThreads), 0, h->stream, L.d, x, r,
+        hipLaunchKernelGGL((k_resid<T, 1, NORM, APPLY>), g, dim3(kThreads), 0, h->stream, L.d, x, r,
                            ops_of<T>(L), out, h->part);
+}
+
+template <typename T, bool NORM>
+void launch_resid(stzmg_t h, Level &L, const T *x, const T *r, T *out) {
+    launch_resid_any<T, NORM, false>(h, L, x, r, out);
 }
 
 template <typename T>
 void launch_apply(stzmg_t h, Level &L, const T *x, T *out) {
-    dim3 g(nblk((long)L.d.m * L.d.n), L.d.o);
-    if (L.mode == STZMG_OP_PLANE)
-        hipLaunchKernelGGL((k_resid<T, 0, false, true>), g, dim3(kThreads), 0, h->stream, L.d, x,
-                           (const T *)nullptr, ops_of<T>(L), out, h->part);
-    else
-        hipLaunchKernelGGL((k_resid<T, 1, false, true>), g, dim3(kThreads), 0, h->stream, L.d, x,
-                           (const T *)nullptr, ops_of<T>(L), out, h->part);
+    launch_resid_any<T, false, true>(h, L, x, (const T *)nullptr, out);
 }
 
 int ncol(int n, int prd) { return (prd && (n & 1)) ? 3 : 2; }
 
-// Tile of the fused smoother: 64 x 16 in FP32 and 32 x 16 in FP64 keep the
-// three staged planes (3 comps, 4-point halo) within 70 KB of LDS.
+// Tile and workgroup size of the fused smoother: 64 x 16 in FP32 and
+// 32 x 16 in FP64 keep the three staged planes (3 comps, 4-point halo)
+// within 70 KB of LDS, so two workgroups fit one CU.
+#ifndef STZ_F32_TX
+#define STZ_F32_TX 64
+#endif
+#ifndef STZ_F32_TY
+#define STZ_F32_TY 16
+#endif
+#ifndef STZ_F32_NT
+#define STZ_F32_NT 256
+#endif
+#ifndef STZ_F64_TX
+#define STZ_F64_TX 32
+#endif
+#ifndef STZ_F64_TY
+#define STZ_F64_TY 16
+#endif
+#ifndef STZ_F64_NT
+#define STZ_F64_NT 256
+#endif
 template <typename T>
 struct FusedTile;
 template <>
 struct FusedTile<float> {
-    static constexpr int TX = 64, TY = 16;
+    static constexpr int TX = STZ_F32_TX, TY = STZ_F32_TY, NT = STZ_F32_NT;
 };
 template <>
 struct FusedTile<double> {
-    static constexpr int TX = 32, TY = 16;
+    static constexpr int TX = STZ_F64_TX, TY = STZ_F64_TY, NT = STZ_F64_NT;
 };
 
 template <typename T>
@@ -193,7 +273,7 @@ bool fused_ok(const Level &L) {
 
 template <typename T, int PAT>
 void launch_fused_pat(stzmg_t h, Level &L) {
-    constexpr int TX = FusedTile<T>::TX, TY = FusedTile<T>::TY;
+    constexpr int TX = FusedTile<T>::TX, TY = FusedTile<T>::TY, NT = FusedTile<T>::NT;
     const Dims &d = L.d;
     T *x = xref_of<T>(L), *xb = xb_of<T>(L);
     for (int P = 0; P < 2; P++) {
@@ -203,7 +283,7 @@ void launch_fused_pat(stzmg_t h, Level &L) {
         int kchunk = 1;
         while (kchunk < 8 && tiles * ((nP + 2 * kchunk - 1) / (2 * kchunk)) >= 2048) kchunk *= 2;
         dim3 g(d.m / TX, d.n / TY, (nP + kchunk - 1) / kchunk);
-        hipLaunchKernelGGL((k_gs_planes<T, PAT, TX, TY>), g, dim3(kFusedThreads), 0, h->stream, d, (const T *)x,
+        hipLaunchKernelGGL((k_gs_planes<T, PAT, TX, TY, NT>), g, dim3(NT), 0, h->stream, d, (const T *)x,
                            (const T *)(P == 0 ? x : xb), xb, (const T *)r_of<T>(L), ops_of<T>(L), inv_of<T>(L),
                            P, kchunk);
     }
@@ -383,7 +463,7 @@ int ensure_part(stzmg_t h, long nb) {
 // The residual is left in t64 of that level.
 int norms64(stzmg_t h, int l, double out[2]) {
     Level &L = h->lv[l];
-    const long nb = (long)nblk((long)L.d.m * L.d.n) * L.d.o;
+    const long nb = resid_parts<double>(L);
     if (ensure_part(h, nb)) return STZMG_ERR_NOMEM;
     launch_resid<double, true>(h, L, L.x64, L.r64, L.t64);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, h->stream, (const double *)h->part, nb,
@@ -434,10 +514,12 @@ int build_hierarchy_ops(stzmg_t h) {
     HIPCHK(h, hipMemcpyAsync(&flag, h->dflag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     // Zero pattern of every per-plane level: the tightest compiled pattern
-    // that holds all nonzero entries selects the fused smoother instance.
+    // that holds all nonzero entries selects the fused smoother and tiled
+    // residual instances.
     for (auto &L : h->lv) {
         L.fused = false;
-        if (!h->fused_smoother || !fused_ok(L) || !L.x64b) continue;
+        L.tiled = false;
+        if (L.mode != STZMG_OP_PLANE) continue;
         std::vector<double> tab((size_t)L.ncls * kBlk);
         HIPCHK(h, hipMemcpy(tab.data(), L.ops64, sizeof(double) * tab.size(), hipMemcpyDeviceToHost));
         unsigned used[27] = {0};
@@ -456,7 +538,8 @@ int build_hierarchy_ops(stzmg_t h) {
             }
         }
         L.pat = pat;
-        L.fused = true;
+        L.fused = h->fused_smoother && fused_ok(L) && L.x64b;
+        L.tiled = h->tiled_resid && tiled_ok(L);
     }
     if (h->g64) { hipGraphExecDestroy(h->g64); h->g64 = nullptr; }
     if (h->g32) { hipGraphExecDestroy(h->g32); h->g32 = nullptr; }
@@ -635,6 +718,9 @@ int stzmg_set_option(stzmg_t h, int option, int value) {
                 if (rc) return rc;
             }
         }
+    } else if (option == STZMG_OPT_TILED_RESIDUAL) {
+        h->tiled_resid = value != 0;
+        for (auto &L : h->lv) L.tiled = h->ops_set && h->tiled_resid && tiled_ok(L);
     } else {
         return fail(h, STZMG_ERR_ARG, "unknown option %d", option);
     }

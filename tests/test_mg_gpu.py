@@ -157,6 +157,47 @@ def test_gpu_fused_smoother_matches_oracle(kind, prd):
     assert relerr(out[0], orc.fields(0)[0]) < 1e-12
 
 
+@pytest.mark.parametrize("kind,prd", [("shear", (1, 1, 0)), ("trans", (1, 1, 1)), ("transT", (1, 1, 0)),
+                                      ("rat", (1, 1, 1)), ("rat", (1, 1, 0))])
+def test_gpu_tiled_residual_matches_oracle(kind, prd):
+    # Plane-marching residual (64 x 32 planes, every zero pattern, walled and
+    # periodic z) against the oracle's residual and the per-point kernel;
+    # the norms against the oracle's.
+    m, n = 64, 32
+    h = 1.0 / n
+    o = 24 + (0 if prd[2] else 1)
+    T = np.array([[1.0, 0.0, 0.5], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+    st, fix = MG.shear_stencil(1.0, 1.5, h, h, h, 1.0)
+    if kind == "trans":
+        st = MG.trans_stencil(T, 1.0, 1.5, h, h, h, 1.0)
+    elif kind == "transT":
+        st = MG.trans_stencil(np.array([[1.1, 0.2, 0.3], [0.0, 0.9, 0.1], [0.0, 0.0, 1.0]]), 1.0, 1.5, h, h, h, 1.0)
+    elif kind == "rat":
+        big = MG.Multigrid(64, 64, 64, (1, 1, 1))
+        big.setup_matrices_const(MG.trans_stencil(T, 1.0, 1.5, h / 2, h / 2, h / 2, 1.0), None)
+        st = big.level_operator(1)[0]
+    planes = O.plane_operator(o, prd[2], st, fix if not prd[2] else None)
+    N = m * n * o
+    x, r = O.det_field(N, 51), O.det_field(N, 52)
+    orc = O.OracleMG(m, n, o, prd, mode=O.PLANE)
+    orc.set_ops(0, planes)
+    orc.set_fields(0, x, r)
+    res = []
+    norms = []
+    for tiled in (1, 0):
+        mg = MG.Multigrid(m, n, o, prd)
+        mg.set_option(MG.OPT_TILED_RESIDUAL, tiled)
+        mg.setup_matrices_planes(planes)
+        mg.setup_fields(x, r)
+        res.append(mg.residual(0))
+        norms.append((mg.norm(0, MG.NORM_QUARTIC), mg.norm(0, MG.NORM_L2SQ)))
+    ref = orc.residual(0)
+    assert relerr(res[0], ref) < 1e-13
+    assert relerr(res[0], res[1]) < 1e-13
+    assert abs(norms[0][0] / orc.norm_quartic(0) - 1) < 1e-12
+    assert abs(norms[0][1] / orc.norm2(0) - 1) < 1e-12
+
+
 @pytest.mark.parametrize("n,prd", [(32, (1, 1, 0)), (64, (1, 1, 1)), (48, (1, 1, 0))])
 @pytest.mark.parametrize("fp32", [False, True])
 def test_gpu_coarse_kernel_matches_launch_path(n, prd, fp32):
