@@ -56,33 +56,83 @@ __host__ __device__ constexpr unsigned pat_cols(unsigned m) {
     return ((m & 0x049) ? 1u : 0u) | ((m & 0x092) ? 2u : 0u) | ((m & 0x124) ? 4u : 0u);
 }
 
-template <typename T, int PAT, int B, typename LD>
-__device__ __forceinline__ void acc_block(const T *__restrict__ A, const LD &ld, T &a0, T &a1, T &a2) {
+// One plane's 27 blocks and inverse centre block (252 values; the inverse
+// is left zero when inv is null) spread over the 64 lanes of a wave: entry
+// q sits in lane q % 64 of word q / 64.  load() runs with all lanes active.  The
+// stencil loop broadcasts a coefficient with v_readlane, a VALU operation.
+// Loaded through the scalar cache instead, every group of coefficients would
+// wait on the counter that the LDS reads share, draining the reads in flight.
+template <typename T>
+struct LaneCoef;
+template <>
+struct LaneCoef<float> {
+    int w[4];
+    __device__ __forceinline__ void load(const float *__restrict__ A, const float *__restrict__ inv) {
+        const int lane = threadIdx.x & 63;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int q = lane + 64 * j;
+            const float v = q < kBlk ? A[q] : (inv && q < kBlk + 9 ? inv[q - kBlk] : 0.0f);
+            w[j] = __float_as_int(v);
+            // pins the load to this point, where every lane is active: a
+            // load sunk into a divergent branch would leave the lanes that
+            // v_readlane reads unwritten
+            asm volatile("" : "+v"(w[j]));
+        }
+    }
+    template <int Q>
+    __device__ __forceinline__ float get() const {
+        return __int_as_float(__builtin_amdgcn_readlane(w[Q >> 6], Q & 63));
+    }
+};
+template <>
+struct LaneCoef<double> {
+    int lo[4], hi[4];
+    __device__ __forceinline__ void load(const double *__restrict__ A, const double *__restrict__ inv) {
+        const int lane = threadIdx.x & 63;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int q = lane + 64 * j;
+            const double v = q < kBlk ? A[q] : (inv && q < kBlk + 9 ? inv[q - kBlk] : 0.0);
+            lo[j] = __double2loint(v);
+            hi[j] = __double2hiint(v);
+            asm volatile("" : "+v"(lo[j]), "+v"(hi[j]));
+        }
+    }
+    template <int Q>
+    __device__ __forceinline__ double get() const {
+        return __hiloint2double(__builtin_amdgcn_readlane(hi[Q >> 6], Q & 63),
+                                __builtin_amdgcn_readlane(lo[Q >> 6], Q & 63));
+    }
+};
+
+template <typename T, int PAT, int B, typename CF, typename LD>
+__device__ __forceinline__ void acc_block(const CF &cf, const LD &ld, T &a0, T &a1, T &a2) {
     constexpr unsigned m = pat_mask(PAT, B);
     if constexpr (B != 13 && m != 0) {
         constexpr int di = B % 3 - 1, dj = (B / 3) % 3 - 1, dk = B / 9 - 1;
         constexpr unsigned cols = pat_cols(m);
-        const T *a = A + 9 * B;
+        constexpr int q = 9 * B;
         T v0 = 0, v1 = 0, v2 = 0;
         if constexpr (cols & 1) v0 = ld(0, di, dj, dk);
         if constexpr (cols & 2) v1 = ld(1, di, dj, dk);
         if constexpr (cols & 4) v2 = ld(2, di, dj, dk);
-        if constexpr (m & 0x001) a0 += a[0] * v0;
-        if constexpr (m & 0x002) a0 += a[1] * v1;
-        if constexpr (m & 0x004) a0 += a[2] * v2;
-        if constexpr (m & 0x008) a1 += a[3] * v0;
-        if constexpr (m & 0x010) a1 += a[4] * v1;
-        if constexpr (m & 0x020) a1 += a[5] * v2;
-        if constexpr (m & 0x040) a2 += a[6] * v0;
-        if constexpr (m & 0x080) a2 += a[7] * v1;
-        if constexpr (m & 0x100) a2 += a[8] * v2;
+        if constexpr (m & 0x001) a0 += cf.template get<q + 0>() * v0;
+        if constexpr (m & 0x002) a0 += cf.template get<q + 1>() * v1;
+        if constexpr (m & 0x004) a0 += cf.template get<q + 2>() * v2;
+        if constexpr (m & 0x008) a1 += cf.template get<q + 3>() * v0;
+        if constexpr (m & 0x010) a1 += cf.template get<q + 4>() * v1;
+        if constexpr (m & 0x020) a1 += cf.template get<q + 5>() * v2;
+        if constexpr (m & 0x040) a2 += cf.template get<q + 6>() * v0;
+        if constexpr (m & 0x080) a2 += cf.template get<q + 7>() * v1;
+        if constexpr (m & 0x100) a2 += cf.template get<q + 8>() * v2;
     }
 }
 
-template <typename T, int PAT, typename LD, int... Bs>
-__device__ __forceinline__ void acc_blocks(const T *__restrict__ A, const LD &ld, T &a0, T &a1, T &a2,
+template <typename T, int PAT, typename CF, typename LD, int... Bs>
+__device__ __forceinline__ void acc_blocks(const CF &cf, const LD &ld, T &a0, T &a1, T &a2,
                                            std::integer_sequence<int, Bs...>) {
-    (acc_block<T, PAT, Bs>(A, ld, a0, a1, a2), ...);
+    (acc_block<T, PAT, Bs>(cf, ld, a0, a1, a2), ...);
 }
 
 // Build-time tuning of the fused smoother: whether the next two planes and
@@ -217,8 +267,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(STZ_FUSED_WA
                 if (u < SP::iters(s) && t < SP::npts(s)) rload(s, t, rr[s][u]);
             }
 #endif
-        const T *A = ops + (long)k * kBlk;
-        const T *Bi = inv + (long)k * 9;
+        LaneCoef<T> cf;
+        cf.load(ops + (long)k * kBlk, inv + (long)k * 9);
 #pragma unroll
         for (int s = 0; s < 4; s++) {
 #pragma unroll
@@ -235,11 +285,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(STZ_FUSED_WA
                     rload(s, t, rr[s][u]);
 #endif
                     T a0 = 0, a1 = 0, a2 = 0;
-                    acc_blocks<T, PAT>(A, ld, a0, a1, a2, std::make_integer_sequence<int, 27>{});
+                    acc_blocks<T, PAT>(cf, ld, a0, a1, a2, std::make_integer_sequence<int, 27>{});
                     const T e0 = rr[s][u][0] - a0, e1 = rr[s][u][1] - a1, e2 = rr[s][u][2] - a2;
-                    lds[li(0, sc, yy, xx)] = Bi[0] * e0 + Bi[1] * e1 + Bi[2] * e2;
-                    lds[li(1, sc, yy, xx)] = Bi[3] * e0 + Bi[4] * e1 + Bi[5] * e2;
-                    lds[li(2, sc, yy, xx)] = Bi[6] * e0 + Bi[7] * e1 + Bi[8] * e2;
+                    constexpr int I = kBlk;
+                    lds[li(0, sc, yy, xx)] =
+                        cf.template get<I + 0>() * e0 + cf.template get<I + 1>() * e1 + cf.template get<I + 2>() * e2;
+                    lds[li(1, sc, yy, xx)] =
+                        cf.template get<I + 3>() * e0 + cf.template get<I + 4>() * e1 + cf.template get<I + 5>() * e2;
+                    lds[li(2, sc, yy, xx)] =
+                        cf.template get<I + 6>() * e0 + cf.template get<I + 7>() * e1 + cf.template get<I + 8>() * e2;
                 }
             }
             __syncthreads();
@@ -320,7 +374,8 @@ __global__ __launch_bounds__(NT) void k_resid_planes(Dims d, const T *__restrict
     __syncthreads();
     for (int k = k0; k < k1; k++) {
         const long base = (long)d.m * d.n * k;
-        const T *A = ops + (long)k * kBlk;
+        LaneCoef<T> cf;
+        cf.load(ops + (long)k * kBlk, nullptr);
 #pragma unroll
         for (int u = 0; u < R; u++) {
             const int yy = yb + u;
@@ -329,14 +384,14 @@ __global__ __launch_bounds__(NT) void k_resid_planes(Dims d, const T *__restrict
                 return lds[sl * PSZ + c * CSZ + (yy + 1 + dj) * WX + (xx + 1 + di)];
             };
             T a0 = 0, a1 = 0, a2 = 0;
-            acc_blocks<T, PAT>(A, ld, a0, a1, a2, std::make_integer_sequence<int, 27>{});
+            acc_blocks<T, PAT>(cf, ld, a0, a1, a2, std::make_integer_sequence<int, 27>{});
             // centre block (acc_blocks leaves it out: the smoother inverts it)
             {
-                const T *c = A + 9 * 13;
+                constexpr int C = 9 * 13;
                 const T v0 = ld(0, 0, 0, 0), v1 = ld(1, 0, 0, 0), v2 = ld(2, 0, 0, 0);
-                a0 += c[0] * v0 + c[1] * v1 + c[2] * v2;
-                a1 += c[3] * v0 + c[4] * v1 + c[5] * v2;
-                a2 += c[6] * v0 + c[7] * v1 + c[8] * v2;
+                a0 += cf.template get<C + 0>() * v0 + cf.template get<C + 1>() * v1 + cf.template get<C + 2>() * v2;
+                a1 += cf.template get<C + 3>() * v0 + cf.template get<C + 4>() * v1 + cf.template get<C + 5>() * v2;
+                a2 += cf.template get<C + 6>() * v0 + cf.template get<C + 7>() * v1 + cf.template get<C + 8>() * v2;
             }
             const long p = base + (x0 + xx) + (long)d.m * (y0 + yy);
             T o0 = a0, o1 = a1, o2 = a2;
