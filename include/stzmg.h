@@ -163,9 +163,12 @@ int stzmg_set_option(stzmg_t h, int option, int value);
 
 /* The HIP stream (hipStream_t) the handle launches on. */
 void *stzmg_stream(stzmg_t h);
-/* Per-kernel timing of the last n launches of one kernel class, for roofline
- * accounting: kind 0 = fine-level GS colour pass, 1 = fine residual. */
-int stzmg_time_kernel(stzmg_t h, int kind, int level, int reps, double *ms_per_launch);
+/* Mean time of `reps` back-to-back repetitions of one operation on a level,
+ * timed with HIP events on the handle's stream, for roofline accounting
+ * (no reference counterpart): kind 0 = one FP64 Gauss-Seidel sweep,
+ * 1 = one FP64 residual r - A x, 2 = one FP32 sweep, 3 = one FP32 residual.
+ * The level's solution is left as it was. */
+int stzmg_time_kernel(stzmg_t h, int kind, int level, int reps, double *ms_per_unit);
 
 /* Operator builders (host side, FP64).
  * stzmg_shear_stencil: stencil = -dt*stencil_base and fix of

@@ -36,6 +36,8 @@ template <typename T>
 struct CoarseArgs {
     int nl;                     // levels handled (relative 0 .. nl-1)
     int bottom;                 // sweeps on the last one
+    int table_vals;             // doubles of dynamic LDS before the bottom fields;
+                                // -1: the bottom fields stay in global memory
     CoarseLevel<T> L[kMaxCoarse];
 };
 
@@ -76,6 +78,75 @@ __device__ void wg_gs_sweep(const Dims &d, T *x, const T *r, const T *ops, const
                 }
                 __syncthreads();
             }
+}
+
+// Bottom solve: nsweeps colour sweeps of one level with x and r held in
+// LDS (xs, rs: 3 N values each) and 16 lanes per point.  Each lane sums up to
+// two of the 26 off-centre blocks, the 16 partial sums are added with
+// shuffles, and the first lane of the group applies the inverse centre block.
+// Colour order and per-point update are those of wg_gs_sweep; only the
+// summation order inside a point differs.  On exit x holds the result.
+constexpr int kBottomLanes = 16;
+template <typename T, int MODE>
+__device__ void wg_bottom_solve(const Dims &d, T *x, const T *r, const T *ops, const T *inv, int nsweeps, T *xs,
+                                T *rs) {
+    const long N = d.N;
+    const int n3 = 3 * (int)N;
+    for (int q = threadIdx.x; q < n3; q += blockDim.x) {
+        xs[q] = x[q];
+        rs[q] = r[q];
+    }
+    __syncthreads();
+    const int sl = threadIdx.x % kBottomLanes, grp = threadIdx.x / kBottomLanes;
+    const int ngrp = blockDim.x / kBottomLanes;
+    const int pxo = d.px && (d.m & 1), pyo = d.py && (d.n & 1), pzo = d.pz && (d.o & 1);
+    for (int sw = 0; sw < nsweeps; sw++)
+        for (int cz = 0; cz < 2 + pzo; cz++)
+            for (int cy = 0; cy < 2 + pyo; cy++)
+                for (int cx = 0; cx < 2 + pxo; cx++) {
+                    const int cntx = col_count(d.m, cx, d.px), cnty = col_count(d.n, cy, d.py),
+                              cntz = col_count(d.o, cz, d.pz);
+                    const int tot = cntx * cnty * cntz;
+                    for (int base = 0; base < tot; base += ngrp) {
+                        const int t = base + grp;
+                        T a0 = 0, a1 = 0, a2 = 0;
+                        long p = 0, c = 0;
+                        if (t < tot) {
+                            const int ii = t % cntx, rest = t / cntx;
+                            const int jj = rest % cnty, kk = rest / cnty;
+                            const int i = col_coord(ii, cx, d.m), j = col_coord(jj, cy, d.n), k = col_coord(kk, cz, d.o);
+                            p = i + (long)d.m * (j + (long)d.n * k);
+                            c = cls_of<MODE>(p, k);
+                            const T *A = ops + c * kBlk;
+#pragma unroll
+                            for (int h2 = 0; h2 < 2; h2++) {
+                                const int b = sl + kBottomLanes * h2;
+                                if (b >= 27 || b == 13) continue;
+                                const int ni = nb(i, b % 3 - 1, d.m, d.px), nj = nb(j, (b / 3) % 3 - 1, d.n, d.py),
+                                          nk = nb(k, b / 9 - 1, d.o, d.pz);
+                                if (ni < 0 || nj < 0 || nk < 0) continue;
+                                const long q = ni + (long)d.m * (nj + (long)d.n * nk);
+                                mv(A + 9 * b, xs[q], xs[q + N], xs[q + 2 * N], a0, a1, a2);
+                            }
+                        }
+#pragma unroll
+                        for (int off = kBottomLanes / 2; off > 0; off >>= 1) {
+                            a0 += __shfl_xor(a0, off, kBottomLanes);
+                            a1 += __shfl_xor(a1, off, kBottomLanes);
+                            a2 += __shfl_xor(a2, off, kBottomLanes);
+                        }
+                        if (t < tot && sl == 0) {
+                            const T e0 = rs[p] - a0, e1 = rs[p + N] - a1, e2 = rs[p + 2 * N] - a2;
+                            const T *B = inv + c * 9;
+                            xs[p] = B[0] * e0 + B[1] * e1 + B[2] * e2;
+                            xs[p + N] = B[3] * e0 + B[4] * e1 + B[5] * e2;
+                            xs[p + 2 * N] = B[6] * e0 + B[7] * e1 + B[8] * e2;
+                        }
+                    }
+                    __syncthreads();
+                }
+    for (int q = threadIdx.x; q < n3; q += blockDim.x) x[q] = xs[q];
+    __syncthreads();
 }
 
 template <typename T, int MODE>
@@ -120,7 +191,8 @@ __device__ void wg_convert(const A *a, B *b, long n) {
 }
 
 // Dynamic LDS: the largest staged table, (243 + 9) * o_max values of the
-// widest type used (double), given at launch.
+// widest type used (double, none for per-point tables), then x and r of the
+// bottom level (a.table_vals >= 0), given at launch.
 template <typename T, int MODE>
 __global__ __launch_bounds__(kCoarseThreads) void k_coarse_cycle(CoarseArgs<T> a) {
     extern __shared__ double coarse_lds[];
@@ -139,13 +211,24 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_cycle(CoarseArgs<T> a
         const long n3 = 3 * B.d.N;
         wg_convert<T, double>(B.r, B.r64, n3);
         for (long q = threadIdx.x; q < n3; q += blockDim.x) B.x64[q] = 0.0;
+        __syncthreads();
         const double *ops_d, *inv_d;
         stage_tables<double, MODE>(B.d, B.ops64, B.inv64, coarse_lds, &ops_d, &inv_d);
-        for (int q = 0; q < a.bottom; q++) wg_gs_sweep<double, MODE>(B.d, B.x64, B.r64, ops_d, inv_d);
+        if (a.table_vals >= 0) {
+            double *xs = coarse_lds + a.table_vals;
+            wg_bottom_solve<double, MODE>(B.d, B.x64, B.r64, ops_d, inv_d, a.bottom, xs, xs + n3);
+        } else {
+            for (int q = 0; q < a.bottom; q++) wg_gs_sweep<double, MODE>(B.d, B.x64, B.r64, ops_d, inv_d);
+        }
         wg_convert<double, T>(B.x64, B.x, n3);
     } else {
         stage_tables<T, MODE>(B.d, B.ops, B.inv, lds, &ops_s, &inv_s);
-        for (int q = 0; q < a.bottom; q++) wg_gs_sweep<T, MODE>(B.d, B.x, B.r, ops_s, inv_s);
+        if (a.table_vals >= 0) {
+            T *xs = reinterpret_cast<T *>(coarse_lds + a.table_vals);
+            wg_bottom_solve<T, MODE>(B.d, B.x, B.r, ops_s, inv_s, a.bottom, xs, xs + 3 * B.d.N);
+        } else {
+            for (int q = 0; q < a.bottom; q++) wg_gs_sweep<T, MODE>(B.d, B.x, B.r, ops_s, inv_s);
+        }
     }
     for (int i = a.nl - 1; i > 0; i--) {
         CoarseLevel<T> &F = a.L[i - 1];
@@ -155,7 +238,10 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_cycle(CoarseArgs<T> a
     }
 }
 
-// LDS bytes k_coarse_cycle needs for levels with at most omax planes.
+// LDS bytes of the staged per-plane tables for levels with at most omax
+// planes, and of the bottom fields (x and r of at most kCoarseMaxPoints
+// points, in double).
 inline size_t coarse_lds_bytes(int omax) { return sizeof(double) * (size_t)(kBlk + 9) * omax; }
+inline size_t bottom_lds_bytes(long n) { return sizeof(double) * 6 * (size_t)n; }
 
 }  // namespace stzmg

@@ -346,33 +346,50 @@ __global__ __launch_bounds__(NT) void k_resid_planes(Dims d, const T *__restrict
     const int tid = threadIdx.x;
     const int xx = tid % TX, yb = (tid / TX) * R;
 
-    auto stage = [&](int slot, int kg) {
+    // plane kg of x (zero outside a walled z range) into registers, then LDS
+    constexpr int NPER = (PSZ + NT - 1) / NT;
+    auto fetch = [&](int kg, T (&v)[NPER]) {
         bool valid = true;
         if (kg < 0 || kg >= d.o) {
             if (d.pz) kg = kg < 0 ? kg + d.o : kg - d.o;
             else valid = false;
         }
         const long base = (long)d.m * d.n * kg;
-        for (int q = tid; q < PSZ; q += NT) {
-            const int c = q / CSZ, rem = q - c * CSZ, yy = rem / WX, xq = rem - yy * WX;
-            T v = 0;
-            if (valid) {
+#pragma unroll
+        for (int u = 0; u < NPER; u++) {
+            const int q = tid + u * NT;
+            T val = 0;
+            if (valid && q < PSZ) {
+                const int c = q / CSZ, rem = q - c * CSZ, yy = rem / WX, xq = rem - yy * WX;
                 const int gi = wrap1(x0 + xq - 1, d.m), gj = wrap1(y0 + yy - 1, d.n);
-                v = x[c * d.N + base + gi + (long)d.m * gj];
+                val = x[c * d.N + base + gi + (long)d.m * gj];
             }
-            lds[slot * PSZ + q] = v;
+            v[u] = val;
+        }
+    };
+    auto store = [&](int slot, const T (&v)[NPER]) {
+#pragma unroll
+        for (int u = 0; u < NPER; u++) {
+            const int q = tid + u * NT;
+            if (q < PSZ) lds[slot * PSZ + q] = v[u];
         }
     };
 
     double q4 = 0, q2 = 0;
     int sm = 0, sc = 1, sp = 2;
+    T va[NPER];
     if (k0 < k1) {
-        stage(sm, k0 - 1);
-        stage(sc, k0);
-        stage(sp, k0 + 1);
+        fetch(k0 - 1, va);
+        store(sm, va);
+        fetch(k0, va);
+        store(sc, va);
+        fetch(k0 + 1, va);
+        store(sp, va);
     }
     __syncthreads();
     for (int k = k0; k < k1; k++) {
+        // the plane after next is in flight while this one is computed
+        if (k + 1 < k1) fetch(k + 2, va);
         const long base = (long)d.m * d.n * k;
         LaneCoef<T> cf;
         cf.load(ops + (long)k * kBlk, nullptr);
@@ -411,7 +428,7 @@ __global__ __launch_bounds__(NT) void k_resid_planes(Dims d, const T *__restrict
         }
         if (k + 1 < k1) {
             __syncthreads();
-            stage(sm, k + 2);
+            store(sm, va);
             const int t = sm;
             sm = sc;
             sc = sp;

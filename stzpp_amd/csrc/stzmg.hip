@@ -378,6 +378,25 @@ int first_coarse_level(stzmg_t h) {
     return l;
 }
 
+// Dynamic LDS limit of the coarse kernel (allowed once per process by
+// coarse_kernel_attrs, outside any stream capture).
+constexpr size_t kCoarseLdsMax = 128 * 1024;
+
+int coarse_kernel_attrs(stzmg_t h) {
+    static hipError_t done = [] {
+        hipError_t e = hipSuccess;
+        const void *fns[] = {(const void *)k_coarse_cycle<float, 0>, (const void *)k_coarse_cycle<float, 1>,
+                             (const void *)k_coarse_cycle<double, 0>, (const void *)k_coarse_cycle<double, 1>};
+        for (const void *f : fns) {
+            const hipError_t e1 = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kCoarseLdsMax);
+            if (e1 != hipSuccess) e = e1;
+        }
+        return e;
+    }();
+    HIPCHK(h, done);
+    return 0;
+}
+
 template <typename T>
 void launch_coarse(stzmg_t h, int first) {
     const int L = (int)h->lv.size();
@@ -401,14 +420,21 @@ void launch_coarse(stzmg_t h, int first) {
         c.ops64 = V.ops64;
         c.inv64 = V.inv64;
     }
-    if (h->lv[first].mode == STZMG_OP_PLANE) {
-        int omax = 0;
+    const bool plane = h->lv[first].mode == STZMG_OP_PLANE;
+    int omax = 0;
+    if (plane)
         for (int i = 0; i < a.nl; i++) omax = a.L[i].d.o > omax ? a.L[i].d.o : omax;
-        hipLaunchKernelGGL((k_coarse_cycle<T, 0>), dim3(1), dim3(kCoarseThreads), coarse_lds_bytes(omax),
-                           h->stream, a);
-    } else {
-        hipLaunchKernelGGL((k_coarse_cycle<T, 1>), dim3(1), dim3(kCoarseThreads), 0, h->stream, a);
+    size_t bytes = coarse_lds_bytes(omax);
+    const size_t fields = bottom_lds_bytes(a.L[a.nl - 1].d.N);
+    a.table_vals = -1;
+    if (bytes + fields <= kCoarseLdsMax) {
+        a.table_vals = (int)(bytes / sizeof(double));
+        bytes += fields;
     }
+    if (plane)
+        hipLaunchKernelGGL((k_coarse_cycle<T, 0>), dim3(1), dim3(kCoarseThreads), bytes, h->stream, a);
+    else
+        hipLaunchKernelGGL((k_coarse_cycle<T, 1>), dim3(1), dim3(kCoarseThreads), bytes, h->stream, a);
 }
 
 // One V-cycle (multigrid::v_cycle, multigrid.cc:253-297) in type T.  In FP32
@@ -487,6 +513,7 @@ int require_ops(stzmg_t h) {
 
 // Galerkin hierarchy and inverse tables after the fine operator changed.
 int build_hierarchy_ops(stzmg_t h) {
+    if (int rc = coarse_kernel_attrs(h)) return rc;
     HIPCHK(h, hipMemsetAsync(h->dflag, 0, sizeof(int), h->stream));
     for (size_t l = 0; l < h->lv.size(); l++) {
         Level &L = h->lv[l];
@@ -1029,7 +1056,7 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
 int stzmg_time_kernel(stzmg_t h, int kind, int level, int reps, double *ms_per_unit) {
     if (int rc = check_level(h, level)) return rc;
     if (int rc = require_ops(h)) return rc;
-    if (reps < 1 || !ms_per_unit || kind < 0 || kind > 2) return STZMG_ERR_ARG;
+    if (reps < 1 || !ms_per_unit || kind < 0 || kind > 3) return STZMG_ERR_ARG;
     Level &L = h->lv[level];
     double *home64 = L.x64;
     float *home32 = L.x32;
@@ -1039,7 +1066,8 @@ int stzmg_time_kernel(stzmg_t h, int kind, int level, int reps, double *ms_per_u
     auto body = [&]() {
         if (kind == 0) launch_gs_sweep<double>(h, L);
         else if (kind == 1) launch_resid<double, false>(h, L, L.x64, L.r64, L.t64);
-        else launch_gs_sweep<float>(h, L);
+        else if (kind == 2) launch_gs_sweep<float>(h, L);
+        else launch_resid<float, false>(h, L, L.x32, L.r32, L.t32);
     };
     body();
     HIPCHK(h, hipEventRecord(a, h->stream));
