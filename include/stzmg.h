@@ -151,9 +151,9 @@ void stzmg_solve_opts_default(stzmg_solve_opts *o);
  *                            separately. */
 enum { STZMG_OPT_COARSE_KERNEL = 0, STZMG_OPT_FUSED_SMOOTHER = 1, STZMG_OPT_TILED_RESIDUAL = 2 };
 /*   STZMG_OPT_FUSED_SMOOTHER 1 (default): per-plane levels with periodic x, y
- *                            (sizes multiples of 64 x 16) sweep with two fused
- *                            launches (one per z parity); 0: eight colour
- *                            launches per sweep.
+ *                            (x and y sizes multiples of 16, more than 1024
+ *                            points) sweep with two fused launches (one per z
+ *                            parity); 0: eight colour launches per sweep.
  *   STZMG_OPT_TILED_RESIDUAL 1 (default): on the same levels the residual
  *                            marches tiles through z with planes staged in
  *                            LDS; 0: one thread per point reading its 27

@@ -38,6 +38,7 @@ struct Level {
     float *x32b = nullptr;
     bool fused = false;   // fused two-launch sweeps on this level
     bool tiled = false;   // LDS plane-marching residual on this level
+    int ftile = 0;        // fused-smoother tile class (fused_tile)
     int pat = 0;          // operator zero pattern (PAT_*)
 };
 
@@ -265,15 +266,23 @@ double *&xref_of<double>(Level &L) { return L.x64; }
 template <>
 float *&xref_of<float>(Level &L) { return L.x32; }
 
-bool fused_ok(const Level &L) {
-    return L.mode == STZMG_OP_PLANE && L.d.px && L.d.py && L.d.o >= 4 && L.d.N > 1024 &&
-           L.d.m % FusedTile<float>::TX == 0 && L.d.n % FusedTile<float>::TY == 0 &&
-           L.d.m % FusedTile<double>::TX == 0 && L.d.n % FusedTile<double>::TY == 0;
+// Small levels (sides multiples of 16, e.g. 32^3 and 16^3) use 16 x 16
+// tiles with 128 threads: one thread per point of the widest sub-pass.
+constexpr int kSmallTile = 16, kSmallThreads = 128;
+
+// Fused-smoother tile class of a level: 0 none, 1 the large tiles, 2 the
+// 16 x 16 tiles.
+int fused_tile(const Level &L) {
+    if (!(L.mode == STZMG_OP_PLANE && L.d.px && L.d.py && L.d.o >= 4 && L.d.N > 1024)) return 0;
+    if (L.d.m % FusedTile<float>::TX == 0 && L.d.n % FusedTile<float>::TY == 0 &&
+        L.d.m % FusedTile<double>::TX == 0 && L.d.n % FusedTile<double>::TY == 0)
+        return 1;
+    if (L.d.m % kSmallTile == 0 && L.d.n % kSmallTile == 0) return 2;
+    return 0;
 }
 
-template <typename T, int PAT>
-void launch_fused_pat(stzmg_t h, Level &L) {
-    constexpr int TX = FusedTile<T>::TX, TY = FusedTile<T>::TY, NT = FusedTile<T>::NT;
+template <typename T, int PAT, int TX, int TY, int NT>
+void launch_fused_tile(stzmg_t h, Level &L) {
     const Dims &d = L.d;
     T *x = xref_of<T>(L), *xb = xb_of<T>(L);
     for (int P = 0; P < 2; P++) {
@@ -288,6 +297,14 @@ void launch_fused_pat(stzmg_t h, Level &L) {
                            P, kchunk);
     }
     std::swap(xref_of<T>(L), xb_of<T>(L));
+}
+
+template <typename T, int PAT>
+void launch_fused_pat(stzmg_t h, Level &L) {
+    if (L.ftile == 1)
+        launch_fused_tile<T, PAT, FusedTile<T>::TX, FusedTile<T>::TY, FusedTile<T>::NT>(h, L);
+    else
+        launch_fused_tile<T, PAT, kSmallTile, kSmallTile, kSmallThreads>(h, L);
 }
 
 template <typename T>
@@ -565,7 +582,8 @@ int build_hierarchy_ops(stzmg_t h) {
             }
         }
         L.pat = pat;
-        L.fused = h->fused_smoother && fused_ok(L) && L.x64b;
+        L.ftile = fused_tile(L);
+        L.fused = h->fused_smoother && L.ftile && L.x64b;
         L.tiled = h->tiled_resid && tiled_ok(L);
     }
     if (h->g64) { hipGraphExecDestroy(h->g64); h->g64 = nullptr; }
@@ -687,7 +705,7 @@ int stzmg_create(const stzmg_desc *d, stzmg_t *out) {
             alloc(&L.inv32, L.ncls * 9) || alloc(&L.x64, n3) || alloc(&L.r64, n3) || alloc(&L.t64, n3) ||
             alloc(&L.x32, n3) || alloc(&L.r32, n3) || alloc(&L.t32, n3))
             return bail(fail(h, STZMG_ERR_NOMEM, "device allocation failed at level %zu", l));
-        if (fused_ok(L) && (alloc(&L.x64b, n3) || alloc(&L.x32b, n3)))
+        if (fused_tile(L) && (alloc(&L.x64b, n3) || alloc(&L.x32b, n3)))
             return bail(fail(h, STZMG_ERR_NOMEM, "device allocation failed at level %zu", l));
         hipMemsetAsync(L.ops64, 0, sizeof(double) * L.ncls * kBlk, h->stream);
         hipMemsetAsync(L.x64, 0, sizeof(double) * n3, h->stream);

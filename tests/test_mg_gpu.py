@@ -118,12 +118,14 @@ def test_gpu_vcycle_matches_oracle(golden, case):
     assert relerr(mg.x(0), orc.fields(0)[0]) < 1e-11
 
 
-@pytest.mark.parametrize("kind,prd", [("shear", (1, 1, 0)), ("trans", (1, 1, 1)), ("transT", (1, 1, 0)),
-                                      ("rat", (1, 1, 1))])
-def test_gpu_fused_smoother_matches_oracle(kind, prd):
-    # Two-launch fused sweeps (64 x 64 planes) against the oracle's 8-colour
-    # sweep and against the eight-launch path, for each zero pattern.
-    n = 64
+@pytest.mark.parametrize("kind,prd,n", [("shear", (1, 1, 0), 64), ("trans", (1, 1, 1), 64),
+                                        ("transT", (1, 1, 0), 64), ("rat", (1, 1, 1), 64),
+                                        ("shear", (1, 1, 0), 48), ("trans", (1, 1, 1), 32),
+                                        ("rat", (1, 1, 0), 16)])
+def test_gpu_fused_smoother_matches_oracle(kind, prd, n):
+    # Two-launch fused sweeps against the oracle's 8-colour sweep and against
+    # the eight-launch path, for each zero pattern; 64 x 64 planes use the
+    # large tiles, 48, 32 and 16 the 16 x 16 tiles.
     h = 1.0 / n
     o = n + (0 if prd[2] else 1)
     T = np.array([[1.0, 0.0, 0.5], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
@@ -134,7 +136,7 @@ def test_gpu_fused_smoother_matches_oracle(kind, prd):
         st = MG.trans_stencil(np.array([[1.1, 0.2, 0.3], [0.0, 0.9, 0.1], [0.0, 0.0, 1.0]]), 1.0, 1.5, h, h, h, 1.0)
     elif kind == "rat":
         # a dense 27-block operator: the Galerkin operator of a 128^3 problem
-        big = MG.Multigrid(2 * n, 2 * n, 2 * n, (1, 1, 1))
+        big = MG.Multigrid(128, 128, 128, (1, 1, 1))
         big.setup_matrices_const(MG.trans_stencil(T, 1.0, 1.5, h / 2, h / 2, h / 2, 1.0), None)
         st = big.level_operator(1)[0]
     planes = O.plane_operator(o, prd[2], st, fix if not prd[2] else None)
