@@ -153,6 +153,21 @@ __device__ __forceinline__ void acc_blocks(const CF &cf, const LD &ld, T &a0, T 
 
 __device__ __forceinline__ int wrap1(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
+// Tile of this workgroup on a (gridDim.x, gridDim.y, gridDim.z) tiling.
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
+// b % 8) and every XCD has its own L2.  Tile number (b % 8) * (nb / 8) + b / 8
+// gives each XCD a contiguous run of tiles (x fastest, then y, then z), so
+// the halo lines neighbouring tiles share are fetched into one L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ void xcd_tile(int &tx, int &ty, int &tz) {
+    const int nb = gridDim.x * gridDim.y * gridDim.z;
+    const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const int t = nb % kXcds ? b : (b % kXcds) * (nb / kXcds) + b / kXcds;
+    tx = t % gridDim.x;
+    ty = (t / gridDim.x) % gridDim.y;
+    tz = t / (gridDim.x * gridDim.y);
+}
+
 // Geometry of in-plane sub-pass s (colour (s & 1, s >> 1), halo 3 - s) on a
 // TX x TY tile: first relative x/y of the colour and the number of points.
 template <int TX, int TY, int NT>
@@ -186,9 +201,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(STZ_FUSED_WA
     constexpr int MI = SP::max_iters;
     __shared__ T lds[3 * PSZ];
 
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    int tx, ty, tz;
+    xcd_tile(tx, ty, tz);
+    const int x0 = tx * TX, y0 = ty * TY;
     const int nP = (d.o - P + 1) / 2;
-    const int kk0 = blockIdx.z * kchunk;
+    const int kk0 = tz * kchunk;
     const int kk1 = kk0 + kchunk < nP ? kk0 + kchunk : nP;
     if (kk0 >= kk1) return;
     const int tid = threadIdx.x;
@@ -340,8 +357,10 @@ __global__ __launch_bounds__(NT) void k_resid_planes(Dims d, const T *__restrict
     constexpr int R = TX * TY / NT;  // rows per thread
     __shared__ T lds[3 * PSZ];
 
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
-    const int k0 = blockIdx.z * kchunk;
+    int tx, ty, tz;
+    xcd_tile(tx, ty, tz);
+    const int x0 = tx * TX, y0 = ty * TY;
+    const int k0 = tz * kchunk;
     const int k1 = k0 + kchunk < d.o ? k0 + kchunk : d.o;
     const int tid = threadIdx.x;
     const int xx = tid % TX, yb = (tid / TX) * R;
