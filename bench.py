@@ -29,6 +29,16 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the dominant kernel (fine-level FP32 fused
+# smoother, one z parity of the 256^3 workload), from two rocprofv3 --pmc
+# runs of `python3 bench.py --steps 1 --warmup 0 --no-cpu --no-graph`:
+# FETCH_SIZE 147.4 MiB x 2.0 + WRITE_SIZE 96.0 MiB x 1.0 per dispatch.  The
+# factors come from the plain copy kernel k_convert<double,double> of the
+# same run (3 x 256^3 doubles = 384 MiB each way), which reads FETCH_SIZE
+# 192.0 MiB and WRITE_SIZE 384.0 MiB.  Summary:
+# profiles/mg_solve_256_trans_kernels.txt.
+TRAFFIC_256_BYTES_PER_LAUNCH = (147.4 * 2.0 + 96.0) * 2 ** 20
+
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
@@ -146,11 +156,15 @@ def main():
     d64 = x64 - x64.mean(axis=0)
     err = float(np.linalg.norm(d32 - d64) / np.linalg.norm(d64))
 
-    # Roofline of the dominant kernel: one FP32 Gauss-Seidel sweep of the
-    # fine level, algorithmic bytes 36 per point (read x, read r, write x),
-    # timed with HIP events on the solver's stream.
+    # Roofline of the dominant kernel, the fine-level FP32 fused smoother:
+    # one sweep is two launches (one per z parity), each updating N/2 points
+    # at 36 algorithmic bytes per point (read x, read r, write x, FP32 vec3),
+    # so 18 N bytes per launch.  The sweep is timed with HIP events on the
+    # solver's stream over 20 back-to-back sweeps.
     sweep_s = mg.time_kernel(2, 0, 20) * 1e-3
-    achieved = 36.0 * N / sweep_s / 1e9
+    launch_s = sweep_s / 2
+    alg_launch = 18.0 * N
+    achieved = alg_launch / launch_s / 1e9
     sweep64_s = mg.time_kernel(0, 0, 10) * 1e-3
     achieved64 = 72.0 * N / sweep64_s / 1e9
 
@@ -202,7 +216,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": TRAFFIC_256_BYTES_PER_LAUNCH if n == 256 else None,
+            "kernel": "k_gs_planes<float, PAT_TRANS_XZ, 64, 16, 256> (fine-level fused GS, one z parity)",
+            "alg_bytes_per_launch": alg_launch,
+            "launch_us": launch_s * 1e6,
         },
         "cpu_baseline": cpu,
     }
