@@ -344,6 +344,15 @@ def ref_lib():
             ("ref_shear_get_state", [vp, _dp, _dp, _dp, _dp, _dp], None),
             ("ref_shear_vcycles", [vp], ctypes.c_int),
             ("ref_shear_destroy", [vp], None),
+            ("ref_trans_create", [ctypes.c_int] * 3 + [_dp, _dp, ctypes.c_double, _ip, ctypes.c_char_p], vp),
+            ("ref_trans_setup", [vp, ctypes.c_double], None),
+            ("ref_trans_get_stencil", [vp, _dp, _dp], None),
+            ("ref_trans_matrices", [vp, _dp, _dp, _dp], None),
+            ("ref_trans_set_state", [vp, ctypes.c_double, _dp, _dp, _dp], None),
+            ("ref_trans_step", [vp, ctypes.c_double, ctypes.c_int], None),
+            ("ref_trans_get_state", [vp, _dp, _dp, _dp, _dp, _dp], None),
+            ("ref_trans_vcycles", [vp], ctypes.c_int),
+            ("ref_trans_destroy", [vp], None),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -458,8 +467,33 @@ def _step_lib():
         L.sto_vcycles.argtypes = [vp]
         L.sto_vcycles.restype = ctypes.c_int
         L.sto_set_solver.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.sto_set_transform.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.sto_set_textbook.argtypes = [vp, ctypes.c_int]
+        L.sto_trans_stencil.argtypes = [_dp] + [ctypes.c_double] * 5 + [_dp]
+        L.sto_transform_at.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       _dp, _dp, _dp]
         L._sto_bound = True
     return L
+
+
+TRANSFORMS = {"none": 0, "pure_shear": 1, "simple_shear": 2, "homogeneous": 3, "identity": 4}
+
+
+def transform_at(family, Ut, bz, t):
+    """(T, T^-1, dT/dt) of a transform family (src/trans_sim_3d.hh:57-125)."""
+    L = _step_lib()
+    T, Ti, dT = np.zeros(9), np.zeros(9), np.zeros(9)
+    L.sto_transform_at(TRANSFORMS[family], Ut, bz, t, _d(T), _d(Ti), _d(dT))
+    return T.reshape(3, 3), Ti.reshape(3, 3), dT.reshape(3, 3)
+
+
+def trans_stencil_c(T, mu, lamb, dx, dy, dz):
+    """The C oracle's closed form of src/stenc_funcs.hh (27 x 3 x 3)."""
+    L = _step_lib()
+    Ta = np.ascontiguousarray(T, dtype=np.float64).reshape(9)
+    out = np.zeros(243)
+    L.sto_trans_stencil(_d(Ta), mu, lamb, dx, dy, dz, _d(out))
+    return out.reshape(27, 3, 3)
 
 
 STEP_PARAM_ORDER = ("mu", "lamb", "sy", "tau0", "c0", "eps0", "delta", "omega", "bath_temp",
@@ -508,6 +542,11 @@ class OracleShear:
         """smoother 0 = lexicographic (reference), 1 = 8-colour; rtol 0 = the
         reference stopping rule, else a relative-residual solve."""
         self.L.sto_set_solver(self.h, smoother, rtol)
+
+    def set_transform(self, family, Ut):
+        """Run the transformed-coordinate step of trans_sim_3d with transform
+        family ``family`` (TRANSFORMS) at rate Ut."""
+        self.L.sto_set_transform(self.h, TRANSFORMS[family] if isinstance(family, str) else family, Ut)
 
     @property
     def vcycles(self):
@@ -564,3 +603,60 @@ class RefShear:
     @property
     def vcycles(self):
         return self.L.ref_shear_vcycles(self.h)
+
+
+class RefTrans:
+    """The reference trans_sim_3d (pure-shear transform at rate Ut) driven
+    through oracle/_ref."""
+
+    def __init__(self, N, params, Ut, outdir, box=(-1, 1, -1, 1, -0.5, 0.5), prd=(1, 1, 0)):
+        self.L = ref_lib()
+        if self.L is None:
+            raise RuntimeError("reference library not built")
+        self.N, self.prd = tuple(N), tuple(prd)
+        p = np.array([params[k] for k in ("mu", "lamb", "sy", "tau0", "c0", "eps0", "delta", "omega",
+                                          "bath_temp", "chi_inf", "ez")] + [21000.0, params["diff_l"],
+                                                                               params["U"]], dtype=np.float64)
+        b = np.array(box, dtype=np.float64)
+        pr = np.array(prd, dtype=np.int32)
+        self.h = self.L.ref_trans_create(N[0], N[1], N[2], _d(b), _d(p), Ut, _i(pr), outdir.encode())
+
+    def close(self):
+        if self.h:
+            self.L.ref_trans_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def setup(self, dt):
+        self.L.ref_trans_setup(self.h, dt)
+
+    def stencil(self):
+        b, f = np.zeros((27, 9)), np.zeros(9)
+        self.L.ref_trans_get_stencil(self.h, _d(b), _d(f))
+        return b.reshape(27, 3, 3), f.reshape(3, 3)
+
+    def matrices(self):
+        T, Ti, dT = np.zeros(9), np.zeros(9), np.zeros(9)
+        self.L.ref_trans_matrices(self.h, _d(T), _d(Ti), _d(dT))
+        return T.reshape(3, 3), Ti.reshape(3, 3), dT.reshape(3, 3)
+
+    def set_state(self, t, u, sig, chi):
+        u, sig, chi = (np.ascontiguousarray(a, dtype=np.float64) for a in (u, sig, chi))
+        self.L.ref_trans_set_state(self.h, t, _d(u), _d(sig), _d(chi))
+
+    def step(self, dt, n=1):
+        self.L.ref_trans_step(self.h, dt, n)
+
+    def state(self):
+        Nx, Ny, Nz = self.N
+        nv = Nx * Ny * _nzv(self.N, self.prd)
+        nc = Nx * Ny * Nz
+        t = np.zeros(1)
+        u, sig, chi, dpl = np.zeros((nv, 3)), np.zeros((nc, 6)), np.zeros(nc), np.zeros(nc)
+        self.L.ref_trans_get_state(self.h, _d(t), _d(u), _d(sig), _d(chi), _d(dpl))
+        return dict(t=float(t[0]), u=u, sig=sig, chi=chi, dpl=dpl)
+
+    @property
+    def vcycles(self):
+        return self.L.ref_trans_vcycles(self.h)

@@ -339,3 +339,146 @@ void ref_shear_destroy(void *hp) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Transformed-coordinate harness: drives trans_sim_3d's QS step (its
+// compiled-in transform is pure shear, src/trans_sim_3d.hh:76-94).
+namespace {
+class ref_trans : public trans_sim_3d {
+   public:
+    using trans_sim_3d::trans_sim_3d;
+    int nx() const { return N_x; }
+    int ny() const { return N_y; }
+    int nz() const { return N_z; }
+    int gx() const { return nxg; }
+    int gxy() const { return nxyg; }
+    void boundary() { set_up_ghost_regions(); set_boundary_conditions(); }
+    const mat3 *stencil_ptr() const { return stencil; }
+    const mat3 *fix_ptr() const { return fix; }
+    void matrices(double *T9, double *Ti9, double *dT9) {
+        update_matrices();
+        const mat3 *m[3] = {&T, &Tinv, &dTdt};
+        double *o[3] = {T9, Ti9, dT9};
+        for (int q = 0; q < 3; q++) {
+            o[q][0] = m[q]->a11; o[q][1] = m[q]->a12; o[q][2] = m[q]->a13;
+            o[q][3] = m[q]->a21; o[q][4] = m[q]->a22; o[q][5] = m[q]->a23;
+            o[q][6] = m[q]->a31; o[q][7] = m[q]->a32; o[q][8] = m[q]->a33;
+        }
+    }
+};
+
+struct trans_handle {
+    MPI_Comm cart;
+    int dims[3];
+    int periods[3];
+    ref_trans *sim;
+};
+
+template <typename SIM>
+void put_state(SIM *s, const int *periods, double t, const double *u, const double *sig, const double *chi) {
+    int Nx = s->nx(), Ny = s->ny(), Nz = s->nz(), gx = s->gx(), gxy = s->gxy();
+    int nzv = Nz + (periods[2] ? 0 : 1);
+    s->sim_time = t;
+    for (int k = 0; k < nzv; k++)
+        for (int j = 0; j < Ny; j++)
+            for (int i = 0; i < Nx; i++) {
+                Field &f = s->grido[i + gx * j + gxy * k];
+                long q = i + (long)Nx * (j + (long)Ny * k);
+                f.u = u[3 * q]; f.v = u[3 * q + 1]; f.w = u[3 * q + 2];
+                if (k < Nz) {
+                    f.s11 = sig[6 * q]; f.s12 = sig[6 * q + 1]; f.s13 = sig[6 * q + 2];
+                    f.s22 = sig[6 * q + 3]; f.s23 = sig[6 * q + 4]; f.s33 = sig[6 * q + 5];
+                    f.chi = chi[q];
+                    f.cu = f.cv = f.cw = f.cs11 = f.cs12 = f.cs13 = f.cs22 = f.cs23 = f.cs33 = 0;
+                    f.cchi = f.cxi_x = f.cxi_y = f.cxi_z = 0;
+                    f.ad_Dpl = 0;
+                    f.sbar = 0;
+                }
+            }
+    s->boundary();
+}
+
+template <typename SIM>
+void take_state(SIM *s, const int *periods, double *t, double *u, double *sig, double *chi, double *dpl) {
+    int Nx = s->nx(), Ny = s->ny(), Nz = s->nz(), gx = s->gx(), gxy = s->gxy();
+    int nzv = Nz + (periods[2] ? 0 : 1);
+    *t = s->sim_time;
+    for (int k = 0; k < nzv; k++)
+        for (int j = 0; j < Ny; j++)
+            for (int i = 0; i < Nx; i++) {
+                const Field &f = s->grido[i + gx * j + gxy * k];
+                long q = i + (long)Nx * (j + (long)Ny * k);
+                u[3 * q] = f.u; u[3 * q + 1] = f.v; u[3 * q + 2] = f.w;
+                if (k < Nz) {
+                    sig[6 * q] = f.s11; sig[6 * q + 1] = f.s12; sig[6 * q + 2] = f.s13;
+                    sig[6 * q + 3] = f.s22; sig[6 * q + 4] = f.s23; sig[6 * q + 5] = f.s33;
+                    chi[q] = f.chi;
+                    dpl[q] = f.ad_Dpl;
+                }
+            }
+}
+}  // namespace
+
+extern "C" {
+
+// p as ref_shear_create; Ut: rate of the pure-shear transform.
+void *ref_trans_create(int Nx, int Ny, int Nz, const double *box, const double *p, double Ut,
+                       const int *periods, const char *outdir) {
+    ensure_mpi();
+    trans_handle *h = new trans_handle;
+    h->dims[0] = h->dims[1] = h->dims[2] = 1;
+    for (int a = 0; a < 3; a++) h->periods[a] = periods[a];
+    MPI_Cart_create(MPI_COMM_WORLD, 3, h->dims, h->periods, 1, &h->cart);
+    h->sim = new ref_trans(Nx, Ny, Nz, box[0], box[2], box[4], box[1], box[3], box[5], 0.0, 1.0, 0.5, 1,
+                           std::string(outdir), p[0], p[1], p[0], 0.0, p[2], p[3], p[4], p[5], p[6], p[7],
+                           p[8], p[9], p[10], p[11], p[12], Ut, p[13], 5, 3, 0.09, 0.01, 1.0, &h->cart, h->dims,
+                           15, h->periods);
+    return h;
+}
+
+void ref_trans_setup(void *hp, double dt) { ((trans_handle *)hp)->sim->set_up_stencil_and_matrices(dt); }
+
+void ref_trans_get_stencil(void *hp, double *blocks27x9, double *fix9) {
+    trans_handle *h = (trans_handle *)hp;
+    const mat3 *s = h->sim->stencil_ptr();
+    for (int b = 0; b < 27; b++) {
+        double *o = blocks27x9 + 9 * b;
+        o[0] = s[b].a11; o[1] = s[b].a12; o[2] = s[b].a13;
+        o[3] = s[b].a21; o[4] = s[b].a22; o[5] = s[b].a23;
+        o[6] = s[b].a31; o[7] = s[b].a32; o[8] = s[b].a33;
+    }
+    const mat3 *f = h->sim->fix_ptr();
+    fix9[0] = f->a11; fix9[1] = f->a12; fix9[2] = f->a13;
+    fix9[3] = f->a21; fix9[4] = f->a22; fix9[5] = f->a23;
+    fix9[6] = f->a31; fix9[7] = f->a32; fix9[8] = f->a33;
+}
+
+void ref_trans_matrices(void *hp, double *T, double *Ti, double *dT) {
+    ((trans_handle *)hp)->sim->matrices(T, Ti, dT);
+}
+
+void ref_trans_set_state(void *hp, double t, const double *u, const double *sig, const double *chi) {
+    trans_handle *h = (trans_handle *)hp;
+    put_state(h->sim, h->periods, t, u, sig, chi);
+}
+
+void ref_trans_step(void *hp, double dt, int nsteps) {
+    trans_handle *h = (trans_handle *)hp;
+    for (int q = 0; q < nsteps; q++) h->sim->step_forward_qs(dt);
+}
+
+void ref_trans_get_state(void *hp, double *t, double *u, double *sig, double *chi, double *dpl) {
+    trans_handle *h = (trans_handle *)hp;
+    take_state(h->sim, h->periods, t, u, sig, chi, dpl);
+}
+
+int ref_trans_vcycles(void *hp) { return ((trans_handle *)hp)->sim->n_vcycles; }
+
+void ref_trans_destroy(void *hp) {
+    trans_handle *h = (trans_handle *)hp;
+    delete h->sim;
+    MPI_Comm_free(&h->cart);
+    delete h;
+}
+
+}  // extern "C"

@@ -48,6 +48,12 @@ typedef struct {
     int mg_ready;
     int smoother;          /* 0 = lexicographic (reference), 1 = 8-colour (GPU order) */
     double rtol;           /* 0 = reference stopping rule, else ||r||_2 <= rtol ||r_0||_2 */
+    /* transformed-coordinate step (trans_sim_3d); tfam 0 = shear_sim_3d */
+    int tfam;
+    double Ut;
+    double ax, ay, az, bx, by, bz;
+    double T[9], Ti[9], dT[9];
+    int textbook;          /* test switch: 1 = textbook x-ENO for chi instead of :985 */
 } sto_sim;
 
 static long gi(const sto_sim *S, int i, int j, int k) {
@@ -63,6 +69,7 @@ void *sto_new(int Nx, int Ny, int Nz, const int *prd, const double *box, const d
     S->dx = (box[1] - box[0]) / Nx;
     S->dy = (box[3] - box[2]) / Ny;
     S->dz = (box[5] - box[4]) / Nz;
+    S->ax = box[0]; S->bx = box[1]; S->ay = box[2]; S->by = box[3]; S->az = box[4]; S->bz = box[5];
     S->mu = p[0]; S->lamb = p[1]; S->sy = p[2]; S->tau0 = p[3]; S->c0 = p[4]; S->eps0 = p[5];
     S->delta = p[6]; S->omega = p[7]; S->bath_temp = p[8]; S->chi_inf = p[9]; S->ez = p[10];
     S->diff_l = p[11]; S->U = p[12];
@@ -111,7 +118,10 @@ static void extrap(sto_sim *S, long d, long a, long b) {
     for (int q = F_CHI + 1; q < F_NF; q++) S->f[q][d] = 0.0;
 }
 
-static double top_u(const sto_sim *S) { return S->t < 1 ? S->U * S->t : S->U; }
+/* Plate speed: ramped U t for t < 1 (src/shear_sim_3d.hh:375); the
+ * transformed simulation holds its walls at rest (src/trans_sim_3d.cpp:
+ * 1122-1184). */
+static double top_u(const sto_sim *S) { return S->tfam ? 0.0 : (S->t < 1 ? S->U * S->t : S->U); }
 
 static void boundary(sto_sim *S) {
     if (S->prd[2]) return;
@@ -166,11 +176,12 @@ static double chi_F(const sto_sim *S, double dpl, double sb, double chi) {
     return 2 * dpl * sb * (S->chi_inf - chi) / (S->sy * S->c0);
 }
 
-/* Returns 1 - sbar_end/sbar_0 and adds the plastic chi change into cchi. */
-static double adaptive(sto_sim *S, long c, double dt) {
+/* Returns 1 - sbar_end/sbar_0 and adds the plastic chi change into cchi;
+ * sb0 is the starting sbar (physical stress in the transformed case). */
+static double adaptive(sto_sim *S, long c, double dt, double sb0) {
     double tr = dt, ts;
     int keep = 1;
-    double sb0 = dev(S, c), chi0 = S->f[F_CHI][c];
+    double chi0 = S->f[F_CHI][c];
     double sb = sb0, chi = chi0;
     while (keep) {
         double dpl = calc_dpl(S, sb, chi);
@@ -234,7 +245,7 @@ static void advection_cell(sto_sim *S, long c, double dt) {
     double dw_dx = g[2][0], dw_dy = g[2][1], dw_dz = g[2][2];
     double sb = dev(S, c);
     S->f[F_SBAR][c] = sb;
-    double adapt = sb > S->sy ? adaptive(S, c, dt) : 0;
+    double adapt = sb > S->sy ? adaptive(S, c, dt, sb) : 0;
     S->f[F_DPL][c] = adapt / 2. / S->mu * sb;
     double vel[3];
     for (int q = 0; q < 3; q++) {
@@ -268,6 +279,255 @@ static void advection_cell(sto_sim *S, long c, double dt) {
     S->f[F_CCHI][c] += chi_diffusion(S, c) / S->c0;
 }
 
+/* ---- transformed-coordinate step (trans_sim_3d) --------------------------
+ *
+ *   transform families       src/trans_sim_3d.hh:57-125 (pure shear is the
+ *                            one compiled in; simple shear, homogeneous and
+ *                            identity are the commented alternatives)
+ *   compute_L                src/trans_sim_3d.cpp:327-401
+ *   advection_grid_step      :880-992 (L term, T term, ENO, plasticity on
+ *                            the physical stress T s' T^t, chi update with
+ *                            the x-ENO quirk of :985)
+ *   adaptive_plastic_term    :740-775 (eta = 0.002)
+ *   chi_diffusion            :779-812 (metric J = T^-1 T^-t; the mixed
+ *                            y-z difference of :802 whose last two terms
+ *                            cancel)
+ *   projection_step          :995-1119 (T div sigma' right-hand side,
+ *                            r_field src/trans_sim_3d.hh:202-248; norm only
+ *                            once num_iters >= std_iters/16; stress update
+ *                            dt T^-1 C:D(L) T^-t)
+ *   set_up_stencil_and_matrices :1187-1489 (stencil from T every step)
+ *   set_boundary_conditions  :1122-1184 (walls at rest)
+ */
+enum { STO_T_NONE = 0, STO_T_PURE_SHEAR = 1, STO_T_SIMPLE_SHEAR = 2, STO_T_HOMOGENEOUS = 3, STO_T_IDENTITY = 4 };
+
+static void m3set(double *a, double a11, double a12, double a13, double a21, double a22, double a23,
+                  double a31, double a32, double a33) {
+    a[0] = a11; a[1] = a12; a[2] = a13; a[3] = a21; a[4] = a22; a[5] = a23; a[6] = a31; a[7] = a32; a[8] = a33;
+}
+
+/* T, T^-1, dT/dt of a transform family at time t (closed forms, as the
+ * reference evaluates them) */
+void sto_transform_at(int fam, double Ut, double bz, double t, double *T, double *Ti, double *dT) {
+    if (fam == STO_T_PURE_SHEAR) {
+        m3set(T, exp(Ut * t), 0, 0, 0, 1, 0, 0, 0, exp(-Ut * t));
+        m3set(Ti, exp(-Ut * t), 0, 0, 0, 1, 0, 0, 0, exp(Ut * t));
+        m3set(dT, Ut * exp(Ut * t), 0, 0, 0, 0, 0, 0, 0, -Ut * exp(-Ut * t));
+    } else if (fam == STO_T_SIMPLE_SHEAR) {
+        m3set(T, 1, 0, Ut * t / bz, 0, 1, 0, 0, 0, 1);
+        m3set(Ti, 1, 0, -Ut * t / bz, 0, 1, 0, 0, 0, 1);
+        m3set(dT, 0, 0, Ut / bz, 0, 0, 0, 0, 0, 0);
+    } else if (fam == STO_T_HOMOGENEOUS) {
+        m3set(T, exp(Ut * t), 0, 0, 0, exp(Ut * t), 0, 0, 0, exp(Ut * t));
+        m3set(Ti, exp(-Ut * t), 0, 0, 0, exp(-Ut * t), 0, 0, 0, exp(-Ut * t));
+        m3set(dT, Ut * exp(Ut * t), 0, 0, 0, Ut * exp(Ut * t), 0, 0, 0, Ut * exp(Ut * t));
+    } else {
+        m3set(T, 1, 0, 0, 0, 1, 0, 0, 0, 1);
+        m3set(Ti, 1, 0, 0, 0, 1, 0, 0, 0, 1);
+        m3set(dT, 0, 0, 0, 0, 0, 0, 0, 0, 0);
+    }
+}
+
+static void update_matrices(sto_sim *S) { sto_transform_at(S->tfam, S->Ut, S->bz, S->t, S->T, S->Ti, S->dT); }
+
+/* mat3 algebra with the reference's summation order (src/mat3.hh:139-187) */
+static void m3mul(const double *a, const double *b, double *o) {
+    double r[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) r[3 * i + j] = a[3 * i] * b[j] + a[3 * i + 1] * b[3 + j] + a[3 * i + 2] * b[6 + j];
+    memcpy(o, r, sizeof(r));
+}
+static void m3t(const double *a, double *o) {
+    double r[9] = {a[0], a[3], a[6], a[1], a[4], a[7], a[2], a[5], a[8]};
+    memcpy(o, r, sizeof(r));
+}
+static void m3vec(const double *a, const double *v, double *o) {
+    double r[3];
+    for (int i = 0; i < 3; i++) r[i] = a[3 * i] * v[0] + a[3 * i + 1] * v[1] + a[3 * i + 2] * v[2];
+    memcpy(o, r, sizeof(r));
+}
+
+/* L = T^-t grad_X(dT/dt X + T U) at the cell whose lowest vertex is c and
+ * whose corner 0 sits at P0 (compute_L, src/trans_sim_3d.cpp:327-401). */
+static void compute_L(const sto_sim *S, long c, const double P0[3], double L[9]) {
+    static const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                  {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    double pn[3] = {(S->prd[0] && P0[0] == S->bx) ? S->ax : P0[0] + S->dx,
+                    (S->prd[1] && P0[1] == S->by) ? S->ay : P0[1] + S->dy,
+                    (S->prd[2] && P0[2] == S->bz) ? S->az : P0[2] + S->dz};
+    double v[8][3];
+    for (int n = 0; n < 8; n++) {
+        double P[3], Vl[3], a[3], b[3];
+        long q = c + off[n][0] + S->nxg * off[n][1] + S->nxyg * off[n][2];
+        for (int d = 0; d < 3; d++) P[d] = off[n][d] ? pn[d] : P0[d];
+        Vl[0] = S->f[F_U][q]; Vl[1] = S->f[F_V][q]; Vl[2] = S->f[F_W][q];
+        m3vec(S->dT, P, a);
+        m3vec(S->T, Vl, b);
+        for (int d = 0; d < 3; d++) v[n][d] = a[d] + b[d];
+    }
+    double g[9], ix = .25 * (1. / S->dx), iy = .25 * (1. / S->dy), iz = .25 * (1. / S->dz);
+    for (int d = 0; d < 3; d++) {
+        g[d] = ix * (((((((v[1][d] - v[0][d]) + v[2][d]) - v[3][d]) + v[5][d]) - v[4][d]) + v[6][d]) - v[7][d]);
+        g[3 + d] = iy * (((((((v[3][d] - v[0][d]) + v[2][d]) - v[1][d]) + v[7][d]) - v[4][d]) + v[6][d]) - v[5][d]);
+        g[6 + d] = iz * (((((((v[4][d] - v[0][d]) + v[5][d]) - v[1][d]) + v[7][d]) - v[3][d]) + v[6][d]) - v[2][d]);
+    }
+    double Tti[9];
+    m3t(S->Ti, Tti);
+    m3mul(Tti, g, L);
+}
+
+/* sbar of a physical stress (calc_devsq, src/trans_sim_3d.hh:151-165) */
+static double sbar_phys(const double *o) {
+    double p = -1. / 3. * (o[0] + o[4] + o[8]);
+    double t11 = o[0] + p, t22 = o[4] + p, t33 = o[8] + p;
+    return sqrt(0.5 * (t11 * t11 + t22 * t22 + t33 * t33) + o[1] * o[1] + o[2] * o[2] + o[5] * o[5]);
+}
+
+static double chi_diffusion_trans(const sto_sim *S, long c) {
+    long X = 1, Y = S->nxg, Z = S->nxyg;
+    const double *D = S->f[F_DPL] + c, *x = S->f[F_CHI] + c;
+    double ix = 1. / S->dx, iy = 1. / S->dy, iz = 1. / S->dz;
+    double dDx = .5 * ix * (D[X] - D[-X]), dDy = .5 * iy * (D[Y] - D[-Y]), dDz = .5 * iz * (D[Z] - D[-Z]);
+    double gc[3] = {.5 * ix * (x[X] - x[-X]), .5 * iy * (x[Y] - x[-Y]), .5 * iz * (x[Z] - x[-Z])};
+    double J[9], Tti[9], Jg[3];
+    m3t(S->Ti, Tti);
+    m3mul(S->Ti, Tti, J);
+    m3vec(J, gc, Jg);
+    double term1 = dDx * Jg[0] + dDy * Jg[1] + dDz * Jg[2];
+    double xx = ix * ix * (x[X] - 2 * x[0] + x[-X]);
+    double yy = iy * iy * (x[Y] - 2 * x[0] + x[-Y]);
+    double zz = iz * iz * (x[Z] - 2 * x[0] + x[-Z]);
+    double xy = .25 * ix * iy * (x[X + Y] + x[-X - Y] - x[X - Y] - x[-X + Y]);
+    double xz = .25 * ix * iz * (x[X + Z] + x[-X - Z] - x[X - Z] - x[-X + Z]);
+    /* src/trans_sim_3d.cpp:802: the last two terms are the same cell */
+    double yz = .25 * iy * iz * (x[Y + Z] + x[-Y - Z] - x[Y - Z] + x[-Z + Y]);
+    double term2 = 0;
+    term2 += J[0] * xx + J[1] * xy + J[2] * xz;
+    term2 += J[3] * xy + J[4] * yy + J[5] * yz;
+    term2 += J[6] * xz + J[7] * yz + J[8] * zz;
+    term2 *= D[0];
+    return S->diff_l * S->diff_l * (term1 + term2);
+}
+
+static void advection_cell_trans(sto_sim *S, long c, const double P0[3], double dt) {
+    long X = 1, Y = S->nxg, Z = S->nxyg;
+    double hx = S->dx / dt, hy = S->dy / dt, hz = S->dz / dt;
+    double L[9];
+    compute_L(S, c, P0, L);
+    double vel[3];
+    for (int q = 0; q < 3; q++) {
+        const double *u = S->f[F_U + q] + c;
+        vel[q] = 0.125 * (u[0] + u[X] + u[Z] + u[X + Z] + u[Y] + u[X + Y] + u[Y + Z] + u[X + Y + Z]);
+    }
+    double s11 = S->f[F_S11][c], s12 = S->f[F_S12][c], s13 = S->f[F_S13][c];
+    double s22 = S->f[F_S22][c], s23 = S->f[F_S23][c], s33 = S->f[F_S33][c];
+    double sp[9] = {s11, s12, s13, s12, s22, s23, s13, s23, s33};
+    double os[9], Tt[9], Tti[9], tmp[9], tmp2[9];
+    m3t(S->T, Tt);
+    m3t(S->Ti, Tti);
+    m3mul(S->T, sp, tmp);
+    m3mul(tmp, Tt, os);
+    /* L term: T^-1 (-tr(L) sigma + sigma L + L^t sigma) T^-t */
+    double trL = L[0] + L[4] + L[8], Lt[9], A[9], B[9], Lterm[9];
+    m3t(L, Lt);
+    m3mul(os, L, A);
+    m3mul(Lt, os, B);
+    for (int e = 0; e < 9; e++) tmp[e] = (os[e] * -(trL) + A[e]) + B[e];
+    m3mul(S->Ti, tmp, tmp2);
+    m3mul(tmp2, Tti, Lterm);
+    /* T term: -(T^-1 dT sigma' + sigma' (T^-1 dT)^t) */
+    double TdT[9], TdTt[9], Tterm[9];
+    m3mul(S->Ti, S->dT, TdT);
+    m3t(TdT, TdTt);
+    m3mul(TdT, sp, A);
+    m3mul(sp, TdTt, B);
+    for (int e = 0; e < 9; e++) Tterm[e] = (A[e] + B[e]) * -1.;
+    static const int comp[6] = {0, 1, 2, 4, 5, 8};
+    for (int q = 0; q < 6; q++) {
+        const double *f = S->f[F_S11 + q] + c;
+        double ddx = eno2(hx, vel[0], f[2 * X], f[X], f[0], f[-X], f[-2 * X]);
+        double ddy = eno2(hy, vel[1], f[2 * Y], f[Y], f[0], f[-Y], f[-2 * Y]);
+        double ddz = eno2(hz, vel[2], f[2 * Z], f[Z], f[0], f[-Z], f[-2 * Z]);
+        double ug = vel[0] * ddx + vel[1] * ddy + vel[2] * ddz;
+        int e = comp[q];
+        S->f[F_CS11 + q][c] += dt * (Lterm[e] + Tterm[e]) - ug;
+    }
+    double sb = sbar_phys(os);
+    S->f[F_SBAR][c] = sb;
+    if (sb > S->sy) {
+        double adapt = adaptive(S, c, dt, sb);
+        S->f[F_DPL][c] = .5 * adapt / S->mu * sb;
+        double third = 1. / 3., tr = os[0] + os[4] + os[8];
+        double dv[9] = {os[0] - third * tr, os[1], os[2], os[1], os[4] - third * tr, os[5], os[2], os[5],
+                        os[8] - third * tr};
+        double Ta[9], P[9];
+        for (int e = 0; e < 9; e++) Ta[e] = S->Ti[e] * adapt;
+        m3mul(Ta, dv, tmp);
+        m3mul(tmp, Tti, P);
+        for (int q = 0; q < 6; q++) S->f[F_CS11 + q][c] -= P[comp[q]];
+    } else {
+        S->f[F_DPL][c] = 0;
+    }
+    const double *x = S->f[F_CHI] + c;
+    /* src/trans_sim_3d.cpp:985 passes me[1].chi as f(+2) */
+    double cx = eno2(hx, vel[0], S->textbook ? x[2 * X] : x[X], x[X], x[0], x[-X], x[-2 * X]);
+    double cy = eno2(hy, vel[1], x[2 * Y], x[Y], x[0], x[-Y], x[-2 * Y]);
+    double cz = eno2(hz, vel[2], x[2 * Z], x[Z], x[0], x[-Z], x[-2 * Z]);
+    S->f[F_CCHI][c] -= vel[0] * cx + vel[1] * cy + vel[2] * cz;
+    S->f[F_CCHI][c] += chi_diffusion_trans(S, c) / S->c0;
+}
+
+/* Transformed elastic operator of src/stenc_funcs.hh (the 171 generated
+ * functions) as one closed form: with S the symmetric matrix built from the
+ * upper triangle of T and W = adj(S)/det(T), the coefficient of d_a d_b U_m
+ * in row i is mu S_im (W W^t)_ab + (lamb + mu) W_ai delta_bm; axis blocks
+ * take it over h_a^2, edge blocks over +-4 h_a h_b, the centre -2 sum_a of
+ * the axis terms.  out: 27 blocks x 9 (row-major), block 13+di+3dj+9dk. */
+void sto_trans_stencil(const double *T, double mu, double lamb, double dx, double dy, double dz, double *out) {
+    double Sm[3][3], adj[3][3], W[3][3], WW[3][3], h[3] = {dx, dy, dz};
+    for (int i = 0; i < 3; i++)
+        for (int m = 0; m < 3; m++) Sm[i][m] = i <= m ? T[3 * i + m] : T[3 * m + i];
+    double D = T[0] * (T[4] * T[8] - T[5] * T[7]) + T[3] * (T[2] * T[7] - T[1] * T[8]) +
+               T[6] * (T[1] * T[5] - T[2] * T[4]);
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            int r0 = b == 0 ? 1 : 0, r1 = b == 2 ? 1 : 2, c0 = a == 0 ? 1 : 0, c1 = a == 2 ? 1 : 2;
+            double mnr = Sm[r0][c0] * Sm[r1][c1] - Sm[r0][c1] * Sm[r1][c0];
+            adj[a][b] = ((a + b) & 1) ? -mnr : mnr;
+            W[a][b] = adj[a][b] / D;
+        }
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) WW[a][b] = W[a][0] * W[b][0] + W[a][1] * W[b][1] + W[a][2] * W[b][2];
+    double K[3][3][3][3];
+    for (int i = 0; i < 3; i++)
+        for (int m = 0; m < 3; m++)
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++)
+                    K[i][m][a][b] = mu * Sm[i][m] * WW[a][b] + (lamb + mu) * W[a][i] * (b == m ? 1.0 : 0.0);
+    memset(out, 0, sizeof(double) * 243);
+    for (int a = 0; a < 3; a++) {
+        int st = a == 0 ? 1 : (a == 1 ? 3 : 9);
+        for (int i = 0; i < 3; i++)
+            for (int m = 0; m < 3; m++) {
+                double v = K[i][m][a][a] / (h[a] * h[a]);
+                out[9 * (13 + st) + 3 * i + m] += v;
+                out[9 * (13 - st) + 3 * i + m] += v;
+                out[9 * 13 + 3 * i + m] -= 2 * v;
+            }
+    }
+    for (int a = 0; a < 3; a++)
+        for (int b = a + 1; b < 3; b++) {
+            int sta = a == 0 ? 1 : (a == 1 ? 3 : 9), stb = b == 0 ? 1 : (b == 1 ? 3 : 9);
+            for (int sa = -1; sa <= 1; sa += 2)
+                for (int sb = -1; sb <= 1; sb += 2)
+                    for (int i = 0; i < 3; i++)
+                        for (int m = 0; m < 3; m++) {
+                            double v = (K[i][m][a][b] + K[i][m][b][a]) / (4 * h[a] * h[b]);
+                            out[9 * (13 + sa * sta + sb * stb) + 3 * i + m] += sa * sb * v;
+                        }
+        }
+}
+
 static void merge(sto_sim *S) {
     for (int k = 0; k < S->Nz; k++)
         for (int j = 0; j < S->Ny; j++)
@@ -285,9 +545,8 @@ static void merge(sto_sim *S) {
 static int mg_o(const sto_sim *S) { return S->Nz + (S->prd[2] ? 0 : 1); }
 
 static void setup_mg(sto_sim *S, double dt) {
-    if (S->mg_ready) mgo_free(&S->mg);
     int o = mg_o(S);
-    mgo_build(&S->mg, S->Nx, S->Ny, o, S->prd, MGO_PLANE, 1024);
+    if (!S->mg_ready) mgo_build(&S->mg, S->Nx, S->Ny, o, S->prd, MGO_PLANE, 1024);
     /* stencil = -dt*stencil_base, fix (src/shear_sim_3d.hh:45-55) */
     double L2 = S->lamb + 2 * S->mu, c = (S->lamb + S->mu) / 4, mu = S->mu;
     double dx = S->dx, dy = S->dy, dz = S->dz;
@@ -312,8 +571,15 @@ static void setup_mg(sto_sim *S, double dt) {
     base[13][0] = -2 * (L2 / dx / dx + mu / dy / dy + mu / dz / dz);
     base[13][4] = -2 * (mu / dx / dx + L2 / dy / dy + mu / dz / dz);
     base[13][8] = -2 * (mu / dx / dx + mu / dy / dy + L2 / dz / dz);
-    for (int b = 0; b < 27; b++)
-        for (int e = 0; e < 9; e++) S->stencil[9 * b + e] = -dt * base[b][e];
+    if (S->tfam) {
+        /* src/trans_sim_3d.cpp:1187-1489: stencil = -dt s_ent(T(t)) */
+        double se[243];
+        sto_trans_stencil(S->T, S->mu, S->lamb, dx, dy, dz, se);
+        for (int q = 0; q < 243; q++) S->stencil[q] = -dt * se[q];
+    } else {
+        for (int b = 0; b < 27; b++)
+            for (int e = 0; e < 9; e++) S->stencil[9 * b + e] = -dt * base[b][e];
+    }
     memset(S->fix, 0, sizeof(S->fix));
     double fv = -dt * 2 * (L2 / dx / dx + mu / dy / dy + mu / dz / dz);
     S->fix[0] = S->fix[4] = S->fix[8] = fv;
@@ -342,8 +608,8 @@ static void projection(sto_sim *S, double dt) {
                 L->x[3 * p + 1] = S->f[F_V][c];
                 L->x[3 * p + 2] = S->f[F_W][c];
                 double r0, r1, r2;
-                if (!S->prd[2] && k == 0) { r0 = -S->fix[0] * ut; r1 = 0; r2 = 0; }
-                else if (!S->prd[2] && k == S->Nz) { r0 = S->fix[0] * ut; r1 = 0; r2 = 0; }
+                if (!S->prd[2] && k == 0) { r0 = S->tfam ? 0 : -S->fix[0] * ut; r1 = 0; r2 = 0; }
+                else if (!S->prd[2] && k == S->Nz) { r0 = S->tfam ? 0 : S->fix[0] * ut; r1 = 0; r2 = 0; }
                 else {
                     /* calculate_first_order_stresses, src/shear_sim_3d.cpp:980-996 */
                     const double *a;
@@ -359,6 +625,12 @@ static void projection(sto_sim *S, double dt) {
                     r0 = d11x + d12y + d13z;
                     r1 = d12x + d22y + d23z;
                     r2 = d13x + d23y + d33z;
+                    if (S->tfam) {
+                        /* T div sigma' (src/trans_sim_3d.hh:202-248) */
+                        double dv[3] = {r0, r1, r2}, tv[3];
+                        m3vec(S->T, dv, tv);
+                        r0 = tv[0]; r1 = tv[1]; r2 = tv[2];
+                    }
                 }
                 L->r[3 * p] = r0;
                 L->r[3 * p + 1] = r1;
@@ -383,7 +655,9 @@ static void projection(sto_sim *S, double dt) {
         while (err > tol) {
             mgo_vcycle(&S->mg);
             S->vcycles++;
-            err = mgo_norm_quartic(L);
+            /* shear checks after every cycle; trans only on the schedule
+             * (src/trans_sim_3d.cpp:1020-1040) */
+            if (!S->tfam) err = mgo_norm_quartic(L);
             if (num_iters >= S->std_iters / 16) err = mgo_norm_quartic(L);
             num_iters++;
             if (S->vcycles > 10000) break;
@@ -404,6 +678,29 @@ static void projection(sto_sim *S, double dt) {
                 S->f[F_W][c] = L->x[3 * p + 2];
             }
     refresh(S);
+    if (S->tfam) {
+        /* sigma' += dt T^-1 C:D(L) T^-t at vertex positions
+         * (src/trans_sim_3d.cpp:1096-1118) */
+        double Tti[9];
+        m3t(S->Ti, Tti);
+        for (int k = 0; k < S->Nz; k++)
+            for (int j = 0; j < S->Ny; j++)
+                for (int i = 0; i < S->Nx; i++) {
+                    long c = gi(S, i, j, k);
+                    double P0[3] = {S->ax + (double)i * S->dx, S->ay + (double)j * S->dy, S->az + (double)k * S->dz};
+                    double Lm[9], Lt[9], CD[9], A[9], B[9];
+                    compute_L(S, c, P0, Lm);
+                    m3t(Lm, Lt);
+                    double tr = Lm[0] + Lm[4] + Lm[8];
+                    for (int e = 0; e < 9; e++) CD[e] = ((e % 4 == 0) ? S->lamb * tr : 0.0) + S->mu * (Lm[e] + Lt[e]);
+                    for (int e = 0; e < 9; e++) A[e] = S->Ti[e] * dt;
+                    m3mul(A, CD, B);
+                    m3mul(B, Tti, A);
+                    static const int comp[6] = {0, 1, 2, 4, 5, 8};
+                    for (int q = 0; q < 6; q++) S->f[F_S11 + q][c] += A[comp[q]];
+                }
+        return;
+    }
     for (int k = 0; k < S->Nz; k++)
         for (int j = 0; j < S->Ny; j++)
             for (int i = 0; i < S->Nx; i++) {
@@ -422,6 +719,27 @@ static void projection(sto_sim *S, double dt) {
 
 void sto_step(void *hp, double dt, int nsteps) {
     sto_sim *S = (sto_sim *)hp;
+    if (S->tfam) {
+        /* step_forward_qs, src/trans_sim_3d.cpp:72-80 */
+        for (int s = 0; s < nsteps; s++) {
+            update_matrices(S);
+            for (int k = 0; k < S->Nz; k++)
+                for (int j = 0; j < S->Ny; j++)
+                    for (int i = 0; i < S->Nx; i++) {
+                        double P0[3] = {S->ax + ((double)i + .5) * S->dx, S->ay + ((double)j + .5) * S->dy,
+                                        S->az + ((double)k + .5) * S->dz};
+                        advection_cell_trans(S, gi(S, i, j, k), P0, dt);
+                    }
+            S->t += dt;
+            merge(S);
+            ghosts(S);
+            update_matrices(S);
+            setup_mg(S, dt);
+            projection(S, dt);
+            refresh(S);
+        }
+        return;
+    }
     if (!S->mg_ready) setup_mg(S, dt);
     for (int s = 0; s < nsteps; s++) {
         for (int k = 0; k < S->Nz; k++)
@@ -481,3 +799,17 @@ void sto_set_solver(void *hp, int smoother, double rtol) {
     S->smoother = smoother;
     S->rtol = rtol;
 }
+
+/* Switch the instance to the transformed-coordinate step with transform
+ * family fam (STO_T_*) and rate Ut. */
+void sto_set_transform(void *hp, int fam, double Ut) {
+    sto_sim *S = (sto_sim *)hp;
+    S->tfam = fam;
+    S->Ut = Ut;
+    S->eta = fam ? 0.002 : 0.0005;
+}
+
+/* Test switch: 1 replaces the reference's chi x-ENO argument (me[1] as
+ * f(+2), src/trans_sim_3d.cpp:985) by the textbook me[2], so that a test can
+ * show the reference fixtures tell the two apart. */
+void sto_set_textbook(void *hp, int on) { ((sto_sim *)hp)->textbook = on; }

@@ -162,6 +162,60 @@ def step_case(name):
     np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
 
 
+def trans_step_case(name):
+    """QS steps of the reference trans_sim_3d (pure-shear transform
+    T = diag(e^{Ut t}, 1, e^{-Ut t}), Ut = 0.5).
+
+    trans_step_walled / trans_step_periodic: 16x16x8 cells, z walled or
+      periodic; config-0 material with eps0 = 0.3, a pre-stress s12 above
+      yield and a small s11, a small initial velocity (walls at rest), 3 steps
+      of dt = 0.01.  Also stored: T, T^-1, dT/dt and the stencil at the final
+      time (set_up_stencil_and_matrices of the last step).
+    """
+    import tempfile
+
+    from stzpp_amd import fields
+
+    TZ, kb = 21000.0, 1.3806503e-23
+    E, nu, sy = 101e9, 0.35, 0.85e9
+    mu_r, rho = E / (2 * (1 + nu)), 6125.0
+    ts = 1e-2 / np.sqrt(mu_r / rho)
+    N = (16, 16, 8)
+    prd = (1, 1, 0) if name == "trans_step_walled" else (1, 1, 1)
+    p = dict(mu=1.0, lamb=2.2 - 2.0 / 3.0, sy=1.0, tau0=1e-13 / ts, c0=0.4, eps0=0.3,
+             delta=8000 / TZ, omega=300 * 26.0 / 17.0 * 1e-30 * sy / (TZ * kb), bath_temp=400 / TZ,
+             chi_inf=0.15, ez=1.0, diff_l=1.5 * 2.0 / N[0], U=0.01)
+    Ut, dt, nsteps = 0.5, 0.01, 3
+    Nx, Ny, Nz = N
+    nc = Nx * Ny * Nz
+    nv = Nx * Ny * (Nz + (0 if prd[2] else 1))
+    chi0 = 0.09 + 0.01 * fields.grf((Nz, Ny, Nx), 3.0, 0).ravel()
+    sig0 = np.zeros((nc, 6))
+    sig0[:, 1] = 1.1 + 0.1 * np.cos(np.arange(nc) * 0.37)
+    sig0[:, 0] = 0.2 * np.sin(np.arange(nc) * 0.11)
+    u0 = 0.001 * np.sin(np.arange(nv * 3) * 0.07).reshape(nv, 3)
+    if not prd[2]:
+        u0[: Nx * Ny] = 0
+        u0[-Nx * Ny:] = 0
+    ref = O.RefTrans(N, p, Ut, tempfile.mkdtemp(), prd=prd)
+    ref.setup(dt)
+    ref.set_state(0.0, u0, sig0, chi0)
+    cyc = []
+    for _ in range(nsteps):
+        before = ref.vcycles
+        ref.step(dt, 1)
+        cyc.append(ref.vcycles - before)
+    s = ref.state()
+    T, Ti, dT = ref.matrices()
+    st, fx = ref.stencil()
+    out = dict(N=np.array(N), prd=np.array(prd), dt=np.array(dt), Ut=np.array(Ut), nsteps=np.array(nsteps),
+               u0=u0, chi0=chi0, sig0=sig0, vcycles=np.array(cyc), t=np.array(s["t"]),
+               u=s["u"], sig=s["sig"], chi=s["chi"], dpl=s["dpl"], T=T, Tinv=Ti, dTdt=dT,
+               stencil=st, fix=fx, **{f"p_{k}": np.array(v) for k, v in p.items()})
+    ref.close()
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+
+
 CASES = {
     "shear_16x16x17": (("shear_16x16x17", (16, 16, 17), (1, 1, 0), "shear", 11),),
     "trans_16x16x16": (("trans_16x16x16", (16, 16, 16), (1, 1, 1), "trans", 12),),
@@ -177,10 +231,12 @@ if __name__ == "__main__":
             trans_case()
         elif which.startswith("step_"):
             step_case(which)
+        elif which.startswith("trans_step_"):
+            trans_step_case(which)
         else:
             mg_case(*CASES[which][0])
         sys.exit(0)
-    for key in ["trans", *CASES, "step_plastic", "step_c0"]:
+    for key in ["trans", *CASES, "step_plastic", "step_c0", "trans_step_walled", "trans_step_periodic"]:
         subprocess.run([sys.executable, __file__, key], check=True, cwd=ROOT,
                        stdout=subprocess.DEVNULL)
         print("wrote", key)
