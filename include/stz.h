@@ -1,9 +1,11 @@
 /* stz.h -- C ABI of the MI355X quasi-static STZ step.
  *
  * Drop-in for the simulation step API of shear_sim_3d<sym_mat3>
- * (src/shear_sim_3d.hh:35-82): state in, N quasi-static steps
- * (step_forward_qs, src/shear_sim_3d.cpp:381-389), state out.  One
- * subdomain on one GPU; periodic in x and y, walls (plates) or periodic in z.
+ * (src/shear_sim_3d.hh:35-82) and trans_sim_3d (src/trans_sim_3d.hh:13-36):
+ * state in, N quasi-static steps (step_forward_qs, src/shear_sim_3d.cpp:
+ * 381-389, src/trans_sim_3d.cpp:72-80), state out.  One subdomain on one
+ * GPU; periodic in x and y, walls or periodic in z.  In the transformed step
+ * u and sig are the transformed velocity U and stress sigma'.
  *
  * State arrays on the host (AoS, x fastest):
  *   u    vertex velocity, 3 doubles per vertex, Nx*Ny*Nzv vertices with
@@ -33,8 +35,20 @@ typedef struct {
     double sy, tau0, c0, eps0, delta, omega, bath_temp, chi_inf, ez; /* STZ */
     double diff_l;         /* plasticity-mediated diffusion length */
     double U;              /* plate speed; the plates ramp as U*t for t < 1 */
-    double eta;            /* adaptive plastic-step tolerance (5e-4, src/shear_sim_3d.cpp:111) */
+    double eta;            /* adaptive plastic-step tolerance; 0 = the reference's
+                              value for the step: 5e-4 (src/shear_sim_3d.cpp:111),
+                              2e-3 transformed (src/trans_sim_3d.cpp:741) */
+    int transform;         /* STZ_T_NONE: shear_sim_3d<sym_mat3> step with moving
+                              plates; otherwise the transformed-coordinate step of
+                              trans_sim_3d with this transform family and walls at rest */
+    double Ut;             /* rate of the transform family */
 } stz_params;
+
+/* Transform families of trans_sim_3d (src/trans_sim_3d.hh:57-125): pure shear
+ * T = diag(e^{Ut t}, 1, e^{-Ut t}) is the one the reference compiles in;
+ * simple shear T = I + (Ut t / bz) e_x e_z^T, homogeneous T = e^{Ut t} I and
+ * the identity are its commented alternatives. */
+enum { STZ_T_NONE = 0, STZ_T_PURE_SHEAR = 1, STZ_T_SIMPLE_SHEAR = 2, STZ_T_HOMOGENEOUS = 3, STZ_T_IDENTITY = 4 };
 
 typedef struct {
     int steps;             /* steps taken */
@@ -63,6 +77,9 @@ int stz_get_state(stz_t s, double *t, double *u, double *sig, double *chi, doubl
 int stz_step_qs(stz_t s, double dt, int nsteps, const stzmg_solve_opts *opts, stz_step_stats *st);
 /* The projection solver of the simulation (valid after the first step). */
 stzmg_t stz_multigrid(stz_t s);
+/* T, T^-1 and dT/dt (row-major 3x3) of a transform family at time t
+ * (compute_T, compute_Tinv, compute_dTdt, src/trans_sim_3d.hh:57-125). */
+int stz_transform(int family, double Ut, double bz, double t, double *T, double *Tinv, double *dTdt);
 
 #ifdef __cplusplus
 }

@@ -53,7 +53,8 @@ class StzParams(ctypes.Structure):
                 ("prd", ctypes.c_int * 3)] + [
         (k, ctypes.c_double) for k in (
             "ax", "bx", "ay", "by", "az", "bz", "mu", "lamb", "sy", "tau0", "c0", "eps0",
-            "delta", "omega", "bath_temp", "chi_inf", "ez", "diff_l", "U", "eta")]
+            "delta", "omega", "bath_temp", "chi_inf", "ez", "diff_l", "U", "eta")] + [
+        ("transform", ctypes.c_int), ("Ut", ctypes.c_double)]
 
 
 class StzStepStats(ctypes.Structure):
@@ -72,6 +73,8 @@ SIGNATURES = {
     "stz_get_state": ([_vp, _dp, _dp, _dp, _dp, _dp], ctypes.c_int),
     "stz_step_qs": ([_vp, ctypes.c_double, ctypes.c_int, _vp, ctypes.POINTER(StzStepStats)], ctypes.c_int),
     "stz_multigrid": ([_vp], _vp),
+    "stz_transform": ([ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp],
+                      ctypes.c_int),
     "stzmg_desc_default": ([ctypes.POINTER(StzmgDesc)] + [ctypes.c_int] * 6, ctypes.c_int),
     "stzmg_create": ([ctypes.POINTER(StzmgDesc), ctypes.POINTER(_vp)], ctypes.c_int),
     "stzmg_destroy": ([_vp], ctypes.c_int),

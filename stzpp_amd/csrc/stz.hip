@@ -4,7 +4,11 @@
 // 381-389): plasticity + advection (two passes, k_plastic then k_advect, so
 // that the reference's in-place ad_Dpl ordering is reproduced without a
 // sequential loop), time advance, projection (right-hand side, multigrid
-// solve, copy-out with plate values, stress update).  Everything runs on the
+// solve, copy-out with plate values, stress update).  The transformed step
+// (trans_sim_3d::step_forward_qs, src/trans_sim_3d.cpp:72-80) has the same
+// shape with the transform T(t) at the advection and T(t+dt) at the
+// projection, whose operator and Galerkin hierarchy are rebuilt every step
+// (set_up_stencil_and_matrices, :1187-1489).  Everything runs on the
 // multigrid handle's HIP stream; the state never leaves HBM between steps.
 #include <hip/hip_runtime.h>
 
@@ -59,7 +63,38 @@ int sfail(stz_t s, int code, const char *fmt, ...) {
 
 inline unsigned nb(long n) { return (unsigned)((n + kT - 1) / kT); }
 
-double top_u(const stz_t s, double t) { return t < 1 ? s->p.U * t : s->p.U; }
+// Plate speed (ramped U t for t < 1, src/shear_sim_3d.hh:375); the
+// transformed simulation's walls are at rest (src/trans_sim_3d.cpp:1122-1184).
+double top_u(const stz_t s, double t) { return s->p.transform ? 0.0 : (t < 1 ? s->p.U * t : s->p.U); }
+
+void m3set(double *a, double a11, double a12, double a13, double a21, double a22, double a23, double a31,
+           double a32, double a33) {
+    a[0] = a11; a[1] = a12; a[2] = a13; a[3] = a21; a[4] = a22; a[5] = a23; a[6] = a31; a[7] = a32; a[8] = a33;
+}
+
+Xform xform_at(const stz_t s, double t) {
+    Xform X;
+    stz_transform(s->p.transform, s->p.Ut, s->p.bz, t, X.T, X.Ti, X.dT);
+    return X;
+}
+
+// Transformed operator at T(t+dt): stencil = -dt s_ent(T), fix on the walls
+// (set_up_stencil_and_matrices, src/trans_sim_3d.cpp:1187-1489), with the
+// Galerkin hierarchy rebuilt on the GPU.
+int trans_operator(stz_t s, double dt, const Xform &X) {
+    const Grid &g = s->g;
+    double B[243], F[9] = {0};
+    int rc = stzmg_trans_stencil(X.T, s->p.mu, s->p.lamb, g.dx, g.dy, g.dz, dt, B);
+    if (rc) return sfail(s, rc, "transformed stencil");
+    const double fv = -dt * 2 * ((s->p.lamb + 2 * s->p.mu) / g.dx / g.dx + s->p.mu / g.dy / g.dy + s->p.mu / g.dz / g.dz);
+    F[0] = F[4] = F[8] = fv;
+    rc = stzmg_set_operator_const(s->mg, B, g.pz ? nullptr : F);
+    if (rc) return sfail(s, rc, "projection operator: %s", stzmg_last_error(s->mg));
+    s->a11 = B[9 * 13];
+    s->fix_a11 = F[0];
+    s->mg_dt = -1;  // the shear operator is no longer loaded
+    return 0;
+}
 
 // (Re)build the projection operator for step size dt
 // (set_up_stencil_and_matrices, src/shear_sim_3d.hh:45-55).
@@ -113,7 +148,32 @@ int stz_params_default(stz_params *p, int Nx, int Ny, int Nz) {
     p->ez = 1;
     p->diff_l = 1.5 * (p->bx - p->ax) / Nx;
     p->U = 1e-7;
-    p->eta = 0.0005;
+    p->eta = 0;
+    p->transform = STZ_T_NONE;
+    p->Ut = 0;
+    return 0;
+}
+
+int stz_transform(int fam, double Ut, double bz, double t, double *T, double *Ti, double *dT) {
+    if (!T || !Ti || !dT || fam < STZ_T_NONE || fam > STZ_T_IDENTITY) return STZMG_ERR_ARG;
+    if (fam == STZ_T_PURE_SHEAR) {
+        m3set(T, std::exp(Ut * t), 0, 0, 0, 1, 0, 0, 0, std::exp(-Ut * t));
+        m3set(Ti, std::exp(-Ut * t), 0, 0, 0, 1, 0, 0, 0, std::exp(Ut * t));
+        m3set(dT, Ut * std::exp(Ut * t), 0, 0, 0, 0, 0, 0, 0, -Ut * std::exp(-Ut * t));
+    } else if (fam == STZ_T_SIMPLE_SHEAR) {
+        m3set(T, 1, 0, Ut * t / bz, 0, 1, 0, 0, 0, 1);
+        m3set(Ti, 1, 0, -Ut * t / bz, 0, 1, 0, 0, 0, 1);
+        m3set(dT, 0, 0, Ut / bz, 0, 0, 0, 0, 0, 0);
+    } else if (fam == STZ_T_HOMOGENEOUS) {
+        const double e = std::exp(Ut * t), ei = std::exp(-Ut * t);
+        m3set(T, e, 0, 0, 0, e, 0, 0, 0, e);
+        m3set(Ti, ei, 0, 0, 0, ei, 0, 0, 0, ei);
+        m3set(dT, Ut * e, 0, 0, 0, Ut * e, 0, 0, 0, Ut * e);
+    } else {
+        m3set(T, 1, 0, 0, 0, 1, 0, 0, 0, 1);
+        m3set(Ti, 1, 0, 0, 0, 1, 0, 0, 0, 1);
+        m3set(dT, 0, 0, 0, 0, 0, 0, 0, 0, 0);
+    }
     return 0;
 }
 
@@ -141,8 +201,14 @@ int stz_create(const stz_params *p, stz_t *out) {
     g.dx = (p->bx - p->ax) / g.Nx;
     g.dy = (p->by - p->ay) / g.Ny;
     g.dz = (p->bz - p->az) / g.Nz;
+    if (p->transform < STZ_T_NONE || p->transform > STZ_T_IDENTITY) {
+        g_stz_err = "unknown transform family";
+        delete s;
+        return STZMG_ERR_ARG;
+    }
+    const double eta = p->eta > 0 ? p->eta : (p->transform ? 0.002 : 0.0005);
     s->M = Mat{p->mu, p->lamb, p->sy, p->tau0, p->c0, p->eps0, p->delta, p->omega, p->bath_temp,
-               p->chi_inf, p->ez, p->diff_l, p->eta};
+               p->chi_inf, p->ez, p->diff_l, eta};
     // The solver owns the stream; create it now with a placeholder dt.
     if (int rc = ensure_operator(s, 1.0)) {
         g_stz_err = s->err;
@@ -228,8 +294,11 @@ int stz_get_state(stz_t s, double *t, double *u, double *sig, double *chi, doubl
 
 int stz_step_qs(stz_t s, double dt, int nsteps, const stzmg_solve_opts *opts, stz_step_stats *st) {
     if (!s || !(dt > 0) || nsteps < 0) return STZMG_ERR_ARG;
-    if (int rc = ensure_operator(s, dt)) return rc;
+    const bool trans = s->p.transform != STZ_T_NONE;
+    if (!trans)
+        if (int rc = ensure_operator(s, dt)) return rc;
     const Grid &g = s->g;
+    const Box box{s->p.ax, s->p.ay, s->p.az, s->p.bx, s->p.by, s->p.bz};
     stz_step_stats S{};
     auto t0 = std::chrono::steady_clock::now();
     stzmg_solve_opts o;
@@ -242,26 +311,52 @@ int stz_step_qs(stz_t s, double dt, int nsteps, const stzmg_solve_opts *opts, st
         const double tol = std::numeric_limits<double>::epsilon() * s->a11 * 1e3;
         o.tol = tol * tol;
         o.extra_cycles = 2;
-        o.ref_norm_schedule = 0;
+        // shear: norm after every cycle; trans: only on the std_iters
+        // schedule (src/trans_sim_3d.cpp:1020-1040)
+        o.ref_norm_schedule = trans ? 1 : 0;
         o.max_cycles = 100000;
     }
     for (int q = 0; q < nsteps; q++) {
         SHIP(s, hipMemsetAsync(s->counters, 0, sizeof(int) * 4, s->stream));
-        hipLaunchKernelGGL(k_plastic, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt, (const double *)s->sig,
-                           (const double *)s->chi, s->adapt, s->dchi, s->dpl2, s->counters);
-        hipLaunchKernelGGL(k_advect, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt, (const double *)s->vel,
-                           (const double *)s->sig, (const double *)s->chi, (const double *)s->adapt,
-                           (const double *)s->dchi, (const double *)s->dpl2, (const double *)s->dpl, s->sig2,
-                           s->chi2);
+        if (trans) {
+            const Xform Xn = xform_at(s, s->t);
+            hipLaunchKernelGGL(k_plastic_trans, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt, Xn,
+                               (const double *)s->sig, (const double *)s->chi, s->adapt, s->dchi, s->dpl2,
+                               s->counters);
+            hipLaunchKernelGGL(k_advect_trans, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt, Xn, box,
+                               (const double *)s->vel, (const double *)s->sig, (const double *)s->chi,
+                               (const double *)s->adapt, (const double *)s->dchi, (const double *)s->dpl2,
+                               (const double *)s->dpl, s->sig2, s->chi2);
+        } else {
+            hipLaunchKernelGGL(k_plastic, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt,
+                               (const double *)s->sig, (const double *)s->chi, s->adapt, s->dchi, s->dpl2,
+                               s->counters);
+            hipLaunchKernelGGL(k_advect, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt,
+                               (const double *)s->vel, (const double *)s->sig, (const double *)s->chi,
+                               (const double *)s->adapt, (const double *)s->dchi, (const double *)s->dpl2,
+                               (const double *)s->dpl, s->sig2, s->chi2);
+        }
         SHIP(s, hipGetLastError());
         std::swap(s->sig, s->sig2);
         std::swap(s->chi, s->chi2);
         std::swap(s->dpl, s->dpl2);
         s->t += dt;
         const double ut = top_u(s, s->t);
-        hipLaunchKernelGGL(k_rhs, dim3(nb(g.Nv)), dim3(kT), 0, s->stream, g, (const double *)s->sig,
-                           (const double *)s->vel, s->fix_a11, ut, stzmg_internal::x0(s->mg),
-                           stzmg_internal::r0(s->mg));
+        Xform X1{};
+        if (trans) {
+            X1 = xform_at(s, s->t);
+            if (int rc = trans_operator(s, dt, X1)) return rc;
+            if (!opts) {
+                const double tol = std::numeric_limits<double>::epsilon() * s->a11 * 1e3;
+                o.tol = tol * tol;
+            }
+            hipLaunchKernelGGL(k_rhs_trans, dim3(nb(g.Nv)), dim3(kT), 0, s->stream, g, X1, (const double *)s->sig,
+                               (const double *)s->vel, stzmg_internal::x0(s->mg), stzmg_internal::r0(s->mg));
+        } else {
+            hipLaunchKernelGGL(k_rhs, dim3(nb(g.Nv)), dim3(kT), 0, s->stream, g, (const double *)s->sig,
+                               (const double *)s->vel, s->fix_a11, ut, stzmg_internal::x0(s->mg),
+                               stzmg_internal::r0(s->mg));
+        }
         SHIP(s, hipGetLastError());
         stzmg_solve_stats ms{};
         int rc = stzmg_solve(s->mg, &o, &ms);
@@ -271,8 +366,12 @@ int stz_step_qs(stz_t s, double dt, int nsteps, const stzmg_solve_opts *opts, st
         if (rc) return sfail(s, rc, "projection solve at t=%g: %s", s->t, stzmg_last_error(s->mg));
         hipLaunchKernelGGL(k_copy_vel, dim3(nb(g.Nv)), dim3(kT), 0, s->stream, g,
                            (const double *)stzmg_internal::x0(s->mg), ut, s->vel);
-        hipLaunchKernelGGL(k_stress_update, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt,
-                           (const double *)s->vel, s->sig);
+        if (trans)
+            hipLaunchKernelGGL(k_stress_update_trans, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt, X1,
+                               box, (const double *)s->vel, s->sig);
+        else
+            hipLaunchKernelGGL(k_stress_update, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt,
+                               (const double *)s->vel, s->sig);
         SHIP(s, hipGetLastError());
         int cnt[4];
         SHIP(s, hipMemcpyAsync(cnt, s->counters, sizeof(cnt), hipMemcpyDeviceToHost, s->stream));

@@ -317,6 +317,327 @@ __global__ __launch_bounds__(kT) void k_stress_update(Grid g, Mat M, double dt, 
     sig[5 * g.Nc + c] += dt * (lam * (G[1][1] + G[0][0]) + L2 * G[2][2]);
 }
 
+// ---- transformed-coordinate step (trans_sim_3d) ---------------------------
+//
+// State is the transformed velocity U (vertices), transformed stress sigma'
+// and chi (cells).  Xform holds T, T^-1 and dT/dt at the time a kernel
+// works at (trans_sim_3d::update_matrices, src/trans_sim_3d.hh:132-140).
+struct Xform {
+    double T[9], Ti[9], dT[9];
+};
+
+// 3x3 algebra in the summation order of src/mat3.hh:139-187.
+__device__ __forceinline__ void m3mul(const double *a, const double *b, double *o) {
+    double r[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) r[3 * i + j] = a[3 * i] * b[j] + a[3 * i + 1] * b[3 + j] + a[3 * i + 2] * b[6 + j];
+#pragma unroll
+    for (int e = 0; e < 9; e++) o[e] = r[e];
+}
+__device__ __forceinline__ void m3t(const double *a, double *o) {
+    const double r[9] = {a[0], a[3], a[6], a[1], a[4], a[7], a[2], a[5], a[8]};
+#pragma unroll
+    for (int e = 0; e < 9; e++) o[e] = r[e];
+}
+__device__ __forceinline__ void m3vec(const double *a, const double *v, double *o) {
+    const double r0 = a[0] * v[0] + a[1] * v[1] + a[2] * v[2];
+    const double r1 = a[3] * v[0] + a[4] * v[1] + a[5] * v[2];
+    const double r2 = a[6] * v[0] + a[7] * v[1] + a[8] * v[2];
+    o[0] = r0; o[1] = r1; o[2] = r2;
+}
+
+struct Box {
+    double ax, ay, az, bx, by, bz;
+};
+
+// L = T^-t grad_X(dT/dt X + T U) over the cell whose lowest vertex is
+// (i,j,k) and whose corner 0 is placed at P0 (compute_L,
+// src/trans_sim_3d.cpp:327-401).  Corner n of the reference's numbering
+// (counter-clockwise on the bottom face, then the top face).
+__device__ __forceinline__ void compute_L(const double *__restrict__ vel, const Grid &g, const Box &b,
+                                          const Xform &X, int i, int j, int k, const double P0[3], double L[9]) {
+    const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    const double pn[3] = {P0[0] == b.bx ? b.ax : P0[0] + g.dx, P0[1] == b.by ? b.ay : P0[1] + g.dy,
+                          (g.pz && P0[2] == b.bz) ? b.az : P0[2] + g.dz};
+    double v[8][3];
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+        double P[3], Vl[3], a[3], c[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            P[d] = off[n][d] ? pn[d] : P0[d];
+            Vl[d] = vertv(vel + d * g.Nv, g, i + off[n][0], j + off[n][1], k + off[n][2]);
+        }
+        m3vec(X.dT, P, a);
+        m3vec(X.T, Vl, c);
+#pragma unroll
+        for (int d = 0; d < 3; d++) v[n][d] = a[d] + c[d];
+    }
+    double G[9];
+    const double ix = .25 * (1. / g.dx), iy = .25 * (1. / g.dy), iz = .25 * (1. / g.dz);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        G[d] = ix * (((((((v[1][d] - v[0][d]) + v[2][d]) - v[3][d]) + v[5][d]) - v[4][d]) + v[6][d]) - v[7][d]);
+        G[3 + d] = iy * (((((((v[3][d] - v[0][d]) + v[2][d]) - v[1][d]) + v[7][d]) - v[4][d]) + v[6][d]) - v[5][d]);
+        G[6 + d] = iz * (((((((v[4][d] - v[0][d]) + v[5][d]) - v[1][d]) + v[7][d]) - v[3][d]) + v[6][d]) - v[2][d]);
+    }
+    double Tti[9];
+    m3t(X.Ti, Tti);
+    m3mul(Tti, G, L);
+}
+
+// Physical stress T sigma' T^t of cell c (compute_old_sig,
+// src/trans_sim_3d.hh:147) and its sbar (calc_devsq, :151-165).
+__device__ __forceinline__ void old_sig(const double *__restrict__ sig, long Nc, long c, const Xform &X,
+                                        double sp[9], double os[9]) {
+    const double s11 = sig[c], s12 = sig[Nc + c], s13 = sig[2 * Nc + c];
+    const double s22 = sig[3 * Nc + c], s23 = sig[4 * Nc + c], s33 = sig[5 * Nc + c];
+    const double q[9] = {s11, s12, s13, s12, s22, s23, s13, s23, s33};
+#pragma unroll
+    for (int e = 0; e < 9; e++) sp[e] = q[e];
+    double Tt[9], tmp[9];
+    m3t(X.T, Tt);
+    m3mul(X.T, sp, tmp);
+    m3mul(tmp, Tt, os);
+}
+__device__ __forceinline__ double sbar_phys(const double os[9]) {
+    const double p = -1. / 3. * (os[0] + os[4] + os[8]);
+    const double t11 = os[0] + p, t22 = os[4] + p, t33 = os[8] + p;
+    return sqrt(0.5 * (t11 * t11 + t22 * t22 + t33 * t33) + os[1] * os[1] + os[2] * os[2] + os[5] * os[5]);
+}
+
+// Pass 1 of the transformed step: STZ plasticity on the physical stress
+// (advection_grid_step, src/trans_sim_3d.cpp:940-969, with
+// adaptive_plastic_term :740-775, eta = M.eta).  Writes adapt, the plastic
+// chi change and this step's ad_Dpl (0 below yield).
+__global__ __launch_bounds__(kT) void k_plastic_trans(Grid g, Mat M, double dt, Xform X,
+                                                      const double *__restrict__ sig,
+                                                      const double *__restrict__ chi, double *__restrict__ adapt,
+                                                      double *__restrict__ dchi, double *__restrict__ dpl_new,
+                                                      int *__restrict__ counters) {
+    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    if (c >= g.Nc) return;
+    double sp[9], os[9];
+    old_sig(sig, g.Nc, c, X, sp, os);
+    const double sb = sbar_phys(os);
+    double a = 0, dc = 0, D = 0;
+    if (sb > M.sy) {
+        int n;
+        a = adaptive(M, sb, chi[c], dt, dc, n);
+        D = .5 * a / M.mu * sb;
+        atomicAdd(&counters[0], 1);
+    }
+    adapt[c] = a;
+    dchi[c] = dc;
+    dpl_new[c] = D;
+}
+
+// Pass 2 of the transformed step (advection_grid_step,
+// src/trans_sim_3d.cpp:880-992): L and T terms, ENO2 advection of sigma'
+// and chi (the x derivative of chi takes f(+1) as its f(+2), :985), the
+// plastic update T^-1 adapt dev(T sigma' T^t) T^-t, and chi diffusion with
+// the metric J = T^-1 T^-t (chi_diffusion, :779-812, including the mixed
+// y-z difference of :802 whose last two terms are the same cell).  ad_Dpl
+// follows the in-place order as in k_advect.
+__global__ __launch_bounds__(kT) void k_advect_trans(Grid g, Mat M, double dt, Xform X, Box B,
+                                                     const double *__restrict__ vel, const double *__restrict__ sig,
+                                                     const double *__restrict__ chi,
+                                                     const double *__restrict__ adapt,
+                                                     const double *__restrict__ dchi, const double *__restrict__ dnew,
+                                                     const double *__restrict__ dold, double *__restrict__ sig_out,
+                                                     double *__restrict__ chi_out) {
+    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    if (c >= g.Nc) return;
+    const int i = (int)(c % g.Nx);
+    const int j = (int)((c / g.Nx) % g.Ny);
+    const int k = (int)(c / ((long)g.Nx * g.Ny));
+    const double P0[3] = {B.ax + ((double)i + .5) * g.dx, B.ay + ((double)j + .5) * g.dy,
+                          B.az + ((double)k + .5) * g.dz};
+    double L[9];
+    compute_L(vel, g, B, X, i, j, k, P0, L);
+    double v[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const double *u = vel + q * g.Nv;
+        v[q] = 0.125 * (vertv(u, g, i, j, k) + vertv(u, g, i + 1, j, k) + vertv(u, g, i, j, k + 1) +
+                        vertv(u, g, i + 1, j, k + 1) + vertv(u, g, i, j + 1, k) + vertv(u, g, i + 1, j + 1, k) +
+                        vertv(u, g, i, j + 1, k + 1) + vertv(u, g, i + 1, j + 1, k + 1));
+    }
+    double sp[9], os[9], Tti[9], tmp[9], tmp2[9];
+    old_sig(sig, g.Nc, c, X, sp, os);
+    m3t(X.Ti, Tti);
+    // L term: T^-1 (-tr(L) sigma + sigma L + L^t sigma) T^-t
+    double Lt[9], A[9], Bm[9], Lterm[9];
+    const double trL = L[0] + L[4] + L[8];
+    m3t(L, Lt);
+    m3mul(os, L, A);
+    m3mul(Lt, os, Bm);
+#pragma unroll
+    for (int e = 0; e < 9; e++) tmp[e] = (os[e] * -(trL) + A[e]) + Bm[e];
+    m3mul(X.Ti, tmp, tmp2);
+    m3mul(tmp2, Tti, Lterm);
+    // T term: -(T^-1 dT sigma' + sigma' (T^-1 dT)^t)
+    double TdT[9], TdTt[9], Tterm[9];
+    m3mul(X.Ti, X.dT, TdT);
+    m3t(TdT, TdTt);
+    m3mul(TdT, sp, A);
+    m3mul(sp, TdTt, Bm);
+#pragma unroll
+    for (int e = 0; e < 9; e++) Tterm[e] = (A[e] + Bm[e]) * -1.;
+    const double hx = g.dx / dt, hy = g.dy / dt, hz = g.dz / dt;
+    const int comp[6] = {0, 1, 2, 4, 5, 8};
+    double cs[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const double *f = sig + q * g.Nc;
+        const double f0 = f[c];
+        const double ddx = eno2(hx, v[0], cellv(f, g, i + 2, j, k), cellv(f, g, i + 1, j, k), f0,
+                                cellv(f, g, i - 1, j, k), cellv(f, g, i - 2, j, k));
+        const double ddy = eno2(hy, v[1], cellv(f, g, i, j + 2, k), cellv(f, g, i, j + 1, k), f0,
+                                cellv(f, g, i, j - 1, k), cellv(f, g, i, j - 2, k));
+        const double ddz = eno2(hz, v[2], cellv(f, g, i, j, k + 2), cellv(f, g, i, j, k + 1), f0,
+                                cellv(f, g, i, j, k - 1), cellv(f, g, i, j, k - 2));
+        const double ug = v[0] * ddx + v[1] * ddy + v[2] * ddz;
+        cs[q] = 0.0 + (dt * (Lterm[comp[q]] + Tterm[comp[q]]) - ug);
+    }
+    const double a = adapt[c];
+    if (a != 0.0) {
+        const double third = 1. / 3., tr = os[0] + os[4] + os[8];
+        const double dv[9] = {os[0] - third * tr, os[1], os[2], os[1], os[4] - third * tr, os[5],
+                              os[2], os[5], os[8] - third * tr};
+        double Ta[9], P[9];
+#pragma unroll
+        for (int e = 0; e < 9; e++) Ta[e] = X.Ti[e] * a;
+        m3mul(Ta, dv, tmp);
+        m3mul(tmp, Tti, P);
+#pragma unroll
+        for (int q = 0; q < 6; q++) cs[q] -= P[comp[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < 6; q++) sig_out[q * g.Nc + c] = sig[q * g.Nc + c] + cs[q];
+    // chi: advection (x quirk) and metric diffusion
+    const double x0 = chi[c];
+    const double cx = eno2(hx, v[0], cellv(chi, g, i + 1, j, k), cellv(chi, g, i + 1, j, k), x0,
+                           cellv(chi, g, i - 1, j, k), cellv(chi, g, i - 2, j, k));
+    const double cy = eno2(hy, v[1], cellv(chi, g, i, j + 2, k), cellv(chi, g, i, j + 1, k), x0,
+                           cellv(chi, g, i, j - 1, k), cellv(chi, g, i, j - 2, k));
+    const double cz = eno2(hz, v[2], cellv(chi, g, i, j, k + 2), cellv(chi, g, i, j, k + 1), x0,
+                           cellv(chi, g, i, j, k - 1), cellv(chi, g, i, j, k - 2));
+    const double Dc = dnew[c];
+    const double Dxm = i > 0 ? dnew[c - 1] : dold[cidx(g, i - 1, j, k)];
+    const double Dxp = dold[cidx(g, i + 1, j, k)];
+    const double Dym = j > 0 ? dnew[c - g.Nx] : dold[cidx(g, i, j - 1, k)];
+    const double Dyp = dold[cidx(g, i, j + 1, k)];
+    double Dzm, Dzp;
+    if (g.pz) {
+        Dzm = k > 0 ? dnew[c - (long)g.Nx * g.Ny] : dold[cidx(g, i, j, g.Nz - 1)];
+        Dzp = dold[cidx(g, i, j, wrapc(k + 1, g.Nz))];
+    } else {
+        Dzm = k > 0 ? dnew[c - (long)g.Nx * g.Ny] : 0.0;
+        Dzp = k < g.Nz - 1 ? dold[c + (long)g.Nx * g.Ny] : 0.0;
+    }
+    const double ix = 1. / g.dx, iy = 1. / g.dy, iz = 1. / g.dz;
+    const double xxm = cellv(chi, g, i - 1, j, k), xxp = cellv(chi, g, i + 1, j, k);
+    const double xym = cellv(chi, g, i, j - 1, k), xyp = cellv(chi, g, i, j + 1, k);
+    const double xzm = cellv(chi, g, i, j, k - 1), xzp = cellv(chi, g, i, j, k + 1);
+    const double dDx = .5 * ix * (Dxp - Dxm), dDy = .5 * iy * (Dyp - Dym), dDz = .5 * iz * (Dzp - Dzm);
+    const double gc[3] = {.5 * ix * (xxp - xxm), .5 * iy * (xyp - xym), .5 * iz * (xzp - xzm)};
+    double J[9], Jg[3];
+    m3mul(X.Ti, Tti, J);
+    m3vec(J, gc, Jg);
+    const double term1 = dDx * Jg[0] + dDy * Jg[1] + dDz * Jg[2];
+    const double dxx = ix * ix * (xxp - 2 * x0 + xxm);
+    const double dyy = iy * iy * (xyp - 2 * x0 + xym);
+    const double dzz = iz * iz * (xzp - 2 * x0 + xzm);
+    const double dxy = .25 * ix * iy *
+                       (cellv(chi, g, i + 1, j + 1, k) + cellv(chi, g, i - 1, j - 1, k) -
+                        cellv(chi, g, i + 1, j - 1, k) - cellv(chi, g, i - 1, j + 1, k));
+    const double dxz = .25 * ix * iz *
+                       (cellv(chi, g, i + 1, j, k + 1) + cellv(chi, g, i - 1, j, k - 1) -
+                        cellv(chi, g, i + 1, j, k - 1) - cellv(chi, g, i - 1, j, k + 1));
+    const double ypzm = cellv(chi, g, i, j + 1, k - 1);
+    const double dyz = .25 * iy * iz *
+                       (cellv(chi, g, i, j + 1, k + 1) + cellv(chi, g, i, j - 1, k - 1) - ypzm + ypzm);
+    double term2 = 0;
+    term2 += J[0] * dxx + J[1] * dxy + J[2] * dxz;
+    term2 += J[3] * dxy + J[4] * dyy + J[5] * dyz;
+    term2 += J[6] * dxz + J[7] * dyz + J[8] * dzz;
+    term2 *= Dc;
+    const double diff = M.diff_l * M.diff_l * (term1 + term2);
+    double cchi = dchi[c];
+    cchi -= v[0] * cx + v[1] * cy + v[2] * cz;
+    cchi += diff / M.c0;
+    chi_out[c] = x0 + cchi;
+}
+
+// Right-hand side T div sigma' at interior vertices, zero on the walls, and
+// the warm start x = U (r_field, src/trans_sim_3d.hh:202-248).
+__global__ __launch_bounds__(kT) void k_rhs_trans(Grid g, Xform X, const double *__restrict__ sig,
+                                                  const double *__restrict__ vel, double *__restrict__ x,
+                                                  double *__restrict__ r) {
+    const long p = (long)blockIdx.x * kT + threadIdx.x;
+    if (p >= g.Nv) return;
+    const int i = (int)(p % g.Nx);
+    const int j = (int)((p / g.Nx) % g.Ny);
+    const int k = (int)(p / ((long)g.Nx * g.Ny));
+#pragma unroll
+    for (int q = 0; q < 3; q++) x[q * g.Nv + p] = vel[q * g.Nv + p];
+    double o[3] = {0, 0, 0};
+    if (g.pz || (k != 0 && k != g.Nz)) {
+        const double ix = .25 * (1.0 / g.dx), iy = .25 * (1.0 / g.dy), iz = .25 * (1.0 / g.dz);
+        const int km = g.pz ? wrapc(k - 1, g.Nz) : k - 1;
+        const long c000 = cidx(g, i, j, k), c100 = cidx(g, i - 1, j, k), c010 = cidx(g, i, j - 1, k),
+                   c110 = cidx(g, i - 1, j - 1, k), c001 = cidx(g, i, j, km), c101 = cidx(g, i - 1, j, km),
+                   c011 = cidx(g, i, j - 1, km), c111 = cidx(g, i - 1, j - 1, km);
+        auto DX = [&](const double *a) {
+            return ix * (a[c000] - a[c100] + a[c010] - a[c110] + a[c001] - a[c101] + a[c011] - a[c111]);
+        };
+        auto DY = [&](const double *a) {
+            return iy * (a[c000] - a[c010] + a[c100] - a[c110] + a[c001] - a[c011] + a[c101] - a[c111]);
+        };
+        auto DZ = [&](const double *a) {
+            return iz * (a[c000] - a[c001] + a[c100] - a[c101] + a[c010] - a[c011] + a[c110] - a[c111]);
+        };
+        const double *S11 = sig, *S12 = sig + g.Nc, *S13 = sig + 2 * g.Nc, *S22 = sig + 3 * g.Nc,
+                     *S23 = sig + 4 * g.Nc, *S33 = sig + 5 * g.Nc;
+        const double d[3] = {DX(S11) + DY(S12) + DZ(S13), DX(S12) + DY(S22) + DZ(S23),
+                             DX(S13) + DY(S23) + DZ(S33)};
+        m3vec(X.T, d, o);
+    }
+    r[p] = o[0];
+    r[g.Nv + p] = o[1];
+    r[2 * g.Nv + p] = o[2];
+}
+
+// sigma' += dt T^-1 C:D(L) T^-t with L at the cell's lowest vertex position
+// (projection_step, src/trans_sim_3d.cpp:1096-1118).
+__global__ __launch_bounds__(kT) void k_stress_update_trans(Grid g, Mat M, double dt, Xform X, Box B,
+                                                            const double *__restrict__ vel,
+                                                            double *__restrict__ sig) {
+    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    if (c >= g.Nc) return;
+    const int i = (int)(c % g.Nx);
+    const int j = (int)((c / g.Nx) % g.Ny);
+    const int k = (int)(c / ((long)g.Nx * g.Ny));
+    const double P0[3] = {B.ax + (double)i * g.dx, B.ay + (double)j * g.dy, B.az + (double)k * g.dz};
+    double L[9], Lt[9], CD[9], A[9], Bm[9], Tti[9];
+    compute_L(vel, g, B, X, i, j, k, P0, L);
+    m3t(L, Lt);
+    m3t(X.Ti, Tti);
+    const double tr = L[0] + L[4] + L[8];
+#pragma unroll
+    for (int e = 0; e < 9; e++) CD[e] = ((e % 4 == 0) ? M.lamb * tr : 0.0) + M.mu * (L[e] + Lt[e]);
+#pragma unroll
+    for (int e = 0; e < 9; e++) A[e] = X.Ti[e] * dt;
+    m3mul(A, CD, Bm);
+    m3mul(Bm, Tti, A);
+    const int comp[6] = {0, 1, 2, 4, 5, 8};
+#pragma unroll
+    for (int q = 0; q < 6; q++) sig[q * g.Nc + c] += A[comp[q]];
+}
+
 // Host AoS <-> device SoA for arbitrary component counts.
 __global__ void k_aos2soa(const double *__restrict__ a, double *__restrict__ s, long n, int nc) {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;

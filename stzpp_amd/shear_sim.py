@@ -41,6 +41,21 @@ class StepError(MultigridError):
     pass
 
 
+# transform families of trans_sim_3d (src/trans_sim_3d.hh:57-125); include/stz.h STZ_T_*
+TRANSFORMS = {"none": 0, "pure_shear": 1, "simple_shear": 2, "homogeneous": 3, "identity": 4}
+
+
+def transform(family, Ut, t, bz=0.5):
+    """(T, T^-1, dT/dt) of a transform family at time t, from the library."""
+    L = _lib.load()
+    T, Ti, dT = np.zeros(9), np.zeros(9), np.zeros(9)
+    fam = TRANSFORMS[family] if isinstance(family, str) else int(family)
+    rc = L.stz_transform(fam, Ut, bz, t, _lib.dptr(T), _lib.dptr(Ti), _lib.dptr(dT))
+    if rc:
+        raise StepError(rc, "unknown transform family")
+    return T.reshape(3, 3), Ti.reshape(3, 3), dT.reshape(3, 3)
+
+
 class ShearSim:
     def __init__(self, N_x, N_y, N_z, params=None, **kw):
         self.L = _lib.load()
@@ -56,6 +71,8 @@ class ShearSim:
                     p.prd[a] = int(v[a])
             elif k in ("Nx", "Ny", "Nz"):
                 continue
+            elif k == "transform":
+                p.transform = TRANSFORMS[v] if isinstance(v, str) else int(v)
             else:
                 setattr(p, k, float(v))
         self.params = prm
@@ -141,5 +158,24 @@ class ShearSim:
         return self.step_forward_qs(dt, steps, **kw)
 
 
-__all__ = ["ShearSim", "reference_params", "config0_params", "SOLVE_REF", "SOLVE_RELRES",
-           "SOLVE_FP32_IR"]
+class TransSim(ShearSim):
+    """Host-side mirror of ``trans_sim_3d`` (src/trans_sim_3d.hh:13-36): the QS
+    step in transformed coordinates X = T(t)^-1 x.  State is the transformed
+    velocity U and stress sigma'; the walls are at rest; T(t) is a transform
+    family (pure shear, the reference's compiled-in choice, by default) at
+    rate Ut.  The projection operator and its Galerkin hierarchy are rebuilt
+    on the GPU every step."""
+
+    def __init__(self, N_x, N_y, N_z, Ut, family="pure_shear", params=None, **kw):
+        super().__init__(N_x, N_y, N_z, params, transform=family, Ut=Ut, **kw)
+        self.family = family
+        self.Ut = Ut
+
+    def matrices(self, t=None):
+        """T, T^-1, dT/dt at time t (default: the current simulation time)."""
+        t = self.state_time() if t is None else t
+        return transform(self.family, self.Ut, t, self.params.get("bz", 0.5))
+
+
+__all__ = ["ShearSim", "TransSim", "transform", "TRANSFORMS", "reference_params", "config0_params",
+           "SOLVE_REF", "SOLVE_RELRES", "SOLVE_FP32_IR"]
