@@ -77,6 +77,29 @@ int stz_get_state(stz_t s, double *t, double *u, double *sig, double *chi, doubl
 int stz_step_qs(stz_t s, double dt, int nsteps, const stzmg_solve_opts *opts, stz_step_stats *st);
 /* The projection solver of the simulation (valid after the first step). */
 stzmg_t stz_multigrid(stz_t s);
+/* Adjoint of the quasi-static step (no reference counterpart).
+ *
+ * Loss J = mean over cells of (ad_Dpl / dt)^2 after the last of nsteps
+ * steps from the current state.  The forward steps solve each projection
+ * to a relative residual (so the discrete model is differentiable; the
+ * reference's sum |r|^4 loop is not), keep the state every
+ * checkpoint_every steps, and the reverse sweep recomputes one segment of
+ * steps from its checkpoint before reversing it.  The reverse projection
+ * solves with the transposed operator (the shear operator is symmetric).
+ * Branches (ENO stencils, the yield test, the plastic sub-step count) are
+ * differentiated for the branch the forward state selects.
+ * Supported: the untransformed step with walled z.  On return the state is
+ * the one after the nsteps forward steps. */
+typedef struct {
+    int checkpoint_every;  /* states kept every k steps (k >= 1) */
+    double rtol;           /* relative residual of every projection solve, forward and adjoint */
+} stz_adjoint_opts;
+void stz_adjoint_opts_default(stz_adjoint_opts *o);
+/* grad_chi0: dJ/dchi at the initial state (Nx*Ny*Nz, may be NULL);
+ * grad_params: dJ/d(sy, eps0, chi_inf) (3 doubles, may be NULL). */
+int stz_adjoint(stz_t s, double dt, int nsteps, const stz_adjoint_opts *o, double *loss, double *grad_chi0,
+                double *grad_params);
+
 /* T, T^-1 and dT/dt (row-major 3x3) of a transform family at time t
  * (compute_T, compute_Tinv, compute_dTdt, src/trans_sim_3d.hh:57-125). */
 int stz_transform(int family, double Ut, double bz, double t, double *T, double *Tinv, double *dTdt);

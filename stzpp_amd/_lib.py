@@ -57,6 +57,10 @@ class StzParams(ctypes.Structure):
         ("transform", ctypes.c_int), ("Ut", ctypes.c_double)]
 
 
+class StzAdjointOpts(ctypes.Structure):
+    _fields_ = [("checkpoint_every", ctypes.c_int), ("rtol", ctypes.c_double)]
+
+
 class StzStepStats(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int), ("vcycles", ctypes.c_int), ("yielded_cells", ctypes.c_int),
                 ("solve_ms", ctypes.c_double), ("step_ms", ctypes.c_double),
@@ -73,6 +77,9 @@ SIGNATURES = {
     "stz_get_state": ([_vp, _dp, _dp, _dp, _dp, _dp], ctypes.c_int),
     "stz_step_qs": ([_vp, ctypes.c_double, ctypes.c_int, _vp, ctypes.POINTER(StzStepStats)], ctypes.c_int),
     "stz_multigrid": ([_vp], _vp),
+    "stz_adjoint_opts_default": ([ctypes.POINTER(StzAdjointOpts)], None),
+    "stz_adjoint": ([_vp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(StzAdjointOpts), _dp, _dp, _dp],
+                    ctypes.c_int),
     "stz_transform": ([ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp],
                       ctypes.c_int),
     "stzmg_desc_default": ([ctypes.POINTER(StzmgDesc)] + [ctypes.c_int] * 6, ctypes.c_int),

@@ -23,25 +23,10 @@
 
 #include "../../include/stz.h"
 #include "mg_internal.hpp"
+#include "stz_internal.hpp"
 #include "stz_kernels.hpp"
 
 using namespace stz;
-
-struct stz_s {
-    stz_params p{};
-    Grid g{};
-    Mat M{};
-    double t = 0;
-    stzmg_t mg = nullptr;
-    double mg_dt = -1;          // dt the operator was built for
-    double a11 = 0;             // stencil[13].a11 of the current operator
-    double fix_a11 = 0;
-    hipStream_t stream = nullptr;
-    double *vel = nullptr, *sig = nullptr, *sig2 = nullptr, *chi = nullptr, *chi2 = nullptr;
-    double *dpl = nullptr, *dpl2 = nullptr, *adapt = nullptr, *dchi = nullptr, *stage = nullptr;
-    int *counters = nullptr;
-    std::string err;
-};
 
 namespace {
 
@@ -96,6 +81,20 @@ int trans_operator(stz_t s, double dt, const Xform &X) {
     return 0;
 }
 
+}  // namespace
+
+namespace stz_detail {
+
+int fail(stz_t s, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (s) s->err = buf;
+    return code;
+}
+
 // (Re)build the projection operator for step size dt
 // (set_up_stencil_and_matrices, src/shear_sim_3d.hh:45-55).
 int ensure_operator(stz_t s, double dt) {
@@ -117,6 +116,12 @@ int ensure_operator(stz_t s, double dt) {
     s->mg_dt = dt;
     return 0;
 }
+
+}  // namespace stz_detail
+
+namespace {
+
+using stz_detail::ensure_operator;
 
 }  // namespace
 
@@ -292,93 +297,112 @@ int stz_get_state(stz_t s, double *t, double *u, double *sig, double *chi, doubl
     return 0;
 }
 
+}  // extern "C"
+
+namespace stz_detail {
+
+void step_options(stz_t s, const stzmg_solve_opts *opts, stzmg_solve_opts &o) {
+    if (opts) {
+        o = *opts;
+        return;
+    }
+    stzmg_solve_opts_default(&o);
+    o.mode = STZMG_SOLVE_REF;
+    // tol = (eps * stencil[13].a11 * 1e3)^2, src/shear_sim_3d.cpp:551-555
+    const double tol = std::numeric_limits<double>::epsilon() * s->a11 * 1e3;
+    o.tol = tol * tol;
+    o.extra_cycles = 2;
+    // shear: norm after every cycle; trans: only on the std_iters schedule
+    // (src/trans_sim_3d.cpp:1020-1040)
+    o.ref_norm_schedule = s->p.transform != STZ_T_NONE ? 1 : 0;
+    o.max_cycles = 100000;
+}
+
+int step_once(stz_t s, double dt, stzmg_solve_opts &o, bool ref_tol, stz_step_stats &S, double *sigstar) {
+    const bool trans = s->p.transform != STZ_T_NONE;
+    const Grid &g = s->g;
+    const Box box{s->p.ax, s->p.ay, s->p.az, s->p.bx, s->p.by, s->p.bz};
+    SHIP(s, hipMemsetAsync(s->counters, 0, sizeof(int) * 4, s->stream));
+    if (trans) {
+        const Xform Xn = xform_at(s, s->t);
+        hipLaunchKernelGGL(k_plastic_trans, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt, Xn,
+                           (const double *)s->sig, (const double *)s->chi, s->adapt, s->dchi, s->dpl2,
+                           s->counters);
+        hipLaunchKernelGGL(k_advect_trans, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt, Xn, box,
+                           (const double *)s->vel, (const double *)s->sig, (const double *)s->chi,
+                           (const double *)s->adapt, (const double *)s->dchi, (const double *)s->dpl2,
+                           (const double *)s->dpl, s->sig2, s->chi2);
+    } else {
+        hipLaunchKernelGGL(k_plastic, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt,
+                           (const double *)s->sig, (const double *)s->chi, s->adapt, s->dchi, s->dpl2,
+                           s->counters);
+        hipLaunchKernelGGL(k_advect, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt,
+                           (const double *)s->vel, (const double *)s->sig, (const double *)s->chi,
+                           (const double *)s->adapt, (const double *)s->dchi, (const double *)s->dpl2,
+                           (const double *)s->dpl, s->sig2, s->chi2);
+    }
+    SHIP(s, hipGetLastError());
+    std::swap(s->sig, s->sig2);
+    std::swap(s->chi, s->chi2);
+    if (sigstar)
+        SHIP(s, hipMemcpyAsync(sigstar, s->sig, sizeof(double) * 6 * g.Nc, hipMemcpyDeviceToDevice, s->stream));
+    std::swap(s->dpl, s->dpl2);
+    s->t += dt;
+    const double ut = top_u(s, s->t);
+    Xform X1{};
+    if (trans) {
+        X1 = xform_at(s, s->t);
+        if (int rc = trans_operator(s, dt, X1)) return rc;
+        if (ref_tol) {
+            const double tol = std::numeric_limits<double>::epsilon() * s->a11 * 1e3;
+            o.tol = tol * tol;
+        }
+        hipLaunchKernelGGL(k_rhs_trans, dim3(nb(g.Nv)), dim3(kT), 0, s->stream, g, X1, (const double *)s->sig,
+                           (const double *)s->vel, stzmg_internal::x0(s->mg), stzmg_internal::r0(s->mg));
+    } else {
+        hipLaunchKernelGGL(k_rhs, dim3(nb(g.Nv)), dim3(kT), 0, s->stream, g, (const double *)s->sig,
+                           (const double *)s->vel, s->fix_a11, ut, stzmg_internal::x0(s->mg),
+                           stzmg_internal::r0(s->mg));
+    }
+    SHIP(s, hipGetLastError());
+    stzmg_solve_stats ms{};
+    int rc = stzmg_solve(s->mg, &o, &ms);
+    S.vcycles += ms.cycles;
+    S.solve_ms += ms.ms;
+    S.last_res = ms.res;
+    if (rc) return sfail(s, rc, "projection solve at t=%g: %s", s->t, stzmg_last_error(s->mg));
+    hipLaunchKernelGGL(k_copy_vel, dim3(nb(g.Nv)), dim3(kT), 0, s->stream, g,
+                       (const double *)stzmg_internal::x0(s->mg), ut, s->vel);
+    if (trans)
+        hipLaunchKernelGGL(k_stress_update_trans, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt, X1,
+                           box, (const double *)s->vel, s->sig);
+    else
+        hipLaunchKernelGGL(k_stress_update, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt,
+                           (const double *)s->vel, s->sig);
+    SHIP(s, hipGetLastError());
+    int cnt[4];
+    SHIP(s, hipMemcpyAsync(cnt, s->counters, sizeof(cnt), hipMemcpyDeviceToHost, s->stream));
+    SHIP(s, hipStreamSynchronize(s->stream));
+    S.yielded_cells = cnt[0];
+    S.steps++;
+    return 0;
+}
+
+}  // namespace stz_detail
+
+extern "C" {
+
 int stz_step_qs(stz_t s, double dt, int nsteps, const stzmg_solve_opts *opts, stz_step_stats *st) {
     if (!s || !(dt > 0) || nsteps < 0) return STZMG_ERR_ARG;
     const bool trans = s->p.transform != STZ_T_NONE;
     if (!trans)
         if (int rc = ensure_operator(s, dt)) return rc;
-    const Grid &g = s->g;
-    const Box box{s->p.ax, s->p.ay, s->p.az, s->p.bx, s->p.by, s->p.bz};
     stz_step_stats S{};
     auto t0 = std::chrono::steady_clock::now();
     stzmg_solve_opts o;
-    if (opts) {
-        o = *opts;
-    } else {
-        stzmg_solve_opts_default(&o);
-        o.mode = STZMG_SOLVE_REF;
-        // tol = (eps * stencil[13].a11 * 1e3)^2, src/shear_sim_3d.cpp:551-555
-        const double tol = std::numeric_limits<double>::epsilon() * s->a11 * 1e3;
-        o.tol = tol * tol;
-        o.extra_cycles = 2;
-        // shear: norm after every cycle; trans: only on the std_iters
-        // schedule (src/trans_sim_3d.cpp:1020-1040)
-        o.ref_norm_schedule = trans ? 1 : 0;
-        o.max_cycles = 100000;
-    }
-    for (int q = 0; q < nsteps; q++) {
-        SHIP(s, hipMemsetAsync(s->counters, 0, sizeof(int) * 4, s->stream));
-        if (trans) {
-            const Xform Xn = xform_at(s, s->t);
-            hipLaunchKernelGGL(k_plastic_trans, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt, Xn,
-                               (const double *)s->sig, (const double *)s->chi, s->adapt, s->dchi, s->dpl2,
-                               s->counters);
-            hipLaunchKernelGGL(k_advect_trans, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt, Xn, box,
-                               (const double *)s->vel, (const double *)s->sig, (const double *)s->chi,
-                               (const double *)s->adapt, (const double *)s->dchi, (const double *)s->dpl2,
-                               (const double *)s->dpl, s->sig2, s->chi2);
-        } else {
-            hipLaunchKernelGGL(k_plastic, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt,
-                               (const double *)s->sig, (const double *)s->chi, s->adapt, s->dchi, s->dpl2,
-                               s->counters);
-            hipLaunchKernelGGL(k_advect, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt,
-                               (const double *)s->vel, (const double *)s->sig, (const double *)s->chi,
-                               (const double *)s->adapt, (const double *)s->dchi, (const double *)s->dpl2,
-                               (const double *)s->dpl, s->sig2, s->chi2);
-        }
-        SHIP(s, hipGetLastError());
-        std::swap(s->sig, s->sig2);
-        std::swap(s->chi, s->chi2);
-        std::swap(s->dpl, s->dpl2);
-        s->t += dt;
-        const double ut = top_u(s, s->t);
-        Xform X1{};
-        if (trans) {
-            X1 = xform_at(s, s->t);
-            if (int rc = trans_operator(s, dt, X1)) return rc;
-            if (!opts) {
-                const double tol = std::numeric_limits<double>::epsilon() * s->a11 * 1e3;
-                o.tol = tol * tol;
-            }
-            hipLaunchKernelGGL(k_rhs_trans, dim3(nb(g.Nv)), dim3(kT), 0, s->stream, g, X1, (const double *)s->sig,
-                               (const double *)s->vel, stzmg_internal::x0(s->mg), stzmg_internal::r0(s->mg));
-        } else {
-            hipLaunchKernelGGL(k_rhs, dim3(nb(g.Nv)), dim3(kT), 0, s->stream, g, (const double *)s->sig,
-                               (const double *)s->vel, s->fix_a11, ut, stzmg_internal::x0(s->mg),
-                               stzmg_internal::r0(s->mg));
-        }
-        SHIP(s, hipGetLastError());
-        stzmg_solve_stats ms{};
-        int rc = stzmg_solve(s->mg, &o, &ms);
-        S.vcycles += ms.cycles;
-        S.solve_ms += ms.ms;
-        S.last_res = ms.res;
-        if (rc) return sfail(s, rc, "projection solve at t=%g: %s", s->t, stzmg_last_error(s->mg));
-        hipLaunchKernelGGL(k_copy_vel, dim3(nb(g.Nv)), dim3(kT), 0, s->stream, g,
-                           (const double *)stzmg_internal::x0(s->mg), ut, s->vel);
-        if (trans)
-            hipLaunchKernelGGL(k_stress_update_trans, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt, X1,
-                               box, (const double *)s->vel, s->sig);
-        else
-            hipLaunchKernelGGL(k_stress_update, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt,
-                               (const double *)s->vel, s->sig);
-        SHIP(s, hipGetLastError());
-        int cnt[4];
-        SHIP(s, hipMemcpyAsync(cnt, s->counters, sizeof(cnt), hipMemcpyDeviceToHost, s->stream));
-        SHIP(s, hipStreamSynchronize(s->stream));
-        S.yielded_cells = cnt[0];
-        S.steps++;
-    }
+    stz_detail::step_options(s, opts, o);
+    for (int q = 0; q < nsteps; q++)
+        if (int rc = stz_detail::step_once(s, dt, o, trans && !opts, S, nullptr)) return rc;
     S.step_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (st) *st = S;
     return 0;

@@ -1,0 +1,42 @@
+// Handle of the quasi-static step (include/stz.h) and the pieces of the
+// step shared by the forward driver (stz.hip) and the adjoint (stz_adjoint.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/stz.h"
+#include "stz_kernels.hpp"
+
+struct stz_s {
+    stz_params p{};
+    stz::Grid g{};
+    stz::Mat M{};
+    double t = 0;
+    stzmg_t mg = nullptr;
+    double mg_dt = -1;          // dt the operator was built for
+    double a11 = 0;             // stencil[13].a11 of the current operator
+    double fix_a11 = 0;
+    hipStream_t stream = nullptr;
+    double *vel = nullptr, *sig = nullptr, *sig2 = nullptr, *chi = nullptr, *chi2 = nullptr;
+    double *dpl = nullptr, *dpl2 = nullptr, *adapt = nullptr, *dchi = nullptr, *stage = nullptr;
+    int *counters = nullptr;
+    std::string err;
+};
+
+namespace stz_detail {
+
+// Records the message on the handle and returns code.
+int fail(stz_t s, int code, const char *fmt, ...);
+// Projection operator of the untransformed step for step size dt.
+int ensure_operator(stz_t s, double dt);
+// Solve options of a step call: opts, or the reference's loop when null.
+void step_options(stz_t s, const stzmg_solve_opts *opts, stzmg_solve_opts &o);
+// One quasi-static step on the handle's stream.  ref_tol: recompute the
+// reference tolerance from the current operator (transformed step).  When
+// sigstar is not null it receives the advected stress sigma* (6 Nc doubles,
+// device) that enters the projection.
+int step_once(stz_t s, double dt, stzmg_solve_opts &o, bool ref_tol, stz_step_stats &S, double *sigstar);
+
+}  // namespace stz_detail

@@ -136,7 +136,7 @@ __device__ __forceinline__ double eno2(double h, double v, double f2p, double fp
 // advection_grid_step before the advective terms, src/shear_sim_3d.cpp:
 // 424-434).  Writes adapt = 2 mu dt Dpl/sbar, the plastic chi change and
 // this step's ad_Dpl.
-__global__ __launch_bounds__(kT) void k_plastic(Grid g, Mat M, double dt, const double *__restrict__ sig,
+static __global__ __launch_bounds__(kT) void k_plastic(Grid g, Mat M, double dt, const double *__restrict__ sig,
                                                 const double *__restrict__ chi, double *__restrict__ adapt,
                                                 double *__restrict__ dchi, double *__restrict__ dpl_new,
                                                 int *__restrict__ counters) {
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(kT) void k_plastic(Grid g, Mat M, double dt, const 
 // in-place lexicographic loop: this step's value at the cell and at its
 // -x/-y/-z neighbours inside the domain, last step's value at +x/+y/+z and
 // at periodic images, zero across a wall.
-__global__ __launch_bounds__(kT) void k_advect(Grid g, Mat M, double dt, const double *__restrict__ vel,
+static __global__ __launch_bounds__(kT) void k_advect(Grid g, Mat M, double dt, const double *__restrict__ vel,
                                                const double *__restrict__ sig, const double *__restrict__ chi,
                                                const double *__restrict__ adapt, const double *__restrict__ dchi,
                                                const double *__restrict__ dnew, const double *__restrict__ dold,
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(kT) void k_advect(Grid g, Mat M, double dt, const d
 // r = div sigma* at interior vertices (backward 8-cell differences,
 // calculate_first_order_stresses, src/shear_sim_3d.cpp:980-996), and
 // (-+ fix_a11 * top_u, 0, 0) on the bottom/top plates.
-__global__ __launch_bounds__(kT) void k_rhs(Grid g, const double *__restrict__ sig, const double *__restrict__ vel,
+static __global__ __launch_bounds__(kT) void k_rhs(Grid g, const double *__restrict__ sig, const double *__restrict__ vel,
                                             double fix_a11, double top_u, double *__restrict__ x,
                                             double *__restrict__ r) {
     const long p = (long)blockIdx.x * kT + threadIdx.x;
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(kT) void k_rhs(Grid g, const double *__restrict__ s
 // Copy the solution back into the velocity field with the plates set
 // exactly (projection copy-out and set_boundary_conditions,
 // src/shear_sim_3d.cpp:615-681, :2362-2470).
-__global__ __launch_bounds__(kT) void k_copy_vel(Grid g, const double *__restrict__ x, double top_u,
+static __global__ __launch_bounds__(kT) void k_copy_vel(Grid g, const double *__restrict__ x, double top_u,
                                                  double *__restrict__ vel) {
     const long p = (long)blockIdx.x * kT + threadIdx.x;
     if (p >= g.Nv) return;
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(kT) void k_copy_vel(Grid g, const double *__restric
 }
 
 // Stress update sigma += dt C:D(u) (src/shear_sim_3d.cpp:684-712).
-__global__ __launch_bounds__(kT) void k_stress_update(Grid g, Mat M, double dt, const double *__restrict__ vel,
+static __global__ __launch_bounds__(kT) void k_stress_update(Grid g, Mat M, double dt, const double *__restrict__ vel,
                                                       double *__restrict__ sig) {
     const long c = (long)blockIdx.x * kT + threadIdx.x;
     if (c >= g.Nc) return;
@@ -412,7 +412,7 @@ __device__ __forceinline__ double sbar_phys(const double os[9]) {
 // (advection_grid_step, src/trans_sim_3d.cpp:940-969, with
 // adaptive_plastic_term :740-775, eta = M.eta).  Writes adapt, the plastic
 // chi change and this step's ad_Dpl (0 below yield).
-__global__ __launch_bounds__(kT) void k_plastic_trans(Grid g, Mat M, double dt, Xform X,
+static __global__ __launch_bounds__(kT) void k_plastic_trans(Grid g, Mat M, double dt, Xform X,
                                                       const double *__restrict__ sig,
                                                       const double *__restrict__ chi, double *__restrict__ adapt,
                                                       double *__restrict__ dchi, double *__restrict__ dpl_new,
@@ -441,7 +441,7 @@ __global__ __launch_bounds__(kT) void k_plastic_trans(Grid g, Mat M, double dt, 
 // the metric J = T^-1 T^-t (chi_diffusion, :779-812, including the mixed
 // y-z difference of :802 whose last two terms are the same cell).  ad_Dpl
 // follows the in-place order as in k_advect.
-__global__ __launch_bounds__(kT) void k_advect_trans(Grid g, Mat M, double dt, Xform X, Box B,
+static __global__ __launch_bounds__(kT) void k_advect_trans(Grid g, Mat M, double dt, Xform X, Box B,
                                                      const double *__restrict__ vel, const double *__restrict__ sig,
                                                      const double *__restrict__ chi,
                                                      const double *__restrict__ adapt,
@@ -574,7 +574,7 @@ __global__ __launch_bounds__(kT) void k_advect_trans(Grid g, Mat M, double dt, X
 
 // Right-hand side T div sigma' at interior vertices, zero on the walls, and
 // the warm start x = U (r_field, src/trans_sim_3d.hh:202-248).
-__global__ __launch_bounds__(kT) void k_rhs_trans(Grid g, Xform X, const double *__restrict__ sig,
+static __global__ __launch_bounds__(kT) void k_rhs_trans(Grid g, Xform X, const double *__restrict__ sig,
                                                   const double *__restrict__ vel, double *__restrict__ x,
                                                   double *__restrict__ r) {
     const long p = (long)blockIdx.x * kT + threadIdx.x;
@@ -613,7 +613,7 @@ __global__ __launch_bounds__(kT) void k_rhs_trans(Grid g, Xform X, const double 
 
 // sigma' += dt T^-1 C:D(L) T^-t with L at the cell's lowest vertex position
 // (projection_step, src/trans_sim_3d.cpp:1096-1118).
-__global__ __launch_bounds__(kT) void k_stress_update_trans(Grid g, Mat M, double dt, Xform X, Box B,
+static __global__ __launch_bounds__(kT) void k_stress_update_trans(Grid g, Mat M, double dt, Xform X, Box B,
                                                             const double *__restrict__ vel,
                                                             double *__restrict__ sig) {
     const long c = (long)blockIdx.x * kT + threadIdx.x;
@@ -639,12 +639,12 @@ __global__ __launch_bounds__(kT) void k_stress_update_trans(Grid g, Mat M, doubl
 }
 
 // Host AoS <-> device SoA for arbitrary component counts.
-__global__ void k_aos2soa(const double *__restrict__ a, double *__restrict__ s, long n, int nc) {
+static __global__ void k_aos2soa(const double *__restrict__ a, double *__restrict__ s, long n, int nc) {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     for (int q = 0; q < nc; q++) s[q * n + p] = a[(long)nc * p + q];
 }
-__global__ void k_soa2aos(const double *__restrict__ s, double *__restrict__ a, long n, int nc) {
+static __global__ void k_soa2aos(const double *__restrict__ s, double *__restrict__ a, long n, int nc) {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     for (int q = 0; q < nc; q++) a[(long)nc * p + q] = s[q * n + p];

@@ -152,6 +152,23 @@ class ShearSim:
         self.last_stats = {k: getattr(st, k) for k, _ in _lib.StzStepStats._fields_}
         return self.last_stats
 
+    def adjoint(self, dt, nsteps, checkpoint_every=5, rtol=1e-12):
+        """Loss J = mean (ad_Dpl/dt)^2 after nsteps steps from the current state
+        and its gradient: returns (J, dJ/dchi0 per cell, dJ/d(s_y, eps0, chi_inf)).
+        Every projection, forward and adjoint, is solved to relative residual
+        rtol; states are kept every checkpoint_every steps and segments are
+        recomputed in the reverse sweep.  The state afterwards is the state
+        after the nsteps forward steps."""
+        o = _lib.StzAdjointOpts()
+        self.L.stz_adjoint_opts_default(ctypes.byref(o))
+        o.checkpoint_every, o.rtol = int(checkpoint_every), float(rtol)
+        J = ctypes.c_double()
+        g = np.zeros(self.ncells)
+        gp = np.zeros(3)
+        self._chk(self.L.stz_adjoint(self.h, dt, nsteps, ctypes.byref(o), ctypes.byref(J), _lib.dptr(g),
+                                     _lib.dptr(gp)))
+        return J.value, g, gp
+
     def solve_qs(self, steps, dt, **kw):
         """The QS time loop (solve_qs, src/shear_sim_3d.cpp:336-377) without
         file output: ``steps`` steps of size dt."""
