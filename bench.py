@@ -77,6 +77,82 @@ def cpu_baseline(n, st, b, budget_s=30.0):
     return dt
 
 
+def step_workload(args):
+    """QS steps of shear_sim_3d at 192 x 192 x 96 cells (MG 192 x 192 x 97),
+    config-0 material, chi0 = 0.09 + 0.01 GRF(seed 3, l = 4 cells), dt = 0.005,
+    the reference's projection loop; value = steps per second."""
+    import ctypes  # noqa: F401
+
+    from stzpp_amd import fields
+    from stzpp_amd import shear_sim as SS
+
+    Nx, Ny, Nz = 192, 192, 96
+    p = SS.config0_params(Nx)
+    sim = SS.ShearSim(Nx, Ny, Nz, params=p)
+    chi0 = 0.09 + 0.01 * fields.grf((Nz, Ny, Nx), 4.0, 3).ravel()
+    sim.set_initial_conditions(chi0)
+    dt = 0.005
+    log(f"[bench] step workload {Nx}x{Ny}x{Nz}, warmup {args.warmup}")
+    for _ in range(args.warmup):
+        sim.step_forward_qs(dt, 1)
+    t0 = time.perf_counter()
+    cyc = 0
+    for _ in range(args.steps):
+        st = sim.step_forward_qs(dt, 1)
+        cyc += st["vcycles"]
+    el = time.perf_counter() - t0
+    out = {
+        "metric": "quasi-static steps per second, shear_sim_3d 192x192x96 cells, reference projection loop",
+        "value": args.steps / el, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "shear_sim_3d QS step 192x192x96 cells (MG 192x192x97, z walled), config-0 "
+                               "material, chi0 GRF seed 3 l=4, dt 0.005",
+                   "vcycles_per_step": cyc / args.steps},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def adjoint_workload(args):
+    """Gradient of J = mean (ad_Dpl/dt)^2 after 20 QS steps at 128^3 cells
+    (MG 128 x 128 x 129) with respect to chi0 and (s_y, eps0, chi_inf);
+    checkpoints every 5 steps, projections to relative residual 1e-12;
+    value = seconds per gradient (forward + reverse)."""
+    from stzpp_amd import fields
+    from stzpp_amd import shear_sim as SS
+
+    N = 128
+    p = SS.config0_params(N)
+    p["eps0"] = 0.3
+    sim = SS.ShearSim(N, N, N, params=p)
+    chi0 = 0.09 + 0.01 * fields.grf((N, N, N), 3.0, 4).ravel()
+    nc = N ** 3
+    sig0 = np.zeros((nc, 6))
+    sig0[:, 1] = 1.05
+    ts = []
+    for q in range(args.warmup + args.steps):
+        sim.set_initial_conditions(chi0, sig=sig0)
+        t0 = time.perf_counter()
+        J, g, gp = sim.adjoint(0.01, 20, checkpoint_every=5, rtol=1e-12)
+        el = time.perf_counter() - t0
+        log(f"[bench] adjoint {q}: {el:.2f} s, J {J:.4e}")
+        if q >= args.warmup:
+            ts.append(el)
+    el = float(np.mean(ts))
+    out = {
+        "metric": "seconds per adjoint gradient, 20 QS steps at 128^3 cells, checkpoints every 5",
+        "value": el, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": el * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "adjoint of 20 shear_sim_3d QS steps, 128^3 cells (MG 128x128x129), "
+                               "eps0 0.3, s12 pre-stress 1.05, chi0 GRF seed 4 l=3, dt 0.01, RELRES 1e-12",
+                   "loss": J, "grad_params": gp.tolist()},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,11 +163,18 @@ def main():
     ap.add_argument("--inner", type=int, default=3)
     ap.add_argument("--no-graph", action="store_true",
                     help="launch V-cycles kernel by kernel instead of replaying a hipGraph")
+    ap.add_argument("--workload", choices=("solve", "step", "adjoint"), default="solve",
+                    help="solve (default): the 256^3 projection solve; step: QS steps at 192x192x96; "
+                         "adjoint: gradient of 20 QS steps at 128^3")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "step":
+        return step_workload(args)
+    if args.workload == "adjoint":
+        return adjoint_workload(args)
 
     from stzpp_amd import multigrid as MG
 
