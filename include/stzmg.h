@@ -52,7 +52,10 @@ enum { STZMG_NORM_QUARTIC = 0, STZMG_NORM_L2SQ = 1 };
 enum {
     STZMG_SOLVE_REF = 0,     /* reference projection loop (sum |r|^4 <= tol, + extra cycles) */
     STZMG_SOLVE_RELRES = 1,  /* FP64 V-cycles to ||r||_2 <= rtol ||b - A x0||_2 */
-    STZMG_SOLVE_FP32_IR = 2  /* FP32 V-cycles on the defect inside FP64 refinement */
+    STZMG_SOLVE_FP32_IR = 2, /* FP32 V-cycles on the defect inside FP64 refinement */
+    STZMG_SOLVE_FP32_GMRES = 3 /* flexible GMRES(restart) in FP64, right-preconditioned by
+                                  inner_cycles FP32 V-cycles from zero (GMRES-based
+                                  iterative refinement); stops on the true FP64 residual */
 };
 
 typedef struct {
@@ -75,11 +78,12 @@ typedef struct {
     int max_cycles;       /* cap on V-cycles (the reference loop has none) */
     int inner_cycles;     /* FP32_IR: FP32 V-cycles per refinement step */
     int use_graph;        /* replay V-cycles from a captured hipGraph */
+    int restart;          /* FP32_GMRES: Krylov vectors per restart (1..32) */
 } stzmg_solve_opts;
 
 typedef struct {
     int cycles;           /* V-cycles run (FP64 or FP32) */
-    int outer;            /* FP64 refinement steps (FP32_IR) */
+    int outer;            /* FP64 refinement steps (FP32_IR) or restarts (FP32_GMRES) */
     int norms;            /* residual-norm evaluations */
     double res0;          /* initial ||r||_2 (RELRES/FP32_IR) or sum |r|^4 (REF) */
     double res;           /* final value of the same quantity */

@@ -38,6 +38,7 @@ class StzmgSolveOpts(ctypes.Structure):
         ("max_cycles", ctypes.c_int),
         ("inner_cycles", ctypes.c_int),
         ("use_graph", ctypes.c_int),
+        ("restart", ctypes.c_int),
     ]
 
 

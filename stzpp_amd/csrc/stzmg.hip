@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/stzmg.h"
@@ -19,6 +20,7 @@
 #include "mg_internal.hpp"
 #include "mg_fused.hpp"
 #include "mg_kernels.hpp"
+#include "mg_krylov.hpp"
 
 using namespace stzmg;
 
@@ -60,6 +62,13 @@ struct stzmg_s {
     long coarse_max = kCoarseMaxPoints;  // levels this small run in one workgroup (0: off)
     int fused_smoother = 1;   // two-launch fused sweeps on eligible levels
     int tiled_resid = 1;      // plane-marching residual on eligible levels
+    // flexible GMRES workspace (allocated on first use)
+    double *kV = nullptr;     // (restart + 1) FP64 basis vectors
+    float *kZ = nullptr;      // restart FP32 preconditioned vectors
+    double *ktmp = nullptr;   // one FP64 vector
+    double *kpart = nullptr;  // dot partials
+    double *kdev = nullptr;   // reduced dots / coefficients
+    int kcap = 0;             // restart length the workspace holds
     hipGraphExec_t g64 = nullptr, g32 = nullptr;
 };
 
@@ -517,6 +526,21 @@ int norms64(stzmg_t h, int l, double out[2]) {
     return 0;
 }
 
+// Workspace of the flexible GMRES solve for restart length m.
+int ensure_krylov(stzmg_t h, int m, long n3) {
+    if (h->kcap >= m) return 0;
+    hipFree(h->kV); hipFree(h->kZ); hipFree(h->ktmp); hipFree(h->kpart); hipFree(h->kdev);
+    h->kV = nullptr; h->kZ = nullptr; h->ktmp = nullptr; h->kpart = nullptr; h->kdev = nullptr;
+    h->kcap = 0;
+    HIPCHK(h, alloc(&h->kV, (long)(m + 1) * n3));
+    HIPCHK(h, alloc(&h->kZ, (long)m * n3));
+    HIPCHK(h, alloc(&h->ktmp, n3));
+    HIPCHK(h, alloc(&h->kpart, (long)kDotBlocks * (kKrylovMax + 1)));
+    HIPCHK(h, alloc(&h->kdev, 2L * (kKrylovMax + 1)));
+    h->kcap = m;
+    return 0;
+}
+
 int check_level(stzmg_t h, int l) {
     if (!h) return STZMG_ERR_ARG;
     if (l < 0 || l >= (int)h->lv.size()) return fail(h, STZMG_ERR_ARG, "level %d out of range", l);
@@ -728,6 +752,7 @@ int stzmg_destroy(stzmg_t h) {
     hipFree(h->part);
     hipFree(h->dres);
     hipFree(h->dflag);
+    hipFree(h->kV); hipFree(h->kZ); hipFree(h->ktmp); hipFree(h->kpart); hipFree(h->kdev);
     hipStreamDestroy(h->stream);
     delete h;
     return 0;
@@ -973,6 +998,7 @@ void stzmg_solve_opts_default(stzmg_solve_opts *o) {
     o->max_cycles = 1000;
     o->inner_cycles = 3;
     o->use_graph = 1;
+    o->restart = 10;
 }
 
 int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) {
@@ -1058,6 +1084,104 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
             if ((rc = norms64(h, 0, v))) return rc;
             s.norms++;
             s.res = std::sqrt(v[1]);
+            if (!std::isfinite(s.res)) { rc = STZMG_ERR_NONFINITE; break; }
+        }
+    } else if (opts->mode == STZMG_SOLVE_FP32_GMRES) {
+        // Right-preconditioned flexible GMRES in FP64 (Saad's FGMRES):
+        // z_j = M^-1 v_j is inner_cycles FP32 V-cycles from zero on
+        // sigma v_j (sigma = sqrt(3N) gives the FP32 right-hand side entries
+        // of order one), kept in FP32; w = A z_j in FP64; classical
+        // Gram-Schmidt with one fused dot pass and one fused update+norm pass;
+        // Givens rotations on the host.  Each restart updates x with the FP32
+        // vectors and recomputes the true FP64 residual, which decides.
+        Level &L = h->lv[0];
+        const long n3 = 3 * L.d.N;
+        const int m = std::min(std::max(opts->restart, 1), kKrylovMax);
+        if ((rc = ensure_krylov(h, m, n3))) return rc;
+        if ((rc = norms64(h, 0, v))) return rc;  // t64 = r64 - A x64
+        s.norms++;
+        double beta = std::sqrt(v[1]);
+        s.res0 = s.res = beta;
+        const double target = opts->rtol * beta;
+        const int inner = opts->inner_cycles > 0 ? opts->inner_cycles : 1;
+        const double sigma = std::sqrt((double)n3);
+        if (!std::isfinite(beta)) rc = STZMG_ERR_NONFINITE;
+        std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), hc(m + 2);
+        auto Hc = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
+        const unsigned nb3 = nblk(n3);
+        while (rc == 0 && s.res > target && beta > 0) {
+            hipLaunchKernelGGL(k_scale, dim3(nb3), dim3(kThreads), 0, h->stream, h->kV, (const double *)L.t64,
+                               1.0 / beta, n3);
+            std::fill(g.begin(), g.end(), 0.0);
+            g[0] = beta;
+            int nj = 0;
+            for (int j = 0; j < m; j++) {
+                if (s.cycles >= opts->max_cycles) { rc = STZMG_ERR_NOT_CONVERGED; break; }
+                double *vj = h->kV + (long)j * n3, *w = h->kV + (long)(j + 1) * n3;
+                float *zj = h->kZ + (long)j * n3;
+                // z_j = M^-1 (sigma v_j)
+                hipLaunchKernelGGL(k_defect_f32, dim3(nb3), dim3(kThreads), 0, h->stream, (const double *)vj, sigma,
+                                   L.r32, L.x32, n3);
+                if ((rc = run_vcycles<float>(h, inner, graph))) return rc;
+                s.cycles += inner;
+                HIPCHK(h, hipMemcpyAsync(zj, L.x32, sizeof(float) * n3, hipMemcpyDeviceToDevice, h->stream));
+                // w' = A z_j (the true w is w' / sigma)
+                hipLaunchKernelGGL((k_convert<float, double>), dim3(nb3), dim3(kThreads), 0, h->stream,
+                                   (const float *)zj, h->ktmp, n3);
+                launch_apply<double>(h, L, (const double *)h->ktmp, w);
+                // d_i = (w', v_i); w' -= sum d_i v_i; |w'|^2
+                hipLaunchKernelGGL(k_mdot, dim3(kDotBlocks), dim3(kThreads), 0, h->stream, (const double *)w,
+                                   (const double *)h->kV, j + 1, n3, h->kpart);
+                hipLaunchKernelGGL(k_reduce_cols, dim3(j + 1), dim3(kThreads), 0, h->stream,
+                                   (const double *)h->kpart, kDotBlocks, j + 1, h->kdev);
+                hipLaunchKernelGGL(k_maxpy_norm, dim3(kDotBlocks), dim3(kThreads), 0, h->stream, w,
+                                   (const double *)h->kV, (const double *)h->kdev, j + 1, n3, h->kpart);
+                hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(kThreads), 0, h->stream, (const double *)h->kpart,
+                                   kDotBlocks, 1, h->kdev + (j + 1));
+                LAUNCHCHK(h);
+                HIPCHK(h, hipMemcpyAsync(hc.data(), h->kdev, sizeof(double) * (j + 2), hipMemcpyDeviceToHost,
+                                         h->stream));
+                HIPCHK(h, hipStreamSynchronize(h->stream));
+                const double wn = std::sqrt(hc[j + 1]);
+                for (int i = 0; i <= j; i++) Hc(i, j) = hc[i] / sigma;
+                Hc(j + 1, j) = wn / sigma;
+                if (!std::isfinite(wn)) { rc = STZMG_ERR_NONFINITE; break; }
+                if (wn > 0)
+                    hipLaunchKernelGGL(k_scale, dim3(nb3), dim3(kThreads), 0, h->stream, w, (const double *)w,
+                                       1.0 / wn, n3);
+                // Givens rotations
+                for (int i = 0; i < j; i++) {
+                    const double a = Hc(i, j), b = Hc(i + 1, j);
+                    Hc(i, j) = cs[i] * a + sn[i] * b;
+                    Hc(i + 1, j) = -sn[i] * a + cs[i] * b;
+                }
+                const double a = Hc(j, j), b = Hc(j + 1, j), r = std::hypot(a, b);
+                cs[j] = r > 0 ? a / r : 1.0;
+                sn[j] = r > 0 ? b / r : 0.0;
+                Hc(j, j) = r;
+                Hc(j + 1, j) = 0;
+                g[j + 1] = -sn[j] * g[j];
+                g[j] = cs[j] * g[j];
+                nj = j + 1;
+                if (std::fabs(g[j + 1]) <= target || wn == 0) break;
+            }
+            if (nj == 0) break;
+            // y = H^-1 g, x += sum (y_i / sigma) z_i
+            for (int i = nj - 1; i >= 0; i--) {
+                double t = g[i];
+                for (int k2 = i + 1; k2 < nj; k2++) t -= Hc(i, k2) * y[k2];
+                y[i] = t / Hc(i, i);
+            }
+            for (int i = 0; i < nj; i++) hc[i] = y[i] / sigma;
+            HIPCHK(h, hipMemcpyAsync(h->kdev, hc.data(), sizeof(double) * nj, hipMemcpyHostToDevice, h->stream));
+            hipLaunchKernelGGL(k_xupdate, dim3(nb3), dim3(kThreads), 0, h->stream, L.x64, (const float *)h->kZ,
+                               (const double *)h->kdev, nj, n3);
+            LAUNCHCHK(h);
+            s.outer++;
+            if ((rc = norms64(h, 0, v))) return rc;
+            s.norms++;
+            beta = std::sqrt(v[1]);
+            s.res = beta;
             if (!std::isfinite(s.res)) { rc = STZMG_ERR_NONFINITE; break; }
         }
     } else {

@@ -307,6 +307,41 @@ def test_gpu_fp32_ir_matches_fp64_at_size():
     assert relerr(mg.x(0), 2 * x64) < 1e-9
 
 
+@pytest.mark.parametrize("kind", ["shear", "trans"])
+def test_gpu_fp32_gmres_matches_fp64(kind):
+    # FGMRES in FP64 preconditioned by FP32 V-cycles reaches the FP64
+    # solution: walled shear operator and the periodic transformed one
+    n = 64
+    h = 1.0 / n
+    if kind == "shear":
+        prd, o = (1, 1, 0), n + 1
+        st, fix = MG.shear_stencil(1.0, 1.5, h, h, h, 1.0)
+    else:
+        prd, o = (1, 1, 1), n
+        st, fix = MG.trans_stencil(np.array([[1.0, 0.0, 0.5], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]]),
+                                   1.0, 1.5, h, h, h, 1.0), None
+    mg = MG.Multigrid(n, n, o, prd)
+    mg.setup_matrices_const(st, fix)
+    N = n * n * o
+    b = O.det_field(N, 61)
+    if prd[2]:
+        b -= b.mean(axis=0)
+    else:
+        b[: n * n] = 0
+        b[-n * n:] = 0
+    xs = []
+    for mode in (MG.SOLVE_RELRES, MG.SOLVE_FP32_GMRES):
+        mg.setup_fields(np.zeros((N, 3)), b)
+        s = mg.solve(mode=mode, rtol=1e-11, max_cycles=400)
+        assert s["res"] <= 1e-11 * s["res0"]
+        xs.append(mg.x(0))
+        if mode == MG.SOLVE_FP32_GMRES:
+            cyc = s["cycles"]
+    a64, a32 = (x - x.mean(axis=0) if prd[2] else x for x in xs)
+    assert relerr(a32, a64) < 1e-8
+    assert cyc < 30
+
+
 def test_gpu_solve_ref_mode_reference_rule():
     n = 32
     mg, b = make_c1(n)
