@@ -78,7 +78,7 @@ typedef struct {
     int max_cycles;       /* cap on V-cycles (the reference loop has none) */
     int inner_cycles;     /* FP32_IR: FP32 V-cycles per refinement step */
     int use_graph;        /* replay V-cycles from a captured hipGraph */
-    int restart;          /* FP32_GMRES: Krylov vectors per restart (1..32) */
+    int restart;          /* FP32_GMRES: Krylov vectors per restart (1..16) */
 } stzmg_solve_opts;
 
 typedef struct {

@@ -9,22 +9,24 @@
 
 namespace stzmg {
 
-constexpr int kKrylovMax = 32;       // largest restart length
+constexpr int kKrylovMax = 16;       // largest restart length
 constexpr int kDotBlocks = 1024;     // blocks of the grid-stride dot kernels
 
-// part[b * nv + j] = sum over this block's share of w . V_j, j < nv.
+// part[b * NV + j] = sum over this block's share of w . V_j, j < NV.
+template <int NV>
 __global__ __launch_bounds__(kThreads) void k_mdot(const double *__restrict__ w, const double *__restrict__ V,
-                                                   int nv, long n3, double *__restrict__ part) {
-    double acc[kKrylovMax + 1];
+                                                   long n3, double *__restrict__ part) {
+    constexpr int nv = NV;
+    double acc[NV];
 #pragma unroll
-    for (int j = 0; j <= kKrylovMax; j++) acc[j] = 0;
+    for (int j = 0; j < NV; j++) acc[j] = 0;
     for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n3; i += (long)gridDim.x * kThreads) {
         const double wi = w[i];
 #pragma unroll
-        for (int j = 0; j <= kKrylovMax; j++)
-            if (j < nv) acc[j] += wi * V[(long)j * n3 + i];
+        for (int j = 0; j < NV; j++) acc[j] += wi * V[(long)j * n3 + i];
     }
     __shared__ double red[kThreads / 64];
+#pragma unroll
     for (int j = 0; j < nv; j++) {
         double v = acc[j];
 #pragma unroll
@@ -40,19 +42,19 @@ __global__ __launch_bounds__(kThreads) void k_mdot(const double *__restrict__ w,
     }
 }
 
-// w -= sum_j coef[j] V_j (j < nv), then part[b] = block share of |w|^2.
+// w -= sum_j coef[j] V_j (j < NV), then part[b] = block share of |w|^2.
+template <int NV>
 __global__ __launch_bounds__(kThreads) void k_maxpy_norm(double *__restrict__ w, const double *__restrict__ V,
-                                                         const double *__restrict__ coef, int nv, long n3,
+                                                         const double *__restrict__ coef, long n3,
                                                          double *__restrict__ part) {
-    double c[kKrylovMax + 1];
+    double c[NV];
 #pragma unroll
-    for (int j = 0; j <= kKrylovMax; j++) c[j] = j < nv ? coef[j] : 0.0;
+    for (int j = 0; j < NV; j++) c[j] = coef[j];
     double acc = 0;
     for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n3; i += (long)gridDim.x * kThreads) {
         double wi = w[i];
 #pragma unroll
-        for (int j = 0; j <= kKrylovMax; j++)
-            if (j < nv) wi -= c[j] * V[(long)j * n3 + i];
+        for (int j = 0; j < NV; j++) wi -= c[j] * V[(long)j * n3 + i];
         w[i] = wi;
         acc += wi * wi;
     }
@@ -65,6 +67,29 @@ __global__ __launch_bounds__(kThreads) void k_maxpy_norm(double *__restrict__ w,
         double t = 0;
         for (int q = 0; q < kThreads / 64; q++) t += red[q];
         part[blockIdx.x] = t;
+    }
+}
+
+// Launch of the NV-vector kernels for a run-time count nv (1..kKrylovMax+1).
+template <int NV>
+void launch_mdot_n(hipStream_t st, const double *w, const double *V, int nv, long n3, double *part) {
+    if constexpr (NV <= kKrylovMax + 1) {
+        if (nv == NV) {
+            hipLaunchKernelGGL(k_mdot<NV>, dim3(kDotBlocks), dim3(kThreads), 0, st, w, V, n3, part);
+            return;
+        }
+        launch_mdot_n<NV + 1>(st, w, V, nv, n3, part);
+    }
+}
+template <int NV>
+void launch_maxpy_n(hipStream_t st, double *w, const double *V, const double *coef, int nv, long n3,
+                    double *part) {
+    if constexpr (NV <= kKrylovMax + 1) {
+        if (nv == NV) {
+            hipLaunchKernelGGL(k_maxpy_norm<NV>, dim3(kDotBlocks), dim3(kThreads), 0, st, w, V, coef, n3, part);
+            return;
+        }
+        launch_maxpy_n<NV + 1>(st, w, V, coef, nv, n3, part);
     }
 }
 

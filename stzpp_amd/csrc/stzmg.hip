@@ -1130,12 +1130,11 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
                                    (const float *)zj, h->ktmp, n3);
                 launch_apply<double>(h, L, (const double *)h->ktmp, w);
                 // d_i = (w', v_i); w' -= sum d_i v_i; |w'|^2
-                hipLaunchKernelGGL(k_mdot, dim3(kDotBlocks), dim3(kThreads), 0, h->stream, (const double *)w,
-                                   (const double *)h->kV, j + 1, n3, h->kpart);
+                launch_mdot_n<1>(h->stream, (const double *)w, (const double *)h->kV, j + 1, n3, h->kpart);
                 hipLaunchKernelGGL(k_reduce_cols, dim3(j + 1), dim3(kThreads), 0, h->stream,
                                    (const double *)h->kpart, kDotBlocks, j + 1, h->kdev);
-                hipLaunchKernelGGL(k_maxpy_norm, dim3(kDotBlocks), dim3(kThreads), 0, h->stream, w,
-                                   (const double *)h->kV, (const double *)h->kdev, j + 1, n3, h->kpart);
+                launch_maxpy_n<1>(h->stream, w, (const double *)h->kV, (const double *)h->kdev, j + 1, n3,
+                                  h->kpart);
                 hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(kThreads), 0, h->stream, (const double *)h->kpart,
                                    kDotBlocks, 1, h->kdev + (j + 1));
                 LAUNCHCHK(h);
