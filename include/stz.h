@@ -85,11 +85,15 @@ stzmg_t stz_multigrid(stz_t s);
  * reference's sum |r|^4 loop is not), keep the state every
  * checkpoint_every steps, and the reverse sweep recomputes one segment of
  * steps from its checkpoint before reversing it.  The reverse projection
- * solves with the transposed operator (the shear operator is symmetric).
+ * solves with the transposed operator: the shear operator is symmetric and
+ * is reused; the transformed step's operator at T(t) is not, and its
+ * transpose is loaded for each step's adjoint solve.
  * Branches (ENO stencils, the yield test, the plastic sub-step count) are
  * differentiated for the branch the forward state selects.
- * Supported: the untransformed step with walled z.  On return the state is
- * the one after the nsteps forward steps. */
+ * Supported: the untransformed and the transformed step (every transform
+ * family) with walled z; periodic z is rejected (its projection is
+ * singular).  On return the state is the one after the nsteps forward
+ * steps. */
 typedef struct {
     int checkpoint_every;  /* states kept every k steps (k >= 1) */
     double rtol;           /* relative residual of every projection solve, forward and adjoint */

@@ -468,6 +468,7 @@ def _step_lib():
         L.sto_vcycles.restype = ctypes.c_int
         L.sto_set_solver.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L.sto_set_transform.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.sto_set_eta.argtypes = [vp, ctypes.c_double]
         L.sto_set_textbook.argtypes = [vp, ctypes.c_int]
         L.sto_trans_stencil.argtypes = [_dp] + [ctypes.c_double] * 5 + [_dp]
         L.sto_transform_at.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -547,6 +548,10 @@ class OracleShear:
         """Run the transformed-coordinate step of trans_sim_3d with transform
         family ``family`` (TRANSFORMS) at rate Ut."""
         self.L.sto_set_transform(self.h, TRANSFORMS[family] if isinstance(family, str) else family, Ut)
+
+    def set_eta(self, eta):
+        """Override the adaptive-step tolerance eta of the plastic update."""
+        self.L.sto_set_eta(self.h, eta)
 
     @property
     def vcycles(self):

@@ -809,6 +809,11 @@ void sto_set_transform(void *hp, int fam, double Ut) {
     S->eta = fam ? 0.002 : 0.0005;
 }
 
+/* Adaptive-step tolerance eta of the plastic update (set by
+ * sto_set_transform to the reference's value for the step; a test may
+ * override it, as the library's stz_params.eta does). */
+void sto_set_eta(void *hp, double eta) { ((sto_sim *)hp)->eta = eta; }
+
 /* Test switch: 1 replaces the reference's chi x-ENO argument (me[1] as
  * f(+2), src/trans_sim_3d.cpp:985) by the textbook me[2], so that a test can
  * show the reference fixtures tell the two apart. */

@@ -12,6 +12,7 @@
 // multigrid handle's HIP stream; the state never leaves HBM between steps.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -65,12 +66,21 @@ Xform xform_at(const stz_t s, double t) {
 
 // Transformed operator at T(t+dt): stencil = -dt s_ent(T), fix on the walls
 // (set_up_stencil_and_matrices, src/trans_sim_3d.cpp:1187-1489), with the
-// Galerkin hierarchy rebuilt on the GPU.
-int trans_operator(stz_t s, double dt, const Xform &X) {
+// Galerkin hierarchy rebuilt on the GPU.  transpose: load the transposed
+// stencil (block(o) <- block(-o)^T) instead; with the same diagonal wall rows
+// its interior block is the transpose of the forward operator's.
+int trans_operator(stz_t s, double dt, const Xform &X, bool transpose = false) {
     const Grid &g = s->g;
     double B[243], F[9] = {0};
     int rc = stzmg_trans_stencil(X.T, s->p.mu, s->p.lamb, g.dx, g.dy, g.dz, dt, B);
     if (rc) return sfail(s, rc, "transformed stencil");
+    if (transpose) {
+        double Bt[243];
+        for (int b = 0; b < 27; b++)
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) Bt[9 * b + 3 * r + c] = B[9 * (26 - b) + 3 * c + r];
+        std::copy(Bt, Bt + 243, B);
+    }
     const double fv = -dt * 2 * ((s->p.lamb + 2 * s->p.mu) / g.dx / g.dx + s->p.mu / g.dy / g.dy + s->p.mu / g.dz / g.dz);
     F[0] = F[4] = F[8] = fv;
     rc = stzmg_set_operator_const(s->mg, B, g.pz ? nullptr : F);
@@ -84,6 +94,10 @@ int trans_operator(stz_t s, double dt, const Xform &X) {
 }  // namespace
 
 namespace stz_detail {
+
+Xform xform(const stz_t s, double t) { return xform_at(s, t); }
+
+int transposed_trans_operator(stz_t s, double dt, double t) { return trans_operator(s, dt, xform_at(s, t), true); }
 
 int fail(stz_t s, int code, const char *fmt, ...) {
     char buf[512];

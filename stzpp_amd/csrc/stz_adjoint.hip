@@ -3,7 +3,10 @@
 // steps; the reverse sweep restores a checkpoint, recomputes its segment of
 // steps keeping each step's input state and advected stress, and reverses
 // the steps of the segment last to first with the kernels of stz_adjoint.hpp.
-// The adjoint projection solve uses the simulation's multigrid handle.
+// The adjoint projection solve uses the simulation's multigrid handle: the
+// shear operator is symmetric and is reused; the transformed step's operator
+// is not (its stencil at T(t) has block(-o) != block(o)^T), so the transposed
+// operator of each step is loaded before that step's adjoint solve.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -106,8 +109,6 @@ void stz_adjoint_opts_default(stz_adjoint_opts *o) {
 int stz_adjoint(stz_t s, double dt, int nsteps, const stz_adjoint_opts *opts, double *loss, double *grad_chi0,
                 double *grad_params) {
     if (!s || !(dt > 0) || nsteps < 1) return STZMG_ERR_ARG;
-    if (s->p.transform != STZ_T_NONE)
-        return stz_detail::fail(s, STZMG_ERR_UNSUPPORTED, "adjoint of the transformed step is not built");
     if (s->g.pz)
         return stz_detail::fail(s, STZMG_ERR_UNSUPPORTED,
                                 "adjoint needs walled z (the periodic projection is singular)");
@@ -115,10 +116,14 @@ int stz_adjoint(stz_t s, double dt, int nsteps, const stz_adjoint_opts *opts, do
     stz_adjoint_opts_default(&ao);
     if (opts) ao = *opts;
     if (ao.checkpoint_every < 1 || !(ao.rtol > 0)) return STZMG_ERR_ARG;
-    if (int rc = stz_detail::ensure_operator(s, dt)) return rc;
-    if (!stencil_symmetric(s, dt))
-        return stz_detail::fail(s, STZMG_ERR_UNSUPPORTED, "projection operator is not symmetric");
+    const bool trans = s->p.transform != STZ_T_NONE;
+    if (!trans) {
+        if (int rc = stz_detail::ensure_operator(s, dt)) return rc;
+        if (!stencil_symmetric(s, dt))
+            return stz_detail::fail(s, STZMG_ERR_UNSUPPORTED, "projection operator is not symmetric");
+    }
     const Grid g = s->g;
+    const Box box{s->p.ax, s->p.ay, s->p.az, s->p.bx, s->p.by, s->p.bz};
     const int K = ao.checkpoint_every;
     const int nck = (nsteps + K - 1) / K;
 
@@ -176,14 +181,24 @@ int stz_adjoint(stz_t s, double dt, int nsteps, const stz_adjoint_opts *opts, do
         }
         for (int q = s1 - 1; q >= s0; q--) {
             const State &pre = seg[q - s0];
-            // sigma' = sigma* + dt C:D(u')
+            // transforms at the advection (t) and at the projection (t + dt)
+            const double t1 = pre.t + dt;
+            const Xform Xn = trans ? stz_detail::xform(s, pre.t) : Xform{};
+            const Xform X1 = trans ? stz_detail::xform(s, t1) : Xform{};
+            // sigma' = sigma* + dt C:D(u')  (trans: dt T^-1 C:D(L) T^-t)
             AHIP(s, hipMemcpyAsync(ubp, ub, sizeof(double) * 3 * g.Nv, hipMemcpyDeviceToDevice, s->stream));
-            hipLaunchKernelGGL(k_adj_stress, dim3(nblk_c), dim3(kT), 0, s->stream, g, s->M, dt, (const double *)sb,
-                               ubp);
+            if (trans)
+                hipLaunchKernelGGL(k_adj_stress_trans, dim3(nblk_c), dim3(kT), 0, s->stream, g, s->M, dt, X1,
+                                   (const double *)sb, ubp);
+            else
+                hipLaunchKernelGGL(k_adj_stress, dim3(nblk_c), dim3(kT), 0, s->stream, g, s->M, dt,
+                                   (const double *)sb, ubp);
             // u' = A^-1 r on the interior rows
             hipLaunchKernelGGL(k_adj_wall_zero, dim3(nbk(g.Nv)), dim3(kT), 0, s->stream, g, (const double *)ubp,
                                rhs);
             AHIP(s, hipGetLastError());
+            if (trans)
+                if (int rc = stz_detail::transposed_trans_operator(s, dt, t1)) return rc;
             if (int rc = stzmg_set_fields_device(s->mg, zero, rhs))
                 return stz_detail::fail(s, rc, "adjoint solve: %s", stzmg_last_error(s->mg));
             stzmg_solve_stats ms{};
@@ -191,26 +206,45 @@ int stz_adjoint(stz_t s, double dt, int nsteps, const stz_adjoint_opts *opts, do
                 return stz_detail::fail(s, rc, "adjoint solve: %s", stzmg_last_error(s->mg));
             if (int rc = stzmg_get_x_device(s->mg, lam))
                 return stz_detail::fail(s, rc, "adjoint solve: %s", stzmg_last_error(s->mg));
-            // r = div sigma*
-            hipLaunchKernelGGL(k_adj_div, dim3(nblk_c), dim3(kT), 0, s->stream, g, (const double *)lam,
-                               (const double *)sb, ssb);
+            // r = div sigma*  (trans: T div sigma*')
+            if (trans)
+                hipLaunchKernelGGL(k_adj_div_trans, dim3(nblk_c), dim3(kT), 0, s->stream, g, X1,
+                                   (const double *)lam, (const double *)sb, ssb);
+            else
+                hipLaunchKernelGGL(k_adj_div, dim3(nblk_c), dim3(kT), 0, s->stream, g, (const double *)lam,
+                                   (const double *)sb, ssb);
             // this step's plastic quantities from its input state
-            hipLaunchKernelGGL(k_plastic, dim3(nblk_c), dim3(kT), 0, s->stream, g, s->M, dt, (const double *)pre.sig,
-                               (const double *)pre.chi, ta, tc, td, s->counters);
+            if (trans)
+                hipLaunchKernelGGL(k_plastic_trans, dim3(nblk_c), dim3(kT), 0, s->stream, g, s->M, dt, Xn,
+                                   (const double *)pre.sig, (const double *)pre.chi, ta, tc, td, s->counters);
+            else
+                hipLaunchKernelGGL(k_plastic, dim3(nblk_c), dim3(kT), 0, s->stream, g, s->M, dt,
+                                   (const double *)pre.sig, (const double *)pre.chi, ta, tc, td, s->counters);
             // advection
             AHIP(s, hipMemcpyAsync(sb2, ssb, sizeof(double) * 6 * g.Nc, hipMemcpyDeviceToDevice, s->stream));
             AHIP(s, hipMemcpyAsync(xb2, xb, sizeof(double) * g.Nc, hipMemcpyDeviceToDevice, s->stream));
             AHIP(s, hipMemcpyAsync(dnb, db, sizeof(double) * g.Nc, hipMemcpyDeviceToDevice, s->stream));
             AHIP(s, hipMemsetAsync(dob, 0, sizeof(double) * g.Nc, s->stream));
             AHIP(s, hipMemsetAsync(ub2, 0, sizeof(double) * 3 * g.Nv, s->stream));
-            hipLaunchKernelGGL(k_adj_advect, dim3(nblk_c), dim3(kT), 0, s->stream, g, s->M, dt,
-                               (const double *)pre.vel, (const double *)pre.sig, (const double *)pre.chi,
-                               (const double *)ta, (const double *)td, (const double *)pre.dpl,
-                               (const double *)ssb, (const double *)xb, sb2, xb2, ub2, dnb, dob, ab, dchib);
+            if (trans)
+                hipLaunchKernelGGL(k_adj_advect_trans, dim3(nblk_c), dim3(kT), 0, s->stream, g, s->M, dt, Xn, box,
+                                   (const double *)pre.vel, (const double *)pre.sig, (const double *)pre.chi,
+                                   (const double *)ta, (const double *)td, (const double *)pre.dpl,
+                                   (const double *)ssb, (const double *)xb, sb2, xb2, ub2, dnb, dob, ab, dchib);
+            else
+                hipLaunchKernelGGL(k_adj_advect, dim3(nblk_c), dim3(kT), 0, s->stream, g, s->M, dt,
+                                   (const double *)pre.vel, (const double *)pre.sig, (const double *)pre.chi,
+                                   (const double *)ta, (const double *)td, (const double *)pre.dpl,
+                                   (const double *)ssb, (const double *)xb, sb2, xb2, ub2, dnb, dob, ab, dchib);
             // plasticity
-            hipLaunchKernelGGL(k_adj_plastic, dim3(nblk_c), dim3(kT), 0, s->stream, g, s->M, dt,
-                               (const double *)pre.sig, (const double *)pre.chi, (const double *)ab,
-                               (const double *)dchib, (const double *)dnb, sb2, xb2, part);
+            if (trans)
+                hipLaunchKernelGGL(k_adj_plastic_trans, dim3(nblk_c), dim3(kT), 0, s->stream, g, s->M, dt, Xn,
+                                   (const double *)pre.sig, (const double *)pre.chi, (const double *)ab,
+                                   (const double *)dchib, (const double *)dnb, sb2, xb2, part);
+            else
+                hipLaunchKernelGGL(k_adj_plastic, dim3(nblk_c), dim3(kT), 0, s->stream, g, s->M, dt,
+                                   (const double *)pre.sig, (const double *)pre.chi, (const double *)ab,
+                                   (const double *)dchib, (const double *)dnb, sb2, xb2, part);
             AHIP(s, hipGetLastError());
             AHIP(s, hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, s->stream));
             AHIP(s, hipStreamSynchronize(s->stream));

@@ -15,6 +15,13 @@
 // vector atomics), so sums are formed in a run-dependent order.  The
 // plastic update is differentiated in forward mode (five directions:
 // sbar_0, chi_0, s_y, eps0, chi_inf) through the adaptive sub-stepping loop.
+//
+// The transformed step (trans_sim_3d) has the same stages with the velocity
+// gradient L = T^-t grad(dT X + T U), the L and T terms in the advection,
+// sbar of the physical stress T sigma' T^t, the right-hand side T div sigma'
+// and the stress update dt T^-1 C:D(L) T^-t; its kernels carry the _trans
+// suffix.  The metric chi diffusion is transposed term by term, including
+// the reference's y-z difference whose last two cells cancel.
 #pragma once
 
 #include "stz_kernels.hpp"
@@ -174,6 +181,16 @@ __device__ __forceinline__ void grad_add(double *ub, const Grid &g, int i, int j
                                iz * (dk ? Gb[q][2] : -Gb[q][2]) + 0.125 * vb[q];
             atomicAdd(&ub[q * g.Nv + p], val);
         }
+    }
+}
+
+// The cell-velocity part of grad_add alone (v = mean of the 8 vertices).
+__device__ __forceinline__ void grad_add_v(double *ub, const Grid &g, int i, int j, int k, const double vb[3]) {
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+        const long p = vidx(g, i + (n & 1), j + ((n >> 1) & 1), k + (n >> 2));
+#pragma unroll
+        for (int q = 0; q < 3; q++) atomicAdd(&ub[q * g.Nv + p], 0.125 * vb[q]);
     }
 }
 
@@ -451,6 +468,407 @@ static __global__ __launch_bounds__(kT) void k_adj_plastic(Grid g, Mat M, double
         }
     }
     // block sums of the parameter gradients
+    __shared__ double red[3][kT / 64];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        double v = pg[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = 0;
+        for (int w = 0; w < kT / 64; w++) t += red[threadIdx.x][w];
+        part[3 * (long)blockIdx.x + threadIdx.x] = t;
+    }
+}
+
+// ---- transformed-coordinate step (trans_sim_3d) ----------------------------
+//
+// 3x3 adjoints.  For C = A B: A-bar += C-bar B^T, B-bar += A^T C-bar.  A
+// symmetric matrix built from six components (upper triangle) collects
+// both triangles of its adjoint.
+
+__device__ __forceinline__ void mat_to_sym6_adj(const double *Mb, double *s6) {
+    s6[0] += Mb[0];
+    s6[1] += Mb[1] + Mb[3];
+    s6[2] += Mb[2] + Mb[6];
+    s6[3] += Mb[4];
+    s6[4] += Mb[5] + Mb[7];
+    s6[5] += Mb[8];
+}
+
+// Transpose of compute_L for one cell: L = T^-t G with
+// G[a][d] = i_a sum_n s_a(n) (dT P_n + T U_n)[d]; Lbar -> U-bar at the 8
+// corner vertices (the positions are constants).
+__device__ __forceinline__ void L_adj(double *ub, const Grid &g, const Xform &X, int i, int j, int k,
+                                      const double Lb[9]) {
+    // Gbar = T^-1 Lbar
+    double Gb[9];
+    m3mul(X.Ti, Lb, Gb);
+    const double ia[3] = {.25 * (1. / g.dx), .25 * (1. / g.dy), .25 * (1. / g.dz)};
+    double Tt[9];
+    m3t(X.T, Tt);
+    const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+        double vb[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            double t = 0;
+#pragma unroll
+            for (int a = 0; a < 3; a++) t += ia[a] * (off[n][a] ? Gb[3 * a + d] : -Gb[3 * a + d]);
+            vb[d] = t;
+        }
+        double u[3];
+        m3vec(Tt, vb, u);
+        const long p = vidx(g, i + off[n][0], j + off[n][1], k + off[n][2]);
+#pragma unroll
+        for (int d = 0; d < 3; d++) atomicAdd(&ub[d * g.Nv + p], u[d]);
+    }
+}
+
+// sigma' += dt T^-1 C:D(L) T^-t: U-bar += its transpose applied to sigma'-bar.
+static __global__ __launch_bounds__(kT) void k_adj_stress_trans(Grid g, Mat M, double dt, Xform X,
+                                                                const double *__restrict__ sb,
+                                                                double *__restrict__ ub) {
+    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    if (c >= g.Nc) return;
+    const int i = (int)(c % g.Nx), j = (int)((c / g.Nx) % g.Ny), k = (int)(c / ((long)g.Nx * g.Ny));
+    const int comp[6] = {0, 1, 2, 4, 5, 8};
+    double Ab[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 6; q++) Ab[comp[q]] = sb[q * g.Nc + c];
+    // A = (dt T^-1) CD T^-t  ->  CDbar = dt T^-t Abar T^-1
+    double Tti[9], tmp[9], CDb[9];
+    m3t(X.Ti, Tti);
+    m3mul(Tti, Ab, tmp);
+    m3mul(tmp, X.Ti, CDb);
+#pragma unroll
+    for (int e = 0; e < 9; e++) CDb[e] *= dt;
+    // CD = lamb tr(L) I + mu (L + L^t)
+    const double trb = CDb[0] + CDb[4] + CDb[8];
+    double Lb[9];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) Lb[3 * a + b] = M.mu * (CDb[3 * a + b] + CDb[3 * b + a]) + (a == b ? M.lamb * trb : 0.0);
+    L_adj(ub, g, X, i, j, k, Lb);
+}
+
+// r = T div sigma*' at interior vertices: sigma*'-bar = sigma'-bar + div^T (T^t lambda).
+static __global__ __launch_bounds__(kT) void k_adj_div_trans(Grid g, Xform X, const double *__restrict__ lam,
+                                                             const double *__restrict__ sb,
+                                                             double *__restrict__ ssb) {
+    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    if (c >= g.Nc) return;
+    const int i = (int)(c % g.Nx), j = (int)((c / g.Nx) % g.Ny), k = (int)(c / ((long)g.Nx * g.Ny));
+    const double ix = .25 * (1.0 / g.dx), iy = .25 * (1.0 / g.dy), iz = .25 * (1.0 / g.dz);
+    double Tt[9];
+    m3t(X.T, Tt);
+    double a11 = 0, a12 = 0, a13 = 0, a22 = 0, a23 = 0, a33 = 0;
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+        const int di = n & 1, dj = (n >> 1) & 1, dk = n >> 2;
+        const int kv = k + dk;
+        if (!g.pz && (kv == 0 || kv == g.Nz)) continue;
+        const long p = vidx(g, i + di, j + dj, kv);
+        const double lv[3] = {lam[p], lam[g.Nv + p], lam[2 * g.Nv + p]};
+        double l[3];
+        m3vec(Tt, lv, l);
+        const double sx = di ? -ix : ix, sy = dj ? -iy : iy, sz = dk ? -iz : iz;
+        a11 += sx * l[0];
+        a12 += sy * l[0] + sx * l[1];
+        a13 += sz * l[0] + sx * l[2];
+        a22 += sy * l[1];
+        a23 += sz * l[1] + sy * l[2];
+        a33 += sz * l[2];
+    }
+    ssb[c] = sb[c] + a11;
+    ssb[g.Nc + c] = sb[g.Nc + c] + a12;
+    ssb[2 * g.Nc + c] = sb[2 * g.Nc + c] + a13;
+    ssb[3 * g.Nc + c] = sb[3 * g.Nc + c] + a22;
+    ssb[4 * g.Nc + c] = sb[4 * g.Nc + c] + a23;
+    ssb[5 * g.Nc + c] = sb[5 * g.Nc + c] + a33;
+}
+
+// Transpose of k_advect_trans (see there for the forward terms).  Writes
+// adapt-bar and dchi-bar of the cell; accumulates sigma'-bar (including the
+// physical-stress part of the plastic update), chi-bar, U-bar, dnew-bar and
+// dold-bar.
+static __global__ __launch_bounds__(kT) void k_adj_advect_trans(
+    Grid g, Mat M, double dt, Xform X, Box B, const double *__restrict__ vel, const double *__restrict__ sig,
+    const double *__restrict__ chi, const double *__restrict__ adapt, const double *__restrict__ dnew,
+    const double *__restrict__ dold, const double *__restrict__ ssb, const double *__restrict__ xsb,
+    double *__restrict__ sb_out, double *__restrict__ xb_out, double *__restrict__ ub_out, double *__restrict__ dnb,
+    double *__restrict__ dob, double *__restrict__ ab, double *__restrict__ dchib) {
+    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    if (c >= g.Nc) return;
+    const int i = (int)(c % g.Nx), j = (int)((c / g.Nx) % g.Ny), k = (int)(c / ((long)g.Nx * g.Ny));
+    const double P0[3] = {B.ax + ((double)i + .5) * g.dx, B.ay + ((double)j + .5) * g.dy,
+                          B.az + ((double)k + .5) * g.dz};
+    double L[9];
+    compute_L(vel, g, B, X, i, j, k, P0, L);
+    double v[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const double *u = vel + q * g.Nv;
+        v[q] = 0.125 * (vertv(u, g, i, j, k) + vertv(u, g, i + 1, j, k) + vertv(u, g, i, j, k + 1) +
+                        vertv(u, g, i + 1, j, k + 1) + vertv(u, g, i, j + 1, k) + vertv(u, g, i + 1, j + 1, k) +
+                        vertv(u, g, i, j + 1, k + 1) + vertv(u, g, i + 1, j + 1, k + 1));
+    }
+    double sp[9], os[9], Tti[9];
+    old_sig(sig, g.Nc, c, X, sp, os);
+    m3t(X.Ti, Tti);
+    const double h[3] = {g.dx / dt, g.dy / dt, g.dz / dt};
+    const int comp[6] = {0, 1, 2, 4, 5, 8};
+    double gbar[7];
+#pragma unroll
+    for (int q = 0; q < 6; q++) gbar[q] = -ssb[q * g.Nc + c];
+    gbar[6] = -xsb[c];
+    // ENO advection of sigma' and chi (x derivative of chi: f(+1) as f(+2))
+    double vb[3] = {0, 0, 0};
+    for (int q = 0; q < 7; q++) {
+        const double *f = q < 6 ? sig + q * g.Nc : chi;
+        double *fb = q < 6 ? sb_out + q * g.Nc : xb_out;
+        const double gq = gbar[q];
+        if (gq == 0.0) continue;
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++) {
+            const int ex = ax == 0, ey = ax == 1, ez = ax == 2;
+            const bool quirk = q == 6 && ax == 0;
+            double fv[5];
+#pragma unroll
+            for (int o = 0; o < 5; o++) {
+                int s = 2 - o;
+                if (quirk && s == 2) s = 1;
+                fv[o] = cellv(f, g, i + s * ex, j + s * ey, k + s * ez);
+            }
+            double w[5];
+            eno2_w(h[ax], v[ax], fv[0], fv[1], fv[2], fv[3], fv[4], w);
+            double E = 0;
+#pragma unroll
+            for (int o = 0; o < 5; o++) E += w[o] * fv[o];
+            vb[ax] += gq * E;
+#pragma unroll
+            for (int o = 0; o < 5; o++)
+                if (w[o] != 0.0) {
+                    int s = 2 - o;
+                    if (quirk && s == 2) s = 1;
+                    cell_add(fb, g, i + s * ex, j + s * ey, k + s * ez, gq * v[ax] * w[o]);
+                }
+        }
+    }
+    // dt (L term + T term) and the plastic update of sigma*'
+    double LTb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Pb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        LTb[comp[q]] = dt * ssb[q * g.Nc + c];
+        Pb[comp[q]] = -ssb[q * g.Nc + c];
+    }
+    double spb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, osb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Lb[9];
+    double tmp[9], tmp2[9];
+    // T term = -(Mt sp + sp Mt^t), Mt = T^-1 dT
+    {
+        double Mt[9], Mtt[9];
+        m3mul(X.Ti, X.dT, Mt);
+        m3t(Mt, Mtt);
+        m3mul(Mtt, LTb, tmp);   // Mt^T LTb
+        m3mul(LTb, Mt, tmp2);   // LTb Mt
+#pragma unroll
+        for (int e = 0; e < 9; e++) spb[e] -= tmp[e] + tmp2[e];
+    }
+    // L term = T^-1 K T^-t, K = -tr(L) os + os L + L^t os
+    {
+        double Kb[9], Lt[9];
+        m3mul(Tti, LTb, tmp);
+        m3mul(tmp, X.Ti, Kb);   // Kbar = T^-t LTb T^-1
+        m3t(L, Lt);
+        const double trL = L[0] + L[4] + L[8];
+        m3mul(Kb, Lt, tmp);     // Kb L^t
+        m3mul(L, Kb, tmp2);     // L Kb
+#pragma unroll
+        for (int e = 0; e < 9; e++) osb[e] += -trL * Kb[e] + tmp[e] + tmp2[e];
+        double KbO = 0;
+#pragma unroll
+        for (int e = 0; e < 9; e++) KbO += Kb[e] * os[e];
+        double osT[9], Kbt[9];
+        m3t(os, osT);
+        m3t(Kb, Kbt);
+        m3mul(osT, Kb, tmp);    // os^T Kb   (from os L)
+        m3mul(os, Kbt, tmp2);   // os Kb^T   (from L^t os)
+#pragma unroll
+        for (int e = 0; e < 9; e++) Lb[e] = tmp[e] + tmp2[e] - (e % 4 == 0 ? KbO : 0.0);
+    }
+    // plastic update T^-1 (a dev(os)) T^-t, dev from the upper triangle
+    const double a = adapt[c];
+    double abar = 0;
+    {
+        double Qb[9];
+        m3mul(Tti, Pb, tmp);
+        m3mul(tmp, X.Ti, Qb);   // Qbar = T^-t Pbar T^-1, Q = (T^-1 a) dev T^-t -> same
+        const double third = 1. / 3., tr = os[0] + os[4] + os[8];
+        const double dv[9] = {os[0] - third * tr, os[1], os[2], os[1], os[4] - third * tr, os[5],
+                              os[2], os[5], os[8] - third * tr};
+#pragma unroll
+        for (int e = 0; e < 9; e++) abar += Qb[e] * dv[e];
+        // dev-bar -> os-bar (upper triangle and diagonal)
+        const double dtr = a * (Qb[0] + Qb[4] + Qb[8]) * third;
+        osb[0] += a * Qb[0] - dtr;
+        osb[4] += a * Qb[4] - dtr;
+        osb[8] += a * Qb[8] - dtr;
+        osb[1] += a * (Qb[1] + Qb[3]);
+        osb[2] += a * (Qb[2] + Qb[6]);
+        osb[5] += a * (Qb[5] + Qb[7]);
+    }
+    ab[c] = abar;
+    // os = T sp T^t  ->  spbar += T^t osbar T
+    {
+        double Tt[9];
+        m3t(X.T, Tt);
+        m3mul(Tt, osb, tmp);
+        m3mul(tmp, X.T, tmp2);
+#pragma unroll
+        for (int e = 0; e < 9; e++) spb[e] += tmp2[e];
+    }
+    double s6[6] = {0, 0, 0, 0, 0, 0};
+    mat_to_sym6_adj(spb, s6);
+#pragma unroll
+    for (int q = 0; q < 6; q++) atomicAdd(&sb_out[q * g.Nc + c], s6[q]);
+    L_adj(ub_out, g, X, i, j, k, Lb);
+    // chi: chi* = chi + dchi - u.grad chi + diff_J / c0
+    dchib[c] = xsb[c];
+    const double wdf = M.diff_l * M.diff_l * xsb[c] / M.c0;
+    if (wdf != 0.0) {
+        const double Dc = dnew[c];
+        const bool xin = i > 0, yin = j > 0;
+        const long nxy = (long)g.Nx * g.Ny;
+        const double Dxm = xin ? dnew[c - 1] : dold[cidx(g, i - 1, j, k)];
+        const double Dxp = dold[cidx(g, i + 1, j, k)];
+        const double Dym = yin ? dnew[c - g.Nx] : dold[cidx(g, i, j - 1, k)];
+        const double Dyp = dold[cidx(g, i, j + 1, k)];
+        double Dzm, Dzp;
+        if (g.pz) {
+            Dzm = k > 0 ? dnew[c - nxy] : dold[cidx(g, i, j, g.Nz - 1)];
+            Dzp = dold[cidx(g, i, j, wrapc(k + 1, g.Nz))];
+        } else {
+            Dzm = k > 0 ? dnew[c - nxy] : 0.0;
+            Dzp = k < g.Nz - 1 ? dold[c + nxy] : 0.0;
+        }
+        const double ix = 1. / g.dx, iy = 1. / g.dy, iz = 1. / g.dz;
+        const double x0 = chi[c];
+        const double xxm = cellv(chi, g, i - 1, j, k), xxp = cellv(chi, g, i + 1, j, k);
+        const double xym = cellv(chi, g, i, j - 1, k), xyp = cellv(chi, g, i, j + 1, k);
+        const double xzm = cellv(chi, g, i, j, k - 1), xzp = cellv(chi, g, i, j, k + 1);
+        double J[9], Jt[9];
+        m3mul(X.Ti, Tti, J);
+        m3t(J, Jt);
+        const double gc[3] = {.5 * ix * (xxp - xxm), .5 * iy * (xyp - xym), .5 * iz * (xzp - xzm)};
+        const double dD[3] = {.5 * ix * (Dxp - Dxm), .5 * iy * (Dyp - Dym), .5 * iz * (Dzp - Dzm)};
+        double Jg[3], JtdD[3];
+        m3vec(J, gc, Jg);
+        m3vec(Jt, dD, JtdD);
+        // term1 = dD . (J gc)
+        const double Db0 = wdf * .5 * ix * Jg[0], Db1 = wdf * .5 * iy * Jg[1], Db2 = wdf * .5 * iz * Jg[2];
+        atomicAdd(&dob[cidx(g, i + 1, j, k)], Db0);
+        if (xin) atomicAdd(&dnb[c - 1], -Db0);
+        else atomicAdd(&dob[cidx(g, i - 1, j, k)], -Db0);
+        atomicAdd(&dob[cidx(g, i, j + 1, k)], Db1);
+        if (yin) atomicAdd(&dnb[c - g.Nx], -Db1);
+        else atomicAdd(&dob[cidx(g, i, j - 1, k)], -Db1);
+        if (g.pz) {
+            atomicAdd(&dob[cidx(g, i, j, wrapc(k + 1, g.Nz))], Db2);
+            if (k > 0) atomicAdd(&dnb[c - nxy], -Db2);
+            else atomicAdd(&dob[cidx(g, i, j, g.Nz - 1)], -Db2);
+        } else {
+            if (k < g.Nz - 1) atomicAdd(&dob[c + nxy], Db2);
+            if (k > 0) atomicAdd(&dnb[c - nxy], -Db2);
+        }
+        cell_add(xb_out, g, i + 1, j, k, wdf * .5 * ix * JtdD[0]);
+        cell_add(xb_out, g, i - 1, j, k, -wdf * .5 * ix * JtdD[0]);
+        cell_add(xb_out, g, i, j + 1, k, wdf * .5 * iy * JtdD[1]);
+        cell_add(xb_out, g, i, j - 1, k, -wdf * .5 * iy * JtdD[1]);
+        cell_add(xb_out, g, i, j, k + 1, wdf * .5 * iz * JtdD[2]);
+        cell_add(xb_out, g, i, j, k - 1, -wdf * .5 * iz * JtdD[2]);
+        // term2 = Dc sum J_ab d_ab chi (the y-z difference keeps only its
+        // first two cells: the other two are the same cell with opposite signs)
+        const double dxx = ix * ix * (xxp - 2 * x0 + xxm), dyy = iy * iy * (xyp - 2 * x0 + xym),
+                     dzz = iz * iz * (xzp - 2 * x0 + xzm);
+        const double dxy = .25 * ix * iy *
+                           (cellv(chi, g, i + 1, j + 1, k) + cellv(chi, g, i - 1, j - 1, k) -
+                            cellv(chi, g, i + 1, j - 1, k) - cellv(chi, g, i - 1, j + 1, k));
+        const double dxz = .25 * ix * iz *
+                           (cellv(chi, g, i + 1, j, k + 1) + cellv(chi, g, i - 1, j, k - 1) -
+                            cellv(chi, g, i + 1, j, k - 1) - cellv(chi, g, i - 1, j, k + 1));
+        const double ypzm = cellv(chi, g, i, j + 1, k - 1);
+        const double dyz = .25 * iy * iz *
+                           (cellv(chi, g, i, j + 1, k + 1) + cellv(chi, g, i, j - 1, k - 1) - ypzm + ypzm);
+        const double t2 = J[0] * dxx + J[1] * dxy + J[2] * dxz + J[3] * dxy + J[4] * dyy + J[5] * dyz +
+                          J[6] * dxz + J[7] * dyz + J[8] * dzz;
+        atomicAdd(&dnb[c], wdf * t2);
+        const double w2 = wdf * Dc;
+        const double cxx = w2 * J[0] * ix * ix, cyy = w2 * J[4] * iy * iy, czz = w2 * J[8] * iz * iz;
+        const double cxy = w2 * (J[1] + J[3]) * .25 * ix * iy, cxz = w2 * (J[2] + J[6]) * .25 * ix * iz,
+                     cyz = w2 * (J[5] + J[7]) * .25 * iy * iz;
+        cell_add(xb_out, g, i + 1, j, k, cxx);
+        cell_add(xb_out, g, i - 1, j, k, cxx);
+        cell_add(xb_out, g, i, j + 1, k, cyy);
+        cell_add(xb_out, g, i, j - 1, k, cyy);
+        cell_add(xb_out, g, i, j, k + 1, czz);
+        cell_add(xb_out, g, i, j, k - 1, czz);
+        atomicAdd(&xb_out[c], -2 * (cxx + cyy + czz));
+        cell_add(xb_out, g, i + 1, j + 1, k, cxy);
+        cell_add(xb_out, g, i - 1, j - 1, k, cxy);
+        cell_add(xb_out, g, i + 1, j - 1, k, -cxy);
+        cell_add(xb_out, g, i - 1, j + 1, k, -cxy);
+        cell_add(xb_out, g, i + 1, j, k + 1, cxz);
+        cell_add(xb_out, g, i - 1, j, k - 1, cxz);
+        cell_add(xb_out, g, i + 1, j, k - 1, -cxz);
+        cell_add(xb_out, g, i - 1, j, k + 1, -cxz);
+        cell_add(xb_out, g, i, j + 1, k + 1, cyz);
+        cell_add(xb_out, g, i, j - 1, k - 1, cyz);
+    }
+    grad_add_v(ub_out, g, i, j, k, vb);
+}
+
+// Transpose of k_plastic_trans: as k_adj_plastic with sbar taken from the
+// physical stress T sigma' T^t.
+static __global__ __launch_bounds__(kT) void k_adj_plastic_trans(
+    Grid g, Mat M, double dt, Xform X, const double *__restrict__ sig, const double *__restrict__ chi,
+    const double *__restrict__ ab, const double *__restrict__ dchib, const double *__restrict__ dnb,
+    double *__restrict__ sb_out, double *__restrict__ xb_out, double *__restrict__ part) {
+    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    double pg[3] = {0, 0, 0};
+    if (c < g.Nc) {
+        double sp[9], os[9];
+        old_sig(sig, g.Nc, c, X, sp, os);
+        const double sb = sbar_phys(os);
+        if (sb > M.sy) {
+            double a, dc, Ja[5], Jc[5];
+            adaptive_jac(M, sb, chi[c], dt, a, dc, Ja, Jc);
+            const double Dbar = dnb[c];
+            const double abar = ab[c] + Dbar * (.5 / M.mu * sb);
+            const double cbar = dchib[c];
+            const double sbb = Dbar * (.5 * a / M.mu) + abar * Ja[0] + cbar * Jc[0];
+            xb_out[c] += abar * Ja[1] + cbar * Jc[1];
+#pragma unroll
+            for (int q = 0; q < 3; q++) pg[q] = abar * Ja[2 + q] + cbar * Jc[2 + q];
+            // d sbar / d os (upper triangle and diagonal), then os = T sp T^t
+            const double p = -1. / 3. * (os[0] + os[4] + os[8]);
+            const double t11 = os[0] + p, t22 = os[4] + p, t33 = os[8] + p;
+            const double f = sbb / sb;
+            double osb[9] = {f * .5 * t11, f * os[1], f * os[2], 0, f * .5 * t22, f * os[5], 0, 0, f * .5 * t33};
+            double Tt[9], tmp[9], spb[9];
+            m3t(X.T, Tt);
+            m3mul(Tt, osb, tmp);
+            m3mul(tmp, X.T, spb);
+            double s6[6] = {0, 0, 0, 0, 0, 0};
+            mat_to_sym6_adj(spb, s6);
+#pragma unroll
+            for (int q = 0; q < 6; q++) sb_out[q * g.Nc + c] += s6[q];
+        }
+    }
     __shared__ double red[3][kT / 64];
 #pragma unroll
     for (int q = 0; q < 3; q++) {

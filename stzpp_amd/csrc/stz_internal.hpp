@@ -29,6 +29,11 @@ namespace stz_detail {
 
 // Records the message on the handle and returns code.
 int fail(stz_t s, int code, const char *fmt, ...);
+// T, T^-1 and dT/dt of the handle's transform at time t.
+stz::Xform xform(const stz_t s, double t);
+// Loads the transpose of the transformed step's projection operator at T(t)
+// (the operator of the step that ends at t) into the handle's solver.
+int transposed_trans_operator(stz_t s, double dt, double t);
 // Projection operator of the untransformed step for step size dt.
 int ensure_operator(stz_t s, double dt);
 // Solve options of a step call: opts, or the reference's loop when null.

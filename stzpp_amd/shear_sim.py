@@ -158,7 +158,9 @@ class ShearSim:
         Every projection, forward and adjoint, is solved to relative residual
         rtol; states are kept every checkpoint_every steps and segments are
         recomputed in the reverse sweep.  The state afterwards is the state
-        after the nsteps forward steps."""
+        after the nsteps forward steps.  On a TransSim this differentiates
+        the transformed step (the adjoint solves use the transposed operator
+        of each step)."""
         o = _lib.StzAdjointOpts()
         self.L.stz_adjoint_opts_default(ctypes.byref(o))
         o.checkpoint_every, o.rtol = int(checkpoint_every), float(rtol)

@@ -6,7 +6,12 @@ Loss J = mean over cells of (ad_Dpl/dt)^2 after 3 steps of a 16x16x8 walled
 case whose pre-stress yields (plasticity, chi diffusion and the in-place
 ad_Dpl ordering all active).  Tolerance of the directional derivatives:
 1e-5 relative (central differences with step h: truncation O(h^2), solve
-noise ~1e-13 J / h)."""
+noise ~1e-13 J / h).
+
+The transformed step (trans_sim_3d) is checked the same way for the pure
+shear transform (the reference's compiled-in family) and the simple shear;
+both have non-symmetric projection operators, so these cases exercise the
+transposed-operator adjoint solve."""
 import numpy as np
 import pytest
 
@@ -34,25 +39,42 @@ def case():
     return p, u, sig, chi
 
 
-def gpu_loss(p, u, sig, chi):
-    sim = SS.ShearSim(*N, params=p)
+# Transform rate of the trans_sim_3d cases.  The transform keeps loading the
+# yielding band, whose cells then cross adaptive sub-step limits under small
+# changes of chi: J is piecewise smooth with kinks about 1e-6..1e-5 apart
+# along a random chi direction at rate 0.5.  At rate 0.2 the central
+# differences at h = 3e-7 and 1e-6 agree to 1e-8 relative (checked with
+# the oracle), so the trans tests use h = 1e-6.
+UT = 0.2
+
+
+def make_sim(p, family):
+    if family is None:
+        return SS.ShearSim(*N, params=p)
+    return SS.TransSim(*N, UT, family=family, params=p)
+
+
+def gpu_loss(p, u, sig, chi, family=None):
+    sim = make_sim(p, family)
     sim.set_initial_conditions(chi, u=u, sig=sig, t=0.3)
     sim.step_forward_qs(DT, NSTEPS, solve=SS.SOLVE_RELRES, rtol=RTOL)
     d = sim.state()["dpl"]
     return float(np.mean((d / DT) ** 2))
 
 
-def oracle_loss(p, u, sig, chi):
+def oracle_loss(p, u, sig, chi, family=None):
     orc = O.OracleShear(N, {k: p[k] for k in O.STEP_PARAM_ORDER})
-    orc.set_solver(0, RTOL)
+    if family is not None:
+        orc.set_transform(family, UT)
+    orc.set_solver(0 if family is None else 1, RTOL)
     orc.set_state(0.3, u, sig, chi)
     orc.step(DT, NSTEPS)
     d = orc.state()["dpl"]
     return float(np.mean((d / DT) ** 2))
 
 
-def adjoint(p, u, sig, chi, k=2):
-    sim = SS.ShearSim(*N, params=p)
+def adjoint(p, u, sig, chi, k=2, family=None):
+    sim = make_sim(p, family)
     sim.set_initial_conditions(chi, u=u, sig=sig, t=0.3)
     return sim.adjoint(DT, NSTEPS, checkpoint_every=k, rtol=RTOL), sim
 
@@ -123,7 +145,71 @@ def test_adjoint_rejects_unsupported():
     sim.set_initial_conditions(chi)
     with pytest.raises(SS.StepError):
         sim.adjoint(DT, 1)
-    tsim = SS.TransSim(16, 16, 8, 0.5, params=p)
+    tsim = SS.TransSim(16, 16, 8, UT, params=p, prd=(1, 1, 1))
     tsim.set_initial_conditions(chi)
     with pytest.raises(SS.StepError):
         tsim.adjoint(DT, 1)
+
+
+# ---- transformed-coordinate step ---------------------------------------------
+
+FAMILIES = ["pure_shear", "simple_shear"]
+
+
+@pytest.fixture(scope="module", params=FAMILIES)
+def tgrad(request):
+    fam = request.param
+    p, u, sig, chi = case()
+    (J, g, gp), sim = adjoint(p, u, sig, chi, family=fam)
+    return fam, p, u, sig, chi, J, g, gp, sim
+
+
+def test_trans_operator_is_not_symmetric():
+    # the reason the transformed adjoint needs its own transposed solve
+    for fam in FAMILIES:
+        T, _, _ = O.transform_at(fam, UT, 0.5, 0.33)
+        B = np.asarray(O.trans_stencil_c(np.asarray(T), 1.0, 1.5, 1 / 8, 1 / 8, 1 / 8)).reshape(27, 3, 3)
+        asym = max(np.abs(B[b] - B[26 - b].T).max() for b in range(27))
+        assert asym > 1e-3 * np.abs(B).max(), fam
+
+
+def test_trans_adjoint_loss_and_state(tgrad):
+    fam, p, u, sig, chi, J, g, gp, sim = tgrad
+    s = sim.state()
+    assert abs(J - np.mean((s["dpl"] / DT) ** 2)) <= 1e-14 * J
+    assert J > 0 and np.abs(g).max() > 0
+    assert abs(J / gpu_loss(p, u, sig, chi, fam) - 1) < 1e-11
+
+
+@pytest.mark.parametrize("forward", ["gpu", "oracle"])
+def test_trans_adjoint_chi0_matches_finite_differences(tgrad, forward):
+    fam, p, u, sig, chi, J, g, gp, _ = tgrad
+    f = gpu_loss if forward == "gpu" else oracle_loss
+    rng = np.random.default_rng(13)
+    for trial in range(2):
+        d = rng.standard_normal(chi.size)
+        h = 1e-6
+        fd = (f(p, u, sig, chi + h * d, fam) - f(p, u, sig, chi - h * d, fam)) / (2 * h)
+        ad = float(g @ d)
+        assert abs(ad - fd) <= 1e-5 * abs(fd), (fam, trial, ad, fd)
+
+
+@pytest.mark.parametrize("which,name", [(1, "eps0"), (2, "chi_inf")])
+def test_trans_adjoint_params_match_finite_differences(tgrad, which, name):
+    fam, p, u, sig, chi, J, g, gp, _ = tgrad
+    # J is smooth in eps0 and chi_inf at this scale; at 1e-6 relative the
+    # solve noise of J (~1e-13 J / h) is already ~1e-5 of the derivative
+    h = 1e-5 * abs(p[name])
+    pp, pm = dict(p), dict(p)
+    pp[name] += h
+    pm[name] -= h
+    fd = (gpu_loss(pp, u, sig, chi, fam) - gpu_loss(pm, u, sig, chi, fam)) / (2 * h)
+    assert abs(gp[which] - fd) <= 1e-5 * abs(fd) + 1e-12 * abs(J), (fam, name, gp[which], fd)
+
+
+def test_trans_adjoint_independent_of_checkpoint_spacing(tgrad):
+    fam, p, u, sig, chi, J, g, gp, _ = tgrad
+    (J2, g2, gp2), _ = adjoint(p, u, sig, chi, k=1, family=fam)
+    assert abs(J2 - J) <= 1e-14 * J
+    assert np.abs(g2 - g).max() <= 1e-10 * np.abs(g).max()
+    assert np.abs(gp2 - gp).max() <= 1e-10 * np.abs(gp).max()
