@@ -108,6 +108,48 @@ int stz_adjoint(stz_t s, double dt, int nsteps, const stz_adjoint_opts *o, doubl
  * (compute_T, compute_Tinv, compute_dTdt, src/trans_sim_3d.hh:57-125). */
 int stz_transform(int family, double Ut, double bz, double t, double *T, double *Tinv, double *dTdt);
 
+/* ---- random initial chi (Gaussian random field) -------------------------
+ *
+ * chi = chi_avg + chi_1 (noise * G) on the Nx x Ny x Nz cells (x fastest),
+ * G(i,j,k) = exp(-(i^2 + j^2 + k^2) / conv_l) on the cube |i|,|j|,|k| <= cut,
+ * cut = (int)((int)gauss_fac * conv_l), chi_1 = sigma / sqrt(sum G^2)
+ * (set_random_initial_conditions, convolve, gauss_normal_factor_long,
+ * src/shear_sim_3d.cpp:2793-3078; the int members gauss_fac and cut,
+ * src/shear_sim_3d.hh:140-142).  The noise grid is nx x ny x nz with
+ * n = N on a periodic axis and N + 2 cut otherwise (x fastest); periodic
+ * axes wrap once, so cut must not exceed a periodic N.  The convolution
+ * runs on the GPU as three 1-D passes.  When the caller passes no noise it
+ * is drawn on the GPU from `seed` (Philox4x64-10 and the polar Box-Muller
+ * method); the reference's own noise comes from unseeded rand(). */
+typedef struct {
+    double gauss_fac;        /* cutoff in correlation lengths (truncated to an int) */
+    double conv_l;           /* correlation length in cells */
+    double chi_avg;          /* mean of chi */
+    double sigma;            /* standard deviation of chi */
+    double TZ;               /* temperature scale: enters only the cache file name */
+    int sim_case;            /* enters only the cache file name */
+    unsigned long long seed; /* noise stream when no noise grid is given */
+} stz_grf_params;
+/* The shear_sim driver's values (src/shear_sim.cpp:8, :83-86). */
+void stz_grf_params_default(stz_grf_params *g);
+/* Cutoff, noise-grid size and chi_1 of a configuration (any may be NULL). */
+int stz_grf_geometry(const int N[3], const int prd[3], const stz_grf_params *g, int *cut, long long *noise_size,
+                     double *chi_1);
+/* The reference's cache file name, e.g. "cg_32_32_32_mu550_sig16.dat"
+ * (src/shear_sim_3d.cpp:2812-2816). */
+int stz_grf_cache_name(const int N[3], const int prd[3], const stz_grf_params *g, char *buf, int len);
+/* chi from host noise (NULL: drawn from g->seed) into host memory. */
+int stz_grf_field(const int N[3], const int prd[3], const stz_grf_params *g, const double *noise, double *chi);
+/* Same on device buffers and a caller's HIP stream (synchronised on return). */
+int stz_grf_field_device(const int N[3], const int prd[3], const stz_grf_params *g, const double *noise_dev,
+                         double *chi_dev, void *stream);
+/* Message of the last failing stz_grf_* call of this thread. */
+const char *stz_grf_last_error(void);
+/* set_random_initial_conditions (src/shear_sim_3d.cpp:2793-2934) on the
+ * simulation's chi: with cache_dir, an existing cache file there is loaded
+ * instead, and a computed field is written to it (raw FP64, x fastest). */
+int stz_set_random_initial_conditions(stz_t s, const stz_grf_params *g, const double *noise, const char *cache_dir);
+
 #ifdef __cplusplus
 }
 #endif

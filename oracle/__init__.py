@@ -353,6 +353,8 @@ def ref_lib():
             ("ref_trans_get_state", [vp, _dp, _dp, _dp, _dp, _dp], None),
             ("ref_trans_vcycles", [vp], ctypes.c_int),
             ("ref_trans_destroy", [vp], None),
+            ("ref_grf", [ctypes.c_int] * 3 + [_ip, _dp, ctypes.c_int, ctypes.c_char_p, _dp, ctypes.c_long, _dp,
+                         _dp], ctypes.c_int),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -665,3 +667,124 @@ class RefTrans:
     @property
     def vcycles(self):
         return self.L.ref_trans_vcycles(self.h)
+
+
+# ---------------------------------------------------------------------------
+# Random initial chi (grf_oracle.c): the reference's cube-cutoff Gaussian
+# convolution (src/shear_sim_3d.cpp:2793-3078) and this library's own
+# Philox4x64-10 noise.
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+def _grf_lib():
+    L = lib()
+    if not hasattr(L, "_grf_bound"):
+        L.grf_philox4x64.argtypes = [_u64p, _u64p, _u64p]
+        L.grf_noise.argtypes = [ctypes.c_uint64, ctypes.c_long, _dp]
+        L.grf_normal_factor.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.grf_normal_factor.restype = ctypes.c_double
+        L.grf_convolve.argtypes = [_ip, _ip, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                   _dp, _dp]
+        L.grf_convolve_points.argtypes = [_ip, _ip, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_double, _dp, ctypes.c_long,
+                                          ctypes.POINTER(ctypes.c_long), _dp]
+        L._grf_bound = True
+    return L
+
+
+def philox4x64(ctr, key):
+    """One Philox4x64-10 block (4 uint64) for counter ctr (4) and key (2)."""
+    c = np.ascontiguousarray(ctr, dtype=np.uint64)
+    k = np.ascontiguousarray(key, dtype=np.uint64)
+    o = np.zeros(4, dtype=np.uint64)
+    _grf_lib().grf_philox4x64(c.ctypes.data_as(_u64p), k.ctypes.data_as(_u64p), o.ctypes.data_as(_u64p))
+    return o
+
+
+def grf_cut(gauss_fac, conv_l):
+    """Cutoff in grid points: the reference's int member initialised from
+    gauss_fac * ll (src/shear_sim_3d.cpp:57), where gauss_fac itself is an int
+    member (src/shear_sim_3d.hh:140): a fractional factor is truncated first."""
+    return int(int(gauss_fac) * conv_l)
+
+
+def grf_noise_shape(N, prd, cut):
+    """(nx, ny, nz) of the noise grid: a non-periodic axis is padded by cut on
+    each side (src/shear_sim_3d.cpp:2802-2805)."""
+    return tuple(int(n) if p else int(n) + 2 * cut for n, p in zip(N, prd))
+
+
+def grf_noise(seed, n):
+    out = np.zeros(int(n))
+    _grf_lib().grf_noise(int(seed), int(n), _d(out))
+    return out
+
+
+def grf_normal_factor(cut, llinv):
+    return _grf_lib().grf_normal_factor(int(cut), float(llinv))
+
+
+def grf_convolve(N, prd, cut, llinv, chi_avg, chi_1, noise):
+    """chi on N (x fastest) by the reference's brute-force convolution."""
+    N3 = np.array(N, dtype=np.int32)
+    P3 = np.array([1 if p else 0 for p in prd], dtype=np.int32)
+    noise = np.ascontiguousarray(noise, dtype=np.float64).ravel()
+    assert noise.size == int(np.prod(grf_noise_shape(N, prd, cut)))
+    out = np.zeros(int(np.prod(N)))
+    _grf_lib().grf_convolve(_i(N3), _i(P3), int(cut), float(llinv), float(chi_avg), float(chi_1), _d(noise),
+                            _d(out))
+    return out
+
+
+def grf_convolve_points(N, prd, cut, llinv, chi_avg, chi_1, noise, idx):
+    """grf_convolve at the cells idx (flat, x fastest) only."""
+    N3 = np.array(N, dtype=np.int32)
+    P3 = np.array([1 if p else 0 for p in prd], dtype=np.int32)
+    noise = np.ascontiguousarray(noise, dtype=np.float64).ravel()
+    assert noise.size == int(np.prod(grf_noise_shape(N, prd, cut)))
+    ix = np.ascontiguousarray(idx, dtype=np.int64)
+    assert ix.min() >= 0 and ix.max() < int(np.prod(N))
+    out = np.zeros(ix.size)
+    _grf_lib().grf_convolve_points(_i(N3), _i(P3), int(cut), float(llinv), float(chi_avg), float(chi_1),
+                                   _d(noise), ix.size, ix.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), _d(out))
+    return out
+
+
+def grf_field(N, prd, gauss_fac, conv_l, chi_avg, sigma, seed=0, noise=None):
+    """The whole initialiser: noise (drawn from seed unless given), chi_1 =
+    sigma / gauss_normal_factor_long, brute-force convolution."""
+    cut = grf_cut(gauss_fac, conv_l)
+    llinv = 1.0 / conv_l
+    if noise is None:
+        noise = grf_noise(seed, int(np.prod(grf_noise_shape(N, prd, cut))))
+    chi_1 = sigma / grf_normal_factor(cut, llinv)
+    return grf_convolve(N, prd, cut, llinv, chi_avg, chi_1, noise)
+
+
+def grf_cache_name(N, prd, sim_case, TZ, chi_avg, chi_1, gnf):
+    """Name of the reference's convolution cache (src/shear_sim_3d.cpp:
+    2812-2816); chi_1 is the normalised scale, gnf gauss_normal_factor_long."""
+    human_mu = int(TZ * chi_avg)
+    human_sig = int(TZ * chi_1 * gnf + 1)
+    pref = "f" if (sim_case == 19 and not prd[0] and not prd[1]) else ("t" if prd[2] else "")
+    return f"{pref}cg_{N[0]}_{N[1]}_{N[2]}_mu{human_mu}_sig{human_sig}.dat"
+
+
+def ref_grf(N, prd, gauss_fac, conv_l, chi_avg, sigma, TZ, sim_case, workdir):
+    """The reference's own noise grid and set_random_initial_conditions (run
+    in workdir, where it writes its cache) through oracle/_ref."""
+    L = ref_lib()
+    if L is None:
+        raise RuntimeError("reference library not built")
+    cut = grf_cut(gauss_fac, conv_l)
+    n = int(np.prod(grf_noise_shape(N, prd, cut)))
+    noise, chi, out3 = np.zeros(n), np.zeros(int(np.prod(N))), np.zeros(3)
+    pr = np.array([1 if p else 0 for p in prd], dtype=np.int32)
+    p = np.array([gauss_fac, conv_l, chi_avg, sigma, TZ], dtype=np.float64)
+    rc = L.ref_grf(N[0], N[1], N[2], _i(pr), _d(p), int(sim_case), workdir.encode(), _d(noise), n, _d(chi),
+                   _d(out3))
+    if rc:
+        raise RuntimeError(f"ref_grf failed ({rc})")
+    return dict(noise=noise, chi=chi, chi_1=float(out3[0]), gnf=float(out3[1]), cut=int(out3[2]),
+                files=sorted(os.listdir(workdir)))

@@ -11,7 +11,10 @@
 // Nothing here is linked into the product library.
 #include <mpi.h>
 
+#include <unistd.h>
+
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -479,6 +482,72 @@ void ref_trans_destroy(void *hp) {
     delete h->sim;
     MPI_Comm_free(&h->cart);
     delete h;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Random initial chi of shear_sim_3d (set_random_initial_conditions,
+// src/shear_sim_3d.cpp:2793-2934).  The reference draws its noise with
+// glibc rand() and never seeds it, i.e. from the state srand(1) sets; the
+// harness resets that state, asks the reference's own fill_noise_grid for
+// the noise grid, resets it again and runs set_random_initial_conditions in
+// workdir, where the reference writes its cg_*.dat cache.
+namespace {
+class ref_grf_sim : public shear_sim_3d<sym_mat3> {
+   public:
+    using shear_sim_3d<sym_mat3>::shear_sim_3d;
+    int cut_pts() const { return cut; }
+    double chi_scale() const { return chi_1; }
+    double normal_factor() { return gauss_normal_factor_long(); }
+    void noise(double *b, double *e) { fill_noise_grid(b, e); }
+    void random_ic() { set_random_initial_conditions(); }
+    double chi_at(int i, int j, int k) const { return grido[i + nxg * j + nxyg * k].chi; }
+};
+}  // namespace
+
+extern "C" {
+
+// p: gauss_fac, conv_l, chi_avg, sigma, TZ.  noise_out holds noise_n values
+// (the padded grid), chi_out Nx Ny Nz (x fastest).  out3: chi_1 after
+// normalisation, gauss_normal_factor_long, cut.  Returns 0, or -1 when
+// noise_n is not the padded size.
+int ref_grf(int Nx, int Ny, int Nz, const int *periods, const double *p, int sim_case, const char *workdir,
+            double *noise_out, long noise_n, double *chi_out, double *out3) {
+    ensure_mpi();
+    MPI_Comm cart;
+    int dims[3] = {1, 1, 1}, per[3] = {periods[0], periods[1], periods[2]};
+    MPI_Cart_create(MPI_COMM_WORLD, 3, dims, per, 1, &cart);
+    ref_grf_sim *s = new ref_grf_sim(Nx, Ny, Nz, -1, -1, -0.5, 1, 1, 0.5, 0.0, 1.0, 0.5, 1, std::string(workdir),
+                                     1.0, 1.5, 1.0, 0.0, 1.0, 1.0, 0.4, 10.0, 1.0, 1.0, 1.0, 0.15, 1.0, p[4], 0.0,
+                                     0.01, p[0], p[1], p[2], p[3], 1.0, &cart, dims, sim_case, 0.0, 0.0, per);
+    const int cut = s->cut_pts();
+    const long nt = (long)(per[0] ? Nx : Nx + 2 * cut) * (per[1] ? Ny : Ny + 2 * cut) * (per[2] ? Nz : Nz + 2 * cut);
+    int rc = 0;
+    if (nt != noise_n) {
+        rc = -1;
+    } else {
+        srand(1);
+        s->noise(noise_out, noise_out + nt);
+        char old[4096];
+        if (!getcwd(old, sizeof(old))) old[0] = 0;
+        if (chdir(workdir) == 0) {
+            srand(1);
+            s->random_ic();
+            if (old[0] && chdir(old) != 0) rc = -2;
+        } else {
+            rc = -3;
+        }
+        for (int k = 0; k < Nz; k++)
+            for (int j = 0; j < Ny; j++)
+                for (int i = 0; i < Nx; i++) chi_out[i + (long)Nx * (j + (long)Ny * k)] = s->chi_at(i, j, k);
+        out3[0] = s->chi_scale();
+        out3[1] = s->normal_factor();
+        out3[2] = cut;
+    }
+    delete s;
+    MPI_Comm_free(&cart);
+    return rc;
 }
 
 }  // extern "C"

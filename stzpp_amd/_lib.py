@@ -62,6 +62,12 @@ class StzAdjointOpts(ctypes.Structure):
     _fields_ = [("checkpoint_every", ctypes.c_int), ("rtol", ctypes.c_double)]
 
 
+class StzGrfParams(ctypes.Structure):
+    _fields_ = [("gauss_fac", ctypes.c_double), ("conv_l", ctypes.c_double), ("chi_avg", ctypes.c_double),
+                ("sigma", ctypes.c_double), ("TZ", ctypes.c_double), ("sim_case", ctypes.c_int),
+                ("seed", ctypes.c_ulonglong)]
+
+
 class StzStepStats(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int), ("vcycles", ctypes.c_int), ("yielded_cells", ctypes.c_int),
                 ("solve_ms", ctypes.c_double), ("step_ms", ctypes.c_double),
@@ -83,6 +89,14 @@ SIGNATURES = {
                     ctypes.c_int),
     "stz_transform": ([ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp],
                       ctypes.c_int),
+    "stz_grf_params_default": ([ctypes.POINTER(StzGrfParams)], None),
+    "stz_grf_geometry": ([_ip, _ip, ctypes.POINTER(StzGrfParams), _ip, ctypes.POINTER(ctypes.c_longlong), _dp],
+                         ctypes.c_int),
+    "stz_grf_cache_name": ([_ip, _ip, ctypes.POINTER(StzGrfParams), ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
+    "stz_grf_field": ([_ip, _ip, ctypes.POINTER(StzGrfParams), _dp, _dp], ctypes.c_int),
+    "stz_grf_field_device": ([_ip, _ip, ctypes.POINTER(StzGrfParams), _vp, _vp, _vp], ctypes.c_int),
+    "stz_grf_last_error": ([], ctypes.c_char_p),
+    "stz_set_random_initial_conditions": ([_vp, ctypes.POINTER(StzGrfParams), _dp, ctypes.c_char_p], ctypes.c_int),
     "stzmg_desc_default": ([ctypes.POINTER(StzmgDesc)] + [ctypes.c_int] * 6, ctypes.c_int),
     "stzmg_create": ([ctypes.POINTER(StzmgDesc), ctypes.POINTER(_vp)], ctypes.c_int),
     "stzmg_destroy": ([_vp], ctypes.c_int),

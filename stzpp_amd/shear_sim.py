@@ -116,6 +116,25 @@ class ShearSim:
 
     set_state = set_initial_conditions
 
+    def set_random_initial_conditions(self, noise=None, cache_dir=None, **grf_params):
+        """chi <- chi_avg + chi_1 (noise * G) on the GPU
+        (set_random_initial_conditions, src/shear_sim_3d.cpp:2793-2934); the
+        velocity and stress are left as they are.  grf_params: gauss_fac,
+        conv_l, chi_avg, sigma, TZ, sim_case, seed (stzpp_amd.grf).  noise:
+        the padded noise grid (default: drawn on the GPU from seed).
+        cache_dir: load the reference's cg_*.dat cache from there when it
+        exists, else write it."""
+        from . import grf as G
+
+        g = G._params(grf_params)
+        nz = None
+        if noise is not None:
+            nz = np.ascontiguousarray(noise, dtype=np.float64).ravel()
+            if nz.size != G.geometry(self.N, self.prd, **grf_params)[1]:
+                raise ValueError("noise grid has the wrong size")
+        cd = None if cache_dir is None else str(cache_dir).encode()
+        self._chk(self.L.stz_set_random_initial_conditions(self.h, ctypes.byref(g), _lib.dptr(nz), cd))
+
     def state(self):
         t = np.zeros(1)
         u = np.zeros((self.nverts, 3))

@@ -216,6 +216,40 @@ def trans_step_case(name):
     np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
 
 
+GRF_CASES = {
+    # name: N, prd, gauss_fac, conv_l, chi_avg, sigma, TZ, sim_case
+    "grf_walled": ((16, 12, 10), (1, 1, 0), 2.0, 2.0, 0.09, 0.01, 21000.0, 0),
+    "grf_periodic": ((12, 10, 8), (1, 1, 1), 1.5, 3.0, 0.09, 0.01, 21000.0, 0),
+    "grf_open": ((10, 8, 6), (0, 0, 0), 1.0, 2.5, 0.1, 0.02, 21000.0, 19),
+    "grf_wrap": ((8, 6, 10), (1, 1, 0), 2.0, 3.0, 0.09, 0.01, 21000.0, 0),
+    "grf_c0": ((32, 32, 16), (1, 1, 0), 5.0, 3.0, 0.09, 0.01, 21000.0, 0),
+}
+
+
+def grf_case(name):
+    """The reference's random initial chi (set_random_initial_conditions,
+    src/shear_sim_3d.cpp:2793-2934) on a noise grid from its own generator
+    (glibc rand() from the unseeded state), and the cache file it writes.
+    grf_wrap has a cutoff equal to the periodic y size (the largest the
+    reference's single wrap handles); grf_c0 uses the driver's gauss_fac = 5
+    and a 3-cell correlation length (cut 15)."""
+    import tempfile
+
+    N, prd, gf, cl, ca, sg, TZ, sc = GRF_CASES[name]
+    d = tempfile.mkdtemp()
+    r = O.ref_grf(N, prd, gf, cl, ca, sg, TZ, sc, d)
+    assert len(r["files"]) == 1, r["files"]
+    raw = np.fromfile(os.path.join(d, r["files"][0]), dtype=np.uint8)
+    assert np.array_equal(raw.view(np.float64), r["chi"]), "cache file is not the chi field"
+    out = dict(N=np.array(N), prd=np.array(prd), gauss_fac=np.array(gf), conv_l=np.array(cl),
+               chi_avg=np.array(ca), sigma=np.array(sg), TZ=np.array(TZ), sim_case=np.array(sc),
+               noise=r["noise"], chi=r["chi"], chi_1=np.array(r["chi_1"]), gnf=np.array(r["gnf"]),
+               cut=np.array(r["cut"]), cache_name=np.array(r["files"][0]))
+    if raw.size <= 1 << 16:
+        out["cache_bytes"] = raw
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+
+
 CASES = {
     "shear_16x16x17": (("shear_16x16x17", (16, 16, 17), (1, 1, 0), "shear", 11),),
     "trans_16x16x16": (("trans_16x16x16", (16, 16, 16), (1, 1, 1), "trans", 12),),
@@ -233,10 +267,13 @@ if __name__ == "__main__":
             step_case(which)
         elif which.startswith("trans_step_"):
             trans_step_case(which)
+        elif which.startswith("grf_"):
+            grf_case(which)
         else:
             mg_case(*CASES[which][0])
         sys.exit(0)
-    for key in ["trans", *CASES, "step_plastic", "step_c0", "trans_step_walled", "trans_step_periodic"]:
+    for key in ["trans", *CASES, "step_plastic", "step_c0", "trans_step_walled", "trans_step_periodic",
+                *GRF_CASES]:
         subprocess.run([sys.executable, __file__, key], check=True, cwd=ROOT,
                        stdout=subprocess.DEVNULL)
         print("wrote", key)
