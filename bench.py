@@ -83,16 +83,14 @@ def step_workload(args):
     """QS steps of shear_sim_3d at 192 x 192 x 96 cells (MG 192 x 192 x 97),
     config-0 material, chi0 = 0.09 + 0.01 GRF(seed 3, l = 4 cells), dt = 0.005,
     the reference's projection loop; value = steps per second."""
-    import ctypes  # noqa: F401
-
-    from stzpp_amd import fields
     from stzpp_amd import shear_sim as SS
 
     Nx, Ny, Nz = 192, 192, 96
     p = SS.config0_params(Nx)
     sim = SS.ShearSim(Nx, Ny, Nz, params=p)
-    chi0 = 0.09 + 0.01 * fields.grf((Nz, Ny, Nx), 4.0, 3).ravel()
-    sim.set_initial_conditions(chi0)
+    # chi0 from the reference's random initial condition (cube-cutoff
+    # Gaussian, l = 4 cells, cut 20) on the GPU, noise seed 3
+    sim.set_random_initial_conditions(gauss_fac=5, conv_l=4.0, chi_avg=0.09, sigma=0.01, seed=3)
     dt = 0.005
     log(f"[bench] step workload {Nx}x{Ny}x{Nz}, warmup {args.warmup}")
     for _ in range(args.warmup):
@@ -112,6 +110,77 @@ def step_workload(args):
                                "material, chi0 GRF seed 3 l=4, dt 0.005",
                    "vcycles_per_step": cyc / args.steps},
         "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def grf_workload(args):
+    """The random initial chi at 192 x 192 x 96 cells, z walled: the
+    reference's cube-cutoff Gaussian (l = 4 cells, gauss_fac 5, cut 20),
+    chi = 0.09 + 0.01 GRF, noise drawn on the GPU (seed 3).  value = cells per
+    second of its four kernels (noise draw, x, y and z passes), each timed
+    with HIP events on the library's stream; the API call with its
+    allocations and the copy-out is timed beside it.  CPU baseline: the
+    reference's brute-force convolution (the oracle) on one thread at
+    sampled cells of the same grid."""
+    from stzpp_amd import grf as G
+
+    N, prd = (192, 192, 96), (1, 1, 0)
+    p = dict(gauss_fac=5, conv_l=4.0, chi_avg=0.09, sigma=0.01, seed=3)
+    cut, nn, chi_1 = G.geometry(N, prd, **p)
+    nx, ny, nz = N
+    nxt, nyt, nzt = G.noise_shape(N, prd, **p)
+    nc = nx * ny * nz
+    log(f"[bench] grf workload {N}, cut {cut}, noise grid {nxt}x{nyt}x{nzt}")
+    for _ in range(args.warmup):
+        G.grf_field(N, prd, **p)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        chi = G.grf_field(N, prd, **p)
+    call_s = (time.perf_counter() - t0) / args.steps
+    ms = G.time_kernels(N, prd, reps=20, **p)
+    total_s = float(ms.sum()) * 1e-3
+    names = ["k_grf_noise (Philox4x64 + polar Box-Muller)", "k_grf_x", "k_grf_axis_lds (y pass)",
+             "k_grf_axis_lds (z pass)"]
+    # algorithmic bytes per launch: each input element read once and each
+    # output written once (windows are re-read from LDS)
+    alg = [8.0 * nn, 8.0 * (nn + nx * nyt * nzt), 8.0 * (nx * nyt * nzt + nx * ny * nzt),
+           8.0 * (nx * ny * nzt + nc)]
+    dom = int(np.argmax(ms))
+    achieved = alg[dom] / (ms[dom] * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        import ctypes
+
+        import oracle as O
+
+        try:
+            ctypes.CDLL("libgomp.so.1").omp_set_num_threads(1)
+        except OSError:
+            pass
+        noise = O.grf_noise(3, nn)
+        rng = np.random.default_rng(1)
+        idx = rng.integers(0, nc, 16384)
+        t0 = time.perf_counter()
+        O.grf_convolve_points(N, prd, cut, 1 / p["conv_l"], p["chi_avg"], chi_1, noise, idx)
+        el = time.perf_counter() - t0
+        cpu = {"value": idx.size / el, "unit": "cells/s", "cores": 1, "kind": "port",
+               "sample": f"brute-force cube convolution (the reference's convolve, {(2 * cut + 1) ** 3} taps per "
+                         f"cell) at {idx.size} random cells of the same grid: {el:.1f} s; the whole field "
+                         f"would take {nc / (idx.size / el) / 3600:.1f} h"}
+    out = {
+        "metric": "random initial chi: cells per second, 192x192x96 cells, reference cube-cutoff Gaussian l=4",
+        "value": nc / total_s, "unit": "cells/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": total_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "set_random_initial_conditions 192x192x96 cells, z walled (noise padded by the "
+                               "cutoff), gauss_fac 5, conv_l 4 (cut 20), chi_avg 0.09, sigma 0.01, seed 3",
+                   "kernel_ms": dict(zip(["noise", "x", "y", "z"], [float(v) for v in ms])),
+                   "api_call_ms": call_s * 1e3, "chi_mean": float(chi.mean()), "chi_std": float(chi.std())},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": names[dom],
+                     "alg_bytes_per_launch": alg[dom], "launch_us": float(ms[dom]) * 1e3},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
@@ -166,9 +235,9 @@ def main():
     ap.add_argument("--restart", type=int, default=10, help="GMRES restart length")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch V-cycles kernel by kernel instead of replaying a hipGraph")
-    ap.add_argument("--workload", choices=("solve", "step", "adjoint"), default="solve",
+    ap.add_argument("--workload", choices=("solve", "step", "adjoint", "grf"), default="solve",
                     help="solve (default): the 256^3 projection solve; step: QS steps at 192x192x96; "
-                         "adjoint: gradient of 20 QS steps at 128^3")
+                         "adjoint: gradient of 20 QS steps at 128^3; grf: random initial chi at 192x192x96")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -178,6 +247,8 @@ def main():
         return step_workload(args)
     if args.workload == "adjoint":
         return adjoint_workload(args)
+    if args.workload == "grf":
+        return grf_workload(args)
 
     from stzpp_amd import multigrid as MG
 

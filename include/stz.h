@@ -143,6 +143,10 @@ int stz_grf_field(const int N[3], const int prd[3], const stz_grf_params *g, con
 /* Same on device buffers and a caller's HIP stream (synchronised on return). */
 int stz_grf_field_device(const int N[3], const int prd[3], const stz_grf_params *g, const double *noise_dev,
                          double *chi_dev, void *stream);
+/* Average device time per launch (ms) of the four kernels -- noise draw,
+ * x pass, y pass, z pass -- over reps back-to-back launches each, from HIP
+ * events on the library's stream (for measurement). */
+int stz_grf_time_kernels(const int N[3], const int prd[3], const stz_grf_params *g, int reps, double ms[4]);
 /* Message of the last failing stz_grf_* call of this thread. */
 const char *stz_grf_last_error(void);
 /* set_random_initial_conditions (src/shear_sim_3d.cpp:2793-2934) on the

@@ -95,6 +95,7 @@ SIGNATURES = {
     "stz_grf_cache_name": ([_ip, _ip, ctypes.POINTER(StzGrfParams), ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
     "stz_grf_field": ([_ip, _ip, ctypes.POINTER(StzGrfParams), _dp, _dp], ctypes.c_int),
     "stz_grf_field_device": ([_ip, _ip, ctypes.POINTER(StzGrfParams), _vp, _vp, _vp], ctypes.c_int),
+    "stz_grf_time_kernels": ([_ip, _ip, ctypes.POINTER(StzGrfParams), ctypes.c_int, _dp], ctypes.c_int),
     "stz_grf_last_error": ([], ctypes.c_char_p),
     "stz_set_random_initial_conditions": ([_vp, ctypes.POINTER(StzGrfParams), _dp, ctypes.c_char_p], ctypes.c_int),
     "stzmg_desc_default": ([ctypes.POINTER(StzmgDesc)] + [ctypes.c_int] * 6, ctypes.c_int),

@@ -225,6 +225,9 @@ double normal_factor(int cut, double llinv) {
 
 struct DevBuf {
     double *p = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() {
         if (p) hipFree(p);
     }
@@ -239,61 +242,84 @@ struct DevBuf {
         }                                                                    \
     } while (0)
 
+// Buffers and launches of one configuration: stage 0 draws the noise,
+// stages 1-3 are the x, y and z passes.
+struct Plan {
+    Geom G{};
+    double chi_avg = 0, chi_1 = 0;
+    uint64_t seed = 0;
+    DevBuf w, noise, t1, t2;
+
+    int setup(const Geom &geom, const stz_grf_params *g, bool own_noise, hipStream_t st) {
+        G = geom;
+        const int cut = G.cut;
+        chi_avg = g->chi_avg;
+        chi_1 = g->sigma / normal_factor(cut, G.llinv);
+        seed = (uint64_t)g->seed;
+        if (sizeof(double) * (kRowOut + 2 * cut) > 64 * 1024)
+            return gfail(STZMG_ERR_UNSUPPORTED, "cutoff too large for the x pass");
+        std::vector<double> wh(cut + 1);
+        for (int i = 0; i <= cut; i++) wh[i] = exp(-G.llinv * (double)(i * i));
+        const int nx = G.N[0], ny = G.N[1];
+        GHIP(hipMalloc((void **)&w.p, sizeof(double) * (cut + 1)));
+        GHIP(hipMalloc((void **)&t1.p, sizeof(double) * (size_t)nx * G.nt[1] * G.nt[2]));
+        GHIP(hipMalloc((void **)&t2.p, sizeof(double) * (size_t)nx * ny * G.nt[2]));
+        if (own_noise) GHIP(hipMalloc((void **)&noise.p, sizeof(double) * (size_t)G.nnoise()));
+        GHIP(hipMemcpyAsync(w.p, wh.data(), sizeof(double) * (cut + 1), hipMemcpyHostToDevice, st));
+        GHIP(hipStreamSynchronize(st));  // wh is released on return
+        return 0;
+    }
+
+    int enqueue(int stage, const double *noise_dev, double *chi, hipStream_t st) const {
+        const int cut = G.cut;
+        const int nx = G.N[0], ny = G.N[1], nz = G.N[2];
+        const int nyt = G.nt[1], nzt = G.nt[2];
+        if (stage == 0) {
+            const long nn = G.nnoise(), np = (nn + 1) / 2;
+            const unsigned nb = (unsigned)std::min<long>((np + 255) / 256, 1L << 16);
+            hipLaunchKernelGGL(k_grf_noise, dim3(nb), dim3(256), 0, st, seed, nn, noise.p);
+        } else if (stage == 1) {
+            const int segs = (nx + kRowOut - 1) / kRowOut;
+            const long rows = (long)nyt * nzt;
+            if (rows * segs > 0x7fffffffL) return gfail(STZMG_ERR_UNSUPPORTED, "grid too large");
+            hipLaunchKernelGGL(k_grf_x, dim3((unsigned)(rows * segs)), dim3(kRowOut),
+                               sizeof(double) * (kRowOut + 2 * cut), st, noise_dev, t1.p, (const double *)w.p, nx,
+                               G.nt[0], G.prd[0], cut, rows, segs);
+        } else {
+            const AxisPass P = stage == 2
+                ? AxisPass{nx, ny, G.prd[1], cut, nzt, nx, (long)nx * nyt, nx, (long)nx * ny, 0.0, 1.0}
+                : AxisPass{nx, nz, G.prd[2], cut, ny, (long)nx * ny, nx, (long)nx * ny, nx, chi_avg, chi_1};
+            const double *in = stage == 2 ? t1.p : t2.p;
+            double *out = stage == 2 ? t2.p : chi;
+            if (cut <= kLdsCut) {
+                const int tx = (P.nx + kColX - 1) / kColX, ta = (P.na_out + kColA - 1) / kColA;
+                const long nb = (long)tx * ta * P.no;
+                if (nb > 0x7fffffffL) return gfail(STZMG_ERR_UNSUPPORTED, "grid too large");
+                hipLaunchKernelGGL(k_grf_axis_lds, dim3((unsigned)nb), dim3(256),
+                                   sizeof(double) * kColX * (kColA + 2 * cut), st, in, out, (const double *)w.p, P,
+                                   tx, ta);
+            } else {
+                const long tot = (long)P.nx * P.na_out * P.no;
+                const unsigned nb = (unsigned)std::min<long>((tot + 255) / 256, 1L << 20);
+                hipLaunchKernelGGL(k_grf_axis_direct, dim3(nb), dim3(256), 0, st, in, out, (const double *)w.p, P);
+            }
+        }
+        GHIP(hipGetLastError());
+        return 0;
+    }
+};
+
 // chi (device, Nx Ny Nz) from noise (device, the padded grid; drawn from
 // the seed when null) on stream st.
 int grf_device(const Geom &G, const stz_grf_params *g, const double *noise_dev, double *chi, hipStream_t st) {
-    const int cut = G.cut;
-    std::vector<double> wh(cut + 1);
-    for (int i = 0; i <= cut; i++) wh[i] = exp(-G.llinv * (double)(i * i));
-    const double chi_1 = g->sigma / normal_factor(cut, G.llinv);
-    const long nn = G.nnoise();
-    const int nx = G.N[0], ny = G.N[1], nz = G.N[2];
-    const int nyt = G.nt[1], nzt = G.nt[2];
-    DevBuf w, noise, t1, t2;
-    GHIP(hipMalloc((void **)&w.p, sizeof(double) * (cut + 1)));
-    GHIP(hipMalloc((void **)&t1.p, sizeof(double) * (size_t)nx * nyt * nzt));
-    GHIP(hipMalloc((void **)&t2.p, sizeof(double) * (size_t)nx * ny * nzt));
-    GHIP(hipMemcpyAsync(w.p, wh.data(), sizeof(double) * (cut + 1), hipMemcpyHostToDevice, st));
+    Plan P;
+    if (int rc = P.setup(G, g, noise_dev == nullptr, st)) return rc;
     if (!noise_dev) {
-        GHIP(hipMalloc((void **)&noise.p, sizeof(double) * (size_t)nn));
-        const long np = (nn + 1) / 2;
-        const unsigned nb = (unsigned)std::min<long>((np + 255) / 256, 1L << 16);
-        hipLaunchKernelGGL(k_grf_noise, dim3(nb), dim3(256), 0, st, (uint64_t)g->seed, nn, noise.p);
-        GHIP(hipGetLastError());
-        noise_dev = noise.p;
+        if (int rc = P.enqueue(0, nullptr, nullptr, st)) return rc;
+        noise_dev = P.noise.p;
     }
-    // x pass
-    const int segs = (nx + kRowOut - 1) / kRowOut;
-    const long rows = (long)nyt * nzt;
-    const size_t lx = sizeof(double) * (kRowOut + 2 * cut);
-    if (lx > 64 * 1024) return gfail(STZMG_ERR_UNSUPPORTED, "cutoff too large for the x pass");
-    if (rows * segs > 0x7fffffffL) return gfail(STZMG_ERR_UNSUPPORTED, "grid too large");
-    hipLaunchKernelGGL(k_grf_x, dim3((unsigned)(rows * segs)), dim3(kRowOut), lx, st, noise_dev, t1.p,
-                       (const double *)w.p, nx, G.nt[0], G.prd[0], cut, rows, segs);
-    GHIP(hipGetLastError());
-    // y and z passes
-    AxisPass Y{nx, ny, G.prd[1], cut, nzt, nx, (long)nx * nyt, nx, (long)nx * ny, 0.0, 1.0};
-    AxisPass Z{nx, nz, G.prd[2], cut, ny, (long)nx * ny, nx, (long)nx * ny, nx, g->chi_avg, chi_1};
-    const double *ins[2] = {t1.p, t2.p};
-    double *outs[2] = {t2.p, chi};
-    const AxisPass *ps[2] = {&Y, &Z};
-    for (int q = 0; q < 2; q++) {
-        const AxisPass &P = *ps[q];
-        if (cut <= kLdsCut) {
-            const int tx = (P.nx + kColX - 1) / kColX, ta = (P.na_out + kColA - 1) / kColA;
-            const long nb = (long)tx * ta * P.no;
-            if (nb > 0x7fffffffL) return gfail(STZMG_ERR_UNSUPPORTED, "grid too large");
-            const size_t lb = sizeof(double) * kColX * (kColA + 2 * cut);
-            hipLaunchKernelGGL(k_grf_axis_lds, dim3((unsigned)nb), dim3(256), lb, st, ins[q], outs[q],
-                               (const double *)w.p, P, tx, ta);
-        } else {
-            const long tot = (long)P.nx * P.na_out * P.no;
-            const unsigned nb = (unsigned)std::min<long>((tot + 255) / 256, 1L << 20);
-            hipLaunchKernelGGL(k_grf_axis_direct, dim3(nb), dim3(256), 0, st, ins[q], outs[q], (const double *)w.p,
-                               P);
-        }
-        GHIP(hipGetLastError());
-    }
+    for (int stage = 1; stage <= 3; stage++)
+        if (int rc = P.enqueue(stage, noise_dev, chi, st)) return rc;
     GHIP(hipStreamSynchronize(st));
     return 0;
 }
@@ -375,6 +401,42 @@ int stz_grf_field_device(const int N[3], const int prd[3], const stz_grf_params 
     if (int rc = make_geom(N, prd, g, G)) return rc;
     if (!chi_dev) return gfail(STZMG_ERR_ARG, "null output");
     return grf_device(G, g, noise_dev, chi_dev, (hipStream_t)stream);
+}
+
+int stz_grf_time_kernels(const int N[3], const int prd[3], const stz_grf_params *g, int reps, double ms[4]) {
+    Geom G;
+    if (int rc = make_geom(N, prd, g, G)) return rc;
+    if (!ms || reps < 1) return gfail(STZMG_ERR_ARG, "bad timing arguments");
+    hipStream_t st = nullptr;
+    GHIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    struct Guard {
+        hipStream_t s;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Guard() {
+            if (e0) hipEventDestroy(e0);
+            if (e1) hipEventDestroy(e1);
+            hipStreamDestroy(s);
+        }
+    } guard{st};
+    GHIP(hipEventCreate(&guard.e0));
+    GHIP(hipEventCreate(&guard.e1));
+    Plan P;
+    DevBuf chi;
+    if (int rc = P.setup(G, g, true, st)) return rc;
+    GHIP(hipMalloc((void **)&chi.p, sizeof(double) * (size_t)G.ncell()));
+    for (int stage = 0; stage <= 3; stage++)
+        if (int rc = P.enqueue(stage, P.noise.p, chi.p, st)) return rc;
+    for (int stage = 0; stage <= 3; stage++) {
+        GHIP(hipEventRecord(guard.e0, st));
+        for (int q = 0; q < reps; q++)
+            if (int rc = P.enqueue(stage, P.noise.p, chi.p, st)) return rc;
+        GHIP(hipEventRecord(guard.e1, st));
+        GHIP(hipEventSynchronize(guard.e1));
+        float el = 0;
+        GHIP(hipEventElapsedTime(&el, guard.e0, guard.e1));
+        ms[stage] = el / reps;
+    }
+    return 0;
 }
 
 int stz_set_random_initial_conditions(stz_t s, const stz_grf_params *g, const double *noise, const char *cache_dir) {

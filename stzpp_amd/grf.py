@@ -96,4 +96,14 @@ def grf_field(N, prd=(1, 1, 0), noise=None, **params):
     return chi
 
 
-__all__ = ["GrfError", "default_params", "geometry", "noise_shape", "cache_name", "grf_field"]
+def time_kernels(N, prd=(1, 1, 0), reps=20, **params):
+    """Average device ms per launch of (noise, x pass, y pass, z pass)."""
+    L = _lib.load()
+    n3, p3 = _geom(N, prd)
+    ms = np.zeros(4)
+    _chk(L.stz_grf_time_kernels(_lib.iptr(n3), _lib.iptr(p3), ctypes.byref(_params(params)), int(reps),
+                                _lib.dptr(ms)))
+    return ms
+
+
+__all__ = ["GrfError", "time_kernels", "default_params", "geometry", "noise_shape", "cache_name", "grf_field"]
