@@ -154,6 +154,47 @@ const char *stz_grf_last_error(void);
  * instead, and a computed field is written to it (raw FP64, x fastest). */
 int stz_set_random_initial_conditions(stz_t s, const stz_grf_params *g, const double *noise, const char *cache_dir);
 
+/* ---- output in the reference's file formats ------------------------------
+ *
+ * Cube modes are those of Field::fval (src/field.hh:158-178) and write_files
+ * (src/shear_sim_3d.cpp:1938-1960): 0-2 u v w, 3-8 s11 s12 s13 s22 s23 s33,
+ * 9 p, 10 dev, 11 chi, 12 tem (chi * 21000).  The reference map (13-15) is
+ * not part of the quasi-static state and is rejected.  A cube is float32,
+ * x slowest, then y, z fastest (output_cube, :1963-2161): Nx*Ny*(Nz+1)
+ * values for the velocity (the top plate plane; the periodic image of plane
+ * 0 when z is periodic), Nx*Ny*Nz otherwise. */
+long long stz_cube_size(stz_t s, int mode);   /* -1 for an unknown mode */
+int stz_output_cube(stz_t s, int mode, float *out, long long n);
+/* Appends the cube to <dir>/<name>_grid.<frame> (name: u v w s11 ... tem),
+ * creating dir (one level) as the reference does. */
+int stz_write_cube(stz_t s, const char *dir, int mode, int frame);
+/* write_files(frame) (src/shear_sim_3d.cpp:1938-1960): the modes whose bit
+ * is set in fflags (the reference's build uses 4096, tem only), the velocity
+ * and stress every n_tp/10 frames, chi when (n_tp/10) % frame == 0 for
+ * frame > 0.  n_tp must be at least 10 (the reference divides by n_tp/10). */
+int stz_write_files(stz_t s, const char *dir, int frame, int n_tp, int fflags);
+/* Wall tractions (compute_net_force, src/shear_sim_3d.cpp:1061-1126,
+ * src/trans_sim_3d.cpp:1575-1660): out = top (s13, s23, s33) * dx dy and
+ * bottom (s13, s23, s33) * (-dx dy) of the stress extrapolated to the walls,
+ * top 1.5 s(Nz-1) - 0.5 s(Nz-2), bottom 1.5 s(x) - 0.5 s(x+1) (the
+ * reference pairs each bottom cell with its +x neighbour); the transformed
+ * step reports the physical stress T s T^t at the current time. */
+int stz_net_force(stz_t s, double out[6]);
+/* Appends the traction line to <dir>/traction.dat ("%.16f" x 7 with the
+ * time first; transformed: "%g" x 6) and, untransformed, the six doubles to
+ * <dir>/trac_grid.bin. */
+int stz_write_traction(stz_t s, const char *dir);
+/* Sum of sym_mat3::modsq of the last untransformed projection's stress
+ * increments (proj_mag, src/shear_sim_3d.cpp:699-724). */
+double stz_proj_mag(stz_t s);
+/* Per-step output during stz_step_qs, as solve_qs and projection_step write
+ * it (src/shear_sim_3d.cpp:336-377, :714-724): STZ_OUT_TRACTION appends the
+ * traction line after every step, STZ_OUT_PROJ_MAG one "%g %g" line (time,
+ * proj_mag) to proj_mag.dat per untransformed projection (the first line of a
+ * handle truncates the file).  flags = 0 turns it off. */
+enum { STZ_OUT_TRACTION = 1, STZ_OUT_PROJ_MAG = 2 };
+int stz_set_output(stz_t s, const char *dir, int flags);
+
 #ifdef __cplusplus
 }
 #endif

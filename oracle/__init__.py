@@ -355,6 +355,10 @@ def ref_lib():
             ("ref_trans_destroy", [vp], None),
             ("ref_grf", [ctypes.c_int] * 3 + [_ip, _dp, ctypes.c_int, ctypes.c_char_p, _dp, ctypes.c_long, _dp,
                          _dp], ctypes.c_int),
+            ("ref_shear_output_cube", [vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int], None),
+            ("ref_shear_net_force", [vp], None),
+            ("ref_trans_output_cube", [vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int], None),
+            ("ref_trans_net_force", [vp], None),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -471,6 +475,8 @@ def _step_lib():
         L.sto_set_solver.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L.sto_set_transform.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L.sto_set_eta.argtypes = [vp, ctypes.c_double]
+        L.sto_proj_mag.argtypes = [vp]
+        L.sto_proj_mag.restype = ctypes.c_double
         L.sto_set_textbook.argtypes = [vp, ctypes.c_int]
         L.sto_trans_stencil.argtypes = [_dp] + [ctypes.c_double] * 5 + [_dp]
         L.sto_transform_at.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -551,6 +557,11 @@ class OracleShear:
         family ``family`` (TRANSFORMS) at rate Ut."""
         self.L.sto_set_transform(self.h, TRANSFORMS[family] if isinstance(family, str) else family, Ut)
 
+    @property
+    def proj_mag(self):
+        """Sum of sym_mat3::modsq of the last projection's stress increments."""
+        return self.L.sto_proj_mag(self.h)
+
     def set_eta(self, eta):
         """Override the adaptive-step tolerance eta of the plastic update."""
         self.L.sto_set_eta(self.h, eta)
@@ -582,6 +593,14 @@ class RefShear:
             self.h = None
 
     __del__ = close
+
+    def output_cube(self, prefix, mode, sn):
+        """The reference's output_cube into the simulation's directory."""
+        self.L.ref_shear_output_cube(self.h, prefix.encode(), int(mode), int(sn))
+
+    def net_force(self):
+        """The reference's compute_net_force (appends to traction.dat)."""
+        self.L.ref_shear_net_force(self.h)
 
     def setup(self, dt):
         self.L.ref_shear_setup(self.h, dt)
@@ -634,6 +653,13 @@ class RefTrans:
             self.h = None
 
     __del__ = close
+
+    def output_cube(self, prefix, mode, sn):
+        self.L.ref_trans_output_cube(self.h, prefix.encode(), int(mode), int(sn))
+
+    def net_force(self):
+        """compute_net_force at the current time (appends to traction.dat)."""
+        self.L.ref_trans_net_force(self.h)
 
     def setup(self, dt):
         self.L.ref_trans_setup(self.h, dt)
@@ -788,3 +814,86 @@ def ref_grf(N, prd, gauss_fac, conv_l, chi_avg, sigma, TZ, sim_case, workdir):
         raise RuntimeError(f"ref_grf failed ({rc})")
     return dict(noise=noise, chi=chi, chi_1=float(out3[0]), gnf=float(out3[1]), cut=int(out3[2]),
                 files=sorted(os.listdir(workdir)))
+
+
+# ---------------------------------------------------------------------------
+# Output of the reference (numpy restatements): output_cube
+# (src/shear_sim_3d.cpp:1963-2161 with Field::fval, src/field.hh:144-178) and
+# compute_net_force (src/shear_sim_3d.cpp:1061-1126, src/trans_sim_3d.cpp:
+# 1575-1660), in the reference's arithmetic order.
+
+CUBE_NAMES = ["u", "v", "w", "s11", "s12", "s13", "s22", "s23", "s33", "p", "dev", "chi", "tem"]
+
+
+def cube_values(mode, N, prd, u, sig, chi):
+    """The float32 cube of write_files mode `mode` (0-12): x slowest, then y,
+    z fastest; velocities carry Nz + 1 z points (the top plane; the periodic
+    image of plane 0 when z is periodic)."""
+    Nx, Ny, Nz = N
+    if mode <= 2:
+        nzv = Nz + (0 if prd[2] else 1)
+        f = np.asarray(u)[:, mode].reshape(nzv, Ny, Nx)
+        if prd[2]:
+            f = np.concatenate([f, f[:1]], axis=0)
+    else:
+        s = np.asarray(sig).reshape(Nz, Ny, Nx, 6)
+        s11, s12, s13, s22, s23, s33 = (s[..., q] for q in range(6))
+        if 3 <= mode <= 8:
+            f = s[..., mode - 3]
+        elif mode in (9, 10):
+            p = -(s11 + s22 + s33) * (1 / 3.0)
+            if mode == 9:
+                f = p
+            else:
+                t11, t22, t33 = s11 + p, s22 + p, s33 + p
+                f = np.sqrt(0.5 * (t11 * t11 + t22 * t22 + t33 * t33) + s12 * s12 + s13 * s13 + s23 * s23)
+        elif mode == 11:
+            f = np.asarray(chi).reshape(Nz, Ny, Nx)
+        elif mode == 12:
+            f = np.asarray(chi).reshape(Nz, Ny, Nx) * 21000
+        else:
+            raise ValueError("mode")
+    return np.ascontiguousarray(np.transpose(f, (2, 1, 0))).astype(np.float32)
+
+
+def net_force(N, box, sig, T=None):
+    """(ts13, ts23, ts33, bs13, bs23, bs33): the wall tractions of
+    compute_net_force.  The bottom 'extrapolation' pairs each cell with its
+    +x neighbour (the reference's grido + 1); with T, the physical stress
+    T s T^t of the extrapolated transformed stress (the trans override)."""
+    Nx, Ny, Nz = N
+    dx, dy = (box[1] - box[0]) / Nx, (box[3] - box[2]) / Ny
+    s = np.asarray(sig).reshape(Nz, Ny, Nx, 6)
+
+    def comp(a, b):
+        v = [1.5 * a[q] - 0.5 * b[q] for q in range(6)]
+        if T is None:
+            return v[2], v[4], v[5]
+        M = np.array([[v[0], v[1], v[2]], [v[1], v[3], v[4]], [v[2], v[4], v[5]]])
+        P = (np.asarray(T) @ M) @ np.asarray(T).T
+        return P[0, 2], P[1, 2], P[2, 2]
+
+    b = [0.0, 0.0, 0.0]
+    t = [0.0, 0.0, 0.0]
+    for yy in range(Ny):
+        for xx in range(Nx):
+            c = comp(s[0, yy, xx], s[0, yy, (xx + 1) % Nx])
+            for q in range(3):
+                b[q] += c[q]
+    for yy in range(Ny):
+        for xx in range(Nx):
+            c = comp(s[Nz - 1, yy, xx], s[Nz - 2, yy, xx])
+            for q in range(3):
+                t[q] += c[q]
+    f = -dx * dy
+    b = [v * f for v in b]
+    g = dx * dy
+    t = [v * g for v in t]
+    return t + b
+
+
+def traction_line(t, vals, trans=False):
+    """One line of traction.dat as the reference prints it."""
+    if trans:
+        return "\t".join("%g" % v for v in vals) + "\n"
+    return "\t".join("%.16f" % v for v in [t] + list(vals)) + "\n"

@@ -230,6 +230,7 @@ class ref_shear : public shear_sim_3d<sym_mat3> {
     int gx() const { return nxg; }
     int gxy() const { return nxyg; }
     void boundary() { set_up_ghost_regions(); set_boundary_conditions(); }
+    void net_force() { compute_net_force(); }
     const sym_mat3 *stencil_ptr() const { return stencil; }
     const sym_mat3 *fix_ptr() const { return fix; }
 };
@@ -356,6 +357,10 @@ class ref_trans : public trans_sim_3d {
     int gx() const { return nxg; }
     int gxy() const { return nxyg; }
     void boundary() { set_up_ghost_regions(); set_boundary_conditions(); }
+    void net_force() {
+        update_matrices();
+        compute_net_force();
+    }
     const mat3 *stencil_ptr() const { return stencil; }
     const mat3 *fix_ptr() const { return fix; }
     void matrices(double *T9, double *Ti9, double *dT9) {
@@ -549,5 +554,25 @@ int ref_grf(int Nx, int Ny, int Nz, const int *periods, const double *p, int sim
     MPI_Comm_free(&cart);
     return rc;
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Output of the reference (output_cube, src/shear_sim_3d.cpp:1963-2161;
+// compute_net_force, :1061-1126 and src/trans_sim_3d.cpp:1575-1660): the
+// files land in the directory the simulation was created with.
+extern "C" {
+
+void ref_shear_output_cube(void *hp, const char *prefix, int mode, int sn) {
+    ((shear_handle *)hp)->sim->output_cube(prefix, mode, sn);
+}
+
+void ref_shear_net_force(void *hp) { ((shear_handle *)hp)->sim->net_force(); }
+
+void ref_trans_output_cube(void *hp, const char *prefix, int mode, int sn) {
+    ((trans_handle *)hp)->sim->output_cube(prefix, mode, sn);
+}
+
+void ref_trans_net_force(void *hp) { ((trans_handle *)hp)->sim->net_force(); }
 
 }  // extern "C"

@@ -51,6 +51,7 @@ typedef struct {
     /* transformed-coordinate step (trans_sim_3d); tfam 0 = shear_sim_3d */
     int tfam;
     double Ut;
+    double proj_mag;       /* sum of modsq of the last projection's stress increments */
     double ax, ay, az, bx, by, bz;
     double T[9], Ti[9], dT[9];
     int textbook;          /* test switch: 1 = textbook x-ENO for chi instead of :985 */
@@ -139,6 +140,14 @@ static void boundary(sto_sim *S) {
             long t0 = gi(S, i, j, Nz), t1 = gi(S, i, j, Nz + 1), tm = gi(S, i, j, Nz - 1);
             S->f[F_U][t0] = ut; S->f[F_V][t0] = 0; S->f[F_W][t0] = 0;
             for (int q = F_U; q <= F_W; q++) S->f[q][t1] = 2 * S->f[q][t0] - S->f[q][tm];
+            /* The shear step's bottom plate: set_boundary_conditions tests
+             * `pz == max_pz ... else if (pz == min_pz)`
+             * (src/shear_sim_3d.cpp:2393, :2468), so with one rank in z only
+             * the top plate is set and the bottom plane keeps its velocity
+             * (after a projection, the solve's Dirichlet row leaves -ut
+             * there).  The transformed step sets both walls
+             * (src/trans_sim_3d.cpp:1138-1179). */
+            if (!S->tfam) continue;
             long b0 = gi(S, i, j, 0), b1 = gi(S, i, j, 1), bm1 = gi(S, i, j, -1), bm2 = gi(S, i, j, -2);
             S->f[F_U][b0] = -ut; S->f[F_V][b0] = 0; S->f[F_W][b0] = 0;
             for (int q = F_U; q <= F_W; q++) {
@@ -701,6 +710,10 @@ static void projection(sto_sim *S, double dt) {
                 }
         return;
     }
+    /* the stress increments and their sum of sym_mat3::modsq, the
+     * reference's proj_mag (src/shear_sim_3d.cpp:684-724; modsq,
+     * src/mat3.hh:338-340), in its loop order */
+    double pm = 0;
     for (int k = 0; k < S->Nz; k++)
         for (int j = 0; j < S->Ny; j++)
             for (int i = 0; i < S->Nx; i++) {
@@ -708,14 +721,26 @@ static void projection(sto_sim *S, double dt) {
                 double g[3][3];
                 vel_grad(S, c, g);
                 double lam = S->lamb, mu = S->mu, L2 = lam + 2 * mu;
-                S->f[F_S11][c] += dt * (lam * (g[2][2] + g[1][1]) + L2 * g[0][0]);
-                S->f[F_S12][c] += dt * mu * (g[0][1] + g[1][0]);
-                S->f[F_S13][c] += dt * mu * (g[0][2] + g[2][0]);
-                S->f[F_S22][c] += dt * (lam * (g[2][2] + g[0][0]) + L2 * g[1][1]);
-                S->f[F_S23][c] += dt * mu * (g[1][2] + g[2][1]);
-                S->f[F_S33][c] += dt * (lam * (g[1][1] + g[0][0]) + L2 * g[2][2]);
+                const double d11 = dt * (lam * (g[2][2] + g[1][1]) + L2 * g[0][0]);
+                const double d12 = dt * mu * (g[0][1] + g[1][0]);
+                const double d13 = dt * mu * (g[0][2] + g[2][0]);
+                const double d22 = dt * (lam * (g[2][2] + g[0][0]) + L2 * g[1][1]);
+                const double d23 = dt * mu * (g[1][2] + g[2][1]);
+                const double d33 = dt * (lam * (g[1][1] + g[0][0]) + L2 * g[2][2]);
+                pm += 0.5 * (d11 * d11 + d12 * d12 + d13 * d13 + d12 * d12 + d22 * d22 + d23 * d23 + d13 * d13 +
+                             d23 * d23 + d33 * d33);
+                S->f[F_S11][c] += d11;
+                S->f[F_S12][c] += d12;
+                S->f[F_S13][c] += d13;
+                S->f[F_S22][c] += d22;
+                S->f[F_S23][c] += d23;
+                S->f[F_S33][c] += d33;
             }
+    S->proj_mag = pm;
 }
+
+/* proj_mag of the last untransformed projection. */
+double sto_proj_mag(void *hp) { return ((sto_sim *)hp)->proj_mag; }
 
 void sto_step(void *hp, double dt, int nsteps) {
     sto_sim *S = (sto_sim *)hp;

@@ -98,6 +98,14 @@ SIGNATURES = {
     "stz_grf_time_kernels": ([_ip, _ip, ctypes.POINTER(StzGrfParams), ctypes.c_int, _dp], ctypes.c_int),
     "stz_grf_last_error": ([], ctypes.c_char_p),
     "stz_set_random_initial_conditions": ([_vp, ctypes.POINTER(StzGrfParams), _dp, ctypes.c_char_p], ctypes.c_int),
+    "stz_cube_size": ([_vp, ctypes.c_int], ctypes.c_longlong),
+    "stz_output_cube": ([_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_longlong], ctypes.c_int),
+    "stz_write_cube": ([_vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "stz_write_files": ([_vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "stz_net_force": ([_vp, _dp], ctypes.c_int),
+    "stz_write_traction": ([_vp, ctypes.c_char_p], ctypes.c_int),
+    "stz_proj_mag": ([_vp], ctypes.c_double),
+    "stz_set_output": ([_vp, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
     "stzmg_desc_default": ([ctypes.POINTER(StzmgDesc)] + [ctypes.c_int] * 6, ctypes.c_int),
     "stzmg_create": ([ctypes.POINTER(StzmgDesc), ctypes.POINTER(_vp)], ctypes.c_int),
     "stzmg_destroy": ([_vp], ctypes.c_int),

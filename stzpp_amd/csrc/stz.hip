@@ -244,6 +244,7 @@ int stz_create(const stz_params *p, stz_t *out) {
     A(&s->dpl, Nc); A(&s->dpl2, Nc); A(&s->adapt, Nc); A(&s->dchi, Nc);
     A(&s->stage, 6 * (Nc > Nv ? Nc : Nv));
     if (e == hipSuccess) e = hipMalloc((void **)&s->counters, sizeof(int) * 4);
+    A(&s->pm_part, (long)nb(Nc) + 1);
     if (e != hipSuccess) {
         g_stz_err = "device allocation failed";
         stz_destroy(s);
@@ -264,6 +265,7 @@ int stz_destroy(stz_t s) {
     for (double *q : {s->vel, s->sig, s->sig2, s->chi, s->chi2, s->dpl, s->dpl2, s->adapt, s->dchi, s->stage})
         hipFree(q);
     hipFree(s->counters);
+    hipFree(s->pm_part);
     if (s->mg) stzmg_destroy(s->mg);
     delete s;
     return 0;
@@ -279,9 +281,14 @@ int stz_set_state(stz_t s, double t, const double *u, const double *sig, const d
     s->t = t;
     SHIP(s, hipMemcpyAsync(s->stage, u, sizeof(double) * 3 * g.Nv, hipMemcpyHostToDevice, s->stream));
     hipLaunchKernelGGL(k_aos2soa, dim3(nb(g.Nv)), dim3(kT), 0, s->stream, (const double *)s->stage, s->sig2, g.Nv, 3);
-    // plates at time t (set_boundary_conditions)
+    // plates at time t (set_boundary_conditions).  The shear step sets only
+    // the top plate: with one rank in z the bottom branch of its
+    // `pz == max_pz ... else if (pz == min_pz)` chain is never taken
+    // (src/shear_sim_3d.cpp:2393, :2468), so the bottom plane keeps the
+    // given velocity until the first projection; the transformed step sets
+    // both walls to rest (src/trans_sim_3d.cpp:1138-1179).
     hipLaunchKernelGGL(k_copy_vel, dim3(nb(g.Nv)), dim3(kT), 0, s->stream, g, (const double *)s->sig2, top_u(s, t),
-                       s->vel);
+                       s->vel, s->p.transform != STZ_T_NONE ? 1 : 0);
     SHIP(s, hipMemcpyAsync(s->stage, sig, sizeof(double) * 6 * g.Nc, hipMemcpyHostToDevice, s->stream));
     hipLaunchKernelGGL(k_aos2soa, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, (const double *)s->stage, s->sig, g.Nc, 6);
     SHIP(s, hipMemcpyAsync(s->chi, chi, sizeof(double) * g.Nc, hipMemcpyHostToDevice, s->stream));
@@ -390,16 +397,22 @@ int step_once(stz_t s, double dt, stzmg_solve_opts &o, bool ref_tol, stz_step_st
     if (trans)
         hipLaunchKernelGGL(k_stress_update_trans, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt, X1,
                            box, (const double *)s->vel, s->sig);
-    else
+    else {
         hipLaunchKernelGGL(k_stress_update, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt,
-                           (const double *)s->vel, s->sig);
+                           (const double *)s->vel, s->sig, s->pm_part);
+        hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kT), 0, s->stream, (const double *)s->pm_part, (long)nb(g.Nc),
+                           s->pm_part + nb(g.Nc));
+    }
     SHIP(s, hipGetLastError());
     int cnt[4];
+    double pm = 0;
     SHIP(s, hipMemcpyAsync(cnt, s->counters, sizeof(cnt), hipMemcpyDeviceToHost, s->stream));
+    if (!trans) SHIP(s, hipMemcpyAsync(&pm, s->pm_part + nb(g.Nc), sizeof(double), hipMemcpyDeviceToHost, s->stream));
     SHIP(s, hipStreamSynchronize(s->stream));
+    s->proj_mag = pm;
     S.yielded_cells = cnt[0];
     S.steps++;
-    return 0;
+    return step_output(s);
 }
 
 }  // namespace stz_detail

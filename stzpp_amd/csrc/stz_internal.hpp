@@ -22,6 +22,11 @@ struct stz_s {
     double *vel = nullptr, *sig = nullptr, *sig2 = nullptr, *chi = nullptr, *chi2 = nullptr;
     double *dpl = nullptr, *dpl2 = nullptr, *adapt = nullptr, *dchi = nullptr, *stage = nullptr;
     int *counters = nullptr;
+    double *pm_part = nullptr;  // proj_mag: per-block partial sums, then the total
+    double proj_mag = 0;        // sum of modsq of the last projection's stress increments
+    std::string out_dir;        // per-step output (stz_set_output)
+    int out_flags = 0;
+    int pm_writes = 0;          // proj_mag.dat lines written by this handle
     std::string err;
 };
 
@@ -43,5 +48,7 @@ void step_options(stz_t s, const stzmg_solve_opts *opts, stzmg_solve_opts &o);
 // sigstar is not null it receives the advected stress sigma* (6 Nc doubles,
 // device) that enters the projection.
 int step_once(stz_t s, double dt, stzmg_solve_opts &o, bool ref_tol, stz_step_stats &S, double *sigstar);
+// The per-step output of stz_set_output after a step (stz_output.hip).
+int step_output(stz_t s);
 
 }  // namespace stz_detail

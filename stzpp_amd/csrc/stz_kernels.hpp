@@ -286,35 +286,72 @@ static __global__ __launch_bounds__(kT) void k_rhs(Grid g, const double *__restr
 // exactly (projection copy-out and set_boundary_conditions,
 // src/shear_sim_3d.cpp:615-681, :2362-2470).
 static __global__ __launch_bounds__(kT) void k_copy_vel(Grid g, const double *__restrict__ x, double top_u,
-                                                 double *__restrict__ vel) {
+                                                 double *__restrict__ vel, int bottom = 1) {
     const long p = (long)blockIdx.x * kT + threadIdx.x;
     if (p >= g.Nv) return;
     const int k = (int)(p / ((long)g.Nx * g.Ny));
     double v0 = x[p], v1 = x[g.Nv + p], v2 = x[2 * g.Nv + p];
-    if (!g.pz && k == 0) { v0 = -top_u; v1 = 0; v2 = 0; }
+    if (!g.pz && k == 0 && bottom) { v0 = -top_u; v1 = 0; v2 = 0; }
     if (!g.pz && k == g.Nz) { v0 = top_u; v1 = 0; v2 = 0; }
     vel[p] = v0;
     vel[g.Nv + p] = v1;
     vel[2 * g.Nv + p] = v2;
 }
 
-// Stress update sigma += dt C:D(u) (src/shear_sim_3d.cpp:684-712).
+// Stress update sigma += dt C:D(u) (src/shear_sim_3d.cpp:684-712), and
+// per block the sum of sym_mat3::modsq of the increments (the reference's
+// proj_mag, :699-724; modsq, src/mat3.hh:338-340) into pm_part[block].
 static __global__ __launch_bounds__(kT) void k_stress_update(Grid g, Mat M, double dt, const double *__restrict__ vel,
-                                                      double *__restrict__ sig) {
+                                                      double *__restrict__ sig, double *__restrict__ pm_part) {
     const long c = (long)blockIdx.x * kT + threadIdx.x;
-    if (c >= g.Nc) return;
-    const int i = (int)(c % g.Nx);
-    const int j = (int)((c / g.Nx) % g.Ny);
-    const int k = (int)(c / ((long)g.Nx * g.Ny));
-    double G[3][3], v[3];
-    vel_grad(vel, g, i, j, k, G, v);
-    const double lam = M.lamb, mu = M.mu, L2 = lam + 2 * mu;
-    sig[c] += dt * (lam * (G[2][2] + G[1][1]) + L2 * G[0][0]);
-    sig[g.Nc + c] += dt * mu * (G[0][1] + G[1][0]);
-    sig[2 * g.Nc + c] += dt * mu * (G[0][2] + G[2][0]);
-    sig[3 * g.Nc + c] += dt * (lam * (G[2][2] + G[0][0]) + L2 * G[1][1]);
-    sig[4 * g.Nc + c] += dt * mu * (G[1][2] + G[2][1]);
-    sig[5 * g.Nc + c] += dt * (lam * (G[1][1] + G[0][0]) + L2 * G[2][2]);
+    double pm = 0;
+    if (c < g.Nc) {
+        const int i = (int)(c % g.Nx);
+        const int j = (int)((c / g.Nx) % g.Ny);
+        const int k = (int)(c / ((long)g.Nx * g.Ny));
+        double G[3][3], v[3];
+        vel_grad(vel, g, i, j, k, G, v);
+        const double lam = M.lamb, mu = M.mu, L2 = lam + 2 * mu;
+        const double d11 = dt * (lam * (G[2][2] + G[1][1]) + L2 * G[0][0]);
+        const double d12 = dt * mu * (G[0][1] + G[1][0]);
+        const double d13 = dt * mu * (G[0][2] + G[2][0]);
+        const double d22 = dt * (lam * (G[2][2] + G[0][0]) + L2 * G[1][1]);
+        const double d23 = dt * mu * (G[1][2] + G[2][1]);
+        const double d33 = dt * (lam * (G[1][1] + G[0][0]) + L2 * G[2][2]);
+        sig[c] += d11;
+        sig[g.Nc + c] += d12;
+        sig[2 * g.Nc + c] += d13;
+        sig[3 * g.Nc + c] += d22;
+        sig[4 * g.Nc + c] += d23;
+        sig[5 * g.Nc + c] += d33;
+        pm = 0.5 * (d11 * d11 + d12 * d12 + d13 * d13 + d12 * d12 + d22 * d22 + d23 * d23 + d13 * d13 +
+                    d23 * d23 + d33 * d33);
+    }
+    __shared__ double red[kT / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) pm += __shfl_down(pm, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = pm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int w = 0; w < kT / 64; w++) t += red[w];
+        pm_part[blockIdx.x] = t;
+    }
+}
+
+// out[0] = sum of part[0 .. n) in a fixed order (one block).
+static __global__ __launch_bounds__(kT) void k_sum_parts(const double *__restrict__ part, long n,
+                                                  double *__restrict__ out) {
+    double t = 0;
+    for (long b = threadIdx.x; b < n; b += kT) t += part[b];
+    __shared__ double red[kT];
+    red[threadIdx.x] = t;
+    __syncthreads();
+    for (int st = kT / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
 }
 
 // ---- transformed-coordinate step (trans_sim_3d) ---------------------------

@@ -190,6 +190,55 @@ class ShearSim:
                                      _lib.dptr(gp)))
         return J.value, g, gp
 
+    # ---- output in the reference's formats (include/stz.h) -------------
+
+    def cube_size(self, mode):
+        n = self.L.stz_cube_size(self.h, int(mode))
+        if n < 0:
+            raise StepError(-1, f"unknown cube mode {mode}")
+        return n
+
+    def output_cube(self, mode):
+        """output_cube's values for write_files mode `mode` (0-12): float32, x
+        slowest, then y, z fastest (src/shear_sim_3d.cpp:1963-2161)."""
+        n = self.L.stz_cube_size(self.h, int(mode))
+        if n < 0:
+            self._chk(self.L.stz_output_cube(self.h, int(mode), None, 0))
+        out = np.zeros(n, dtype=np.float32)
+        self._chk(self.L.stz_output_cube(self.h, int(mode), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n))
+        return out
+
+    def write_cube(self, out_dir, mode, frame):
+        """Append the cube to <out_dir>/<name>_grid.<frame>."""
+        self._chk(self.L.stz_write_cube(self.h, str(out_dir).encode(), int(mode), int(frame)))
+
+    def write_files(self, frame, out_dir, n_tp, fflags=4096):
+        """write_files(frame) (src/shear_sim_3d.cpp:1938-1960) into out_dir;
+        the reference's build uses fflags = 4096 (tem)."""
+        self._chk(self.L.stz_write_files(self.h, str(out_dir).encode(), int(frame), int(n_tp), int(fflags)))
+
+    def compute_net_force(self):
+        """Wall tractions (ts13, ts23, ts33, bs13, bs23, bs33) of
+        compute_net_force (src/shear_sim_3d.cpp:1061-1126)."""
+        out = np.zeros(6)
+        self._chk(self.L.stz_net_force(self.h, _lib.dptr(out)))
+        return out
+
+    def write_traction(self, out_dir):
+        """Append the traction.dat line (and the trac_grid.bin record)."""
+        self._chk(self.L.stz_write_traction(self.h, str(out_dir).encode()))
+
+    @property
+    def proj_mag(self):
+        """Sum of modsq of the last projection's stress increments."""
+        return self.L.stz_proj_mag(self.h)
+
+    def set_output(self, out_dir, traction=True, proj_mag=True):
+        """Per-step traction.dat / proj_mag.dat in out_dir during
+        step_forward_qs, as the reference's solve_qs writes them."""
+        flags = (1 if traction else 0) | (2 if proj_mag else 0)
+        self._chk(self.L.stz_set_output(self.h, str(out_dir).encode() if flags else None, flags))
+
     def solve_qs(self, steps, dt, **kw):
         """The QS time loop (solve_qs, src/shear_sim_3d.cpp:336-377) without
         file output: ``steps`` steps of size dt."""

@@ -250,6 +250,72 @@ def grf_case(name):
     np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
 
 
+CUBE_NAMES = ["u", "v", "w", "s11", "s12", "s13", "s22", "s23", "s33", "p", "dev", "chi", "tem"]
+
+
+def output_case(name):
+    """The reference's output files for a given state.
+
+    output_walled / output_periodic (shear_sim_3d, 16 x 12 x 8 cells, z walled
+      or periodic): the cubes of write_files' modes 0-12 at frame 3
+      (output_cube), traction.dat and trac_grid.bin from compute_net_force at
+      the set state; walled only: then two QS steps of dt = 0.01 each
+      followed by compute_net_force (two more traction lines; proj_mag.dat
+      from the projections) and the state after the steps.
+    output_trans (trans_sim_3d, pure shear Ut = 0.5, 16 x 12 x 8, z walled):
+      traction.dat from compute_net_force at t = 0.3 (physical stress).
+    """
+    import tempfile
+
+    from stzpp_amd import shear_sim as SS
+
+    N = (16, 12, 8)
+    prd = (1, 1, 1) if name == "output_periodic" else (1, 1, 0)
+    Nx, Ny, Nz = N
+    nc = Nx * Ny * Nz
+    nv = Nx * Ny * (Nz + (0 if prd[2] else 1))
+    p = SS.config0_params(16)
+    p["eps0"] = 0.3
+    rng = np.random.default_rng({"output_walled": 21, "output_periodic": 22, "output_trans": 23}[name])
+    sig0 = 0.3 * rng.standard_normal((nc, 6))
+    sig0[:, 1] += 0.9
+    u0 = 1e-3 * rng.standard_normal((nv, 3))
+    chi0 = 0.09 + 0.01 * rng.standard_normal(nc)
+    t0 = 0.3 if name == "output_trans" else 0.25
+    d = tempfile.mkdtemp()
+    out = dict(N=np.array(N), prd=np.array(prd), t0=np.array(t0), u0=u0, sig0=sig0, chi0=chi0,
+               **{f"p_{k}": np.array(v) for k, v in p.items() if k in O.STEP_PARAM_ORDER})
+    if name == "output_trans":
+        Ut = 0.5
+        ref = O.RefTrans(N, {k: p[k] for k in O.STEP_PARAM_ORDER}, Ut, d, prd=prd)
+        ref.setup(0.01)
+        ref.set_state(t0, u0, sig0, chi0)
+        ref.net_force()
+        out["Ut"] = np.array(Ut)
+        out["traction_txt"] = np.array(open(os.path.join(d, "traction.dat")).read())
+    else:
+        ref = O.RefShear(N, {k: p[k] for k in O.STEP_PARAM_ORDER}, d, prd=prd)
+        ref.setup(0.01)
+        ref.set_state(t0, u0, sig0, chi0)
+        for mode, nm in enumerate(CUBE_NAMES):
+            ref.output_cube(nm, mode, 3)
+            out[f"cube_{nm}"] = np.fromfile(os.path.join(d, f"{nm}_grid.3"), dtype=np.uint8)
+        ref.net_force()
+        if not prd[2]:
+            # (the reference's uncapped loop does not end on the singular
+            # periodic-z projection, so only the walled case steps)
+            for _ in range(2):
+                ref.step(0.01, 1)
+                ref.net_force()
+            s = ref.state()
+            out.update(u1=s["u"], sig1=s["sig"], chi1=s["chi"], dpl1=s["dpl"], t1=np.array(s["t"]))
+            out["proj_mag_txt"] = np.array(open(os.path.join(d, "proj_mag.dat")).read())
+        out["traction_txt"] = np.array(open(os.path.join(d, "traction.dat")).read())
+        out["trac_bin"] = np.fromfile(os.path.join(d, "trac_grid.bin"), dtype=np.uint8)
+    ref.close()
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+
+
 CASES = {
     "shear_16x16x17": (("shear_16x16x17", (16, 16, 17), (1, 1, 0), "shear", 11),),
     "trans_16x16x16": (("trans_16x16x16", (16, 16, 16), (1, 1, 1), "trans", 12),),
@@ -269,11 +335,13 @@ if __name__ == "__main__":
             trans_step_case(which)
         elif which.startswith("grf_"):
             grf_case(which)
+        elif which.startswith("output_"):
+            output_case(which)
         else:
             mg_case(*CASES[which][0])
         sys.exit(0)
     for key in ["trans", *CASES, "step_plastic", "step_c0", "trans_step_walled", "trans_step_periodic",
-                *GRF_CASES]:
+                *GRF_CASES, "output_walled", "output_periodic", "output_trans"]:
         subprocess.run([sys.executable, __file__, key], check=True, cwd=ROOT,
                        stdout=subprocess.DEVNULL)
         print("wrote", key)
