@@ -356,6 +356,11 @@ def ref_lib():
             ("ref_grf", [ctypes.c_int] * 3 + [_ip, _dp, ctypes.c_int, ctypes.c_char_p, _dp, ctypes.c_long, _dp,
                          _dp], ctypes.c_int),
             ("ref_shear_output_cube", [vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int], None),
+            ("ref_shear_create_direct", [ctypes.c_int] * 3 + [_dp, _dp, ctypes.c_double, ctypes.c_double, _ip,
+                                                              ctypes.c_char_p], vp),
+            ("ref_shear_init_ref", [vp], None),
+            ("ref_shear_step_direct", [vp, ctypes.c_double, ctypes.c_int], None),
+            ("ref_shear_get_ref", [vp, _dp], None),
             ("ref_shear_net_force", [vp], None),
             ("ref_trans_output_cube", [vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int], None),
             ("ref_trans_net_force", [vp], None),
@@ -477,6 +482,10 @@ def _step_lib():
         L.sto_set_eta.argtypes = [vp, ctypes.c_double]
         L.sto_proj_mag.argtypes = [vp]
         L.sto_proj_mag.restype = ctypes.c_double
+        L.sto_set_direct.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+        L.sto_init_ref.argtypes = [vp]
+        L.sto_step_direct.argtypes = [vp, ctypes.c_double, ctypes.c_int]
+        L.sto_get_ref.argtypes = [vp, _dp]
         L.sto_set_textbook.argtypes = [vp, ctypes.c_int]
         L.sto_trans_stencil.argtypes = [_dp] + [ctypes.c_double] * 5 + [_dp]
         L.sto_transform_at.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -562,6 +571,23 @@ class OracleShear:
         """Sum of sym_mat3::modsq of the last projection's stress increments."""
         return self.L.sto_proj_mag(self.h)
 
+    # direct (explicit dynamics) step, src/shear_sim_3d.cpp:166-213, :774-1030
+    def set_direct(self, rho, kappa):
+        self.L.sto_set_direct(self.h, float(rho), float(kappa))
+
+    def init_ref(self):
+        """init_ref: the reference map at the grid positions."""
+        self.L.sto_init_ref(self.h)
+
+    def step_direct(self, dt, n=1):
+        self.L.sto_step_direct(self.h, float(dt), int(n))
+
+    def ref_map(self):
+        Nx, Ny, Nz = self.N
+        xi = np.zeros((Nx * Ny * _nzv(self.N, self.prd), 3))
+        self.L.sto_get_ref(self.h, _d(xi))
+        return xi
+
     def set_eta(self, eta):
         """Override the adaptive-step tolerance eta of the plastic update."""
         self.L.sto_set_eta(self.h, eta)
@@ -593,6 +619,37 @@ class RefShear:
             self.h = None
 
     __del__ = close
+
+    @classmethod
+    def direct(cls, N, params, outdir, rho, kappa, box=(-1, 1, -1, 1, -0.5, 0.5), prd=(1, 1, 0)):
+        """A reference shear_sim_3d for the direct step (density rho,
+        viscous damping kappa)."""
+        self = cls.__new__(cls)
+        self.L = ref_lib()
+        if self.L is None:
+            raise RuntimeError("reference library not built")
+        self.N, self.prd = tuple(N), tuple(prd)
+        p = np.array([params[k] for k in ("mu", "lamb", "sy", "tau0", "c0", "eps0", "delta", "omega",
+                                          "bath_temp", "chi_inf", "ez")] + [21000.0, params["diff_l"],
+                                                                               params["U"]], dtype=np.float64)
+        b = np.array(box, dtype=np.float64)
+        pr = np.array(prd, dtype=np.int32)
+        self.h = self.L.ref_shear_create_direct(N[0], N[1], N[2], _d(b), _d(p), float(rho), float(kappa), _i(pr),
+                                                outdir.encode())
+        return self
+
+    def init_ref(self):
+        self.L.ref_shear_init_ref(self.h)
+
+    def step_direct(self, dt, n=1):
+        """n calls of the reference's step_forward(dt)."""
+        self.L.ref_shear_step_direct(self.h, float(dt), int(n))
+
+    def ref_map(self):
+        Nx, Ny, Nz = self.N
+        xi = np.zeros((Nx * Ny * _nzv(self.N, self.prd), 3))
+        self.L.ref_shear_get_ref(self.h, _d(xi))
+        return xi
 
     def output_cube(self, prefix, mode, sn):
         """The reference's output_cube into the simulation's directory."""

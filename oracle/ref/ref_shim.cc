@@ -231,6 +231,11 @@ class ref_shear : public shear_sim_3d<sym_mat3> {
     int gxy() const { return nxyg; }
     void boundary() { set_up_ghost_regions(); set_boundary_conditions(); }
     void net_force() { compute_net_force(); }
+    void ref_init() { init_ref(); }
+    double xi(int i, int j, int k, int a) const {
+        const Field &f = grido[i + nxg * j + nxyg * k];
+        return a == 0 ? f.xi_x : (a == 1 ? f.xi_y : f.xi_z);
+    }
     const sym_mat3 *stencil_ptr() const { return stencil; }
     const sym_mat3 *fix_ptr() const { return fix; }
 };
@@ -574,5 +579,50 @@ void ref_trans_output_cube(void *hp, const char *prefix, int mode, int sn) {
 }
 
 void ref_trans_net_force(void *hp) { ((trans_handle *)hp)->sim->net_force(); }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Direct (explicit dynamics) step of shear_sim_3d (step_forward,
+// src/shear_sim_3d.cpp:206-213) with density rho and viscous damping kappa.
+extern "C" {
+
+void *ref_shear_create_direct(int Nx, int Ny, int Nz, const double *box, const double *p, double rho, double kappa,
+                              const int *periods, const char *outdir) {
+    ensure_mpi();
+    shear_handle *h = new shear_handle;
+    h->dims[0] = h->dims[1] = h->dims[2] = 1;
+    for (int a = 0; a < 3; a++) h->periods[a] = periods[a];
+    MPI_Cart_create(MPI_COMM_WORLD, 3, h->dims, h->periods, 1, &h->cart);
+    h->sim = new ref_shear(Nx, Ny, Nz, box[0], box[2], box[4], box[1], box[3], box[5], 0.0, 1.0, 0.5, 1,
+                           std::string(outdir), p[0], p[1], rho, kappa, p[2], p[3], p[4], p[5], p[6], p[7], p[8],
+                           p[9], p[10], p[11], p[12], p[13], 5, 3, 0.09, 0.01, 1.0, &h->cart, h->dims, 15, 0.0, 0.0,
+                           h->periods);
+    return h;
+}
+
+void ref_shear_init_ref(void *hp) {
+    shear_handle *h = (shear_handle *)hp;
+    h->sim->ref_init();
+    h->sim->boundary();
+}
+
+void ref_shear_step_direct(void *hp, double dt, int nsteps) {
+    shear_handle *h = (shear_handle *)hp;
+    for (int q = 0; q < nsteps; q++) h->sim->step_forward(dt);
+}
+
+void ref_shear_get_ref(void *hp, double *xi) {
+    shear_handle *h = (shear_handle *)hp;
+    ref_shear *s = h->sim;
+    int Nx = s->nx(), Ny = s->ny(), Nz = s->nz();
+    int nzv = Nz + (h->periods[2] ? 0 : 1);
+    for (int k = 0; k < nzv; k++)
+        for (int j = 0; j < Ny; j++)
+            for (int i = 0; i < Nx; i++) {
+                long q = i + (long)Nx * (j + (long)Ny * k);
+                for (int a = 0; a < 3; a++) xi[3 * q + a] = s->xi(i, j, k, a);
+            }
+}
 
 }  // extern "C"

@@ -30,7 +30,8 @@
 #include "mg_oracle.h"
 
 enum { F_U, F_V, F_W, F_S11, F_S12, F_S13, F_S22, F_S23, F_S33, F_CHI,
-       F_CS11, F_CS12, F_CS13, F_CS22, F_CS23, F_CS33, F_CCHI, F_SBAR, F_DPL, F_NF };
+       F_CS11, F_CS12, F_CS13, F_CS22, F_CS23, F_CS33, F_CCHI, F_SBAR, F_DPL,
+       F_CU, F_CV, F_CW, F_XI, F_XIY, F_XIZ, F_CXI, F_CXIY, F_CXIZ, F_NF };
 
 typedef struct {
     int Nx, Ny, Nz, prd[3];
@@ -55,6 +56,7 @@ typedef struct {
     double ax, ay, az, bx, by, bz;
     double T[9], Ti[9], dT[9];
     int textbook;          /* test switch: 1 = textbook x-ENO for chi instead of :985 */
+    double rho, kappa;     /* density and viscous damping of the direct step */
 } sto_sim;
 
 static long gi(const sto_sim *S, int i, int j, int k) {
@@ -109,6 +111,13 @@ static void ghosts(sto_sim *S) {
                 if (!ghost || !ok) continue;
                 long d = gi(S, i, j, k), s = gi(S, src[0], src[1], src[2]);
                 for (int q = 0; q < F_NF; q++) S->f[q][d] = S->f[q][s];
+                /* the periodic image of the reference map is shifted by the
+                 * domain length (src/shear_sim_3d.cpp:1470-1500) */
+                const double Lb[3] = {S->bx - S->ax, S->by - S->ay, S->bz - S->az};
+                for (int a = 0; a < 3; a++) {
+                    if (c[a] >= N[a]) S->f[F_XI + a][d] += Lb[a];
+                    if (c[a] < 0) S->f[F_XI + a][d] -= Lb[a];
+                }
             }
 }
 
@@ -139,6 +148,12 @@ static void boundary(sto_sim *S) {
         for (int i = -2; i < S->Nx + 2; i++) {
             long t0 = gi(S, i, j, Nz), t1 = gi(S, i, j, Nz + 1), tm = gi(S, i, j, Nz - 1);
             S->f[F_U][t0] = ut; S->f[F_V][t0] = 0; S->f[F_W][t0] = 0;
+            if (!S->tfam) {
+                /* the top plate's reference map (xi_z keeps the zero of the
+                 * extrapolated Field) */
+                S->f[F_XI][t0] = S->ax + i * S->dx + (S->t < 1 ? S->U * S->t * S->t * .5 : S->U * (S->t - 0.5));
+                S->f[F_XIY][t0] = S->ay + j * S->dy;
+            }
             for (int q = F_U; q <= F_W; q++) S->f[q][t1] = 2 * S->f[q][t0] - S->f[q][tm];
             /* The shear step's bottom plate: set_boundary_conditions tests
              * `pz == max_pz ... else if (pz == min_pz)`
@@ -843,3 +858,213 @@ void sto_set_eta(void *hp, double eta) { ((sto_sim *)hp)->eta = eta; }
  * f(+2), src/trans_sim_3d.cpp:985) by the textbook me[2], so that a test can
  * show the reference fixtures tell the two apart. */
 void sto_set_textbook(void *hp, int on) { ((sto_sim *)hp)->textbook = on; }
+
+/* ---- direct (explicit dynamics) step of shear_sim_3d -------------------
+ *
+ *   solve_direct / step_forward   src/shear_sim_3d.cpp:166-213
+ *   calc_updates                  :89-104 (cells in place in lexicographic
+ *                                 order; the vertex update skips the plane
+ *                                 zz = 0 whenever pcoords[0] == 0, i.e.
+ *                                 always with one rank)
+ *   step_stress                   :774-896 (ad_Dpl without the factor dt)
+ *   step_chi                      :1193-1215 (dt chi_diffusion / c0)
+ *   step_velocity                 :919-968, first-order stresses :971-987,
+ *                                 Laplacian :990-994
+ *   step_ref                      :997-1030
+ *   merge_updates / Field::update :154-162, src/field.hh:31-47
+ */
+void sto_set_direct(void *hp, double rho, double kappa) {
+    sto_sim *S = (sto_sim *)hp;
+    S->rho = rho;
+    S->kappa = kappa;
+}
+
+/* init_ref (src/shear_sim_3d.cpp:1036-1057): the reference map at the
+ * grid positions of the interior points. */
+void sto_init_ref(void *hp) {
+    sto_sim *S = (sto_sim *)hp;
+    for (int k = 0; k < S->Nz; k++)
+        for (int j = 0; j < S->Ny; j++)
+            for (int i = 0; i < S->Nx; i++) {
+                long c = gi(S, i, j, k);
+                S->f[F_XI][c] = S->ax + S->dx * i;
+                S->f[F_XIY][c] = S->ay + S->dy * j;
+                S->f[F_XIZ][c] = S->az + S->dz * k;
+            }
+    refresh(S);
+}
+
+static void direct_stress(sto_sim *S, long c, double dt) {
+    long X = 1, Y = S->nxg, Z = S->nxyg;
+    double dtlamb = dt * S->lamb, twomudt = 2 * S->mu * dt, dxdt = S->dx / dt, dydt = S->dy / dt,
+           dzdt = S->dz / dt, dtmu = dt * S->mu;
+    double vel[3];
+    for (int q = 0; q < 3; q++) {
+        const double *u = S->f[F_U + q] + c;
+        vel[q] = 0.125 * (u[0] + u[X] + u[Z] + u[X + Z] + u[Y] + u[X + Y] + u[Y + Z] + u[X + Y + Z]);
+    }
+    double g[3][3];
+    vel_grad(S, c, g);
+    double du_dx = g[0][0], du_dy = g[0][1], du_dz = g[0][2];
+    double dv_dx = g[1][0], dv_dy = g[1][1], dv_dz = g[1][2];
+    double dw_dx = g[2][0], dw_dy = g[2][1], dw_dz = g[2][2];
+    double d_trace = (du_dx + dv_dy + dw_dz);
+    double ug[6];
+    for (int q = 0; q < 6; q++) {
+        const double *f = S->f[F_S11 + q] + c;
+        double dx_ = eno2(dxdt, vel[0], f[2 * X], f[X], f[0], f[-X], f[-2 * X]);
+        double dy_ = eno2(dydt, vel[1], f[2 * Y], f[Y], f[0], f[-Y], f[-2 * Y]);
+        double dz_ = eno2(dzdt, vel[2], f[2 * Z], f[Z], f[0], f[-Z], f[-2 * Z]);
+        ug[q] = vel[0] * dx_ + vel[1] * dy_ + vel[2] * dz_;
+    }
+    double s11 = S->f[F_S11][c], s12 = S->f[F_S12][c], s13 = S->f[F_S13][c];
+    double s22 = S->f[F_S22][c], s23 = S->f[F_S23][c], s33 = S->f[F_S33][c];
+    double t11 = s11 * (du_dx - dv_dy - dw_dz) + 2 * s13 * du_dz + 2 * s12 * du_dy;
+    double t12 = s23 * du_dz + s13 * dv_dz - s12 * dw_dz + s22 * du_dy + s11 * dv_dx;
+    double t13 = s33 * du_dz + s11 * dw_dx + s23 * du_dy + s12 * dw_dy - s13 * dv_dy;
+    double t22 = 2 * s23 * dv_dz + s22 * (dv_dy - du_dx - dw_dz) + 2 * s12 * dv_dx;
+    double t23 = s33 * dv_dz + s22 * dw_dy + s13 * dv_dx - s23 * du_dx + s12 * dw_dx;
+    double t33 = s33 * (dw_dz - dv_dy - du_dx) + 2 * s23 * dw_dy + 2 * s13 * dw_dx;
+    double cs[6];
+    cs[0] = dtlamb * d_trace + twomudt * du_dx - ug[0] + dt * t11;
+    cs[3] = dtlamb * d_trace + twomudt * dv_dy - ug[3] + dt * t22;
+    cs[5] = dtlamb * d_trace + twomudt * dw_dz - ug[5] + dt * t33;
+    cs[1] = dtmu * (du_dy + dv_dx) - ug[1] + dt * t12;
+    cs[2] = dtmu * (du_dz + dw_dx) - ug[2] + dt * t13;
+    cs[4] = dtmu * (dv_dz + dw_dy) - ug[4] + dt * t23;
+    double sig_trace = s11 + s22 + s33;
+    double sb = dev(S, c);
+    S->f[F_SBAR][c] = sb;
+    if (sb > S->sy) {
+        double adapt = adaptive(S, c, dt, sb);
+        S->f[F_DPL][c] = adapt / 2. / S->mu / dt * sb;
+        double third = 1. / 3.;
+        cs[0] -= adapt * (s11 - third * sig_trace);
+        cs[1] -= adapt * (s12);
+        cs[2] -= adapt * (s13);
+        cs[3] -= adapt * (s22 - third * sig_trace);
+        cs[4] -= adapt * (s23);
+        cs[5] -= adapt * (s33 - third * sig_trace);
+    } else {
+        S->f[F_DPL][c] = 0;
+    }
+    for (int q = 0; q < 6; q++) S->f[F_CS11 + q][c] = cs[q];
+}
+
+static void direct_chi(sto_sim *S, long c, double dt) {
+    long X = 1, Y = S->nxg, Z = S->nxyg;
+    double dxdt = S->dx / dt, dydt = S->dy / dt, dzdt = S->dz / dt;
+    double vel[3];
+    for (int q = 0; q < 3; q++) {
+        const double *u = S->f[F_U + q] + c;
+        vel[q] = 0.125 * (u[0] + u[X] + u[Z] + u[X + Z] + u[Y] + u[X + Y] + u[Y + Z] + u[X + Y + Z]);
+    }
+    const double *x = S->f[F_CHI] + c;
+    double dchi_dx = eno2(dxdt, vel[0], x[2 * X], x[X], x[0], x[-X], x[-2 * X]);
+    double dchi_dy = eno2(dydt, vel[1], x[2 * Y], x[Y], x[0], x[-Y], x[-2 * Y]);
+    double dchi_dz = eno2(dzdt, vel[2], x[2 * Z], x[Z], x[0], x[-Z], x[-2 * Z]);
+    S->f[F_CCHI][c] -= vel[0] * dchi_dx + vel[1] * dchi_dy + vel[2] * dchi_dz;
+    S->f[F_CCHI][c] += dt * chi_diffusion(S, c) / S->c0;
+}
+
+static void direct_ref(sto_sim *S, long c, double dt) {
+    long X = 1, Y = S->nxg, Z = S->nxyg;
+    double u = S->f[F_U][c], v = S->f[F_V][c], w = S->f[F_W][c];
+    double dxdt = S->dx / dt, dydt = S->dy / dt, dzdt = S->dz / dt;
+    for (int a = 0; a < 3; a++) {
+        const double *x = S->f[F_XI + a] + c;
+        double ddx = eno2(dxdt, u, x[2 * X], x[X], x[0], x[-X], x[-2 * X]);
+        double ddy = eno2(dydt, v, x[2 * Y], x[Y], x[0], x[-Y], x[-2 * Y]);
+        double ddz = eno2(dzdt, w, x[2 * Z], x[Z], x[0], x[-Z], x[-2 * Z]);
+        S->f[F_CXI + a][c] -= (u * ddx + v * ddy + w * ddz);
+    }
+}
+
+static void direct_velocity(sto_sim *S, long c, double dt) {
+    long X = 1, Y = S->nxg, Z = S->nxyg;
+    double dx_inv = 1. / S->dx, dy_inv = 1. / S->dy, dz_inv = 1. / S->dz;
+    double dx_inv_sq = dx_inv * dx_inv, dy_inv_sq = dy_inv * dy_inv, dz_inv_sq = dz_inv * dz_inv;
+    double rho_inv = 1. / S->rho;
+    double dtrho_inv = dt * rho_inv, dtkaprho_inv = S->kappa * dtrho_inv, dxdt = S->dx / dt, dydt = S->dy / dt,
+           dzdt = S->dz / dt;
+    /* backward staggered stress derivatives (calculate_first_order_stresses) */
+#define SD(fld, A, B, C) (.25 * (A) * (S->f[fld][c] - S->f[fld][c - (B)] + S->f[fld][c - (C)] - S->f[fld][c - (B) - (C)] + \
+                         S->f[fld][c - (D_)] - S->f[fld][c - (B) - (D_)] + S->f[fld][c - (C) - (D_)] -            \
+                         S->f[fld][c - (B) - (C) - (D_)]))
+    long D_;
+    D_ = Z;
+    double d_s11_dx = SD(F_S11, dx_inv, X, Y), d_s12_dx = SD(F_S12, dx_inv, X, Y), d_s13_dx = SD(F_S13, dx_inv, X, Y);
+    D_ = Z;
+    double d_s12_dy = SD(F_S12, dy_inv, Y, X), d_s22_dy = SD(F_S22, dy_inv, Y, X), d_s23_dy = SD(F_S23, dy_inv, Y, X);
+    D_ = Y;
+    double d_s13_dz = SD(F_S13, dz_inv, Z, X), d_s23_dz = SD(F_S23, dz_inv, Z, X), d_s33_dz = SD(F_S33, dz_inv, Z, X);
+#undef SD
+    double gsq[3];
+    for (int q = 0; q < 3; q++) {
+        const double *u = S->f[F_U + q] + c;
+        gsq[q] = dx_inv_sq * (u[X] - 2 * u[0] + u[-X]) + dy_inv_sq * (u[Y] - 2 * u[0] + u[-Y]) +
+                 dz_inv_sq * (u[Z] - 2 * u[0] + u[-Z]);
+    }
+    double u = S->f[F_U][c], v = S->f[F_V][c], w = S->f[F_W][c];
+    double ug[3];
+    for (int q = 0; q < 3; q++) {
+        const double *f = S->f[F_U + q] + c;
+        double ddx = eno2(dxdt, u, f[2 * X], f[X], f[0], f[-X], f[-2 * X]);
+        double ddy = eno2(dydt, v, f[2 * Y], f[Y], f[0], f[-Y], f[-2 * Y]);
+        double ddz = eno2(dzdt, w, f[2 * Z], f[Z], f[0], f[-Z], f[-2 * Z]);
+        ug[q] = u * ddx + v * ddy + w * ddz;
+    }
+    S->f[F_CU][c] = dtrho_inv * (d_s11_dx + d_s12_dy + d_s13_dz) + dtkaprho_inv * gsq[0] - ug[0];
+    S->f[F_CV][c] = dtrho_inv * (d_s12_dx + d_s22_dy + d_s23_dz) + dtkaprho_inv * gsq[1] - ug[1];
+    S->f[F_CW][c] = dtrho_inv * (d_s13_dx + d_s23_dy + d_s33_dz) + dtkaprho_inv * gsq[2] - ug[2];
+}
+
+void sto_step_direct(void *hp, double dt, int nsteps) {
+    sto_sim *S = (sto_sim *)hp;
+    for (int s = 0; s < nsteps; s++) {
+        for (int k = 0; k < S->Nz; k++)
+            for (int j = 0; j < S->Ny; j++)
+                for (int i = 0; i < S->Nx; i++) {
+                    long c = gi(S, i, j, k);
+                    direct_stress(S, c, dt);
+                    direct_chi(S, c, dt);
+                    if (k != 0) {
+                        direct_ref(S, c, dt);
+                        direct_velocity(S, c, dt);
+                    }
+                }
+        /* merge_updates: Field::update on every interior point */
+        for (int k = 0; k < S->Nz; k++)
+            for (int j = 0; j < S->Ny; j++)
+                for (int i = 0; i < S->Nx; i++) {
+                    long c = gi(S, i, j, k);
+                    for (int q = 0; q < 3; q++) {
+                        S->f[F_U + q][c] += S->f[F_CU + q][c];
+                        S->f[F_CU + q][c] = 0;
+                        S->f[F_XI + q][c] += S->f[F_CXI + q][c];
+                        S->f[F_CXI + q][c] = 0;
+                    }
+                    for (int q = 0; q < 6; q++) {
+                        S->f[F_S11 + q][c] += S->f[F_CS11 + q][c];
+                        S->f[F_CS11 + q][c] = 0;
+                    }
+                    S->f[F_CHI][c] += S->f[F_CCHI][c];
+                    S->f[F_CCHI][c] = 0;
+                }
+        S->t += dt;
+        refresh(S);
+    }
+}
+
+/* The reference map on the vertex grid (Nz + 1 planes when z is walled),
+ * 3 doubles per vertex. */
+void sto_get_ref(void *hp, double *xi) {
+    sto_sim *S = (sto_sim *)hp;
+    int o = mg_o(S);
+    for (int k = 0; k < o; k++)
+        for (int j = 0; j < S->Ny; j++)
+            for (int i = 0; i < S->Nx; i++) {
+                long q = i + (long)S->Nx * (j + (long)S->Ny * k), c = gi(S, i, j, k);
+                for (int a = 0; a < 3; a++) xi[3 * q + a] = S->f[F_XI + a][c];
+            }
+}

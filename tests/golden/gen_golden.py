@@ -316,6 +316,55 @@ def output_case(name):
     np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
 
 
+def direct_case(name):
+    """The reference's direct (explicit dynamics) step (step_forward,
+    src/shear_sim_3d.cpp:206-213) from a set state with the reference map
+    initialised at the grid positions (init_ref): 16 x 12 x 8 cells, density
+    1, viscous damping 0.35 (the driver's values), config-0 material with
+    eps0 = 0.3, the reference's time step min(tmult dx^2 / (6 kappa), dx / 4)
+    with tmult = 0.5, and 5 steps; the state after step 1 and after step 5.
+    direct_walled: z walled, a pre-stress above yield and velocity noise;
+    direct_periodic: z periodic, milder noise (the reference's own run of
+    the walled case's noise level blows up when z is periodic)."""
+    import tempfile
+
+    from stzpp_amd import shear_sim as SS
+
+    N = (16, 12, 8)
+    prd = (1, 1, 1) if name == "direct_periodic" else (1, 1, 0)
+    p = SS.config0_params(16)
+    p["eps0"] = 0.3
+    P = {k: p[k] for k in O.STEP_PARAM_ORDER}
+    rho, kappa, tmult = 1.0, 0.35, 0.5
+    dx = 2.0 / N[0]
+    dt = min(tmult * (dx * dx / 3.0 / kappa / 2.0), dx / 4.0)
+    nc = N[0] * N[1] * N[2]
+    nv = N[0] * N[1] * (N[2] + (0 if prd[2] else 1))
+    rng = np.random.default_rng(31 if prd[2] else 30)
+    amp_s, amp_u = (0.05, 2e-3) if prd[2] else (0.3, 1e-2)
+    sig0 = amp_s * rng.standard_normal((nc, 6))
+    sig0[:, 1] += 0.9
+    u0 = amp_u * rng.standard_normal((nv, 3))
+    chi0 = 0.09 + 0.01 * rng.standard_normal(nc)
+    t0 = 0.3
+    ref = O.RefShear.direct(N, P, tempfile.mkdtemp(), rho, kappa, prd=prd)
+    ref.set_state(t0, u0, sig0, chi0)
+    ref.init_ref()
+    out = dict(N=np.array(N), prd=np.array(prd), t0=np.array(t0), dt=np.array(dt), rho=np.array(rho),
+               kappa=np.array(kappa), tmult=np.array(tmult), nsteps=np.array(5), u0=u0, sig0=sig0, chi0=chi0,
+               xi0=ref.ref_map(), **{f"p_{k}": np.array(v) for k, v in P.items()})
+    for s in range(5):
+        ref.step_direct(dt, 1)
+        if s in (0, 4):
+            st = ref.state()
+            tag = "1" if s == 0 else "5"
+            out.update({f"u{tag}": st["u"], f"sig{tag}": st["sig"], f"chi{tag}": st["chi"], f"dpl{tag}": st["dpl"],
+                        f"xi{tag}": ref.ref_map(), f"t{tag}": np.array(st["t"])})
+    assert all(np.isfinite(out[k]).all() for k in ("u5", "sig5", "chi5", "xi5"))
+    ref.close()
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+
+
 CASES = {
     "shear_16x16x17": (("shear_16x16x17", (16, 16, 17), (1, 1, 0), "shear", 11),),
     "trans_16x16x16": (("trans_16x16x16", (16, 16, 16), (1, 1, 1), "trans", 12),),
@@ -337,11 +386,14 @@ if __name__ == "__main__":
             grf_case(which)
         elif which.startswith("output_"):
             output_case(which)
+        elif which.startswith("direct_"):
+            direct_case(which)
         else:
             mg_case(*CASES[which][0])
         sys.exit(0)
     for key in ["trans", *CASES, "step_plastic", "step_c0", "trans_step_walled", "trans_step_periodic",
-                *GRF_CASES, "output_walled", "output_periodic", "output_trans"]:
+                *GRF_CASES, "output_walled", "output_periodic", "output_trans", "direct_walled",
+                "direct_periodic"]:
         subprocess.run([sys.executable, __file__, key], check=True, cwd=ROOT,
                        stdout=subprocess.DEVNULL)
         print("wrote", key)
