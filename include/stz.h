@@ -42,6 +42,10 @@ typedef struct {
                               plates; otherwise the transformed-coordinate step of
                               trans_sim_3d with this transform family and walls at rest */
     double Ut;             /* rate of the transform family */
+    double rho;            /* density; direct step only (shear_sim driver: rho = mu) */
+    double kappa;          /* viscous damping of the direct step (driver: 0.35; the
+                              quasi-static driver passes 0, src/shear_sim.cpp:122) */
+    double tmult;          /* direct time-step multiplier (src/shear_sim.cpp:53) */
 } stz_params;
 
 /* Transform families of trans_sim_3d (src/trans_sim_3d.hh:57-125): pure shear
@@ -158,11 +162,12 @@ int stz_set_random_initial_conditions(stz_t s, const stz_grf_params *g, const do
  *
  * Cube modes are those of Field::fval (src/field.hh:158-178) and write_files
  * (src/shear_sim_3d.cpp:1938-1960): 0-2 u v w, 3-8 s11 s12 s13 s22 s23 s33,
- * 9 p, 10 dev, 11 chi, 12 tem (chi * 21000).  The reference map (13-15) is
- * not part of the quasi-static state and is rejected.  A cube is float32,
- * x slowest, then y, z fastest (output_cube, :1963-2161): Nx*Ny*(Nz+1)
- * values for the velocity (the top plate plane; the periodic image of plane
- * 0 when z is periodic), Nx*Ny*Nz otherwise. */
+ * 9 p, 10 dev, 11 chi, 12 tem (chi * 21000), 13-15 xi_x xi_y xi_z (the
+ * reference map, kept by the direct step only; rejected without one).  A
+ * cube is float32, x slowest, then y, z fastest (output_cube, :1963-2161):
+ * Nx*Ny*(Nz+1) values for the velocity and the reference map (the top plate
+ * plane; the periodic image of plane 0 when z is periodic, xi_z shifted by
+ * bz - az), Nx*Ny*Nz otherwise. */
 long long stz_cube_size(stz_t s, int mode);   /* -1 for an unknown mode */
 int stz_output_cube(stz_t s, int mode, float *out, long long n);
 /* Appends the cube to <dir>/<name>_grid.<frame> (name: u v w s11 ... tem),
@@ -194,6 +199,37 @@ double stz_proj_mag(stz_t s);
  * handle truncates the file).  flags = 0 turns it off. */
 enum { STZ_OUT_TRACTION = 1, STZ_OUT_PROJ_MAG = 2 };
 int stz_set_output(stz_t s, const char *dir, int flags);
+
+/* ---- direct (explicit dynamics) step -------------------------------------
+ *
+ * shear_sim_3d::step_forward (src/shear_sim_3d.cpp:206-213) of the
+ * untransformed simulation: forward Euler of the hypo-elastoplastic
+ * equations with density rho and viscous damping kappa (stz_params).  Per
+ * step every cell takes the elastic, Truesdell, ENO2-advective and STZ
+ * plastic stress change and the advected, plastic and diffused chi change
+ * (step_stress, step_chi, :774-896, :1193-1215); every vertex 0 < k < Nz
+ * takes dt/rho div sigma + dt kappa/rho Lap u - u.grad u (step_velocity,
+ * :919-994) and the advected reference map (step_ref, :997-1030); then the
+ * top plate moves to U t (U for t >= 1) with reference map
+ * (a_x + i dx + U t^2/2 or + U (t - 1/2), a_y + j dy, 0).  As in the
+ * reference with one rank, the vertex plane k = 0 is never updated (the
+ * skip of zz = 0, :101, also with z periodic) and the bottom plate is not
+ * set (:2393, :2468).  Beyond a wall the reference map reads zero (an
+ * extrapolated Field carries only its ten primary members,
+ * src/field.hh:49-112); periodic images of it are shifted by the domain
+ * length.  The reference map is allocated by the first call of
+ * stz_init_ref, stz_set_ref_map or stz_step_direct (zero when not set). */
+/* dt = min(tmult dx^2 / (6 kappa), dx / 4) (solve_direct, :166-172). */
+double stz_direct_dt(stz_t s);
+/* init_ref (:1036-1057): the reference map at the vertex positions below
+ * the top plate, and the plate's map at the current time. */
+int stz_init_ref(stz_t s);
+/* Reference map in/out: 3 doubles per vertex, Nx*Ny*(Nz+1) vertices
+ * (Nx*Ny*Nz when z is periodic), x fastest. */
+int stz_set_ref_map(stz_t s, const double *xi);
+int stz_get_ref_map(stz_t s, double *xi);
+/* nsteps direct steps of size dt (stats: steps and yielded_cells). */
+int stz_step_direct(stz_t s, double dt, int nsteps, stz_step_stats *st);
 
 #ifdef __cplusplus
 }

@@ -55,7 +55,8 @@ class StzParams(ctypes.Structure):
         (k, ctypes.c_double) for k in (
             "ax", "bx", "ay", "by", "az", "bz", "mu", "lamb", "sy", "tau0", "c0", "eps0",
             "delta", "omega", "bath_temp", "chi_inf", "ez", "diff_l", "U", "eta")] + [
-        ("transform", ctypes.c_int), ("Ut", ctypes.c_double)]
+        ("transform", ctypes.c_int), ("Ut", ctypes.c_double), ("rho", ctypes.c_double),
+        ("kappa", ctypes.c_double), ("tmult", ctypes.c_double)]
 
 
 class StzAdjointOpts(ctypes.Structure):
@@ -106,6 +107,11 @@ SIGNATURES = {
     "stz_write_traction": ([_vp, ctypes.c_char_p], ctypes.c_int),
     "stz_proj_mag": ([_vp], ctypes.c_double),
     "stz_set_output": ([_vp, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
+    "stz_direct_dt": ([_vp], ctypes.c_double),
+    "stz_init_ref": ([_vp], ctypes.c_int),
+    "stz_set_ref_map": ([_vp, _dp], ctypes.c_int),
+    "stz_get_ref_map": ([_vp, _dp], ctypes.c_int),
+    "stz_step_direct": ([_vp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(StzStepStats)], ctypes.c_int),
     "stzmg_desc_default": ([ctypes.POINTER(StzmgDesc)] + [ctypes.c_int] * 6, ctypes.c_int),
     "stzmg_create": ([ctypes.POINTER(StzmgDesc), ctypes.POINTER(_vp)], ctypes.c_int),
     "stzmg_destroy": ([_vp], ctypes.c_int),

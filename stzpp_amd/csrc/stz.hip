@@ -170,6 +170,9 @@ int stz_params_default(stz_params *p, int Nx, int Ny, int Nz) {
     p->eta = 0;
     p->transform = STZ_T_NONE;
     p->Ut = 0;
+    p->rho = p->mu;
+    p->kappa = .35;
+    p->tmult = .5;
     return 0;
 }
 
@@ -262,7 +265,8 @@ int stz_create(const stz_params *p, stz_t *out) {
 int stz_destroy(stz_t s) {
     if (!s) return STZMG_ERR_ARG;
     if (s->stream) hipStreamSynchronize(s->stream);
-    for (double *q : {s->vel, s->sig, s->sig2, s->chi, s->chi2, s->dpl, s->dpl2, s->adapt, s->dchi, s->stage})
+    for (double *q : {s->vel, s->sig, s->sig2, s->chi, s->chi2, s->dpl, s->dpl2, s->adapt, s->dchi, s->stage, s->xi,
+                      s->xi2, s->vel2})
         hipFree(q);
     hipFree(s->counters);
     hipFree(s->pm_part);

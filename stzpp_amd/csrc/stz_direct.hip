@@ -1,0 +1,414 @@
+// Direct (explicit dynamics) step of shear_sim_3d (include/stz.h,
+// stz_step_direct): step_forward, src/shear_sim_3d.cpp:206-213, with
+// calc_updates :89-104, step_stress :774-896, step_chi :1193-1215,
+// step_velocity :919-994, step_ref :997-1030 and merge_updates :154-162.
+//
+// Per step, on the handle's stream:
+//   k_direct_plastic  STZ plasticity of every cell: adapt, the plastic chi
+//                     change and this step's ad_Dpl (adapt / 2 / mu / dt sbar)
+//   k_direct_cell     stress and chi: elastic, Truesdell, ENO2 advection and
+//                     plastic terms; chi advection and dt chi_diffusion / c0
+//                     with the reference's in-place ad_Dpl order (this step's
+//                     value at -x/-y/-z inside the domain, last step's at
+//                     +x/+y/+z and at periodic images, zero across a wall)
+//   k_direct_vertex   velocity (dt/rho div sigma + dt kappa/rho Lap u -
+//                     u.grad u) and reference map (- u.grad xi) at the
+//                     vertices 0 < k < Nz; the plane k = 0 is left as it is
+//                     (the reference skips zz = 0 when its x coordinate in
+//                     the rank grid is 0, :101, i.e. always with one rank)
+//   k_direct_plate    top plate velocity and reference map at t + dt
+//                     (set_boundary_conditions, :2393-2466; with one rank in
+//                     z the bottom branch is never taken)
+// Ghosts are computed where they are read: periodic images (the reference
+// map shifted by the domain length, :1470-1500), linear extrapolation of
+// velocity across the walls (the top plate's image from the plate value,
+// :2461-2463), and zero reference map beyond the walls (an extrapolated
+// Field carries only the ten primary members, src/field.hh:49-112).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <utility>
+
+#include "../../include/stz.h"
+
+// Every product and sum rounds on its own, as the reference's host code
+// does, so that the explicit step matches it bit for bit where no
+// transcendental function enters.
+#pragma clang fp contract(off)
+
+#include "stz_internal.hpp"
+
+using namespace stz;
+
+namespace {
+
+inline unsigned nbk(long n) { return (unsigned)((n + kT - 1) / kT); }
+
+struct Dyn {
+    double rho, kappa, U;
+    double ax, ay, az, Lx, Ly, Lz;
+};
+
+// Velocity component at vertex (i, j, k) with the wall ghosts of the
+// reference's grid: k = -1, -2 extrapolated from planes 0, 1; k = Nz + 1 from
+// the top plate and plane Nz - 1.
+__device__ __forceinline__ double vel_g(const double *__restrict__ u, const Grid &g, int i, int j, int k) {
+    if (g.pz || (k >= 0 && k <= g.Nz)) return vertv(u, g, i, j, k);
+    if (k < 0) {
+        const double a = vertv(u, g, i, j, 0), b = vertv(u, g, i, j, 1);
+        const double m1 = (a + a) - b;
+        return k == -1 ? m1 : (m1 + m1) - a;
+    }
+    const double a = vertv(u, g, i, j, g.Nz), b = vertv(u, g, i, j, g.Nz - 1);
+    return 2 * a - b;
+}
+
+// Reference-map component q at vertex (i, j, k): periodic images shifted by
+// the domain length, zero beyond a wall.
+__device__ __forceinline__ double xi_g(const double *__restrict__ xi, const Grid &g, const Dyn &D, int q, int i,
+                                       int j, int k) {
+    double off = 0;
+    if (q == 0) off = i >= g.Nx ? D.Lx : (i < 0 ? -D.Lx : 0.0);
+    if (q == 1) off = j >= g.Ny ? D.Ly : (j < 0 ? -D.Ly : 0.0);
+    if (g.pz) {
+        if (q == 2) off = k >= g.Nz ? D.Lz : (k < 0 ? -D.Lz : 0.0);
+    } else if (k < 0 || k > g.Nz) {
+        return 0.0;
+    }
+    const double v = vertv(xi + (long)q * g.Nv, g, i, j, k);
+    return off == 0.0 ? v : v + off;
+}
+
+static __global__ __launch_bounds__(kT) void k_direct_plastic(Grid g, Mat M, double dt,
+                                                              const double *__restrict__ sig,
+                                                              const double *__restrict__ chi,
+                                                              double *__restrict__ adapt, double *__restrict__ dchi,
+                                                              double *__restrict__ dpl_new,
+                                                              int *__restrict__ counters) {
+    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    if (c >= g.Nc) return;
+    double s[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) s[q] = sig[q * g.Nc + c];
+    const double sb = sbar_of(s);
+    double a = 0, dc = 0, D = 0;
+    if (sb > M.sy) {
+        int n;
+        a = adaptive(M, sb, chi[c], dt, dc, n);
+        D = a / 2. / M.mu / dt * sb;
+        atomicAdd(&counters[0], 1);
+    }
+    adapt[c] = a;
+    dchi[c] = dc;
+    dpl_new[c] = D;
+}
+
+static __global__ __launch_bounds__(kT) void k_direct_cell(Grid g, Mat M, double dt, const double *__restrict__ vel,
+                                                           const double *__restrict__ sig,
+                                                           const double *__restrict__ chi,
+                                                           const double *__restrict__ adapt,
+                                                           const double *__restrict__ dchi,
+                                                           const double *__restrict__ dnew,
+                                                           const double *__restrict__ dold,
+                                                           double *__restrict__ sig_out, double *__restrict__ chi_out) {
+    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    if (c >= g.Nc) return;
+    const int i = (int)(c % g.Nx);
+    const int j = (int)((c / g.Nx) % g.Ny);
+    const int k = (int)(c / ((long)g.Nx * g.Ny));
+    double G[3][3], v[3];
+    vel_grad(vel, g, i, j, k, G, v);
+    const double dtlamb = dt * M.lamb, twomudt = 2 * M.mu * dt, dtmu = dt * M.mu;
+    const double hx = g.dx / dt, hy = g.dy / dt, hz = g.dz / dt;
+    double ug[7];
+#pragma unroll
+    for (int q = 0; q < 7; q++) {
+        const double *f = q < 6 ? sig + q * g.Nc : chi;
+        const double f0 = f[c];
+        const double ddx = eno2(hx, v[0], cellv(f, g, i + 2, j, k), cellv(f, g, i + 1, j, k), f0,
+                                cellv(f, g, i - 1, j, k), cellv(f, g, i - 2, j, k));
+        const double ddy = eno2(hy, v[1], cellv(f, g, i, j + 2, k), cellv(f, g, i, j + 1, k), f0,
+                                cellv(f, g, i, j - 1, k), cellv(f, g, i, j - 2, k));
+        const double ddz = eno2(hz, v[2], cellv(f, g, i, j, k + 2), cellv(f, g, i, j, k + 1), f0,
+                                cellv(f, g, i, j, k - 1), cellv(f, g, i, j, k - 2));
+        ug[q] = v[0] * ddx + v[1] * ddy + v[2] * ddz;
+    }
+    const double s11 = sig[c], s12 = sig[g.Nc + c], s13 = sig[2 * g.Nc + c];
+    const double s22 = sig[3 * g.Nc + c], s23 = sig[4 * g.Nc + c], s33 = sig[5 * g.Nc + c];
+    const double du_dx = G[0][0], du_dy = G[0][1], du_dz = G[0][2];
+    const double dv_dx = G[1][0], dv_dy = G[1][1], dv_dz = G[1][2];
+    const double dw_dx = G[2][0], dw_dy = G[2][1], dw_dz = G[2][2];
+    const double d_trace = (du_dx + dv_dy + dw_dz);
+    const double t11 = s11 * (du_dx - dv_dy - dw_dz) + 2 * s13 * du_dz + 2 * s12 * du_dy;
+    const double t12 = s23 * du_dz + s13 * dv_dz - s12 * dw_dz + s22 * du_dy + s11 * dv_dx;
+    const double t13 = s33 * du_dz + s11 * dw_dx + s23 * du_dy + s12 * dw_dy - s13 * dv_dy;
+    const double t22 = 2 * s23 * dv_dz + s22 * (dv_dy - du_dx - dw_dz) + 2 * s12 * dv_dx;
+    const double t23 = s33 * dv_dz + s22 * dw_dy + s13 * dv_dx - s23 * du_dx + s12 * dw_dx;
+    const double t33 = s33 * (dw_dz - dv_dy - du_dx) + 2 * s23 * dw_dy + 2 * s13 * dw_dx;
+    double cs11 = dtlamb * d_trace + twomudt * du_dx - ug[0] + dt * t11;
+    double cs22 = dtlamb * d_trace + twomudt * dv_dy - ug[3] + dt * t22;
+    double cs33 = dtlamb * d_trace + twomudt * dw_dz - ug[5] + dt * t33;
+    double cs12 = dtmu * (du_dy + dv_dx) - ug[1] + dt * t12;
+    double cs13 = dtmu * (du_dz + dw_dx) - ug[2] + dt * t13;
+    double cs23 = dtmu * (dv_dz + dw_dy) - ug[4] + dt * t23;
+    const double a = adapt[c];
+    if (a != 0.0) {
+        const double sig_trace = s11 + s22 + s33, third = 1. / 3.;
+        cs11 -= a * (s11 - third * sig_trace);
+        cs12 -= a * (s12);
+        cs13 -= a * (s13);
+        cs22 -= a * (s22 - third * sig_trace);
+        cs23 -= a * (s23);
+        cs33 -= a * (s33 - third * sig_trace);
+    }
+    sig_out[c] = s11 + cs11;
+    sig_out[g.Nc + c] = s12 + cs12;
+    sig_out[2 * g.Nc + c] = s13 + cs13;
+    sig_out[3 * g.Nc + c] = s22 + cs22;
+    sig_out[4 * g.Nc + c] = s23 + cs23;
+    sig_out[5 * g.Nc + c] = s33 + cs33;
+    // chi diffusion (chi_diffusion, src/shear_sim_3d.cpp:1217-1237)
+    const double Dc = dnew[c];
+    const double Dxm = i > 0 ? dnew[c - 1] : dold[cidx(g, i - 1, j, k)];
+    const double Dxp = dold[cidx(g, i + 1, j, k)];
+    const double Dym = j > 0 ? dnew[c - g.Nx] : dold[cidx(g, i, j - 1, k)];
+    const double Dyp = dold[cidx(g, i, j + 1, k)];
+    double Dzm, Dzp;
+    if (g.pz) {
+        Dzm = k > 0 ? dnew[c - (long)g.Nx * g.Ny] : dold[cidx(g, i, j, g.Nz - 1)];
+        Dzp = dold[cidx(g, i, j, wrapc(k + 1, g.Nz))];
+    } else {
+        Dzm = k > 0 ? dnew[c - (long)g.Nx * g.Ny] : 0.0;
+        Dzp = k < g.Nz - 1 ? dold[c + (long)g.Nx * g.Ny] : 0.0;
+    }
+    const double x0 = chi[c];
+    const double xxm = cellv(chi, g, i - 1, j, k), xxp = cellv(chi, g, i + 1, j, k);
+    const double xym = cellv(chi, g, i, j - 1, k), xyp = cellv(chi, g, i, j + 1, k);
+    const double xzm = cellv(chi, g, i, j, k - 1), xzp = cellv(chi, g, i, j, k + 1);
+    const double ix = 1 / g.dx, iy = 1 / g.dy, iz = 1 / g.dz;
+    const double ddx = .5 * ix * (Dxp - Dxm) * (.5 * ix * (xxp - xxm)) + Dc * (ix * ix * (xxp - 2 * x0 + xxm));
+    const double ddy = .5 * iy * (Dyp - Dym) * (.5 * iy * (xyp - xym)) + Dc * (iy * iy * (xyp - 2 * x0 + xym));
+    const double ddz = .5 * iz * (Dzp - Dzm) * (.5 * iz * (xzp - xzm)) + Dc * (iz * iz * (xzp - 2 * x0 + xzm));
+    const double diff = M.diff_l * M.diff_l * (ddx + ddy + ddz);
+    double cchi = dchi[c];
+    cchi -= ug[6];
+    cchi += dt * diff / M.c0;
+    chi_out[c] = x0 + cchi;
+}
+
+static __global__ __launch_bounds__(kT) void k_direct_vertex(Grid g, Dyn D, double dt, const double *__restrict__ vel,
+                                                             const double *__restrict__ sig,
+                                                             const double *__restrict__ xi,
+                                                             double *__restrict__ vel_out,
+                                                             double *__restrict__ xi_out) {
+    const long p = (long)blockIdx.x * kT + threadIdx.x;
+    if (p >= g.Nv) return;
+    const int i = (int)(p % g.Nx);
+    const int j = (int)((p / g.Nx) % g.Ny);
+    const int k = (int)(p / ((long)g.Nx * g.Ny));
+    if (k == 0 || k >= g.Nz) {
+        // the plane zz = 0 and the top plate are not updated here
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            vel_out[q * g.Nv + p] = vel[q * g.Nv + p];
+            xi_out[q * g.Nv + p] = xi[q * g.Nv + p];
+        }
+        return;
+    }
+    const double dx_inv = 1. / g.dx, dy_inv = 1. / g.dy, dz_inv = 1. / g.dz;
+    const double dx_inv_sq = dx_inv * dx_inv, dy_inv_sq = dy_inv * dy_inv, dz_inv_sq = dz_inv * dz_inv;
+    const double rho_inv = 1. / D.rho;
+    const double dtrho_inv = dt * rho_inv, dtkaprho_inv = D.kappa * dtrho_inv;
+    const double dxdt = g.dx / dt, dydt = g.dy / dt, dzdt = g.dz / dt;
+    // backward staggered stress derivatives (calculate_first_order_stresses,
+    // src/shear_sim_3d.cpp:971-987): cells (i-1..i, j-1..j, k-1..k)
+    const long c000 = cidx(g, i, j, k), c100 = cidx(g, i - 1, j, k), c010 = cidx(g, i, j - 1, k),
+               c110 = cidx(g, i - 1, j - 1, k), c001 = cidx(g, i, j, k - 1), c101 = cidx(g, i - 1, j, k - 1),
+               c011 = cidx(g, i, j - 1, k - 1), c111 = cidx(g, i - 1, j - 1, k - 1);
+    auto DX = [&](const double *a) {
+        return .25 * dx_inv * (a[c000] - a[c100] + a[c010] - a[c110] + a[c001] - a[c101] + a[c011] - a[c111]);
+    };
+    auto DY = [&](const double *a) {
+        return .25 * dy_inv * (a[c000] - a[c010] + a[c100] - a[c110] + a[c001] - a[c011] + a[c101] - a[c111]);
+    };
+    auto DZ = [&](const double *a) {
+        return .25 * dz_inv * (a[c000] - a[c001] + a[c100] - a[c101] + a[c010] - a[c011] + a[c110] - a[c111]);
+    };
+    const double *S11 = sig, *S12 = sig + g.Nc, *S13 = sig + 2 * g.Nc, *S22 = sig + 3 * g.Nc, *S23 = sig + 4 * g.Nc,
+                 *S33 = sig + 5 * g.Nc;
+    const double div[3] = {DX(S11) + DY(S12) + DZ(S13), DX(S12) + DY(S22) + DZ(S23), DX(S13) + DY(S23) + DZ(S33)};
+    const double u = vel[p], v = vel[g.Nv + p], w = vel[2 * g.Nv + p];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const double *f = vel + q * g.Nv;
+        const double f0 = f[p];
+        const double gsq = dx_inv_sq * (vel_g(f, g, i + 1, j, k) - 2 * f0 + vel_g(f, g, i - 1, j, k)) +
+                           dy_inv_sq * (vel_g(f, g, i, j + 1, k) - 2 * f0 + vel_g(f, g, i, j - 1, k)) +
+                           dz_inv_sq * (vel_g(f, g, i, j, k + 1) - 2 * f0 + vel_g(f, g, i, j, k - 1));
+        const double ddx = eno2(dxdt, u, vel_g(f, g, i + 2, j, k), vel_g(f, g, i + 1, j, k), f0,
+                                vel_g(f, g, i - 1, j, k), vel_g(f, g, i - 2, j, k));
+        const double ddy = eno2(dydt, v, vel_g(f, g, i, j + 2, k), vel_g(f, g, i, j + 1, k), f0,
+                                vel_g(f, g, i, j - 1, k), vel_g(f, g, i, j - 2, k));
+        const double ddz = eno2(dzdt, w, vel_g(f, g, i, j, k + 2), vel_g(f, g, i, j, k + 1), f0,
+                                vel_g(f, g, i, j, k - 1), vel_g(f, g, i, j, k - 2));
+        const double ugq = u * ddx + v * ddy + w * ddz;
+        vel_out[q * g.Nv + p] = f0 + (dtrho_inv * div[q] + dtkaprho_inv * gsq - ugq);
+        const double x0 = xi[q * g.Nv + p];
+        const double ex = eno2(dxdt, u, xi_g(xi, g, D, q, i + 2, j, k), xi_g(xi, g, D, q, i + 1, j, k), x0,
+                               xi_g(xi, g, D, q, i - 1, j, k), xi_g(xi, g, D, q, i - 2, j, k));
+        const double ey = eno2(dydt, v, xi_g(xi, g, D, q, i, j + 2, k), xi_g(xi, g, D, q, i, j + 1, k), x0,
+                               xi_g(xi, g, D, q, i, j - 1, k), xi_g(xi, g, D, q, i, j - 2, k));
+        const double ez = eno2(dzdt, w, xi_g(xi, g, D, q, i, j, k + 2), xi_g(xi, g, D, q, i, j, k + 1), x0,
+                               xi_g(xi, g, D, q, i, j, k - 1), xi_g(xi, g, D, q, i, j, k - 2));
+        xi_out[q * g.Nv + p] = x0 + (0.0 - (u * ex + v * ey + w * ez));
+    }
+}
+
+// Top plate at time t: velocity (U t, 0, 0) (U for t >= 1) and reference map
+// (a_x + i dx + U t^2 / 2 or + U (t - 1/2), a_y + j dy, 0).
+static __global__ __launch_bounds__(kT) void k_direct_plate(Grid g, Dyn D, double t, double *__restrict__ vel,
+                                                            double *__restrict__ xi) {
+    const long q = (long)blockIdx.x * kT + threadIdx.x;
+    const long nxy = (long)g.Nx * g.Ny;
+    if (g.pz || q >= nxy) return;
+    const int i = (int)(q % g.Nx), j = (int)(q / g.Nx);
+    const long p = q + nxy * g.Nz;
+    vel[p] = t < 1 ? D.U * t : D.U;
+    vel[g.Nv + p] = 0;
+    vel[2 * g.Nv + p] = 0;
+    xi[p] = D.ax + i * g.dx + (t < 1 ? D.U * t * t * .5 : D.U * (t - 0.5));
+    xi[g.Nv + p] = D.ay + j * g.dy;
+    xi[2 * g.Nv + p] = 0;
+}
+
+// The reference map at the grid positions (init_ref, src/shear_sim_3d.cpp:
+// 1036-1057) below the top plate.
+static __global__ __launch_bounds__(kT) void k_init_ref(Grid g, Dyn D, double *__restrict__ xi) {
+    const long p = (long)blockIdx.x * kT + threadIdx.x;
+    if (p >= g.Nv) return;
+    const int i = (int)(p % g.Nx);
+    const int j = (int)((p / g.Nx) % g.Ny);
+    const int k = (int)(p / ((long)g.Nx * g.Ny));
+    if (k >= g.Nz) return;
+    xi[p] = D.ax + g.dx * i;
+    xi[g.Nv + p] = D.ay + g.dy * j;
+    xi[2 * g.Nv + p] = D.az + g.dz * k;
+}
+
+#define DHIP(s, call)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return stz_detail::fail((s), STZMG_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+Dyn dyn_of(const stz_t s) {
+    const stz_params &p = s->p;
+    return Dyn{p.rho, p.kappa, p.U, p.ax, p.ay, p.az, p.bx - p.ax, p.by - p.ay, p.bz - p.az};
+}
+
+int ensure_ref_map(stz_t s) {
+    if (s->xi) return 0;
+    const Grid &g = s->g;
+    DHIP(s, hipMalloc((void **)&s->xi, sizeof(double) * 3 * g.Nv));
+    DHIP(s, hipMalloc((void **)&s->xi2, sizeof(double) * 3 * g.Nv));
+    DHIP(s, hipMalloc((void **)&s->vel2, sizeof(double) * 3 * g.Nv));
+    DHIP(s, hipMemsetAsync(s->xi, 0, sizeof(double) * 3 * g.Nv, s->stream));
+    return 0;
+}
+
+int check_direct(stz_t s) {
+    if (s->p.transform != STZ_T_NONE)
+        return stz_detail::fail(s, STZMG_ERR_UNSUPPORTED, "the direct step is the untransformed shear_sim_3d one");
+    if (!(s->p.rho > 0) || !(s->p.kappa >= 0))
+        return stz_detail::fail(s, STZMG_ERR_ARG, "direct step needs rho > 0 and kappa >= 0");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+double stz_direct_dt(stz_t s) {
+    if (!s) return 0;
+    // solve_direct (src/shear_sim_3d.cpp:166-172)
+    const double dx = s->g.dx, kappa = s->p.kappa, tmult = s->p.tmult;
+    const double cfl = dx / 4.;
+    if (!(kappa > 0)) return cfl;
+    const double visc = dx * dx / 3. / kappa / 2.;
+    return (tmult * visc < cfl) ? tmult * visc : cfl;
+}
+
+int stz_init_ref(stz_t s) {
+    if (!s) return STZMG_ERR_ARG;
+    if (int rc = ensure_ref_map(s)) return rc;
+    const Grid &g = s->g;
+    const Dyn D = dyn_of(s);
+    hipLaunchKernelGGL(k_init_ref, dim3(nbk(g.Nv)), dim3(kT), 0, s->stream, g, D, s->xi);
+    hipLaunchKernelGGL(k_direct_plate, dim3(nbk((long)g.Nx * g.Ny)), dim3(kT), 0, s->stream, g, D, s->t, s->vel,
+                       s->xi);
+    DHIP(s, hipGetLastError());
+    DHIP(s, hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int stz_set_ref_map(stz_t s, const double *xi) {
+    if (!s || !xi) return STZMG_ERR_ARG;
+    if (int rc = ensure_ref_map(s)) return rc;
+    const Grid &g = s->g;
+    DHIP(s, hipMemcpyAsync(s->stage, xi, sizeof(double) * 3 * g.Nv, hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_aos2soa, dim3(nbk(g.Nv)), dim3(kT), 0, s->stream, (const double *)s->stage, s->xi, g.Nv, 3);
+    DHIP(s, hipGetLastError());
+    DHIP(s, hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int stz_get_ref_map(stz_t s, double *xi) {
+    if (!s || !xi) return STZMG_ERR_ARG;
+    if (!s->xi) return stz_detail::fail(s, STZMG_ERR_STATE, "no reference map (stz_init_ref or stz_set_ref_map)");
+    const Grid &g = s->g;
+    hipLaunchKernelGGL(k_soa2aos, dim3(nbk(g.Nv)), dim3(kT), 0, s->stream, (const double *)s->xi, s->stage, g.Nv, 3);
+    DHIP(s, hipGetLastError());
+    DHIP(s, hipMemcpyAsync(xi, s->stage, sizeof(double) * 3 * g.Nv, hipMemcpyDeviceToHost, s->stream));
+    DHIP(s, hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int stz_step_direct(stz_t s, double dt, int nsteps, stz_step_stats *st) {
+    if (!s || !(dt > 0) || nsteps < 0) return STZMG_ERR_ARG;
+    if (int rc = check_direct(s)) return rc;
+    if (int rc = ensure_ref_map(s)) return rc;
+    const Grid &g = s->g;
+    const Dyn D = dyn_of(s);
+    stz_step_stats S{};
+    for (int q = 0; q < nsteps; q++) {
+        DHIP(s, hipMemsetAsync(s->counters, 0, sizeof(int) * 4, s->stream));
+        hipLaunchKernelGGL(k_direct_plastic, dim3(nbk(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt,
+                           (const double *)s->sig, (const double *)s->chi, s->adapt, s->dchi, s->dpl2, s->counters);
+        hipLaunchKernelGGL(k_direct_cell, dim3(nbk(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt,
+                           (const double *)s->vel, (const double *)s->sig, (const double *)s->chi,
+                           (const double *)s->adapt, (const double *)s->dchi, (const double *)s->dpl2,
+                           (const double *)s->dpl, s->sig2, s->chi2);
+        hipLaunchKernelGGL(k_direct_vertex, dim3(nbk(g.Nv)), dim3(kT), 0, s->stream, g, D, dt,
+                           (const double *)s->vel, (const double *)s->sig, (const double *)s->xi, s->vel2, s->xi2);
+        DHIP(s, hipGetLastError());
+        std::swap(s->sig, s->sig2);
+        std::swap(s->chi, s->chi2);
+        std::swap(s->dpl, s->dpl2);
+        std::swap(s->vel, s->vel2);
+        std::swap(s->xi, s->xi2);
+        s->t += dt;
+        hipLaunchKernelGGL(k_direct_plate, dim3(nbk((long)g.Nx * g.Ny)), dim3(kT), 0, s->stream, g, D, s->t, s->vel,
+                           s->xi);
+        DHIP(s, hipGetLastError());
+        int cnt[4];
+        DHIP(s, hipMemcpyAsync(cnt, s->counters, sizeof(cnt), hipMemcpyDeviceToHost, s->stream));
+        DHIP(s, hipStreamSynchronize(s->stream));
+        S.yielded_cells = cnt[0];
+        S.steps++;
+    }
+    if (st) *st = S;
+    return 0;
+}
+
+}  // extern "C"

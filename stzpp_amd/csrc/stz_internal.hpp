@@ -22,6 +22,9 @@ struct stz_s {
     double *vel = nullptr, *sig = nullptr, *sig2 = nullptr, *chi = nullptr, *chi2 = nullptr;
     double *dpl = nullptr, *dpl2 = nullptr, *adapt = nullptr, *dchi = nullptr, *stage = nullptr;
     int *counters = nullptr;
+    // direct (explicit dynamics) step: reference map and its double buffer,
+    // second velocity buffer; allocated by the first direct-step call
+    double *xi = nullptr, *xi2 = nullptr, *vel2 = nullptr;
     double *pm_part = nullptr;  // proj_mag: per-block partial sums, then the total
     double proj_mag = 0;        // sum of modsq of the last projection's stress increments
     std::string out_dir;        // per-step output (stz_set_output)

@@ -4,8 +4,9 @@
 // * Cubes (output_cube, src/shear_sim_3d.cpp:1963-2161, with the values of
 //   Field::fval, src/field.hh:144-178): one float32 per grid point, x
 //   slowest, then y, z fastest, appended to <dir>/<name>_grid.<frame>; the
-//   velocity components carry Nz + 1 z points (the top plate plane, or the
-//   periodic image of plane 0).  The state is SoA with x fastest, so the
+//   velocity and reference-map components carry Nz + 1 z points (the top
+//   plate plane, or the periodic image of plane 0).  The state is SoA with x
+//   fastest, so the
 //   kernel transposes (z, x) tiles of one y row through LDS to write whole
 //   z runs.
 // * Wall tractions (compute_net_force, src/shear_sim_3d.cpp:1061-1126;
@@ -33,9 +34,11 @@ using namespace stz;
 
 namespace {
 
-constexpr int kCubeNames = 13;
-const char *const kCubeName[kCubeNames] = {"u",   "v",   "w", "s11", "s12", "s13", "s22",
-                                           "s23", "s33", "p", "dev", "chi", "tem"};
+constexpr int kCubeNames = 16;
+const char *const kCubeName[kCubeNames] = {"u",   "v",   "w",   "s11", "s12", "s13",  "s22",  "s23",
+                                           "s33", "p",   "dev", "chi", "tem", "xi_x", "xi_y", "xi_z"};
+
+inline bool vertex_mode(int mode) { return mode <= 2 || mode >= 13; }
 
 constexpr int kTile = 32;  // (z, x) transpose tile; 256 threads = 32 x 8
 
@@ -57,10 +60,12 @@ __device__ __forceinline__ double cell_value(int mode, const double *__restrict_
 }
 
 // out[(x Ny + y) nzo + z] = value(x, y, z) for one y per block row of tiles.
-// Grid: (ceil(Nx / 32), Ny, ceil(nzo / 32)).
+// Grid: (ceil(Nx / 32), Ny, ceil(nzo / 32)).  Vertex modes (velocity 0-2,
+// reference map 13-15) have Nz + 1 planes; with z periodic plane Nz is the
+// periodic image of plane 0, whose z reference map is shifted by lz.
 __global__ __launch_bounds__(256) void k_cube(Grid g, int mode, int nzo, const double *__restrict__ vel,
                                               const double *__restrict__ sig, const double *__restrict__ chi,
-                                              float *__restrict__ out) {
+                                              const double *__restrict__ xi, double lz, float *__restrict__ out) {
     __shared__ float tile[kTile][kTile + 1];
     const int x0 = blockIdx.x * kTile, y = blockIdx.y, z0 = blockIdx.z * kTile;
     const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;  // ty: 0..7
@@ -71,6 +76,10 @@ __global__ __launch_bounds__(256) void k_cube(Grid g, int mode, int nzo, const d
             if (mode <= 2) {
                 const int zw = z < g.Nzv ? z : z - g.Nzv;  // periodic: plane Nz is plane 0
                 v = (float)vel[mode * g.Nv + x + (long)g.Nx * (y + (long)g.Ny * zw)];
+            } else if (mode >= 13) {
+                const int zw = z < g.Nzv ? z : z - g.Nzv;
+                const double r = xi[(mode - 13) * g.Nv + x + (long)g.Nx * (y + (long)g.Ny * zw)];
+                v = (float)(mode == 15 && zw != z ? r + lz : r);
             } else {
                 v = (float)cell_value(mode, sig, chi, g.Nc, x + (long)g.Nx * (y + (long)g.Ny * z));
             }
@@ -131,11 +140,12 @@ __global__ __launch_bounds__(256) void k_traction(Grid g, const double *__restri
 }
 
 int cube_size(const stz_t s, int mode, long *n) {
-    if (mode < 0 || mode >= kCubeNames)
-        return stz_detail::fail(s, STZMG_ERR_ARG, "cube mode %d: modes 0-12 (the reference map, 13-15, is not kept)",
-                                mode);
+    if (mode < 0 || mode >= kCubeNames) return stz_detail::fail(s, STZMG_ERR_ARG, "cube mode %d: modes 0-15", mode);
+    if (mode >= 13 && !s->xi)
+        return stz_detail::fail(s, STZMG_ERR_STATE,
+                                "cube mode %d: the reference map is kept by the direct step only (stz_init_ref)", mode);
     const Grid &g = s->g;
-    *n = (long)g.Nx * g.Ny * (mode <= 2 ? g.Nz + 1 : g.Nz);
+    *n = (long)g.Nx * g.Ny * (vertex_mode(mode) ? g.Nz + 1 : g.Nz);
     return 0;
 }
 
@@ -154,7 +164,7 @@ int cube_host(stz_t s, int mode, std::vector<float> &h) {
     float *d = reinterpret_cast<float *>(s->stage);  // 6 max(Nc, Nv) doubles: room for any cube
     hipLaunchKernelGGL(k_cube, dim3((g.Nx + kTile - 1) / kTile, g.Ny, (nzo + kTile - 1) / kTile), dim3(256), 0,
                        s->stream, g, mode, nzo, (const double *)s->vel, (const double *)s->sig,
-                       (const double *)s->chi, d);
+                       (const double *)s->chi, (const double *)s->xi, s->p.bz - s->p.az, d);
     OHIP(s, hipGetLastError());
     h.resize(n);
     OHIP(s, hipMemcpyAsync(h.data(), d, sizeof(float) * n, hipMemcpyDeviceToHost, s->stream));
@@ -305,8 +315,10 @@ int stz_write_files(stz_t s, const char *dir, int frame, int n_tp, int fflags) {
     if (!s) return STZMG_ERR_ARG;
     if (n_tp < 10 || frame < 0)
         return stz_detail::fail(s, STZMG_ERR_ARG, "write_files needs n_tp >= 10 (frames every n_tp/10) and frame >= 0");
-    if (fflags >> kCubeNames)
-        return stz_detail::fail(s, STZMG_ERR_ARG, "fflags: bits 13-15 select the reference map, which is not kept");
+    if (fflags >> kCubeNames) return stz_detail::fail(s, STZMG_ERR_ARG, "fflags: unknown bits above 15");
+    if ((fflags >> 13) && !s->xi)
+        return stz_detail::fail(s, STZMG_ERR_STATE,
+                                "fflags: bits 13-15 select the reference map, kept by the direct step only");
     // write_files (src/shear_sim_3d.cpp:1938-1960): bit `mode` of fflags, the
     // velocity and stress every n_tp/10 frames, chi when (n_tp/10) % frame
     // == 0 for frame > 0 (the reference's order of the operands)

@@ -244,6 +244,62 @@ class ShearSim:
         file output: ``steps`` steps of size dt."""
         return self.step_forward_qs(dt, steps, **kw)
 
+    # ---- direct (explicit dynamics) step (include/stz.h) ----------------
+
+    def direct_dt(self):
+        """min(tmult dx^2 / (6 kappa), dx / 4) (solve_direct,
+        src/shear_sim_3d.cpp:166-172)."""
+        return self.L.stz_direct_dt(self.h)
+
+    def init_ref(self):
+        """The reference map at the vertex positions (init_ref,
+        src/shear_sim_3d.cpp:1036-1057) and the top plate's at sim_time."""
+        self._chk(self.L.stz_init_ref(self.h))
+
+    def set_ref_map(self, xi):
+        xi = np.ascontiguousarray(xi, dtype=np.float64).reshape(self.nverts, 3)
+        self._chk(self.L.stz_set_ref_map(self.h, _lib.dptr(xi)))
+
+    def ref_map(self):
+        """The reference map, (vertices, 3), x fastest."""
+        xi = np.zeros((self.nverts, 3))
+        self._chk(self.L.stz_get_ref_map(self.h, _lib.dptr(xi)))
+        return xi
+
+    def step_forward(self, dt, nsteps=1):
+        """nsteps direct Euler steps (step_forward, src/shear_sim_3d.cpp:206-213)."""
+        st = _lib.StzStepStats()
+        self._chk(self.L.stz_step_direct(self.h, float(dt), int(nsteps), ctypes.byref(st)))
+        self.last_stats = {k: getattr(st, k) for k, _ in _lib.StzStepStats._fields_}
+        return self.last_stats
+
+    def solve_direct(self, tf, n_tp, out_dir=None, fflags=4096):
+        """The direct time loop (solve_direct, src/shear_sim_3d.cpp:166-203)
+        from the current state (reference map at the positions when none is
+        set) to tf in n_tp frames, each ending at the last step that stays
+        below its target time; with out_dir, write_files at frame 0 and after
+        every frame.  Returns the step count."""
+        dt = self.direct_dt()
+        t0 = self.sim_time
+        if self.L.stz_cube_size(self.h, 13) < 0:
+            self.init_ref()
+        if out_dir is not None:
+            self.write_files(0, out_dir, n_tp, fflags)
+        steps, t = 0, t0
+        for kk in range(1, n_tp + 1):
+            target = t0 + (tf - t0) / n_tp * kk
+            n = 0
+            while t + dt * (1 + 1e-8) < target:
+                n += 1
+                t += dt
+            if n:
+                self.step_forward(dt, n)
+                t = self.sim_time
+            steps += n
+            if out_dir is not None:
+                self.write_files(kk, out_dir, n_tp, fflags)
+        return steps
+
 
 class TransSim(ShearSim):
     """Host-side mirror of ``trans_sim_3d`` (src/trans_sim_3d.hh:13-36): the QS
