@@ -224,6 +224,81 @@ def adjoint_workload(args):
     print(json.dumps(out), flush=True)
 
 
+DIRECT_N = (192, 192, 96)
+
+
+def direct_workload(args):
+    """Direct (explicit dynamics) steps of shear_sim_3d at 192 x 192 x 96
+    cells (vertices 192 x 192 x 97, z walled): config-0 material, density 1,
+    viscous damping 0.35, the reference's dt = min(tmult dx^2 / (6 kappa),
+    dx / 4) with tmult 0.5; chi0 = 0.09 + 0.01 GRF (seed 3, l = 4 cells),
+    s12 pre-stress 1.05 (every cell above yield: the plastic path
+    everywhere), at rest.  value = steps per second over K steps in one call.
+    The four kernels are timed with HIP events on the handle's stream.  CPU
+    baseline: the oracle's step (bit-exact to the reference's step_forward
+    where no exp enters) on one thread, on the same grid."""
+    from stzpp_amd import shear_sim as SS
+
+    Nx, Ny, Nz = DIRECT_N
+    p = SS.config0_params(Nx)
+    p.update(rho=1.0, kappa=0.35, tmult=0.5)
+    nc, nv = Nx * Ny * Nz, Nx * Ny * (Nz + 1)
+    sim = SS.ShearSim(Nx, Ny, Nz, params=p)
+    sig0 = np.zeros((nc, 6))
+    sig0[:, 1] = 1.05
+    sim.set_initial_conditions(np.zeros(nc), sig=sig0)
+    sim.set_random_initial_conditions(gauss_fac=5, conv_l=4.0, chi_avg=0.09, sigma=0.01, seed=3)
+    sim.init_ref()
+    dt = sim.direct_dt()
+    log(f"[bench] direct workload {Nx}x{Ny}x{Nz}, dt {dt:.4g}, warmup {args.warmup}")
+    if args.warmup:
+        sim.step_forward(dt, args.warmup)
+    ms = sim.time_direct_kernels(dt, reps=20)
+    t0 = time.perf_counter()
+    st = sim.step_forward(dt, args.steps)
+    el = time.perf_counter() - t0
+    log(f"[bench] {args.steps} steps in {el * 1e3:.2f} ms; kernels (ms) {ms.round(4).tolist()}")
+    names = ["k_direct_plastic", "k_direct_cell", "k_direct_vertex", "k_direct_plate"]
+    # algorithmic bytes per launch: every input read once, every output
+    # written once (neighbour re-reads come from cache)
+    alg = [80.0 * nc, 144.0 * nc + 24.0 * nv, 96.0 * nv + 48.0 * nc, 48.0 * Nx * Ny]
+    dom = int(np.argmax(ms))
+    achieved = alg[dom] / (ms[dom] * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+
+        P = {k: p[k] for k in O.STEP_PARAM_ORDER}
+        orc = O.OracleShear((Nx, Ny, Nz), P, prd=(1, 1, 0))
+        orc.set_direct(p["rho"], p["kappa"])
+        s = sim.state()
+        orc.set_state(s["t"], s["u"], s["sig"], s["chi"])
+        orc.init_ref()
+        n_cpu = 6
+        t0 = time.perf_counter()
+        orc.step_direct(dt, n_cpu)
+        el_cpu = time.perf_counter() - t0
+        cpu = {"value": n_cpu / el_cpu, "unit": "steps/s", "cores": 1, "kind": "port",
+               "sample": f"{n_cpu} direct steps of the oracle (the reference's step_forward restated in C, one "
+                         f"thread) on the same 192x192x96 grid from the GPU's state after warmup: {el_cpu:.1f} s"}
+    out = {
+        "metric": "direct (explicit dynamics) steps per second, shear_sim_3d 192x192x96 cells",
+        "value": args.steps / el, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "shear_sim_3d direct step 192x192x96 cells (vertices 192x192x97, z walled), "
+                               "config-0 material, rho 1, kappa 0.35, tmult 0.5, chi0 GRF seed 3 l=4, "
+                               "s12 pre-stress 1.05",
+                   "dt": dt, "yielded_cells": st["yielded_cells"],
+                   "kernel_ms": dict(zip(["plastic", "cell", "vertex", "plate"], [float(v) for v in ms]))},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": names[dom],
+                     "alg_bytes_per_launch": alg[dom], "launch_us": float(ms[dom]) * 1e3},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -235,9 +310,10 @@ def main():
     ap.add_argument("--restart", type=int, default=10, help="GMRES restart length")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch V-cycles kernel by kernel instead of replaying a hipGraph")
-    ap.add_argument("--workload", choices=("solve", "step", "adjoint", "grf"), default="solve",
+    ap.add_argument("--workload", choices=("solve", "step", "adjoint", "grf", "direct"), default="solve",
                     help="solve (default): the 256^3 projection solve; step: QS steps at 192x192x96; "
-                         "adjoint: gradient of 20 QS steps at 128^3; grf: random initial chi at 192x192x96")
+                         "adjoint: gradient of 20 QS steps at 128^3; grf: random initial chi at 192x192x96; "
+                         "direct: explicit dynamics steps at 192x192x96")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -249,6 +325,8 @@ def main():
         return adjoint_workload(args)
     if args.workload == "grf":
         return grf_workload(args)
+    if args.workload == "direct":
+        return direct_workload(args)
 
     from stzpp_amd import multigrid as MG
 

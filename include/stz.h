@@ -230,6 +230,12 @@ int stz_set_ref_map(stz_t s, const double *xi);
 int stz_get_ref_map(stz_t s, double *xi);
 /* nsteps direct steps of size dt (stats: steps and yielded_cells). */
 int stz_step_direct(stz_t s, double dt, int nsteps, stz_step_stats *st);
+/* Average device time per launch (ms) of the step's four kernels -- cell
+ * plasticity, cell stress and chi, vertex velocity and reference map, top
+ * plate -- over reps back-to-back launches each on the current state, from
+ * HIP events on the handle's stream (for measurement; the state is left as
+ * it was). */
+int stz_direct_time_kernels(stz_t s, double dt, int reps, double ms[4]);
 
 #ifdef __cplusplus
 }

@@ -218,7 +218,7 @@ __device__ __forceinline__ void eno2_w(double h, double v, double f2p, double fp
 // sigma' = sigma* + dt C:D(u'): u'-bar += (dt C:D)^T sigma'-bar (cell scatter).
 static __global__ __launch_bounds__(kT) void k_adj_stress(Grid g, Mat M, double dt, const double *__restrict__ sb,
                                                    double *__restrict__ ub) {
-    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    const long c = xcd_block() * kT + threadIdx.x;
     if (c >= g.Nc) return;
     const int i = (int)(c % g.Nx), j = (int)((c / g.Nx) % g.Ny), k = (int)(c / ((long)g.Nx * g.Ny));
     const double b11 = sb[c], b12 = sb[g.Nc + c], b13 = sb[2 * g.Nc + c];
@@ -239,7 +239,7 @@ static __global__ __launch_bounds__(kT) void k_adj_stress(Grid g, Mat M, double 
 // (the walls are fixed values, not unknowns).
 static __global__ __launch_bounds__(kT) void k_adj_wall_zero(Grid g, const double *__restrict__ ub,
                                                       double *__restrict__ rhs) {
-    const long p = (long)blockIdx.x * kT + threadIdx.x;
+    const long p = xcd_block() * kT + threadIdx.x;
     if (p >= g.Nv) return;
     const int k = (int)(p / ((long)g.Nx * g.Ny));
     const bool wall = !g.pz && (k == 0 || k == g.Nz);
@@ -251,7 +251,7 @@ static __global__ __launch_bounds__(kT) void k_adj_wall_zero(Grid g, const doubl
 // (gather over the 8 vertices of each cell; lambda = A^-T u'-bar).
 static __global__ __launch_bounds__(kT) void k_adj_div(Grid g, const double *__restrict__ lam,
                                                 const double *__restrict__ sb, double *__restrict__ ssb) {
-    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    const long c = xcd_block() * kT + threadIdx.x;
     if (c >= g.Nc) return;
     const int i = (int)(c % g.Nx), j = (int)((c / g.Nx) % g.Ny), k = (int)(c / ((long)g.Nx * g.Ny));
     const double ix = .25 * (1.0 / g.dx), iy = .25 * (1.0 / g.dy), iz = .25 * (1.0 / g.dz);
@@ -292,7 +292,7 @@ static __global__ __launch_bounds__(kT) void k_adj_advect(Grid g, Mat M, double 
                                                    double *__restrict__ xb_out, double *__restrict__ ub_out,
                                                    double *__restrict__ dnb, double *__restrict__ dob,
                                                    double *__restrict__ ab, double *__restrict__ dchib) {
-    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    const long c = xcd_block() * kT + threadIdx.x;
     if (c >= g.Nc) return;
     const int i = (int)(c % g.Nx), j = (int)((c / g.Nx) % g.Ny), k = (int)(c / ((long)g.Nx * g.Ny));
     double G[3][3], v[3];
@@ -436,7 +436,7 @@ static __global__ __launch_bounds__(kT) void k_adj_plastic(Grid g, Mat M, double
                                                     const double *__restrict__ dchib,
                                                     const double *__restrict__ dnb, double *__restrict__ sb_out,
                                                     double *__restrict__ xb_out, double *__restrict__ part) {
-    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    const long c = xcd_block() * kT + threadIdx.x;
     double pg[3] = {0, 0, 0};
     if (c < g.Nc) {
         double s[6];
@@ -533,7 +533,7 @@ __device__ __forceinline__ void L_adj(double *ub, const Grid &g, const Xform &X,
 static __global__ __launch_bounds__(kT) void k_adj_stress_trans(Grid g, Mat M, double dt, Xform X,
                                                                 const double *__restrict__ sb,
                                                                 double *__restrict__ ub) {
-    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    const long c = xcd_block() * kT + threadIdx.x;
     if (c >= g.Nc) return;
     const int i = (int)(c % g.Nx), j = (int)((c / g.Nx) % g.Ny), k = (int)(c / ((long)g.Nx * g.Ny));
     const int comp[6] = {0, 1, 2, 4, 5, 8};
@@ -561,7 +561,7 @@ static __global__ __launch_bounds__(kT) void k_adj_stress_trans(Grid g, Mat M, d
 static __global__ __launch_bounds__(kT) void k_adj_div_trans(Grid g, Xform X, const double *__restrict__ lam,
                                                              const double *__restrict__ sb,
                                                              double *__restrict__ ssb) {
-    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    const long c = xcd_block() * kT + threadIdx.x;
     if (c >= g.Nc) return;
     const int i = (int)(c % g.Nx), j = (int)((c / g.Nx) % g.Ny), k = (int)(c / ((long)g.Nx * g.Ny));
     const double ix = .25 * (1.0 / g.dx), iy = .25 * (1.0 / g.dy), iz = .25 * (1.0 / g.dz);
@@ -603,7 +603,7 @@ static __global__ __launch_bounds__(kT) void k_adj_advect_trans(
     const double *__restrict__ dold, const double *__restrict__ ssb, const double *__restrict__ xsb,
     double *__restrict__ sb_out, double *__restrict__ xb_out, double *__restrict__ ub_out, double *__restrict__ dnb,
     double *__restrict__ dob, double *__restrict__ ab, double *__restrict__ dchib) {
-    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    const long c = xcd_block() * kT + threadIdx.x;
     if (c >= g.Nc) return;
     const int i = (int)(c % g.Nx), j = (int)((c / g.Nx) % g.Ny), k = (int)(c / ((long)g.Nx * g.Ny));
     const double P0[3] = {B.ax + ((double)i + .5) * g.dx, B.ay + ((double)j + .5) * g.dy,
@@ -838,7 +838,7 @@ static __global__ __launch_bounds__(kT) void k_adj_plastic_trans(
     Grid g, Mat M, double dt, Xform X, const double *__restrict__ sig, const double *__restrict__ chi,
     const double *__restrict__ ab, const double *__restrict__ dchib, const double *__restrict__ dnb,
     double *__restrict__ sb_out, double *__restrict__ xb_out, double *__restrict__ part) {
-    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    const long c = xcd_block() * kT + threadIdx.x;
     double pg[3] = {0, 0, 0};
     if (c < g.Nc) {
         double sp[9], os[9];
@@ -888,7 +888,7 @@ static __global__ __launch_bounds__(kT) void k_adj_plastic_trans(
 // Adjoint of the loss J = mean_c (ad_Dpl_c / dt)^2 at the final step.
 static __global__ __launch_bounds__(kT) void k_adj_loss(Grid g, double dt, const double *__restrict__ dpl,
                                                  double *__restrict__ db) {
-    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    const long c = xcd_block() * kT + threadIdx.x;
     if (c >= g.Nc) return;
     db[c] = 2.0 * dpl[c] / ((double)g.Nc * dt * dt);
 }

@@ -85,7 +85,7 @@ static __global__ __launch_bounds__(kT) void k_direct_plastic(Grid g, Mat M, dou
                                                               double *__restrict__ adapt, double *__restrict__ dchi,
                                                               double *__restrict__ dpl_new,
                                                               int *__restrict__ counters) {
-    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    const long c = xcd_block() * kT + threadIdx.x;
     if (c >= g.Nc) return;
     double s[6];
 #pragma unroll
@@ -96,8 +96,8 @@ static __global__ __launch_bounds__(kT) void k_direct_plastic(Grid g, Mat M, dou
         int n;
         a = adaptive(M, sb, chi[c], dt, dc, n);
         D = a / 2. / M.mu / dt * sb;
-        atomicAdd(&counters[0], 1);
     }
+    count_wave(&counters[0], sb > M.sy);
     adapt[c] = a;
     dchi[c] = dc;
     dpl_new[c] = D;
@@ -111,7 +111,7 @@ static __global__ __launch_bounds__(kT) void k_direct_cell(Grid g, Mat M, double
                                                            const double *__restrict__ dnew,
                                                            const double *__restrict__ dold,
                                                            double *__restrict__ sig_out, double *__restrict__ chi_out) {
-    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    const long c = xcd_block() * kT + threadIdx.x;
     if (c >= g.Nc) return;
     const int i = (int)(c % g.Nx);
     const int j = (int)((c / g.Nx) % g.Ny);
@@ -201,7 +201,7 @@ static __global__ __launch_bounds__(kT) void k_direct_vertex(Grid g, Dyn D, doub
                                                              const double *__restrict__ xi,
                                                              double *__restrict__ vel_out,
                                                              double *__restrict__ xi_out) {
-    const long p = (long)blockIdx.x * kT + threadIdx.x;
+    const long p = xcd_block() * kT + threadIdx.x;
     if (p >= g.Nv) return;
     const int i = (int)(p % g.Nx);
     const int j = (int)((p / g.Nx) % g.Ny);
@@ -268,7 +268,7 @@ static __global__ __launch_bounds__(kT) void k_direct_vertex(Grid g, Dyn D, doub
 // (a_x + i dx + U t^2 / 2 or + U (t - 1/2), a_y + j dy, 0).
 static __global__ __launch_bounds__(kT) void k_direct_plate(Grid g, Dyn D, double t, double *__restrict__ vel,
                                                             double *__restrict__ xi) {
-    const long q = (long)blockIdx.x * kT + threadIdx.x;
+    const long q = xcd_block() * kT + threadIdx.x;
     const long nxy = (long)g.Nx * g.Ny;
     if (g.pz || q >= nxy) return;
     const int i = (int)(q % g.Nx), j = (int)(q / g.Nx);
@@ -284,7 +284,7 @@ static __global__ __launch_bounds__(kT) void k_direct_plate(Grid g, Dyn D, doubl
 // The reference map at the grid positions (init_ref, src/shear_sim_3d.cpp:
 // 1036-1057) below the top plate.
 static __global__ __launch_bounds__(kT) void k_init_ref(Grid g, Dyn D, double *__restrict__ xi) {
-    const long p = (long)blockIdx.x * kT + threadIdx.x;
+    const long p = xcd_block() * kT + threadIdx.x;
     if (p >= g.Nv) return;
     const int i = (int)(p % g.Nx);
     const int j = (int)((p / g.Nx) % g.Ny);
@@ -401,14 +401,63 @@ int stz_step_direct(stz_t s, double dt, int nsteps, stz_step_stats *st) {
         hipLaunchKernelGGL(k_direct_plate, dim3(nbk((long)g.Nx * g.Ny)), dim3(kT), 0, s->stream, g, D, s->t, s->vel,
                            s->xi);
         DHIP(s, hipGetLastError());
-        int cnt[4];
-        DHIP(s, hipMemcpyAsync(cnt, s->counters, sizeof(cnt), hipMemcpyDeviceToHost, s->stream));
-        DHIP(s, hipStreamSynchronize(s->stream));
-        S.yielded_cells = cnt[0];
         S.steps++;
     }
+    // one synchronisation per call: the counters hold the last step's count
+    int cnt[4] = {0, 0, 0, 0};
+    DHIP(s, hipMemcpyAsync(cnt, s->counters, sizeof(cnt), hipMemcpyDeviceToHost, s->stream));
+    DHIP(s, hipStreamSynchronize(s->stream));
+    S.yielded_cells = nsteps ? cnt[0] : 0;
     if (st) *st = S;
     return 0;
+}
+
+int stz_direct_time_kernels(stz_t s, double dt, int reps, double ms[4]) {
+    if (!s || !(dt > 0) || reps < 1 || !ms) return STZMG_ERR_ARG;
+    if (int rc = check_direct(s)) return rc;
+    if (int rc = ensure_ref_map(s)) return rc;
+    const Grid &g = s->g;
+    const Dyn D = dyn_of(s);
+    hipEvent_t e0, e1;
+    DHIP(s, hipEventCreate(&e0));
+    DHIP(s, hipEventCreate(&e1));
+    // Every kernel reads the current state and writes the second buffers (the
+    // plate kernel rewrites the plate's values at the current time), so the
+    // repeated launches leave the state as it was.
+    auto launch = [&](int which) {
+        if (which == 0)
+            hipLaunchKernelGGL(k_direct_plastic, dim3(nbk(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt,
+                               (const double *)s->sig, (const double *)s->chi, s->adapt, s->dchi, s->dpl2,
+                               s->counters);
+        else if (which == 1)
+            hipLaunchKernelGGL(k_direct_cell, dim3(nbk(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt,
+                               (const double *)s->vel, (const double *)s->sig, (const double *)s->chi,
+                               (const double *)s->adapt, (const double *)s->dchi, (const double *)s->dpl2,
+                               (const double *)s->dpl, s->sig2, s->chi2);
+        else if (which == 2)
+            hipLaunchKernelGGL(k_direct_vertex, dim3(nbk(g.Nv)), dim3(kT), 0, s->stream, g, D, dt,
+                               (const double *)s->vel, (const double *)s->sig, (const double *)s->xi, s->vel2,
+                               s->xi2);
+        else
+            hipLaunchKernelGGL(k_direct_plate, dim3(nbk((long)g.Nx * g.Ny)), dim3(kT), 0, s->stream, g, D, s->t,
+                               s->vel, s->xi);
+    };
+    int rc = 0;
+    for (int w = 0; w < 4 && !rc; w++) {
+        launch(w);  // warm
+        hipEventRecord(e0, s->stream);
+        for (int r = 0; r < reps; r++) launch(w);
+        hipEventRecord(e1, s->stream);
+        hipError_t e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipGetLastError();
+        float t = 0;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        if (e != hipSuccess) rc = stz_detail::fail(s, STZMG_ERR_HIP, "direct kernel timing: %s", hipGetErrorString(e));
+        ms[w] = (double)t / reps;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
 }
 
 }  // extern "C"

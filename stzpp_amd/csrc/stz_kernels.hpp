@@ -30,10 +30,24 @@ struct Mat {
     double mu, lamb, sy, tau0, c0, eps0, delta, omega, bath_temp, chi_inf, ez, diff_l, eta;
 };
 
+// Logical block of a 1-D launch.  Workgroups are dealt round-robin over the
+// 8 XCDs (block b runs on XCD b % 8), each with its own L2; this gives XCD x
+// the contiguous range of logical blocks starting at x q + min(x, r)
+// (q = n / 8, r = n % 8), so the neighbour rows and planes a stencil reads
+// were mostly fetched into the same L2 by the blocks beside it.  A bijection
+// of [0, n) for every n.
+__device__ __forceinline__ long xcd_block() {
+    const unsigned n = gridDim.x, b = blockIdx.x;
+    const unsigned x = b & 7, q = n >> 3, r = n & 7;
+    return (long)(x * q + (x < r ? x : r) + (b >> 3));
+}
+
+// Periodic index for -n <= i < 2n: every stencil here reaches at most two
+// points past an edge and stz_create requires n >= 4, so one correction each
+// way suffices (selects, no loop).
 __device__ __forceinline__ int wrapc(int i, int n) {
-    while (i < 0) i += n;
-    while (i >= n) i -= n;
-    return i;
+    i = i < 0 ? i + n : i;
+    return i >= n ? i - n : i;
 }
 
 __device__ __forceinline__ long cidx(const Grid &g, int i, int j, int k) {
@@ -123,6 +137,14 @@ __device__ __forceinline__ double adaptive(const Mat &M, double sb0, double chi0
     return 1 - sb / sb0;
 }
 
+// Adds the number of lanes of the wave with `flag` set to *counter, one
+// atomic per wave.  Call it outside divergent code: lane 0 of a wave is
+// active whenever any of its lanes is (cells are numbered with the lane).
+__device__ __forceinline__ void count_wave(int *counter, bool flag) {
+    const unsigned long long b = __ballot(flag);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(counter, (int)__popcll(b));
+}
+
 // eno2 (src/shear_sim_3d.cpp:1257-1266), h already divided by dt.
 __device__ __forceinline__ double eno2(double h, double v, double f2p, double fp, double f, double fm, double f2m) {
     const double c2 = fabs(fp - 2 * f + fm);
@@ -140,7 +162,7 @@ static __global__ __launch_bounds__(kT) void k_plastic(Grid g, Mat M, double dt,
                                                 const double *__restrict__ chi, double *__restrict__ adapt,
                                                 double *__restrict__ dchi, double *__restrict__ dpl_new,
                                                 int *__restrict__ counters) {
-    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    const long c = xcd_block() * kT + threadIdx.x;
     if (c >= g.Nc) return;
     double s[6];
 #pragma unroll
@@ -150,8 +172,8 @@ static __global__ __launch_bounds__(kT) void k_plastic(Grid g, Mat M, double dt,
     if (sb > M.sy) {
         int n;
         a = adaptive(M, sb, chi[c], dt, dc, n);
-        atomicAdd(&counters[0], 1);
     }
+    count_wave(&counters[0], sb > M.sy);
     adapt[c] = a;
     dchi[c] = dc;
     dpl_new[c] = a / 2. / M.mu * sb;
@@ -168,7 +190,7 @@ static __global__ __launch_bounds__(kT) void k_advect(Grid g, Mat M, double dt, 
                                                const double *__restrict__ adapt, const double *__restrict__ dchi,
                                                const double *__restrict__ dnew, const double *__restrict__ dold,
                                                double *__restrict__ sig_out, double *__restrict__ chi_out) {
-    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    const long c = xcd_block() * kT + threadIdx.x;
     if (c >= g.Nc) return;
     const int i = (int)(c % g.Nx);
     const int j = (int)((c / g.Nx) % g.Ny);
@@ -244,7 +266,7 @@ static __global__ __launch_bounds__(kT) void k_advect(Grid g, Mat M, double dt, 
 static __global__ __launch_bounds__(kT) void k_rhs(Grid g, const double *__restrict__ sig, const double *__restrict__ vel,
                                             double fix_a11, double top_u, double *__restrict__ x,
                                             double *__restrict__ r) {
-    const long p = (long)blockIdx.x * kT + threadIdx.x;
+    const long p = xcd_block() * kT + threadIdx.x;
     if (p >= g.Nv) return;
     const int i = (int)(p % g.Nx);
     const int j = (int)((p / g.Nx) % g.Ny);
@@ -287,7 +309,7 @@ static __global__ __launch_bounds__(kT) void k_rhs(Grid g, const double *__restr
 // src/shear_sim_3d.cpp:615-681, :2362-2470).
 static __global__ __launch_bounds__(kT) void k_copy_vel(Grid g, const double *__restrict__ x, double top_u,
                                                  double *__restrict__ vel, int bottom = 1) {
-    const long p = (long)blockIdx.x * kT + threadIdx.x;
+    const long p = xcd_block() * kT + threadIdx.x;
     if (p >= g.Nv) return;
     const int k = (int)(p / ((long)g.Nx * g.Ny));
     double v0 = x[p], v1 = x[g.Nv + p], v2 = x[2 * g.Nv + p];
@@ -303,7 +325,7 @@ static __global__ __launch_bounds__(kT) void k_copy_vel(Grid g, const double *__
 // proj_mag, :699-724; modsq, src/mat3.hh:338-340) into pm_part[block].
 static __global__ __launch_bounds__(kT) void k_stress_update(Grid g, Mat M, double dt, const double *__restrict__ vel,
                                                       double *__restrict__ sig, double *__restrict__ pm_part) {
-    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    const long c = xcd_block() * kT + threadIdx.x;
     double pm = 0;
     if (c < g.Nc) {
         const int i = (int)(c % g.Nx);
@@ -454,7 +476,7 @@ static __global__ __launch_bounds__(kT) void k_plastic_trans(Grid g, Mat M, doub
                                                       const double *__restrict__ chi, double *__restrict__ adapt,
                                                       double *__restrict__ dchi, double *__restrict__ dpl_new,
                                                       int *__restrict__ counters) {
-    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    const long c = xcd_block() * kT + threadIdx.x;
     if (c >= g.Nc) return;
     double sp[9], os[9];
     old_sig(sig, g.Nc, c, X, sp, os);
@@ -464,8 +486,8 @@ static __global__ __launch_bounds__(kT) void k_plastic_trans(Grid g, Mat M, doub
         int n;
         a = adaptive(M, sb, chi[c], dt, dc, n);
         D = .5 * a / M.mu * sb;
-        atomicAdd(&counters[0], 1);
     }
+    count_wave(&counters[0], sb > M.sy);
     adapt[c] = a;
     dchi[c] = dc;
     dpl_new[c] = D;
@@ -485,7 +507,7 @@ static __global__ __launch_bounds__(kT) void k_advect_trans(Grid g, Mat M, doubl
                                                      const double *__restrict__ dchi, const double *__restrict__ dnew,
                                                      const double *__restrict__ dold, double *__restrict__ sig_out,
                                                      double *__restrict__ chi_out) {
-    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    const long c = xcd_block() * kT + threadIdx.x;
     if (c >= g.Nc) return;
     const int i = (int)(c % g.Nx);
     const int j = (int)((c / g.Nx) % g.Ny);
@@ -614,7 +636,7 @@ static __global__ __launch_bounds__(kT) void k_advect_trans(Grid g, Mat M, doubl
 static __global__ __launch_bounds__(kT) void k_rhs_trans(Grid g, Xform X, const double *__restrict__ sig,
                                                   const double *__restrict__ vel, double *__restrict__ x,
                                                   double *__restrict__ r) {
-    const long p = (long)blockIdx.x * kT + threadIdx.x;
+    const long p = xcd_block() * kT + threadIdx.x;
     if (p >= g.Nv) return;
     const int i = (int)(p % g.Nx);
     const int j = (int)((p / g.Nx) % g.Ny);
@@ -653,7 +675,7 @@ static __global__ __launch_bounds__(kT) void k_rhs_trans(Grid g, Xform X, const 
 static __global__ __launch_bounds__(kT) void k_stress_update_trans(Grid g, Mat M, double dt, Xform X, Box B,
                                                             const double *__restrict__ vel,
                                                             double *__restrict__ sig) {
-    const long c = (long)blockIdx.x * kT + threadIdx.x;
+    const long c = xcd_block() * kT + threadIdx.x;
     if (c >= g.Nc) return;
     const int i = (int)(c % g.Nx);
     const int j = (int)((c / g.Nx) % g.Ny);

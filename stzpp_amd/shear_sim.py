@@ -273,6 +273,13 @@ class ShearSim:
         self.last_stats = {k: getattr(st, k) for k, _ in _lib.StzStepStats._fields_}
         return self.last_stats
 
+    def time_direct_kernels(self, dt, reps=20):
+        """Average ms per launch of the direct step's kernels (plastic, cell,
+        vertex, plate) on the current state, which they leave unchanged."""
+        ms = np.zeros(4)
+        self._chk(self.L.stz_direct_time_kernels(self.h, float(dt), int(reps), _lib.dptr(ms)))
+        return ms
+
     def solve_direct(self, tf, n_tp, out_dir=None, fflags=4096):
         """The direct time loop (solve_direct, src/shear_sim_3d.cpp:166-203)
         from the current state (reference map at the positions when none is
