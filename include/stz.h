@@ -236,6 +236,15 @@ int stz_step_direct(stz_t s, double dt, int nsteps, stz_step_stats *st);
  * HIP events on the handle's stream (for measurement; the state is left as
  * it was). */
 int stz_direct_time_kernels(stz_t s, double dt, int reps, double ms[4]);
+/* l2_comparison (src/shear_sim_3d.cpp:3122-3158), the comparison of a
+ * direct against a quasi-static simulation in shear_compare
+ * (src/shear_compare.cpp:196-205): l2 = the squared differences of a and b
+ * (same grid) in velocity (trapezoid rule in z: vertex plane 0 and plane Nz
+ * weigh 1/2), stress (six components) and chi, each times
+ * dx dy dz / volume, the velocity's divided by U^2 and chi's by chi_inf^2
+ * (a's parameters).  The reference's accumulators start from an
+ * indeterminate value (:3127); these start from zero. */
+int stz_l2_comparison(stz_t a, stz_t b, double l2[3]);
 
 #ifdef __cplusplus
 }

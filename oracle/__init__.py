@@ -362,6 +362,7 @@ def ref_lib():
             ("ref_shear_step_direct", [vp, ctypes.c_double, ctypes.c_int], None),
             ("ref_shear_get_ref", [vp, _dp], None),
             ("ref_shear_net_force", [vp], None),
+            ("ref_shear_l2_comparison", [vp, vp, _dp], None),
             ("ref_trans_output_cube", [vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int], None),
             ("ref_trans_net_force", [vp], None),
         ]:
@@ -659,6 +660,13 @@ class RefShear:
         """The reference's compute_net_force (appends to traction.dat)."""
         self.L.ref_shear_net_force(self.h)
 
+    def l2_comparison(self, other):
+        """The reference's l2_comparison of this simulation against other
+        (three squared, normalised differences)."""
+        out = np.zeros(3)
+        self.L.ref_shear_l2_comparison(self.h, other.h, _d(out))
+        return out
+
     def setup(self, dt):
         self.L.ref_shear_setup(self.h, dt)
 
@@ -947,6 +955,31 @@ def net_force(N, box, sig, T=None):
     g = dx * dy
     t = [v * g for v in t]
     return t + b
+
+
+def l2_comparison(N, prd, box, U, chi_inf, a, b):
+    """l2_comparison (src/shear_sim_3d.cpp:3122-3158): the squared,
+    normalised differences (velocity, stress, chi) of states a and b (dicts
+    with u, sig, chi) on the same grid.  Velocity by the trapezoid rule in z:
+    vertex plane 0 and plane Nz (the top plate, or with z periodic the
+    periodic image of plane 0) weigh 1/2 (:3134, :3142-3149); stress (the six
+    components) and chi over every cell.  Each sum is scaled by
+    dx dy dz / volume, the velocity's divided by U^2 and chi's by chi_inf^2
+    (:3152-3155).  The reference starts its three accumulators from one
+    indeterminate value (local_l2[2] is read before it is set, :3127); here
+    they start from zero."""
+    Nx, Ny, Nz = N
+    nxy = Nx * Ny
+    du = (np.asarray(a["u"], dtype=np.float64) - np.asarray(b["u"], dtype=np.float64)).reshape(-1, nxy, 3)
+    sq = (du * du).sum(axis=2)
+    w = np.ones(Nz)
+    w[0] = 0.5
+    v = float((sq[:Nz].sum(axis=1) * w).sum()) + 0.5 * float((sq[0] if prd[2] else sq[Nz]).sum())
+    ds = np.asarray(a["sig"], dtype=np.float64) - np.asarray(b["sig"], dtype=np.float64)
+    dc = np.asarray(a["chi"], dtype=np.float64) - np.asarray(b["chi"], dtype=np.float64)
+    Lx, Ly, Lz = box[1] - box[0], box[3] - box[2], box[5] - box[4]
+    nv = (Lx / Nx) * (Ly / Ny) * (Lz / Nz) / (Lx * Ly * Lz)
+    return [v * nv / (U * U), float((ds * ds).sum()) * nv, float((dc * dc).sum()) * nv / (chi_inf * chi_inf)]
 
 
 def traction_line(t, vals, trans=False):

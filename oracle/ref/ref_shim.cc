@@ -612,6 +612,13 @@ void ref_shear_step_direct(void *hp, double dt, int nsteps) {
     for (int q = 0; q < nsteps; q++) h->sim->step_forward(dt);
 }
 
+// l2_comparison (src/shear_sim_3d.cpp:3122-3158) of simulation a against b.
+void ref_shear_l2_comparison(void *ap, void *bp, double *l2) {
+    shear_handle *a = (shear_handle *)ap, *b = (shear_handle *)bp;
+    l2[0] = l2[1] = l2[2] = 0;
+    a->sim->l2_comparison(*b->sim, l2);
+}
+
 void ref_shear_get_ref(void *hp, double *xi) {
     shear_handle *h = (shear_handle *)hp;
     ref_shear *s = h->sim;

@@ -113,6 +113,7 @@ SIGNATURES = {
     "stz_get_ref_map": ([_vp, _dp], ctypes.c_int),
     "stz_step_direct": ([_vp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(StzStepStats)], ctypes.c_int),
     "stz_direct_time_kernels": ([_vp, ctypes.c_double, ctypes.c_int, _dp], ctypes.c_int),
+    "stz_l2_comparison": ([_vp, _vp, _dp], ctypes.c_int),
     "stzmg_desc_default": ([ctypes.POINTER(StzmgDesc)] + [ctypes.c_int] * 6, ctypes.c_int),
     "stzmg_create": ([ctypes.POINTER(StzmgDesc), ctypes.POINTER(_vp)], ctypes.c_int),
     "stzmg_destroy": ([_vp], ctypes.c_int),

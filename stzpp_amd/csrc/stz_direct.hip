@@ -295,6 +295,54 @@ static __global__ __launch_bounds__(kT) void k_init_ref(Grid g, Dyn D, double *_
     xi[2 * g.Nv + p] = D.az + g.dz * k;
 }
 
+// Block partials of l2_comparison (src/shear_sim_3d.cpp:3122-3158) over the
+// points p < Nx Ny (Nz + 1): vertex velocity with the trapezoid weight in z
+// (1/2 on plane 0 and on plane Nz, the top plate or the periodic image of
+// plane 0), and for p < Nc the cell's six stress and chi differences.
+// part[q nblocks + block], q = velocity, stress, chi (unscaled sums).
+static __global__ __launch_bounds__(kT) void k_l2_parts(Grid g, const double *__restrict__ ua,
+                                                        const double *__restrict__ sa, const double *__restrict__ xa,
+                                                        const double *__restrict__ ub, const double *__restrict__ sb,
+                                                        const double *__restrict__ xb, double *__restrict__ part) {
+    const long p = xcd_block() * kT + threadIdx.x;
+    const long nxy = (long)g.Nx * g.Ny;
+    double v[3] = {0, 0, 0};
+    if (p < nxy * (g.Nz + 1)) {
+        const int k = (int)(p / nxy);
+        const long pv = (g.pz && k == g.Nz) ? p - nxy * g.Nz : p;
+        const double du = ua[pv] - ub[pv], dv = ua[g.Nv + pv] - ub[g.Nv + pv];
+        const double dw = ua[2 * g.Nv + pv] - ub[2 * g.Nv + pv];
+        if (k < g.Nz) {
+            v[0] = (du * du + dv * dv + dw * dw) * (k == 0 ? .5 : 1);
+            double t = 0;
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                const double d = sa[q * g.Nc + p] - sb[q * g.Nc + p];
+                t += d * d;
+            }
+            v[1] = t;
+            const double dc = xa[p] - xb[p];
+            v[2] = dc * dc;
+        } else {
+            v[0] = .5 * (du * du + dv * dv + dw * dw);
+        }
+    }
+    __shared__ double red[3][kT / 64];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        double t = v[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+        if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = 0;
+        for (int w = 0; w < kT / 64; w++) t += red[threadIdx.x][w];
+        part[(long)threadIdx.x * gridDim.x + blockIdx.x] = t;
+    }
+}
+
 #define DHIP(s, call)                                                                          \
     do {                                                                                       \
         hipError_t e_ = (call);                                                                \
@@ -409,6 +457,36 @@ int stz_step_direct(stz_t s, double dt, int nsteps, stz_step_stats *st) {
     DHIP(s, hipStreamSynchronize(s->stream));
     S.yielded_cells = nsteps ? cnt[0] : 0;
     if (st) *st = S;
+    return 0;
+}
+
+int stz_l2_comparison(stz_t a, stz_t b, double l2[3]) {
+    if (!a || !b || !l2) return STZMG_ERR_ARG;
+    const Grid &g = a->g;
+    if (b->g.Nx != g.Nx || b->g.Ny != g.Ny || b->g.Nz != g.Nz || b->g.pz != g.pz)
+        return stz_detail::fail(a, STZMG_ERR_ARG, "l2_comparison: the two simulations have different grids");
+    // b's fields must be complete before a's stream reads them
+    DHIP(a, hipStreamSynchronize(b->stream));
+    const long np = (long)g.Nx * g.Ny * (g.Nz + 1);
+    const unsigned nb = nbk(np);
+    double *part = a->stage;  // 3 nb partials + 3 sums: within 6 max(Nc, Nv) doubles
+    double *sums = part + 3 * (long)nb;
+    hipLaunchKernelGGL(k_l2_parts, dim3(nb), dim3(kT), 0, a->stream, g, (const double *)a->vel,
+                       (const double *)a->sig, (const double *)a->chi, (const double *)b->vel,
+                       (const double *)b->sig, (const double *)b->chi, part);
+    for (int q = 0; q < 3; q++)
+        hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kT), 0, a->stream, (const double *)(part + (long)q * nb),
+                           (long)nb, sums + q);
+    DHIP(a, hipGetLastError());
+    double h[3];
+    DHIP(a, hipMemcpyAsync(h, sums, sizeof(h), hipMemcpyDeviceToHost, a->stream));
+    DHIP(a, hipStreamSynchronize(a->stream));
+    // normalisation (:3152-3155) with a's grid, plate speed and chi_inf
+    const stz_params &p = a->p;
+    const double norm_val = g.dx * g.dy * g.dz / ((p.bx - p.ax) * (p.by - p.ay) * (p.bz - p.az));
+    l2[0] = h[0] * norm_val / (p.U * p.U);
+    l2[1] = h[1] * norm_val;
+    l2[2] = h[2] * norm_val / (p.chi_inf * p.chi_inf);
     return 0;
 }
 

@@ -273,6 +273,14 @@ class ShearSim:
         self.last_stats = {k: getattr(st, k) for k, _ in _lib.StzStepStats._fields_}
         return self.last_stats
 
+    def l2_comparison(self, other):
+        """Squared, normalised differences (velocity / U^2, stress, chi /
+        chi_inf^2) of this simulation against other on the same grid
+        (l2_comparison, src/shear_sim_3d.cpp:3122-3158)."""
+        out = np.zeros(3)
+        self._chk(self.L.stz_l2_comparison(self.h, other.h, _lib.dptr(out)))
+        return out
+
     def time_direct_kernels(self, dt, reps=20):
         """Average ms per launch of the direct step's kernels (plastic, cell,
         vertex, plate) on the current state, which they leave unchanged."""
@@ -327,5 +335,45 @@ class TransSim(ShearSim):
         return transform(self.family, self.Ut, t, self.params.get("bz", 0.5))
 
 
+def shear_compare(dsim, qsim, ncomps, time_interval, qdt, ddt=None, out_path=None, time_scale=1.0, **qs_kw):
+    """The comparison loop of shear_compare (src/shear_compare.cpp:182-217):
+    a direct simulation dsim against a quasi-static one qsim from their
+    current states.  At each of ncomps comparison times t0 + tt time_interval
+    dsim steps ddt (default: its direct_dt) while it stays below the target,
+    then one shortened step onto it; qsim takes int(time_interval / qdt) QS
+    steps; then l2_comparison.  With out_path, writes the reference's
+    sc<ii>.dat lines "%g %.12g %.12g %.12g" (time x time_scale and the square
+    roots), the first for the initial states.  Returns the rows."""
+    import math
+
+    ddt = dsim.direct_dt() if ddt is None else ddt
+    qs_steps = int(time_interval / qdt)
+    t0 = dsim.sim_time
+    rows = []
+
+    def record():
+        l2 = dsim.l2_comparison(qsim)
+        rows.append((dsim.sim_time * time_scale, math.sqrt(l2[0]), math.sqrt(l2[1]), math.sqrt(l2[2])))
+
+    record()
+    for tt in range(1, ncomps + 1):
+        target = t0 + tt * time_interval
+        t, n = dsim.sim_time, 0
+        while t + ddt * (1 + 1e-8) < target:
+            n += 1
+            t += ddt
+        if n:
+            dsim.step_forward(ddt, n)
+        dsim.step_forward(target - dsim.sim_time)
+        if qs_steps:
+            qsim.step_forward_qs(qdt, qs_steps, **qs_kw)
+        record()
+    if out_path is not None:
+        with open(out_path, "w") as f:
+            for r in rows:
+                f.write("%g %.12g %.12g %.12g\n" % r)
+    return rows
+
+
 __all__ = ["ShearSim", "TransSim", "transform", "TRANSFORMS", "reference_params", "config0_params",
-           "SOLVE_REF", "SOLVE_RELRES", "SOLVE_FP32_IR"]
+           "shear_compare", "SOLVE_REF", "SOLVE_RELRES", "SOLVE_FP32_IR"]

@@ -365,6 +365,65 @@ def direct_case(name):
     np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
 
 
+def l2_case(name):
+    """The reference's l2_comparison (src/shear_sim_3d.cpp:3122-3158) of a
+    direct simulation against a quasi-static one from the same state, as
+    shear_compare does it (src/shear_compare.cpp:196-205): 16 x 12 x 8 cells,
+    z walled or periodic, config-0 material with eps0 = 0.3; the QS run takes
+    2 steps of 0.01, the direct run (density 1, damping 0.35) steps of the
+    reference's dt up to the same time, the last one shortened to land on
+    it (z periodic: the QS run stays at the set state, since the reference's
+    projection loop does not end on the singular periodic-z operator).
+    Stored: both end states and the three squared differences."""
+    import tempfile
+
+    from stzpp_amd import shear_sim as SS
+
+    N = (16, 12, 8)
+    prd = (1, 1, 1) if name == "l2_periodic" else (1, 1, 0)
+    Nx, Ny, Nz = N
+    nc = Nx * Ny * Nz
+    nv = Nx * Ny * (Nz + (0 if prd[2] else 1))
+    p = SS.config0_params(16)
+    p["eps0"] = 0.3
+    P = {k: p[k] for k in O.STEP_PARAM_ORDER}
+    rng = np.random.default_rng(41 if prd[2] else 40)
+    sig0 = 0.05 * rng.standard_normal((nc, 6))
+    sig0[:, 1] += 0.9
+    u0 = 2e-3 * rng.standard_normal((nv, 3))
+    chi0 = 0.09 + 0.01 * rng.standard_normal(nc)
+    t0, qdt, nq = 0.3, 0.01, 2
+    rho, kappa, tmult = 1.0, 0.35, 0.5
+    dx = 2.0 / Nx
+    ddt = min(tmult * (dx * dx / 3.0 / kappa / 2.0), dx / 4.0)
+    q = O.RefShear(N, P, tempfile.mkdtemp(), prd=prd)
+    q.setup(qdt)
+    q.set_state(t0, u0, sig0, chi0)
+    d = O.RefShear.direct(N, P, tempfile.mkdtemp(), rho, kappa, prd=prd)
+    d.set_state(t0, u0, sig0, chi0)
+    d.init_ref()
+    l2_0 = d.l2_comparison(q)
+    target = t0 + nq * qdt
+    t, nd = t0, 0
+    while t + ddt * (1 + 1e-8) < target:
+        d.step_direct(ddt, 1)
+        t = d.state()["t"]
+        nd += 1
+    d.step_direct(target - t, 1)
+    if not prd[2]:  # the reference's projection loop does not end on a periodic-z grid
+        q.step(qdt, nq)
+    sd, sq = d.state(), q.state()
+    out = dict(N=np.array(N), prd=np.array(prd), t0=np.array(t0), qdt=np.array(qdt), nq=np.array(nq),
+               ddt=np.array(ddt), nd=np.array(nd), rho=np.array(rho), kappa=np.array(kappa),
+               tmult=np.array(tmult), u0=u0, sig0=sig0, chi0=chi0, l2_0=l2_0, l2=d.l2_comparison(q),
+               **{f"d_{k}": np.asarray(sd[k]) for k in ("t", "u", "sig", "chi")},
+               **{f"q_{k}": np.asarray(sq[k]) for k in ("t", "u", "sig", "chi")},
+               **{f"p_{k}": np.array(v) for k, v in P.items()})
+    q.close()
+    d.close()
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+
+
 CASES = {
     "shear_16x16x17": (("shear_16x16x17", (16, 16, 17), (1, 1, 0), "shear", 11),),
     "trans_16x16x16": (("trans_16x16x16", (16, 16, 16), (1, 1, 1), "trans", 12),),
@@ -388,12 +447,14 @@ if __name__ == "__main__":
             output_case(which)
         elif which.startswith("direct_"):
             direct_case(which)
+        elif which.startswith("l2_"):
+            l2_case(which)
         else:
             mg_case(*CASES[which][0])
         sys.exit(0)
     for key in ["trans", *CASES, "step_plastic", "step_c0", "trans_step_walled", "trans_step_periodic",
                 *GRF_CASES, "output_walled", "output_periodic", "output_trans", "direct_walled",
-                "direct_periodic"]:
+                "direct_periodic", "l2_walled", "l2_periodic"]:
         subprocess.run([sys.executable, __file__, key], check=True, cwd=ROOT,
                        stdout=subprocess.DEVNULL)
         print("wrote", key)

@@ -145,3 +145,48 @@ def test_gpu_direct_argument_errors(golden):
     tr = SS.TransSim(16, 12, 8, 0.2, params=params_of(g), prd=(1, 1, 0))
     with pytest.raises(SS.StepError):
         tr.step_forward(1e-3)
+
+
+# ---- l2_comparison and the shear_compare loop --------------------------------
+
+def l2_sims(g):
+    N, prd = tuple(int(v) for v in g["N"]), tuple(int(v) for v in g["prd"])
+    p = params_of(g)
+    d = SS.ShearSim(*N, params=p, prd=prd)
+    q = SS.ShearSim(*N, params=p, prd=prd)
+    return d, q
+
+
+@pytest.mark.parametrize("case", ["l2_walled", "l2_periodic"])
+def test_gpu_l2_comparison_is_the_reference(golden, case):
+    g = golden(f"{case}.npz")
+    d, q = l2_sims(g)
+    d.set_initial_conditions(g["d_chi"], u=g["d_u"], sig=g["d_sig"], t=float(g["d_t"]))
+    q.set_initial_conditions(g["q_chi"], u=g["q_u"], sig=g["q_sig"], t=float(g["q_t"]))
+    got = d.l2_comparison(q)
+    assert np.allclose(got, g["l2"], rtol=1e-13, atol=0), (got, g["l2"])
+    assert np.array_equal(d.l2_comparison(d), np.zeros(3))
+
+
+def test_gpu_shear_compare_loop_follows_the_reference(golden, tmp_path):
+    # direct: the reference's steps to t0 + 0.02 (5 full steps and a
+    # shortened one), bit-for-bit up to exp's last bit; QS: the GPU's
+    # 8-colour projection loop against the reference's lexicographic one
+    g = golden("l2_walled.npz")
+    d, q = l2_sims(g)
+    for s in (d, q):
+        s.set_initial_conditions(g["chi0"], u=g["u0"], sig=g["sig0"], t=float(g["t0"]))
+    d.init_ref()
+    rows = SS.shear_compare(d, q, 1, 2 * float(g["qdt"]), float(g["qdt"]), out_path=tmp_path / "sc0.dat")
+    st = d.state()
+    assert st["t"] == float(g["d_t"])
+    for key in ("u", "sig", "chi"):
+        assert close(st[key], g[f"d_{key}"]), key
+    sq = q.state()
+    assert abs(sq["t"] - float(g["q_t"])) < 1e-15
+    l2 = d.l2_comparison(q)
+    assert np.allclose(l2, g["l2"], rtol=1e-6), (l2, g["l2"])
+    lines = (tmp_path / "sc0.dat").read_text().splitlines()
+    assert len(lines) == 2 and len(rows) == 2
+    assert lines[0].split()[1:] == ["0", "0", "0"]
+    assert np.allclose([float(v) for v in lines[1].split()[1:]], np.sqrt(g["l2"]), rtol=1e-6)
