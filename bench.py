@@ -93,13 +93,15 @@ def step_workload(args):
     sim.set_random_initial_conditions(gauss_fac=5, conv_l=4.0, chi_avg=0.09, sigma=0.01, seed=3)
     dt = 0.005
     log(f"[bench] step workload {Nx}x{Ny}x{Nz}, warmup {args.warmup}")
+    graph = not args.no_graph
     for _ in range(args.warmup):
-        sim.step_forward_qs(dt, 1)
+        sim.step_forward_qs(dt, 1, use_graph=graph)
     t0 = time.perf_counter()
-    cyc = 0
+    cyc, solve_ms = 0, 0.0
     for _ in range(args.steps):
-        st = sim.step_forward_qs(dt, 1)
+        st = sim.step_forward_qs(dt, 1, use_graph=graph)
         cyc += st["vcycles"]
+        solve_ms += st["solve_ms"]
     el = time.perf_counter() - t0
     out = {
         "metric": "quasi-static steps per second, shear_sim_3d 192x192x96 cells, reference projection loop",
@@ -108,7 +110,8 @@ def step_workload(args):
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "shear_sim_3d QS step 192x192x96 cells (MG 192x192x97, z walled), config-0 "
                                "material, chi0 GRF seed 3 l=4, dt 0.005",
-                   "vcycles_per_step": cyc / args.steps},
+                   "vcycles_per_step": cyc / args.steps, "solve_ms_per_step": solve_ms / args.steps,
+                   "v_cycle_graph": graph},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

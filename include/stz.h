@@ -77,7 +77,9 @@ int stz_set_state(stz_t s, double t, const double *u, const double *sig, const d
 int stz_get_state(stz_t s, double *t, double *u, double *sig, double *chi, double *dpl);
 /* nsteps quasi-static steps of size dt.  opts == NULL selects the reference
  * projection loop (sum |r|^4 <= (eps*stencil[13].a11*1e3)^2, then two more
- * cycles, src/shear_sim_3d.cpp:551-600); otherwise the given solve mode. */
+ * cycles, src/shear_sim_3d.cpp:551-600), and so does opts with mode
+ * STZMG_SOLVE_REF and tol 0, taking only use_graph from opts; otherwise the
+ * given solve mode and options. */
 int stz_step_qs(stz_t s, double dt, int nsteps, const stzmg_solve_opts *opts, stz_step_stats *st);
 /* The projection solver of the simulation (valid after the first step). */
 stzmg_t stz_multigrid(stz_t s);

@@ -153,16 +153,27 @@ void stzmg_solve_opts_default(stzmg_solve_opts *o);
  *                            single-workgroup kernel per V-cycle; 1 selects
  *                            the built-in threshold, 0 launches every pass
  *                            separately. */
-enum { STZMG_OPT_COARSE_KERNEL = 0, STZMG_OPT_FUSED_SMOOTHER = 1, STZMG_OPT_TILED_RESIDUAL = 2 };
+enum { STZMG_OPT_COARSE_KERNEL = 0, STZMG_OPT_FUSED_SMOOTHER = 1, STZMG_OPT_TILED_RESIDUAL = 2,
+       STZMG_OPT_BOTTOM_MAP = 3 };
 /*   STZMG_OPT_FUSED_SMOOTHER 1 (default): per-plane levels with periodic x, y
- *                            (x and y sizes multiples of 16, more than 1024
+ *                            (x and y sizes multiples of 8, more than 1024
  *                            points) sweep with two fused launches (one per z
  *                            parity); 0: eight colour launches per sweep.
  *   STZMG_OPT_TILED_RESIDUAL 1 (default): on the same levels the residual
  *                            marches tiles through z with planes staged in
  *                            LDS; 0: one thread per point reading its 27
  *                            neighbours through the caches.  Sums may differ
- *                            in the last bits (another summation order). */
+ *                            in the last bits (another summation order).
+ *   STZMG_OPT_BOTTOM_MAP     1 (default): when the operator is set, the
+ *                            coarsest level's bottom sweeps (from x = 0, as
+ *                            every V-cycle enters that level) are tabulated
+ *                            as the matrix Q of order 3 N (N <= 1024 points,
+ *                            at most 75 MB), and each V-cycle applies x = Q r
+ *                            across the GPU instead of sweeping in one
+ *                            workgroup.  Building Q costs a few milliseconds
+ *                            per operator; 0 keeps the sweeps (for operators
+ *                            that change every solve).  Results differ in the
+ *                            last bits (another summation order). */
 int stzmg_set_option(stzmg_t h, int option, int value);
 
 /* The HIP stream (hipStream_t) the handle launches on. */

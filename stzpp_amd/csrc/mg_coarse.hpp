@@ -88,15 +88,8 @@ __device__ void wg_gs_sweep(const Dims &d, T *x, const T *r, const T *ops, const
 // summation order inside a point differs.  On exit x holds the result.
 constexpr int kBottomLanes = 16;
 template <typename T, int MODE>
-__device__ void wg_bottom_solve(const Dims &d, T *x, const T *r, const T *ops, const T *inv, int nsweeps, T *xs,
-                                T *rs) {
+__device__ void wg_bottom_sweeps(const Dims &d, const T *ops, const T *inv, int nsweeps, T *xs, const T *rs) {
     const long N = d.N;
-    const int n3 = 3 * (int)N;
-    for (int q = threadIdx.x; q < n3; q += blockDim.x) {
-        xs[q] = x[q];
-        rs[q] = r[q];
-    }
-    __syncthreads();
     const int sl = threadIdx.x % kBottomLanes, grp = threadIdx.x / kBottomLanes;
     const int ngrp = blockDim.x / kBottomLanes;
     const int pxo = d.px && (d.m & 1), pyo = d.py && (d.n & 1), pzo = d.pz && (d.o & 1);
@@ -145,8 +138,63 @@ __device__ void wg_bottom_solve(const Dims &d, T *x, const T *r, const T *ops, c
                     }
                     __syncthreads();
                 }
+}
+
+template <typename T, int MODE>
+__device__ void wg_bottom_solve(const Dims &d, T *x, const T *r, const T *ops, const T *inv, int nsweeps, T *xs,
+                                T *rs) {
+    const int n3 = 3 * (int)d.N;
+    for (int q = threadIdx.x; q < n3; q += blockDim.x) {
+        xs[q] = x[q];
+        rs[q] = r[q];
+    }
+    __syncthreads();
+    wg_bottom_sweeps<T, MODE>(d, ops, inv, nsweeps, xs, rs);
     for (int q = threadIdx.x; q < n3; q += blockDim.x) x[q] = xs[q];
     __syncthreads();
+}
+
+// The coarsest level is always entered with x = 0 (the restriction clears
+// it), and nsweeps colour sweeps from x = 0 are a fixed linear map of r:
+// x = Q r with Q of order 3 N.  k_bottom_map computes column j of Q -- the
+// sweeps applied to r = e_j -- in workgroup j with the same per-point update
+// as wg_bottom_solve, and stores it at q[i 3N + j]; k_bottom_apply then
+// replaces the single-workgroup bottom solve of every V-cycle by a
+// matrix-vector product spread over the GPU.  Dynamic LDS: the staged
+// per-plane tables (MODE 0; table_vals doubles) and x, r of the level.
+template <int MODE>
+__global__ __launch_bounds__(kCoarseThreads) void k_bottom_map(Dims d, const double *ops, const double *inv,
+                                                               int nsweeps, int table_vals, double *q) {
+    extern __shared__ double coarse_lds[];
+    const double *ops_s, *inv_s;
+    stage_tables<double, MODE>(d, ops, inv, coarse_lds, &ops_s, &inv_s);
+    const int n3 = 3 * (int)d.N;
+    double *xs = coarse_lds + table_vals, *rs = xs + n3;
+    const int j = blockIdx.x;
+    for (int t = threadIdx.x; t < n3; t += blockDim.x) {
+        xs[t] = 0.0;
+        rs[t] = t == j ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    wg_bottom_sweeps<double, MODE>(d, ops_s, inv_s, nsweeps, xs, rs);
+    for (int i = threadIdx.x; i < n3; i += blockDim.x) q[(long)i * n3 + j] = xs[i];
+}
+
+// x = Q r on the coarsest level (3 N values of type T, component-major):
+// one wave per row, FP64 sums.  Grid: ceil(3 N / 4) workgroups of 256.
+constexpr int kApplyThreads = 256;
+template <typename T>
+__global__ __launch_bounds__(kApplyThreads) void k_bottom_apply(const double *__restrict__ q,
+                                                                const T *__restrict__ r, T *__restrict__ x,
+                                                                int n3) {
+    const int row = blockIdx.x * (kApplyThreads / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (row >= n3) return;  // whole waves leave together
+    const double *qr = q + (long)row * n3;
+    double s = 0;
+    for (int j = lane; j < n3; j += 64) s += qr[j] * (double)r[j];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) x[row] = (T)s;
 }
 
 template <typename T, int MODE>

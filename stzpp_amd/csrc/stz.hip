@@ -120,6 +120,9 @@ int ensure_operator(stz_t s, double dt) {
         int rc = stzmg_create(&d, &s->mg);
         if (rc) return sfail(s, rc, "projection solver: %s", stzmg_last_error(nullptr));
         s->stream = (hipStream_t)stzmg_stream(s->mg);
+        // the transformed step sets a new operator every step: keep its
+        // bottom sweeps rather than tabulate them each time
+        if (s->p.transform != STZ_T_NONE) stzmg_set_option(s->mg, STZMG_OPT_BOTTOM_MAP, 0);
     }
     double B[243], F[9];
     stzmg_shear_stencil(s->p.mu, s->p.lamb, g.dx, g.dy, g.dz, dt, 0.0, B, F);
@@ -327,11 +330,14 @@ int stz_get_state(stz_t s, double *t, double *u, double *sig, double *chi, doubl
 namespace stz_detail {
 
 void step_options(stz_t s, const stzmg_solve_opts *opts, stzmg_solve_opts &o) {
-    if (opts) {
+    if (opts && !(opts->mode == STZMG_SOLVE_REF && !(opts->tol > 0))) {
         o = *opts;
         return;
     }
+    // the reference's loop; with opts (mode STZMG_SOLVE_REF, tol 0) its
+    // use_graph is kept
     stzmg_solve_opts_default(&o);
+    if (opts) o.use_graph = opts->use_graph;
     o.mode = STZMG_SOLVE_REF;
     // tol = (eps * stencil[13].a11 * 1e3)^2, src/shear_sim_3d.cpp:551-555
     const double tol = std::numeric_limits<double>::epsilon() * s->a11 * 1e3;

@@ -62,6 +62,10 @@ struct stzmg_s {
     long coarse_max = kCoarseMaxPoints;  // levels this small run in one workgroup (0: off)
     int fused_smoother = 1;   // two-launch fused sweeps on eligible levels
     int tiled_resid = 1;      // plane-marching residual on eligible levels
+    int bottom_map = 1;       // coarsest solve as x = Q r (mg_coarse.hpp)
+    double *qmap = nullptr;   // Q, row-major, qmap_n x qmap_n
+    long qmap_n = 0;
+    bool qmap_ok = false;     // Q belongs to the current operator
     // flexible GMRES workspace (allocated on first use)
     double *kV = nullptr;     // (restart + 1) FP64 basis vectors
     float *kZ = nullptr;      // restart FP32 preconditioned vectors
@@ -277,16 +281,20 @@ float *&xref_of<float>(Level &L) { return L.x32; }
 
 // Small levels (sides multiples of 16, e.g. 32^3 and 16^3) use 16 x 16
 // tiles with 128 threads: one thread per point of the widest sub-pass.
+// Sides that are multiples of 8 only (e.g. 24 x 24, a level of the
+// 192 x 192 grids) use 8 x 8 tiles with one wave.
 constexpr int kSmallTile = 16, kSmallThreads = 128;
+constexpr int kTinyTile = 8, kTinyThreads = 64;
 
 // Fused-smoother tile class of a level: 0 none, 1 the large tiles, 2 the
-// 16 x 16 tiles.
+// 16 x 16 tiles, 3 the 8 x 8 tiles.
 int fused_tile(const Level &L) {
     if (!(L.mode == STZMG_OP_PLANE && L.d.px && L.d.py && L.d.o >= 4 && L.d.N > 1024)) return 0;
     if (L.d.m % FusedTile<float>::TX == 0 && L.d.n % FusedTile<float>::TY == 0 &&
         L.d.m % FusedTile<double>::TX == 0 && L.d.n % FusedTile<double>::TY == 0)
         return 1;
     if (L.d.m % kSmallTile == 0 && L.d.n % kSmallTile == 0) return 2;
+    if (L.d.m % kTinyTile == 0 && L.d.n % kTinyTile == 0) return 3;
     return 0;
 }
 
@@ -312,8 +320,10 @@ template <typename T, int PAT>
 void launch_fused_pat(stzmg_t h, Level &L) {
     if (L.ftile == 1)
         launch_fused_tile<T, PAT, FusedTile<T>::TX, FusedTile<T>::TY, FusedTile<T>::NT>(h, L);
-    else
+    else if (L.ftile == 2)
         launch_fused_tile<T, PAT, kSmallTile, kSmallTile, kSmallThreads>(h, L);
+    else
+        launch_fused_tile<T, PAT, kTinyTile, kTinyTile, kTinyThreads>(h, L);
 }
 
 template <typename T>
@@ -412,7 +422,8 @@ int coarse_kernel_attrs(stzmg_t h) {
     static hipError_t done = [] {
         hipError_t e = hipSuccess;
         const void *fns[] = {(const void *)k_coarse_cycle<float, 0>, (const void *)k_coarse_cycle<float, 1>,
-                             (const void *)k_coarse_cycle<double, 0>, (const void *)k_coarse_cycle<double, 1>};
+                             (const void *)k_coarse_cycle<double, 0>, (const void *)k_coarse_cycle<double, 1>,
+                             (const void *)k_bottom_map<0>, (const void *)k_bottom_map<1>};
         for (const void *f : fns) {
             const hipError_t e1 = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kCoarseLdsMax);
             if (e1 != hipSuccess) e = e1;
@@ -478,7 +489,14 @@ void enqueue_vcycle(stzmg_t h) {
         for (int q = 0; q < sweeps_down(h, i); q++) launch_gs_sweep<T>(h, h->lv[i]);
         launch_restrict<T>(h, i + 1);
     }
-    if (Lc < L) {
+    if (h->qmap_ok && L > 1 && Lc >= L - 1) {
+        // the coarsest level, entered with x = 0: x = Q r
+        Level &B = h->lv[L - 1];
+        const int n3 = (int)(3 * B.d.N);
+        hipLaunchKernelGGL((k_bottom_apply<T>), dim3((n3 + kApplyThreads / 64 - 1) / (kApplyThreads / 64)),
+                           dim3(kApplyThreads), 0, h->stream, (const double *)h->qmap, (const T *)r_of<T>(B),
+                           x_of<T>(B), n3);
+    } else if (Lc < L) {
         launch_coarse<T>(h, Lc);
     } else {
         Level &B = h->lv[L - 1];
@@ -552,6 +570,37 @@ int require_ops(stzmg_t h) {
     return 0;
 }
 
+// Q of the coarsest level for the current operator (k_bottom_map): only for
+// a hierarchy of several levels whose coarsest level fits one workgroup.
+int build_bottom_map(stzmg_t h) {
+    h->qmap_ok = false;
+    if (!h->bottom_map || h->lv.size() < 2) return 0;
+    Level &B = h->lv.back();
+    if (B.d.N > kCoarseMaxPoints) return 0;
+    const bool plane = B.mode == STZMG_OP_PLANE;
+    const size_t tab = plane ? coarse_lds_bytes(B.d.o) : 0;
+    const size_t bytes = tab + bottom_lds_bytes(B.d.N);
+    if (bytes > kCoarseLdsMax) return 0;
+    const long n3 = 3 * B.d.N;
+    if (h->qmap_n != n3) {
+        hipFree(h->qmap);
+        h->qmap = nullptr;
+        h->qmap_n = 0;
+        HIPCHK(h, alloc(&h->qmap, n3 * n3));
+        h->qmap_n = n3;
+    }
+    const int tv = (int)(tab / sizeof(double));
+    if (plane)
+        hipLaunchKernelGGL(k_bottom_map<0>, dim3((unsigned)n3), dim3(kCoarseThreads), bytes, h->stream, B.d,
+                           (const double *)B.ops64, (const double *)B.inv64, h->desc.bottom_sweeps, tv, h->qmap);
+    else
+        hipLaunchKernelGGL(k_bottom_map<1>, dim3((unsigned)n3), dim3(kCoarseThreads), bytes, h->stream, B.d,
+                           (const double *)B.ops64, (const double *)B.inv64, h->desc.bottom_sweeps, tv, h->qmap);
+    LAUNCHCHK(h);
+    h->qmap_ok = true;
+    return 0;
+}
+
 // Galerkin hierarchy and inverse tables after the fine operator changed.
 int build_hierarchy_ops(stzmg_t h) {
     if (int rc = coarse_kernel_attrs(h)) return rc;
@@ -613,8 +662,9 @@ int build_hierarchy_ops(stzmg_t h) {
     if (h->g64) { hipGraphExecDestroy(h->g64); h->g64 = nullptr; }
     if (h->g32) { hipGraphExecDestroy(h->g32); h->g32 = nullptr; }
     h->ops_set = true;
+    h->qmap_ok = false;
     if (flag) return fail(h, STZMG_ERR_SINGULAR, "singular or non-finite centre block on some level");
-    return 0;
+    return build_bottom_map(h);
 }
 
 template <typename T>
@@ -753,6 +803,7 @@ int stzmg_destroy(stzmg_t h) {
     hipFree(h->dres);
     hipFree(h->dflag);
     hipFree(h->kV); hipFree(h->kZ); hipFree(h->ktmp); hipFree(h->kpart); hipFree(h->kdev);
+    hipFree(h->qmap);
     hipStreamDestroy(h->stream);
     delete h;
     return 0;
@@ -791,6 +842,11 @@ int stzmg_set_option(stzmg_t h, int option, int value) {
     } else if (option == STZMG_OPT_TILED_RESIDUAL) {
         h->tiled_resid = value != 0;
         for (auto &L : h->lv) L.tiled = h->ops_set && h->tiled_resid && tiled_ok(L);
+    } else if (option == STZMG_OPT_BOTTOM_MAP) {
+        h->bottom_map = value != 0;
+        if (h->ops_set) {
+            if (int rc = build_bottom_map(h)) return rc;
+        }
     } else {
         return fail(h, STZMG_ERR_ARG, "unknown option %d", option);
     }

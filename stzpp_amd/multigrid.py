@@ -19,7 +19,7 @@ from . import _lib
 PLANE, POINT = 0, 1
 NORM_QUARTIC, NORM_L2SQ = 0, 1
 SOLVE_REF, SOLVE_RELRES, SOLVE_FP32_IR, SOLVE_FP32_GMRES = 0, 1, 2, 3
-OPT_COARSE_KERNEL, OPT_FUSED_SMOOTHER, OPT_TILED_RESIDUAL = 0, 1, 2
+OPT_COARSE_KERNEL, OPT_FUSED_SMOOTHER, OPT_TILED_RESIDUAL, OPT_BOTTOM_MAP = 0, 1, 2, 3
 
 
 class MultigridError(RuntimeError):

@@ -154,16 +154,23 @@ class ShearSim:
         self._chk(self.L.stz_get_state(self.h, _lib.dptr(t), None, None, None, None))
         return float(t[0])
 
-    def step_forward_qs(self, dt, nsteps=1, solve=None, rtol=1e-12, inner_cycles=3):
+    def step_forward_qs(self, dt, nsteps=1, solve=None, rtol=1e-12, inner_cycles=3, use_graph=True):
         """nsteps QS steps.  solve=None runs the reference projection loop;
-        SOLVE_RELRES / SOLVE_FP32_IR solve each projection to rtol."""
+        SOLVE_RELRES / SOLVE_FP32_IR solve each projection to rtol.
+        use_graph=False launches the V-cycles kernel by kernel."""
         st = _lib.StzStepStats()
-        if solve is None or solve == SOLVE_REF:
+        if (solve is None or solve == SOLVE_REF) and use_graph:
             opts = None
+        elif solve is None or solve == SOLVE_REF:
+            o = _lib.StzmgSolveOpts()
+            self.L.stzmg_solve_opts_default(ctypes.byref(o))
+            o.mode, o.tol, o.use_graph = SOLVE_REF, 0.0, 0
+            opts = ctypes.byref(o)
         else:
             o = _lib.StzmgSolveOpts()
             self.L.stzmg_solve_opts_default(ctypes.byref(o))
             o.mode, o.rtol, o.inner_cycles, o.max_cycles = solve, rtol, inner_cycles, 10000
+            o.use_graph = 1 if use_graph else 0
             opts = ctypes.byref(o)
         self._chk(self.L.stz_step_qs(self.h, dt, nsteps, opts, ctypes.byref(st)))
         self.n_vcycles += st.vcycles

@@ -121,11 +121,12 @@ def test_gpu_vcycle_matches_oracle(golden, case):
 @pytest.mark.parametrize("kind,prd,n", [("shear", (1, 1, 0), 64), ("trans", (1, 1, 1), 64),
                                         ("transT", (1, 1, 0), 64), ("rat", (1, 1, 1), 64),
                                         ("shear", (1, 1, 0), 48), ("trans", (1, 1, 1), 32),
-                                        ("rat", (1, 1, 0), 16)])
+                                        ("rat", (1, 1, 0), 16), ("shear", (1, 1, 0), 24),
+                                        ("rat", (1, 1, 1), 24)])
 def test_gpu_fused_smoother_matches_oracle(kind, prd, n):
     # Two-launch fused sweeps against the oracle's 8-colour sweep and against
     # the eight-launch path, for each zero pattern; 64 x 64 planes use the
-    # large tiles, 48, 32 and 16 the 16 x 16 tiles.
+    # large tiles, 48, 32 and 16 the 16 x 16 tiles, 24 the 8 x 8 tiles.
     h = 1.0 / n
     o = n + (0 if prd[2] else 1)
     T = np.array([[1.0, 0.0, 0.5], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
@@ -229,6 +230,62 @@ def test_gpu_coarse_kernel_matches_launch_path(n, prd, fp32):
         xs.append(mg.x(0))
     # FP64: identical passes; FP32+IR: both reach 1e-9 relative residual
     assert relerr(xs[1], xs[0]) < (1e-7 if fp32 else 1e-12)
+
+
+@pytest.mark.parametrize("n,prd", [(32, (1, 1, 0)), (64, (1, 1, 1)), (48, (1, 1, 0))])
+@pytest.mark.parametrize("fp32", [False, True])
+def test_gpu_bottom_map_matches_sweeps(n, prd, fp32):
+    # x = Q r on the coarsest level (Q tabulated from the bottom sweeps)
+    # against the sweeps themselves: V-cycles agree to rounding, solves reach
+    # the same point in the same number of cycles.
+    h = 1.0 / n
+    T = np.array([[1.0, 0.0, 0.5], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+    st, fix = MG.shear_stencil(1.0, 1.5, h, h, h, 1.0)
+    if prd == (1, 1, 1):
+        st, fix = MG.trans_stencil(T, 1.0, 1.5, h, h, h, 1.0), None
+    o = n + (0 if prd[2] else 1)
+    xs, cyc = [], []
+    for opt in (0, 1):
+        mg = MG.Multigrid(n, n, o, prd)
+        mg.set_option(MG.OPT_BOTTOM_MAP, opt)
+        mg.setup_matrices_const(st, fix)
+        N = mg.npts(0)
+        b = O.det_field(N, 33)
+        if prd == (1, 1, 1):
+            b -= b.mean(axis=0)
+        mg.setup_fields(np.zeros((N, 3)), b)
+        if fp32:
+            s = mg.solve(mode=MG.SOLVE_FP32_GMRES, rtol=1e-10)
+        else:
+            mg.v_cycle(3)
+            s = mg.solve(mode=MG.SOLVE_RELRES, rtol=1e-11)
+        xs.append(mg.x(0))
+        cyc.append(s["cycles"])
+    assert cyc[0] == cyc[1]
+    assert relerr(xs[1], xs[0]) < (1e-8 if fp32 else 1e-11)
+
+
+def test_gpu_bottom_map_of_a_changed_operator():
+    # Setting a new operator rebuilds Q: after switching from one stencil to
+    # another, V-cycles equal those of a handle that only saw the second.
+    n = 32
+    h = 1.0 / n
+    a_st, a_fix = MG.shear_stencil(1.0, 1.5, h, h, h, 1.0)
+    b_st, b_fix = MG.shear_stencil(2.0, 0.5, h, h, h, 0.5)
+    N = n * n * (n + 1)
+    r = O.det_field(N, 34)
+    out = []
+    for first in (True, False):
+        mg = MG.Multigrid(n, n, n + 1, (1, 1, 0))
+        if first:
+            mg.setup_matrices_const(a_st, a_fix)
+            mg.setup_fields(np.zeros((N, 3)), r)
+            mg.v_cycle(2)
+        mg.setup_matrices_const(b_st, b_fix)
+        mg.setup_fields(np.zeros((N, 3)), r)
+        mg.v_cycle(3)
+        out.append(mg.x(0))
+    assert np.array_equal(out[0], out[1])
 
 
 def test_gpu_vcycle_converges_like_reference(golden):
