@@ -164,7 +164,8 @@ enum { STZMG_OPT_COARSE_KERNEL = 0, STZMG_OPT_FUSED_SMOOTHER = 1, STZMG_OPT_TILE
  *                            LDS; 0: one thread per point reading its 27
  *                            neighbours through the caches.  Sums may differ
  *                            in the last bits (another summation order).
- *   STZMG_OPT_BOTTOM_MAP     1 (default): when the operator is set, the
+ *   STZMG_OPT_BOTTOM_MAP     1 (default): at the first V-cycle after the
+ *                            operator is set, the
  *                            coarsest level's bottom sweeps (from x = 0, as
  *                            every V-cycle enters that level) are tabulated
  *                            as the matrix Q of order 3 N (N <= 1024 points,

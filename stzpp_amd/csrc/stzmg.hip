@@ -287,15 +287,29 @@ constexpr int kSmallTile = 16, kSmallThreads = 128;
 constexpr int kTinyTile = 8, kTinyThreads = 64;
 
 // Fused-smoother tile class of a level: 0 none, 1 the large tiles, 2 the
-// 16 x 16 tiles, 3 the 8 x 8 tiles.
+// 16 x 16 tiles, 3 the 8 x 8 tiles.  The largest tile that divides the
+// plane is taken unless one launch would then have fewer than
+// STZ_FUSED_MIN_GROUPS workgroups (tiles times planes of one parity) and a
+// smaller tile divides the plane too.
+#ifndef STZ_FUSED_MIN_GROUPS
+#define STZ_FUSED_MIN_GROUPS 0
+#endif
 int fused_tile(const Level &L) {
     if (!(L.mode == STZMG_OP_PLANE && L.d.px && L.d.py && L.d.o >= 4 && L.d.N > 1024)) return 0;
+    const long nP = (L.d.o + 1) / 2;
+    auto groups = [&](int tx, int ty) { return (long)(L.d.m / tx) * (L.d.n / ty) * nP; };
+    int best = 0;
     if (L.d.m % FusedTile<float>::TX == 0 && L.d.n % FusedTile<float>::TY == 0 &&
-        L.d.m % FusedTile<double>::TX == 0 && L.d.n % FusedTile<double>::TY == 0)
-        return 1;
-    if (L.d.m % kSmallTile == 0 && L.d.n % kSmallTile == 0) return 2;
-    if (L.d.m % kTinyTile == 0 && L.d.n % kTinyTile == 0) return 3;
-    return 0;
+        L.d.m % FusedTile<double>::TX == 0 && L.d.n % FusedTile<double>::TY == 0) {
+        best = 1;
+        if (groups(FusedTile<float>::TX, FusedTile<float>::TY) >= STZ_FUSED_MIN_GROUPS) return best;
+    }
+    if (L.d.m % kSmallTile == 0 && L.d.n % kSmallTile == 0) {
+        best = 2;
+        if (groups(kSmallTile, kSmallTile) >= STZ_FUSED_MIN_GROUPS) return best;
+    }
+    if (L.d.m % kTinyTile == 0 && L.d.n % kTinyTile == 0) best = 3;
+    return best;
 }
 
 template <typename T, int PAT, int TX, int TY, int NT>
@@ -572,6 +586,8 @@ int require_ops(stzmg_t h) {
 
 // Q of the coarsest level for the current operator (k_bottom_map): only for
 // a hierarchy of several levels whose coarsest level fits one workgroup.
+// Called before the first V-cycle after the operator changed; a level that
+// does not qualify leaves qmap_ok false and is checked again (no GPU work).
 int build_bottom_map(stzmg_t h) {
     h->qmap_ok = false;
     if (!h->bottom_map || h->lv.size() < 2) return 0;
@@ -662,9 +678,9 @@ int build_hierarchy_ops(stzmg_t h) {
     if (h->g64) { hipGraphExecDestroy(h->g64); h->g64 = nullptr; }
     if (h->g32) { hipGraphExecDestroy(h->g32); h->g32 = nullptr; }
     h->ops_set = true;
-    h->qmap_ok = false;
+    h->qmap_ok = false;  // tabulated at the first V-cycle with this operator
     if (flag) return fail(h, STZMG_ERR_SINGULAR, "singular or non-finite centre block on some level");
-    return build_bottom_map(h);
+    return 0;
 }
 
 template <typename T>
@@ -683,6 +699,9 @@ int capture_vcycle(stzmg_t h, hipGraphExec_t *exec) {
 
 template <typename T>
 int run_vcycles(stzmg_t h, int n, bool graph) {
+    // Q of the current operator, before any capture that would use it
+    if (h->bottom_map && !h->qmap_ok)
+        if (int rc = build_bottom_map(h)) return rc;
     if (graph) {
         hipGraphExec_t *ex = sizeof(T) == 8 ? &h->g64 : &h->g32;
         int rc = capture_vcycle<T>(h, ex);
@@ -844,9 +863,7 @@ int stzmg_set_option(stzmg_t h, int option, int value) {
         for (auto &L : h->lv) L.tiled = h->ops_set && h->tiled_resid && tiled_ok(L);
     } else if (option == STZMG_OPT_BOTTOM_MAP) {
         h->bottom_map = value != 0;
-        if (h->ops_set) {
-            if (int rc = build_bottom_map(h)) return rc;
-        }
+        h->qmap_ok = false;  // (re)built at the next V-cycle when on
     } else {
         return fail(h, STZMG_ERR_ARG, "unknown option %d", option);
     }
