@@ -110,12 +110,11 @@ static __global__ __launch_bounds__(kT) void k_direct_cell(Grid g, Mat M, double
                                                            const double *__restrict__ dchi,
                                                            const double *__restrict__ dnew,
                                                            const double *__restrict__ dold,
-                                                           double *__restrict__ sig_out, double *__restrict__ chi_out) {
-    const long c = xcd_block() * kT + threadIdx.x;
-    if (c >= g.Nc) return;
-    const int i = (int)(c % g.Nx);
-    const int j = (int)((c / g.Nx) % g.Ny);
-    const int k = (int)(c / ((long)g.Nx * g.Ny));
+                                                           double *__restrict__ sig_out, double *__restrict__ chi_out,
+                                                           BrickGrid B) {
+    int i, j, k;
+    if (!brick_point(B, g.Nx, g.Ny, i, j, k)) return;
+    const long c = i + (long)g.Nx * (j + (long)g.Ny * k);
     double G[3][3], v[3];
     vel_grad(vel, g, i, j, k, G, v);
     const double dtlamb = dt * M.lamb, twomudt = 2 * M.mu * dt, dtmu = dt * M.mu;
@@ -200,12 +199,10 @@ static __global__ __launch_bounds__(kT) void k_direct_vertex(Grid g, Dyn D, doub
                                                              const double *__restrict__ sig,
                                                              const double *__restrict__ xi,
                                                              double *__restrict__ vel_out,
-                                                             double *__restrict__ xi_out) {
-    const long p = xcd_block() * kT + threadIdx.x;
-    if (p >= g.Nv) return;
-    const int i = (int)(p % g.Nx);
-    const int j = (int)((p / g.Nx) % g.Ny);
-    const int k = (int)(p / ((long)g.Nx * g.Ny));
+                                                             double *__restrict__ xi_out, BrickGrid B) {
+    int i, j, k;
+    if (!brick_point(B, g.Nx, g.Ny, i, j, k)) return;
+    const long p = i + (long)g.Nx * (j + (long)g.Ny * k);
     if (k == 0 || k >= g.Nz) {
         // the plane zz = 0 and the top plate are not updated here
 #pragma unroll
@@ -428,17 +425,19 @@ int stz_step_direct(stz_t s, double dt, int nsteps, stz_step_stats *st) {
     if (int rc = ensure_ref_map(s)) return rc;
     const Grid &g = s->g;
     const Dyn D = dyn_of(s);
+    const BrickGrid bc = brick_grid(g.Nx, g.Ny, g.Nz), bv = brick_grid(g.Nx, g.Ny, g.Nzv);
     stz_step_stats S{};
     for (int q = 0; q < nsteps; q++) {
         DHIP(s, hipMemsetAsync(s->counters, 0, sizeof(int) * 4, s->stream));
         hipLaunchKernelGGL(k_direct_plastic, dim3(nbk(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt,
                            (const double *)s->sig, (const double *)s->chi, s->adapt, s->dchi, s->dpl2, s->counters);
-        hipLaunchKernelGGL(k_direct_cell, dim3(nbk(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt,
+        hipLaunchKernelGGL(k_direct_cell, dim3(brick_blocks(bc)), dim3(kT), 0, s->stream, g, s->M, dt,
                            (const double *)s->vel, (const double *)s->sig, (const double *)s->chi,
                            (const double *)s->adapt, (const double *)s->dchi, (const double *)s->dpl2,
-                           (const double *)s->dpl, s->sig2, s->chi2);
-        hipLaunchKernelGGL(k_direct_vertex, dim3(nbk(g.Nv)), dim3(kT), 0, s->stream, g, D, dt,
-                           (const double *)s->vel, (const double *)s->sig, (const double *)s->xi, s->vel2, s->xi2);
+                           (const double *)s->dpl, s->sig2, s->chi2, bc);
+        hipLaunchKernelGGL(k_direct_vertex, dim3(brick_blocks(bv)), dim3(kT), 0, s->stream, g, D, dt,
+                           (const double *)s->vel, (const double *)s->sig, (const double *)s->xi, s->vel2, s->xi2,
+                           bv);
         DHIP(s, hipGetLastError());
         std::swap(s->sig, s->sig2);
         std::swap(s->chi, s->chi2);
@@ -496,6 +495,7 @@ int stz_direct_time_kernels(stz_t s, double dt, int reps, double ms[4]) {
     if (int rc = ensure_ref_map(s)) return rc;
     const Grid &g = s->g;
     const Dyn D = dyn_of(s);
+    const BrickGrid bc = brick_grid(g.Nx, g.Ny, g.Nz), bv = brick_grid(g.Nx, g.Ny, g.Nzv);
     hipEvent_t e0, e1;
     DHIP(s, hipEventCreate(&e0));
     DHIP(s, hipEventCreate(&e1));
@@ -508,14 +508,14 @@ int stz_direct_time_kernels(stz_t s, double dt, int reps, double ms[4]) {
                                (const double *)s->sig, (const double *)s->chi, s->adapt, s->dchi, s->dpl2,
                                s->counters);
         else if (which == 1)
-            hipLaunchKernelGGL(k_direct_cell, dim3(nbk(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt,
+            hipLaunchKernelGGL(k_direct_cell, dim3(brick_blocks(bc)), dim3(kT), 0, s->stream, g, s->M, dt,
                                (const double *)s->vel, (const double *)s->sig, (const double *)s->chi,
                                (const double *)s->adapt, (const double *)s->dchi, (const double *)s->dpl2,
-                               (const double *)s->dpl, s->sig2, s->chi2);
+                               (const double *)s->dpl, s->sig2, s->chi2, bc);
         else if (which == 2)
-            hipLaunchKernelGGL(k_direct_vertex, dim3(nbk(g.Nv)), dim3(kT), 0, s->stream, g, D, dt,
+            hipLaunchKernelGGL(k_direct_vertex, dim3(brick_blocks(bv)), dim3(kT), 0, s->stream, g, D, dt,
                                (const double *)s->vel, (const double *)s->sig, (const double *)s->xi, s->vel2,
-                               s->xi2);
+                               s->xi2, bv);
         else
             hipLaunchKernelGGL(k_direct_plate, dim3(nbk((long)g.Nx * g.Ny)), dim3(kT), 0, s->stream, g, D, s->t,
                                s->vel, s->xi);

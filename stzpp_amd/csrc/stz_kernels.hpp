@@ -42,6 +42,53 @@ __device__ __forceinline__ long xcd_block() {
     return (long)(x * q + (x < r ? x : r) + (b >> 3));
 }
 
+// Brick order of a 3-D stencil launch: each workgroup of kT threads takes a
+// kBT x kBT tile of one plane, and consecutive logical workgroups
+// (xcd_block) form bricks of kBX x kBX tiles x kBZ planes, bricks ordered
+// x, y, z.  The workgroups resident on one XCD then cover a compact block
+// whose stencil neighbourhood (two planes and two points around it) stays in
+// that XCD's L2, and the y neighbours of a tile sit in its own rows.
+#ifndef STZ_BRICK_X
+#define STZ_BRICK_X 4
+#endif
+#ifndef STZ_BRICK_Z
+#define STZ_BRICK_Z 2
+#endif
+constexpr int kBT = 16, kBX = STZ_BRICK_X, kBZ = STZ_BRICK_Z;
+static_assert(kBT * kBT == kT, "one thread per tile point");
+struct BrickGrid {
+    int ntx, nty, nz;    // tiles per plane in x and y, planes
+    int nbx, nby;        // bricks in x and y
+};
+inline BrickGrid brick_grid(int nx, int ny, int nz) {
+    BrickGrid b;
+    b.ntx = (nx + kBT - 1) / kBT;
+    b.nty = (ny + kBT - 1) / kBT;
+    b.nz = nz;
+    b.nbx = (b.ntx + kBX - 1) / kBX;
+    b.nby = (b.nty + kBX - 1) / kBX;
+    return b;
+}
+inline unsigned brick_blocks(const BrickGrid &b) {
+    const long nbz = (b.nz + kBZ - 1) / kBZ;
+    return (unsigned)((long)b.nbx * b.nby * nbz * (kBX * kBX * kBZ));
+}
+// (i, j, k) of this thread; false when it lies outside nx x ny x nz.  Lane 0
+// of a wave is inside whenever any of its lanes is (a wave covers four rows
+// of a tile from its first column).
+__device__ __forceinline__ bool brick_point(const BrickGrid &b, int nx, int ny, int &i, int &j, int &k) {
+    const long L = xcd_block();
+    const int per = kBX * kBX * kBZ;
+    const long br = L / per;
+    const int w = (int)(L - br * per);
+    const int bi = (int)(br % b.nbx), bj = (int)((br / b.nbx) % b.nby), bk = (int)(br / ((long)b.nbx * b.nby));
+    const int ti = bi * kBX + w % kBX, tj = bj * kBX + (w / kBX) % kBX;
+    k = bk * kBZ + w / (kBX * kBX);
+    i = ti * kBT + (int)(threadIdx.x % kBT);
+    j = tj * kBT + (int)(threadIdx.x / kBT);
+    return ti < b.ntx && tj < b.nty && k < b.nz && i < nx && j < ny;
+}
+
 // Periodic index for -n <= i < 2n: every stencil here reaches at most two
 // points past an edge and stz_create requires n >= 4, so one correction each
 // way suffices (selects, no loop).
