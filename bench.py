@@ -228,6 +228,14 @@ def adjoint_workload(args):
 
 
 DIRECT_N = (192, 192, 96)
+# HBM bytes per launch of the direct step's cell and vertex kernels at
+# 192 x 192 x 96, from two rocprofv3 --pmc runs of
+# `python3 bench.py --workload direct --steps 20 --warmup 2 --no-cpu`
+# (FETCH_SIZE x 2.0 + WRITE_SIZE, the copy-kernel factors of the solve
+# profile): cell 448.5 MiB x 2 + 189.0 MiB, vertex 397.0 MiB x 2 + 163.7 MiB.
+# Summary: profiles/direct_step_192x192x96_kernels.txt.
+TRAFFIC_DIRECT_BYTES_PER_LAUNCH = {"k_direct_cell": (448.5 * 2 + 189.0) * 2 ** 20,
+                                   "k_direct_vertex": (397.0 * 2 + 163.7) * 2 ** 20}
 
 
 def direct_workload(args):
@@ -235,8 +243,10 @@ def direct_workload(args):
     cells (vertices 192 x 192 x 97, z walled): config-0 material, density 1,
     viscous damping 0.35, the reference's dt = min(tmult dx^2 / (6 kappa),
     dx / 4) with tmult 0.5; chi0 = 0.09 + 0.01 GRF (seed 3, l = 4 cells),
-    s12 pre-stress 1.05 (every cell above yield: the plastic path
-    everywhere), at rest.  value = steps per second over K steps in one call.
+    s12 pre-stress 1.05, at rest: every cell yields in the first step and
+    relaxes onto the yield surface within it, so the timed steps run near
+    yield (the last step's yielding count is reported).  value = steps per
+    second over K steps in one call.
     The four kernels are timed with HIP events on the handle's stream.  CPU
     baseline: the oracle's step (bit-exact to the reference's step_forward
     where no exp enters) on one thread, on the same grid."""
@@ -295,8 +305,9 @@ def direct_workload(args):
                    "dt": dt, "yielded_cells": st["yielded_cells"],
                    "kernel_ms": dict(zip(["plastic", "cell", "vertex", "plate"], [float(v) for v in ms]))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": names[dom],
-                     "alg_bytes_per_launch": alg[dom], "launch_us": float(ms[dom]) * 1e3},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": TRAFFIC_DIRECT_BYTES_PER_LAUNCH.get(names[dom]) if (Nx, Ny, Nz) == DIRECT_N else None,
+                     "kernel": names[dom], "alg_bytes_per_launch": alg[dom], "launch_us": float(ms[dom]) * 1e3},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
