@@ -227,6 +227,110 @@ def adjoint_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def trans_workload(args):
+    """QS steps of trans_sim_3d at the trans_shear_sim driver's size,
+    256 x 256 x 128 cells, fully periodic (src/trans_shear_sim.cpp:19-21,
+    :96): pure shear T = diag(e^{Ut t}, 1, e^{-Ut t}) at Ut = 0.5, config-0
+    material with eps0 = 0.3, chi0 = 0.09 + 0.01 GRF (seed 3, l = 4), dt =
+    0.005, the reference's projection loop (norm on the std_iters schedule).
+    Every step rebuilds the transformed operator from T(t + dt), its Galerkin
+    hierarchy (RAT) and inverse tables on the GPU before the solve.  value =
+    steps per second."""
+    from stzpp_amd import shear_sim as SS
+
+    N = (256, 256, 128)
+    p = SS.config0_params(N[0])
+    p["eps0"] = 0.3
+    sim = SS.TransSim(*N, 0.5, family="pure_shear", params=p, prd=(1, 1, 1))
+    sim.set_random_initial_conditions(gauss_fac=5, conv_l=4.0, chi_avg=0.09, sigma=0.01, seed=3)
+    dt = 0.005
+    graph = not args.no_graph
+    log(f"[bench] trans workload {N}, warmup {args.warmup}")
+    for _ in range(args.warmup):
+        sim.step_forward_qs(dt, 1, use_graph=graph)
+    t0 = time.perf_counter()
+    cyc, solve_ms = 0, 0.0
+    for _ in range(args.steps):
+        st = sim.step_forward_qs(dt, 1, use_graph=graph)
+        cyc += st["vcycles"]
+        solve_ms += st["solve_ms"]
+    el = time.perf_counter() - t0
+    out = {
+        "metric": "transformed QS steps per second, trans_sim_3d 256x256x128 cells fully periodic, per-step "
+                  "operator and Galerkin rebuild, reference projection loop",
+        "value": args.steps / el, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "trans_sim_3d QS step 256x256x128 cells (MG 256x256x128, periodic), pure shear "
+                               "Ut 0.5, config-0 material eps0 0.3, chi0 GRF seed 3 l=4, dt 0.005",
+                   "vcycles_per_step": cyc / args.steps, "solve_ms_per_step": solve_ms / args.steps,
+                   "v_cycle_graph": graph},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def output_workload(args):
+    """The reference's output cubes (write_files modes 0-12: u v w, the six
+    stresses, p, dev, chi, tem) of a 192 x 192 x 96 state (vertices
+    192 x 192 x 97, z walled): float32, x slowest and z fastest, through
+    stz_output_cube (device kernel, copy to host).  value = cubes per second
+    of the API calls; each mode's kernel is timed on the device beside it.
+    CPU baseline: the oracle's numpy restatement of the same cubes on one
+    core."""
+    from stzpp_amd import shear_sim as SS
+
+    Nx, Ny, Nz = 192, 192, 96
+    nc, nv = Nx * Ny * Nz, Nx * Ny * (Nz + 1)
+    sim = SS.ShearSim(Nx, Ny, Nz, params=SS.config0_params(Nx))
+    rng = np.random.default_rng(4)
+    u0, sig0, chi0 = 1e-3 * rng.standard_normal((nv, 3)), rng.standard_normal((nc, 6)), 0.09 + 0.01 * rng.random(nc)
+    sim.set_initial_conditions(chi0, u=u0, sig=sig0, t=0.25)
+    modes = list(range(13))
+    log(f"[bench] output workload {Nx}x{Ny}x{Nz}, {len(modes)} cubes")
+    for _ in range(args.warmup):
+        for m in modes:
+            sim.output_cube(m)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for m in modes:
+            sim.output_cube(m)
+    el = time.perf_counter() - t0
+    ms = np.array([sim.time_cube_kernel(m, reps=20) for m in modes])
+    # algorithmic bytes per launch: the FP64 values read once, the FP32 cube
+    # written once (p reads three stresses, dev six)
+    alg = np.array([nv * 12.0 if m <= 2 else nc * (28.0 if m == 9 else (52.0 if m == 10 else 12.0))
+                    for m in modes])
+    dom = int(np.argmax(ms))
+    achieved = alg[dom] / (ms[dom] * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+
+        s = sim.state()
+        t0 = time.perf_counter()
+        n_cpu = 0
+        while time.perf_counter() - t0 < 10.0 and n_cpu < len(modes):
+            O.cube_values(modes[n_cpu], (Nx, Ny, Nz), (1, 1, 0), s["u"], s["sig"], s["chi"])
+            n_cpu += 1
+        el_cpu = time.perf_counter() - t0
+        cpu = {"value": n_cpu / el_cpu, "unit": "cubes/s", "cores": 1, "kind": "port",
+               "sample": f"{n_cpu} cubes of the same state by the oracle's numpy restatement: {el_cpu:.1f} s"}
+    out = {
+        "metric": "output cubes per second (write_files modes 0-12, float32 to host), 192x192x96 cells",
+        "value": args.steps * len(modes) / el, "unit": "cubes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64->f32", "data": "synthetic",
+        "config": {"workload": "output_cube modes 0-12 of a 192x192x96 state (vertices 192x192x97, z walled)",
+                   "kernel_ms": {str(m): float(v) for m, v in zip(modes, ms)}},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": f"k_cube (mode {modes[dom]})",
+                     "alg_bytes_per_launch": float(alg[dom]), "launch_us": float(ms[dom]) * 1e3},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 DIRECT_N = (192, 192, 96)
 # HBM bytes per launch of the direct step's cell and vertex kernels at
 # 192 x 192 x 96, from two rocprofv3 --pmc runs of
@@ -324,10 +428,11 @@ def main():
     ap.add_argument("--restart", type=int, default=10, help="GMRES restart length")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch V-cycles kernel by kernel instead of replaying a hipGraph")
-    ap.add_argument("--workload", choices=("solve", "step", "adjoint", "grf", "direct"), default="solve",
+    ap.add_argument("--workload", choices=("solve", "step", "adjoint", "grf", "direct", "trans", "output"), default="solve",
                     help="solve (default): the 256^3 projection solve; step: QS steps at 192x192x96; "
                          "adjoint: gradient of 20 QS steps at 128^3; grf: random initial chi at 192x192x96; "
-                         "direct: explicit dynamics steps at 192x192x96")
+                         "direct: explicit dynamics steps at 192x192x96; trans: transformed QS steps at 256x256x128; "
+                         "output: the reference's output cubes at 192x192x96")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -341,6 +446,10 @@ def main():
         return grf_workload(args)
     if args.workload == "direct":
         return direct_workload(args)
+    if args.workload == "trans":
+        return trans_workload(args)
+    if args.workload == "output":
+        return output_workload(args)
 
     from stzpp_amd import multigrid as MG
 

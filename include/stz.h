@@ -172,6 +172,10 @@ int stz_set_random_initial_conditions(stz_t s, const stz_grf_params *g, const do
  * bz - az), Nx*Ny*Nz otherwise. */
 long long stz_cube_size(stz_t s, int mode);   /* -1 for an unknown mode */
 int stz_output_cube(stz_t s, int mode, float *out, long long n);
+/* Average device time per launch (ms) of the cube kernel of `mode` over
+ * reps back-to-back launches into the handle's staging buffer, from HIP
+ * events on its stream (for measurement). */
+int stz_output_time_kernel(stz_t s, int mode, int reps, double *ms);
 /* Appends the cube to <dir>/<name>_grid.<frame> (name: u v w s11 ... tem),
  * creating dir (one level) as the reference does. */
 int stz_write_cube(stz_t s, const char *dir, int mode, int frame);

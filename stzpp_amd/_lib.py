@@ -100,6 +100,7 @@ SIGNATURES = {
     "stz_grf_last_error": ([], ctypes.c_char_p),
     "stz_set_random_initial_conditions": ([_vp, ctypes.POINTER(StzGrfParams), _dp, ctypes.c_char_p], ctypes.c_int),
     "stz_cube_size": ([_vp, ctypes.c_int], ctypes.c_longlong),
+    "stz_output_time_kernel": ([_vp, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
     "stz_output_cube": ([_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_longlong], ctypes.c_int),
     "stz_write_cube": ([_vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "stz_write_files": ([_vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),

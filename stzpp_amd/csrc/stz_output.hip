@@ -311,6 +311,37 @@ int stz_set_output(stz_t s, const char *dir, int flags) {
     return 0;
 }
 
+int stz_output_time_kernel(stz_t s, int mode, int reps, double *ms) {
+    if (!s || reps < 1 || !ms) return STZMG_ERR_ARG;
+    long n = 0;
+    if (int rc = cube_size(s, mode, &n)) return rc;
+    const Grid &g = s->g;
+    const int nzo = (int)(n / ((long)g.Nx * g.Ny));
+    float *d = reinterpret_cast<float *>(s->stage);
+    const dim3 grid((g.Nx + kTile - 1) / kTile, g.Ny, (nzo + kTile - 1) / kTile);
+    hipEvent_t e0, e1;
+    OHIP(s, hipEventCreate(&e0));
+    OHIP(s, hipEventCreate(&e1));
+    auto launch = [&] {
+        hipLaunchKernelGGL(k_cube, grid, dim3(256), 0, s->stream, g, mode, nzo, (const double *)s->vel,
+                           (const double *)s->sig, (const double *)s->chi, (const double *)s->xi,
+                           s->p.bz - s->p.az, d);
+    };
+    launch();  // warm
+    hipEventRecord(e0, s->stream);
+    for (int r = 0; r < reps; r++) launch();
+    hipEventRecord(e1, s->stream);
+    hipError_t e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipGetLastError();
+    float t = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (e != hipSuccess) return stz_detail::fail(s, STZMG_ERR_HIP, "cube kernel timing: %s", hipGetErrorString(e));
+    *ms = (double)t / reps;
+    return 0;
+}
+
 int stz_write_files(stz_t s, const char *dir, int frame, int n_tp, int fflags) {
     if (!s) return STZMG_ERR_ARG;
     if (n_tp < 10 || frame < 0)

@@ -215,6 +215,12 @@ class ShearSim:
         self._chk(self.L.stz_output_cube(self.h, int(mode), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n))
         return out
 
+    def time_cube_kernel(self, mode, reps=20):
+        """Average ms per launch of the cube kernel of `mode` (device only)."""
+        v = np.zeros(1)
+        self._chk(self.L.stz_output_time_kernel(self.h, int(mode), int(reps), _lib.dptr(v)))
+        return float(v[0])
+
     def write_cube(self, out_dir, mode, frame):
         """Append the cube to <out_dir>/<name>_grid.<frame>."""
         self._chk(self.L.stz_write_cube(self.h, str(out_dir).encode(), int(mode), int(frame)))
