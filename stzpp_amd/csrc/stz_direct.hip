@@ -115,21 +115,34 @@ static __global__ __launch_bounds__(kT) void k_direct_cell(Grid g, Mat M, double
     int i, j, k;
     if (!brick_point(B, g.Nx, g.Ny, i, j, k)) return;
     const long c = i + (long)g.Nx * (j + (long)g.Ny * k);
+    const Stencil5 S = stencil5(g, i, j, k, g.Nz);
+    // away from the walls (uniform over a brick tile, which is one plane)
+    const bool zin = g.pz || (k >= 2 && k <= g.Nz - 3);
     double G[3][3], v[3];
-    vel_grad(vel, g, i, j, k, G, v);
+    vel_grad_o(vel, g, c, S.ox[3], S.oy[3], S.oz[3], G, v);
     const double dtlamb = dt * M.lamb, twomudt = 2 * M.mu * dt, dtmu = dt * M.mu;
     const double hx = g.dx / dt, hy = g.dy / dt, hz = g.dz / dt;
     double ug[7];
 #pragma unroll
     for (int q = 0; q < 7; q++) {
         const double *f = q < 6 ? sig + q * g.Nc : chi;
-        const double f0 = f[c];
-        const double ddx = eno2(hx, v[0], cellv(f, g, i + 2, j, k), cellv(f, g, i + 1, j, k), f0,
-                                cellv(f, g, i - 1, j, k), cellv(f, g, i - 2, j, k));
-        const double ddy = eno2(hy, v[1], cellv(f, g, i, j + 2, k), cellv(f, g, i, j + 1, k), f0,
-                                cellv(f, g, i, j - 1, k), cellv(f, g, i, j - 2, k));
-        const double ddz = eno2(hz, v[2], cellv(f, g, i, j, k + 2), cellv(f, g, i, j, k + 1), f0,
-                                cellv(f, g, i, j, k - 1), cellv(f, g, i, j, k - 2));
+        const double *fc = f + c;
+        const double f0 = fc[0];
+        const double ddx = eno2(hx, v[0], fc[S.ox[4]], fc[S.ox[3]], f0, fc[S.ox[1]], fc[S.ox[0]]);
+        const double ddy = eno2(hy, v[1], fc[S.oy[4]], fc[S.oy[3]], f0, fc[S.oy[1]], fc[S.oy[0]]);
+        double z2p, z1p, z1m, z2m;
+        if (zin) {
+            z2p = fc[S.oz[4]];
+            z1p = fc[S.oz[3]];
+            z1m = fc[S.oz[1]];
+            z2m = fc[S.oz[0]];
+        } else {
+            z2p = cellv(f, g, i, j, k + 2);
+            z1p = cellv(f, g, i, j, k + 1);
+            z1m = cellv(f, g, i, j, k - 1);
+            z2m = cellv(f, g, i, j, k - 2);
+        }
+        const double ddz = eno2(hz, v[2], z2p, z1p, f0, z1m, z2m);
         ug[q] = v[0] * ddx + v[1] * ddy + v[2] * ddz;
     }
     const double s11 = sig[c], s12 = sig[g.Nc + c], s13 = sig[2 * g.Nc + c];
@@ -168,22 +181,24 @@ static __global__ __launch_bounds__(kT) void k_direct_cell(Grid g, Mat M, double
     sig_out[5 * g.Nc + c] = s33 + cs33;
     // chi diffusion (chi_diffusion, src/shear_sim_3d.cpp:1217-1237)
     const double Dc = dnew[c];
-    const double Dxm = i > 0 ? dnew[c - 1] : dold[cidx(g, i - 1, j, k)];
-    const double Dxp = dold[cidx(g, i + 1, j, k)];
-    const double Dym = j > 0 ? dnew[c - g.Nx] : dold[cidx(g, i, j - 1, k)];
-    const double Dyp = dold[cidx(g, i, j + 1, k)];
+    const double Dxm = i > 0 ? dnew[c - 1] : dold[c + S.ox[1]];
+    const double Dxp = dold[c + S.ox[3]];
+    const double Dym = j > 0 ? dnew[c - g.Nx] : dold[c + S.oy[1]];
+    const double Dyp = dold[c + S.oy[3]];
     double Dzm, Dzp;
     if (g.pz) {
-        Dzm = k > 0 ? dnew[c - (long)g.Nx * g.Ny] : dold[cidx(g, i, j, g.Nz - 1)];
-        Dzp = dold[cidx(g, i, j, wrapc(k + 1, g.Nz))];
+        Dzm = k > 0 ? dnew[c - (long)g.Nx * g.Ny] : dold[c + S.oz[1]];
+        Dzp = dold[c + S.oz[3]];
     } else {
         Dzm = k > 0 ? dnew[c - (long)g.Nx * g.Ny] : 0.0;
         Dzp = k < g.Nz - 1 ? dold[c + (long)g.Nx * g.Ny] : 0.0;
     }
-    const double x0 = chi[c];
-    const double xxm = cellv(chi, g, i - 1, j, k), xxp = cellv(chi, g, i + 1, j, k);
-    const double xym = cellv(chi, g, i, j - 1, k), xyp = cellv(chi, g, i, j + 1, k);
-    const double xzm = cellv(chi, g, i, j, k - 1), xzp = cellv(chi, g, i, j, k + 1);
+    const double *xc = chi + c;
+    const double x0 = xc[0];
+    const double xxm = xc[S.ox[1]], xxp = xc[S.ox[3]];
+    const double xym = xc[S.oy[1]], xyp = xc[S.oy[3]];
+    const double xzm = zin ? xc[S.oz[1]] : cellv(chi, g, i, j, k - 1);
+    const double xzp = zin ? xc[S.oz[3]] : cellv(chi, g, i, j, k + 1);
     const double ix = 1 / g.dx, iy = 1 / g.dy, iz = 1 / g.dz;
     const double ddx = .5 * ix * (Dxp - Dxm) * (.5 * ix * (xxp - xxm)) + Dc * (ix * ix * (xxp - 2 * x0 + xxm));
     const double ddy = .5 * iy * (Dyp - Dym) * (.5 * iy * (xyp - xym)) + Dc * (iy * iy * (xyp - 2 * x0 + xym));
@@ -219,9 +234,13 @@ static __global__ __launch_bounds__(kT) void k_direct_vertex(Grid g, Dyn D, doub
     const double dxdt = g.dx / dt, dydt = g.dy / dt, dzdt = g.dz / dt;
     // backward staggered stress derivatives (calculate_first_order_stresses,
     // src/shear_sim_3d.cpp:971-987): cells (i-1..i, j-1..j, k-1..k)
-    const long c000 = cidx(g, i, j, k), c100 = cidx(g, i - 1, j, k), c010 = cidx(g, i, j - 1, k),
-               c110 = cidx(g, i - 1, j - 1, k), c001 = cidx(g, i, j, k - 1), c101 = cidx(g, i - 1, j, k - 1),
-               c011 = cidx(g, i, j - 1, k - 1), c111 = cidx(g, i - 1, j - 1, k - 1);
+    // (0 < k < Nz here, so the cell planes k - 1 and k exist and the cell
+    // index of (i, j, k) is p)
+    const Stencil5 S = stencil5(g, i, j, k, g.Nzv);
+    const bool zin = g.pz || (k >= 2 && k <= g.Nzv - 3);  // no wall ghost in the z stencils
+    const int mx = S.ox[1], my = S.oy[1], mz = S.oz[1];
+    const long c000 = p, c100 = p + mx, c010 = p + my, c110 = p + mx + my, c001 = p + mz, c101 = p + mx + mz,
+               c011 = p + my + mz, c111 = p + mx + my + mz;
     auto DX = [&](const double *a) {
         return .25 * dx_inv * (a[c000] - a[c100] + a[c010] - a[c110] + a[c001] - a[c101] + a[c011] - a[c111]);
     };
@@ -238,25 +257,49 @@ static __global__ __launch_bounds__(kT) void k_direct_vertex(Grid g, Dyn D, doub
 #pragma unroll
     for (int q = 0; q < 3; q++) {
         const double *f = vel + q * g.Nv;
-        const double f0 = f[p];
-        const double gsq = dx_inv_sq * (vel_g(f, g, i + 1, j, k) - 2 * f0 + vel_g(f, g, i - 1, j, k)) +
-                           dy_inv_sq * (vel_g(f, g, i, j + 1, k) - 2 * f0 + vel_g(f, g, i, j - 1, k)) +
-                           dz_inv_sq * (vel_g(f, g, i, j, k + 1) - 2 * f0 + vel_g(f, g, i, j, k - 1));
-        const double ddx = eno2(dxdt, u, vel_g(f, g, i + 2, j, k), vel_g(f, g, i + 1, j, k), f0,
-                                vel_g(f, g, i - 1, j, k), vel_g(f, g, i - 2, j, k));
-        const double ddy = eno2(dydt, v, vel_g(f, g, i, j + 2, k), vel_g(f, g, i, j + 1, k), f0,
-                                vel_g(f, g, i, j - 1, k), vel_g(f, g, i, j - 2, k));
-        const double ddz = eno2(dzdt, w, vel_g(f, g, i, j, k + 2), vel_g(f, g, i, j, k + 1), f0,
-                                vel_g(f, g, i, j, k - 1), vel_g(f, g, i, j, k - 2));
+        const double *fp = f + p;
+        const double f0 = fp[0];
+        double z2p, z1p, z1m, z2m;
+        if (zin) {
+            z2p = fp[S.oz[4]];
+            z1p = fp[S.oz[3]];
+            z1m = fp[S.oz[1]];
+            z2m = fp[S.oz[0]];
+        } else {
+            z2p = vel_g(f, g, i, j, k + 2);
+            z1p = vel_g(f, g, i, j, k + 1);
+            z1m = vel_g(f, g, i, j, k - 1);
+            z2m = vel_g(f, g, i, j, k - 2);
+        }
+        const double gsq = dx_inv_sq * (fp[S.ox[3]] - 2 * f0 + fp[S.ox[1]]) +
+                           dy_inv_sq * (fp[S.oy[3]] - 2 * f0 + fp[S.oy[1]]) + dz_inv_sq * (z1p - 2 * f0 + z1m);
+        const double ddx = eno2(dxdt, u, fp[S.ox[4]], fp[S.ox[3]], f0, fp[S.ox[1]], fp[S.ox[0]]);
+        const double ddy = eno2(dydt, v, fp[S.oy[4]], fp[S.oy[3]], f0, fp[S.oy[1]], fp[S.oy[0]]);
+        const double ddz = eno2(dzdt, w, z2p, z1p, f0, z1m, z2m);
         const double ugq = u * ddx + v * ddy + w * ddz;
         vel_out[q * g.Nv + p] = f0 + (dtrho_inv * div[q] + dtkaprho_inv * gsq - ugq);
-        const double x0 = xi[q * g.Nv + p];
-        const double ex = eno2(dxdt, u, xi_g(xi, g, D, q, i + 2, j, k), xi_g(xi, g, D, q, i + 1, j, k), x0,
-                               xi_g(xi, g, D, q, i - 1, j, k), xi_g(xi, g, D, q, i - 2, j, k));
-        const double ey = eno2(dydt, v, xi_g(xi, g, D, q, i, j + 2, k), xi_g(xi, g, D, q, i, j + 1, k), x0,
-                               xi_g(xi, g, D, q, i, j - 1, k), xi_g(xi, g, D, q, i, j - 2, k));
-        const double ez = eno2(dzdt, w, xi_g(xi, g, D, q, i, j, k + 2), xi_g(xi, g, D, q, i, j, k + 1), x0,
-                               xi_g(xi, g, D, q, i, j, k - 1), xi_g(xi, g, D, q, i, j, k - 2));
+        // reference map: the periodic image of component q along axis q is
+        // shifted by the domain length, as xi_g does it
+        const double *xp = xi + q * g.Nv + p;
+        const double x0 = xp[0];
+        auto img = [&](const int(&o)[5], int n, int a, double Lq, bool axis) {
+            const double val = xp[o[a]];
+            if (!axis || o[a] == (a - 2) * n) return val;
+            const double off = a > 2 ? Lq : -Lq;
+            return off == 0.0 ? val : val + off;
+        };
+        const int nxy = g.Nx * g.Ny;
+        const double ex = eno2(dxdt, u, img(S.ox, 1, 4, D.Lx, q == 0), img(S.ox, 1, 3, D.Lx, q == 0), x0,
+                               img(S.ox, 1, 1, D.Lx, q == 0), img(S.ox, 1, 0, D.Lx, q == 0));
+        const double ey = eno2(dydt, v, img(S.oy, g.Nx, 4, D.Ly, q == 1), img(S.oy, g.Nx, 3, D.Ly, q == 1), x0,
+                               img(S.oy, g.Nx, 1, D.Ly, q == 1), img(S.oy, g.Nx, 0, D.Ly, q == 1));
+        double ez;
+        if (zin)
+            ez = eno2(dzdt, w, img(S.oz, nxy, 4, D.Lz, q == 2), img(S.oz, nxy, 3, D.Lz, q == 2), x0,
+                      img(S.oz, nxy, 1, D.Lz, q == 2), img(S.oz, nxy, 0, D.Lz, q == 2));
+        else
+            ez = eno2(dzdt, w, xi_g(xi, g, D, q, i, j, k + 2), xi_g(xi, g, D, q, i, j, k + 1), x0,
+                      xi_g(xi, g, D, q, i, j, k - 1), xi_g(xi, g, D, q, i, j, k - 2));
         xi_out[q * g.Nv + p] = x0 + (0.0 - (u * ex + v * ey + w * ez));
     }
 }

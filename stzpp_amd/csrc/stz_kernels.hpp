@@ -141,6 +141,45 @@ __device__ __forceinline__ void vel_grad(const double *__restrict__ vel, const G
     }
 }
 
+// Offsets of the points up to two either side of point (i, j, k) of a
+// field of nz planes of Nx Ny points (index a + 2 for a = -2..2): x and y
+// periodic, z periodic when pz and plain otherwise -- valid there for
+// 2 <= k <= nz - 3, where no wall ghost is involved.  The same offsets
+// serve the vertex field at the same (i, j, k) for a = +1.
+struct Stencil5 {
+    int ox[5], oy[5], oz[5];
+};
+__device__ __forceinline__ Stencil5 stencil5(const Grid &g, int i, int j, int k, int nz) {
+    Stencil5 s;
+    const int nxy = g.Nx * g.Ny;
+#pragma unroll
+    for (int a = -2; a <= 2; a++) {
+        s.ox[a + 2] = wrapc(i + a, g.Nx) - i;
+        s.oy[a + 2] = (wrapc(j + a, g.Ny) - j) * g.Nx;
+        s.oz[a + 2] = (g.pz ? wrapc(k + a, nz) - k : a) * nxy;
+    }
+    return s;
+}
+
+// vel_grad for vertex index c of (i, j, k) with the offsets of (i+1, j+1,
+// k+1) (same arithmetic).
+__device__ __forceinline__ void vel_grad_o(const double *__restrict__ vel, const Grid &g, long c, int o1x, int o1y,
+                                           int o1z, double G[3][3], double vc[3]) {
+    const double ix = .25 * (1.0 / g.dx), iy = .25 * (1.0 / g.dy), iz = .25 * (1.0 / g.dz);
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const double *u = vel + q * g.Nv + c;
+        const double u000 = u[0], u100 = u[o1x];
+        const double u010 = u[o1y], u110 = u[o1x + o1y];
+        const double u001 = u[o1z], u101 = u[o1x + o1z];
+        const double u011 = u[o1y + o1z], u111 = u[o1x + o1y + o1z];
+        G[q][0] = ix * (u100 - u000 + u110 - u010 + u101 - u001 + u111 - u011);
+        G[q][1] = iy * (u010 - u000 + u110 - u100 + u011 - u001 + u111 - u101);
+        G[q][2] = iz * (u001 - u000 + u101 - u100 + u011 - u010 + u111 - u110);
+        vc[q] = 0.125 * (u000 + u100 + u001 + u101 + u010 + u110 + u011 + u111);
+    }
+}
+
 // Field::dev (src/field.hh:149-184).
 __device__ __forceinline__ double sbar_of(const double s[6]) {
     const double p = -(s[0] + s[3] + s[5]) * (1 / 3.);
@@ -242,20 +281,32 @@ static __global__ __launch_bounds__(kT) void k_advect(Grid g, Mat M, double dt, 
     const int i = (int)(c % g.Nx);
     const int j = (int)((c / g.Nx) % g.Ny);
     const int k = (int)(c / ((long)g.Nx * g.Ny));
+    const Stencil5 S = stencil5(g, i, j, k, g.Nz);
+    const bool zin = g.pz || (k >= 2 && k <= g.Nz - 3);  // no wall ghost in the z stencil
     double G[3][3], v[3];
-    vel_grad(vel, g, i, j, k, G, v);
+    vel_grad_o(vel, g, c, S.ox[3], S.oy[3], S.oz[3], G, v);
     const double hx = g.dx / dt, hy = g.dy / dt, hz = g.dz / dt;
     double ug[7];
 #pragma unroll
     for (int q = 0; q < 7; q++) {
         const double *f = q < 6 ? sig + q * g.Nc : chi;
-        const double f0 = f[c];
-        const double ddx = eno2(hx, v[0], cellv(f, g, i + 2, j, k), cellv(f, g, i + 1, j, k), f0,
-                                cellv(f, g, i - 1, j, k), cellv(f, g, i - 2, j, k));
-        const double ddy = eno2(hy, v[1], cellv(f, g, i, j + 2, k), cellv(f, g, i, j + 1, k), f0,
-                                cellv(f, g, i, j - 1, k), cellv(f, g, i, j - 2, k));
-        const double ddz = eno2(hz, v[2], cellv(f, g, i, j, k + 2), cellv(f, g, i, j, k + 1), f0,
-                                cellv(f, g, i, j, k - 1), cellv(f, g, i, j, k - 2));
+        const double *fc = f + c;
+        const double f0 = fc[0];
+        const double ddx = eno2(hx, v[0], fc[S.ox[4]], fc[S.ox[3]], f0, fc[S.ox[1]], fc[S.ox[0]]);
+        const double ddy = eno2(hy, v[1], fc[S.oy[4]], fc[S.oy[3]], f0, fc[S.oy[1]], fc[S.oy[0]]);
+        double z2p, z1p, z1m, z2m;
+        if (zin) {
+            z2p = fc[S.oz[4]];
+            z1p = fc[S.oz[3]];
+            z1m = fc[S.oz[1]];
+            z2m = fc[S.oz[0]];
+        } else {
+            z2p = cellv(f, g, i, j, k + 2);
+            z1p = cellv(f, g, i, j, k + 1);
+            z1m = cellv(f, g, i, j, k - 1);
+            z2m = cellv(f, g, i, j, k - 2);
+        }
+        const double ddz = eno2(hz, v[2], z2p, z1p, f0, z1m, z2m);
         ug[q] = v[0] * ddx + v[1] * ddy + v[2] * ddz;
     }
     const double s11 = sig[c], s12 = sig[g.Nc + c], s13 = sig[2 * g.Nc + c];
@@ -278,22 +329,24 @@ static __global__ __launch_bounds__(kT) void k_advect(Grid g, Mat M, double dt, 
     sig_out[5 * g.Nc + c] = s33 + (a * third * (s11 + s22 - 2 * s33) - ug[5] + dt * t33);
     // chi diffusion (chi_diffusion, src/shear_sim_3d.cpp:1217-1237)
     const double Dc = dnew[c];
-    const double Dxm = i > 0 ? dnew[c - 1] : dold[cidx(g, i - 1, j, k)];
-    const double Dxp = dold[cidx(g, i + 1, j, k)];
-    const double Dym = j > 0 ? dnew[c - g.Nx] : dold[cidx(g, i, j - 1, k)];
-    const double Dyp = dold[cidx(g, i, j + 1, k)];
+    const double Dxm = i > 0 ? dnew[c - 1] : dold[c + S.ox[1]];
+    const double Dxp = dold[c + S.ox[3]];
+    const double Dym = j > 0 ? dnew[c - g.Nx] : dold[c + S.oy[1]];
+    const double Dyp = dold[c + S.oy[3]];
     double Dzm, Dzp;
     if (g.pz) {
-        Dzm = k > 0 ? dnew[c - (long)g.Nx * g.Ny] : dold[cidx(g, i, j, g.Nz - 1)];
-        Dzp = dold[cidx(g, i, j, wrapc(k + 1, g.Nz))];
+        Dzm = k > 0 ? dnew[c - (long)g.Nx * g.Ny] : dold[c + S.oz[1]];
+        Dzp = dold[c + S.oz[3]];
     } else {
         Dzm = k > 0 ? dnew[c - (long)g.Nx * g.Ny] : 0.0;
         Dzp = k < g.Nz - 1 ? dold[c + (long)g.Nx * g.Ny] : 0.0;
     }
-    const double x0 = chi[c];
-    const double xxm = cellv(chi, g, i - 1, j, k), xxp = cellv(chi, g, i + 1, j, k);
-    const double xym = cellv(chi, g, i, j - 1, k), xyp = cellv(chi, g, i, j + 1, k);
-    const double xzm = cellv(chi, g, i, j, k - 1), xzp = cellv(chi, g, i, j, k + 1);
+    const double *xc = chi + c;
+    const double x0 = xc[0];
+    const double xxm = xc[S.ox[1]], xxp = xc[S.ox[3]];
+    const double xym = xc[S.oy[1]], xyp = xc[S.oy[3]];
+    const double xzm = zin ? xc[S.oz[1]] : cellv(chi, g, i, j, k - 1);
+    const double xzp = zin ? xc[S.oz[3]] : cellv(chi, g, i, j, k + 1);
     const double ix = 1 / g.dx, iy = 1 / g.dy, iz = 1 / g.dz;
     const double ddx = .5 * ix * (Dxp - Dxm) * (.5 * ix * (xxp - xxm)) + Dc * (ix * ix * (xxp - 2 * x0 + xxm));
     const double ddy = .5 * iy * (Dyp - Dym) * (.5 * iy * (xyp - xym)) + Dc * (iy * iy * (xyp - 2 * x0 + xym));
@@ -494,6 +547,42 @@ __device__ __forceinline__ void compute_L(const double *__restrict__ vel, const 
     m3mul(Tti, G, L);
 }
 
+// compute_L for cell c of (i, j, k) with the vertex offsets of (i+1, j+1,
+// k+1) (same arithmetic).
+__device__ __forceinline__ void compute_L_o(const double *__restrict__ vel, const Grid &g, const Box &b,
+                                            const Xform &X, long c, int o1x, int o1y, int o1z, const double P0[3],
+                                            double L[9]) {
+    const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    const double pn[3] = {P0[0] == b.bx ? b.ax : P0[0] + g.dx, P0[1] == b.by ? b.ay : P0[1] + g.dy,
+                          (g.pz && P0[2] == b.bz) ? b.az : P0[2] + g.dz};
+    double v[8][3];
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+        double P[3], Vl[3], a[3], cc[3];
+        const long q = c + (off[n][0] ? o1x : 0) + (off[n][1] ? o1y : 0) + (off[n][2] ? o1z : 0);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            P[d] = off[n][d] ? pn[d] : P0[d];
+            Vl[d] = vel[d * g.Nv + q];
+        }
+        m3vec(X.dT, P, a);
+        m3vec(X.T, Vl, cc);
+#pragma unroll
+        for (int d = 0; d < 3; d++) v[n][d] = a[d] + cc[d];
+    }
+    double G[9];
+    const double ix = .25 * (1. / g.dx), iy = .25 * (1. / g.dy), iz = .25 * (1. / g.dz);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        G[d] = ix * (((((((v[1][d] - v[0][d]) + v[2][d]) - v[3][d]) + v[5][d]) - v[4][d]) + v[6][d]) - v[7][d]);
+        G[3 + d] = iy * (((((((v[3][d] - v[0][d]) + v[2][d]) - v[1][d]) + v[7][d]) - v[4][d]) + v[6][d]) - v[5][d]);
+        G[6 + d] = iz * (((((((v[4][d] - v[0][d]) + v[5][d]) - v[1][d]) + v[7][d]) - v[3][d]) + v[6][d]) - v[2][d]);
+    }
+    double Tti[9];
+    m3t(X.Ti, Tti);
+    m3mul(Tti, G, L);
+}
+
 // Physical stress T sigma' T^t of cell c (compute_old_sig,
 // src/trans_sim_3d.hh:147) and its sbar (calc_devsq, :151-165).
 __device__ __forceinline__ void old_sig(const double *__restrict__ sig, long Nc, long c, const Xform &X,
@@ -561,15 +650,17 @@ static __global__ __launch_bounds__(kT) void k_advect_trans(Grid g, Mat M, doubl
     const int k = (int)(c / ((long)g.Nx * g.Ny));
     const double P0[3] = {B.ax + ((double)i + .5) * g.dx, B.ay + ((double)j + .5) * g.dy,
                           B.az + ((double)k + .5) * g.dz};
+    const Stencil5 S = stencil5(g, i, j, k, g.Nz);
+    const bool zin = g.pz || (k >= 2 && k <= g.Nz - 3);  // no wall ghost in the z stencil
+    const int o1x = S.ox[3], o1y = S.oy[3], o1z = S.oz[3];
     double L[9];
-    compute_L(vel, g, B, X, i, j, k, P0, L);
+    compute_L_o(vel, g, B, X, c, o1x, o1y, o1z, P0, L);
     double v[3];
 #pragma unroll
     for (int q = 0; q < 3; q++) {
-        const double *u = vel + q * g.Nv;
-        v[q] = 0.125 * (vertv(u, g, i, j, k) + vertv(u, g, i + 1, j, k) + vertv(u, g, i, j, k + 1) +
-                        vertv(u, g, i + 1, j, k + 1) + vertv(u, g, i, j + 1, k) + vertv(u, g, i + 1, j + 1, k) +
-                        vertv(u, g, i, j + 1, k + 1) + vertv(u, g, i + 1, j + 1, k + 1));
+        const double *u = vel + q * g.Nv + c;
+        v[q] = 0.125 * (u[0] + u[o1x] + u[o1z] + u[o1x + o1z] + u[o1y] + u[o1x + o1y] + u[o1y + o1z] +
+                        u[o1x + o1y + o1z]);
     }
     double sp[9], os[9], Tti[9], tmp[9], tmp2[9];
     old_sig(sig, g.Nc, c, X, sp, os);
@@ -598,13 +689,13 @@ static __global__ __launch_bounds__(kT) void k_advect_trans(Grid g, Mat M, doubl
 #pragma unroll
     for (int q = 0; q < 6; q++) {
         const double *f = sig + q * g.Nc;
-        const double f0 = f[c];
-        const double ddx = eno2(hx, v[0], cellv(f, g, i + 2, j, k), cellv(f, g, i + 1, j, k), f0,
-                                cellv(f, g, i - 1, j, k), cellv(f, g, i - 2, j, k));
-        const double ddy = eno2(hy, v[1], cellv(f, g, i, j + 2, k), cellv(f, g, i, j + 1, k), f0,
-                                cellv(f, g, i, j - 1, k), cellv(f, g, i, j - 2, k));
-        const double ddz = eno2(hz, v[2], cellv(f, g, i, j, k + 2), cellv(f, g, i, j, k + 1), f0,
-                                cellv(f, g, i, j, k - 1), cellv(f, g, i, j, k - 2));
+        const double *fc = f + c;
+        const double f0 = fc[0];
+        const double ddx = eno2(hx, v[0], fc[S.ox[4]], fc[S.ox[3]], f0, fc[S.ox[1]], fc[S.ox[0]]);
+        const double ddy = eno2(hy, v[1], fc[S.oy[4]], fc[S.oy[3]], f0, fc[S.oy[1]], fc[S.oy[0]]);
+        const double ddz = zin ? eno2(hz, v[2], fc[S.oz[4]], fc[S.oz[3]], f0, fc[S.oz[1]], fc[S.oz[0]])
+                               : eno2(hz, v[2], cellv(f, g, i, j, k + 2), cellv(f, g, i, j, k + 1), f0,
+                                      cellv(f, g, i, j, k - 1), cellv(f, g, i, j, k - 2));
         const double ug = v[0] * ddx + v[1] * ddy + v[2] * ddz;
         cs[q] = 0.0 + (dt * (Lterm[comp[q]] + Tterm[comp[q]]) - ug);
     }
@@ -624,30 +715,31 @@ static __global__ __launch_bounds__(kT) void k_advect_trans(Grid g, Mat M, doubl
 #pragma unroll
     for (int q = 0; q < 6; q++) sig_out[q * g.Nc + c] = sig[q * g.Nc + c] + cs[q];
     // chi: advection (x quirk) and metric diffusion
-    const double x0 = chi[c];
-    const double cx = eno2(hx, v[0], cellv(chi, g, i + 1, j, k), cellv(chi, g, i + 1, j, k), x0,
-                           cellv(chi, g, i - 1, j, k), cellv(chi, g, i - 2, j, k));
-    const double cy = eno2(hy, v[1], cellv(chi, g, i, j + 2, k), cellv(chi, g, i, j + 1, k), x0,
-                           cellv(chi, g, i, j - 1, k), cellv(chi, g, i, j - 2, k));
-    const double cz = eno2(hz, v[2], cellv(chi, g, i, j, k + 2), cellv(chi, g, i, j, k + 1), x0,
-                           cellv(chi, g, i, j, k - 1), cellv(chi, g, i, j, k - 2));
+    const double *xc = chi + c;
+    const double x0 = xc[0];
+    const double cx = eno2(hx, v[0], xc[S.ox[3]], xc[S.ox[3]], x0, xc[S.ox[1]], xc[S.ox[0]]);
+    const double cy = eno2(hy, v[1], xc[S.oy[4]], xc[S.oy[3]], x0, xc[S.oy[1]], xc[S.oy[0]]);
+    const double cz = zin ? eno2(hz, v[2], xc[S.oz[4]], xc[S.oz[3]], x0, xc[S.oz[1]], xc[S.oz[0]])
+                          : eno2(hz, v[2], cellv(chi, g, i, j, k + 2), cellv(chi, g, i, j, k + 1), x0,
+                                 cellv(chi, g, i, j, k - 1), cellv(chi, g, i, j, k - 2));
     const double Dc = dnew[c];
-    const double Dxm = i > 0 ? dnew[c - 1] : dold[cidx(g, i - 1, j, k)];
-    const double Dxp = dold[cidx(g, i + 1, j, k)];
-    const double Dym = j > 0 ? dnew[c - g.Nx] : dold[cidx(g, i, j - 1, k)];
-    const double Dyp = dold[cidx(g, i, j + 1, k)];
+    const double Dxm = i > 0 ? dnew[c - 1] : dold[c + S.ox[1]];
+    const double Dxp = dold[c + S.ox[3]];
+    const double Dym = j > 0 ? dnew[c - g.Nx] : dold[c + S.oy[1]];
+    const double Dyp = dold[c + S.oy[3]];
     double Dzm, Dzp;
     if (g.pz) {
-        Dzm = k > 0 ? dnew[c - (long)g.Nx * g.Ny] : dold[cidx(g, i, j, g.Nz - 1)];
-        Dzp = dold[cidx(g, i, j, wrapc(k + 1, g.Nz))];
+        Dzm = k > 0 ? dnew[c - (long)g.Nx * g.Ny] : dold[c + S.oz[1]];
+        Dzp = dold[c + S.oz[3]];
     } else {
         Dzm = k > 0 ? dnew[c - (long)g.Nx * g.Ny] : 0.0;
         Dzp = k < g.Nz - 1 ? dold[c + (long)g.Nx * g.Ny] : 0.0;
     }
     const double ix = 1. / g.dx, iy = 1. / g.dy, iz = 1. / g.dz;
-    const double xxm = cellv(chi, g, i - 1, j, k), xxp = cellv(chi, g, i + 1, j, k);
-    const double xym = cellv(chi, g, i, j - 1, k), xyp = cellv(chi, g, i, j + 1, k);
-    const double xzm = cellv(chi, g, i, j, k - 1), xzp = cellv(chi, g, i, j, k + 1);
+    const double xxm = xc[S.ox[1]], xxp = xc[S.ox[3]];
+    const double xym = xc[S.oy[1]], xyp = xc[S.oy[3]];
+    const double xzm = zin ? xc[S.oz[1]] : cellv(chi, g, i, j, k - 1);
+    const double xzp = zin ? xc[S.oz[3]] : cellv(chi, g, i, j, k + 1);
     const double dDx = .5 * ix * (Dxp - Dxm), dDy = .5 * iy * (Dyp - Dym), dDz = .5 * iz * (Dzp - Dzm);
     const double gc[3] = {.5 * ix * (xxp - xxm), .5 * iy * (xyp - xym), .5 * iz * (xzp - xzm)};
     double J[9], Jg[3];
@@ -658,14 +750,18 @@ static __global__ __launch_bounds__(kT) void k_advect_trans(Grid g, Mat M, doubl
     const double dyy = iy * iy * (xyp - 2 * x0 + xym);
     const double dzz = iz * iz * (xzp - 2 * x0 + xzm);
     const double dxy = .25 * ix * iy *
-                       (cellv(chi, g, i + 1, j + 1, k) + cellv(chi, g, i - 1, j - 1, k) -
-                        cellv(chi, g, i + 1, j - 1, k) - cellv(chi, g, i - 1, j + 1, k));
-    const double dxz = .25 * ix * iz *
-                       (cellv(chi, g, i + 1, j, k + 1) + cellv(chi, g, i - 1, j, k - 1) -
-                        cellv(chi, g, i + 1, j, k - 1) - cellv(chi, g, i - 1, j, k + 1));
-    const double ypzm = cellv(chi, g, i, j + 1, k - 1);
-    const double dyz = .25 * iy * iz *
-                       (cellv(chi, g, i, j + 1, k + 1) + cellv(chi, g, i, j - 1, k - 1) - ypzm + ypzm);
+                       (xc[S.ox[3] + S.oy[3]] + xc[S.ox[1] + S.oy[1]] - xc[S.ox[3] + S.oy[1]] -
+                        xc[S.ox[1] + S.oy[3]]);
+    const double dxz = zin ? .25 * ix * iz *
+                                 (xc[S.ox[3] + S.oz[3]] + xc[S.ox[1] + S.oz[1]] - xc[S.ox[3] + S.oz[1]] -
+                                  xc[S.ox[1] + S.oz[3]])
+                           : .25 * ix * iz *
+                                 (cellv(chi, g, i + 1, j, k + 1) + cellv(chi, g, i - 1, j, k - 1) -
+                                  cellv(chi, g, i + 1, j, k - 1) - cellv(chi, g, i - 1, j, k + 1));
+    const double ypzm = zin ? xc[S.oy[3] + S.oz[1]] : cellv(chi, g, i, j + 1, k - 1);
+    const double yzpp = zin ? xc[S.oy[3] + S.oz[3]] : cellv(chi, g, i, j + 1, k + 1);
+    const double yzmm = zin ? xc[S.oy[1] + S.oz[1]] : cellv(chi, g, i, j - 1, k - 1);
+    const double dyz = .25 * iy * iz * (yzpp + yzmm - ypzm + ypzm);
     double term2 = 0;
     term2 += J[0] * dxx + J[1] * dxy + J[2] * dxz;
     term2 += J[3] * dxy + J[4] * dyy + J[5] * dyz;
