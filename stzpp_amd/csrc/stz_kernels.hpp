@@ -380,10 +380,11 @@ static __global__ __launch_bounds__(kT) void k_rhs(Grid g, const double *__restr
         r0 = fix_a11 * top_u; r1 = 0; r2 = 0;
     } else {
         const double ix = .25 * (1.0 / g.dx), iy = .25 * (1.0 / g.dy), iz = .25 * (1.0 / g.dz);
-        const int km = g.pz ? wrapc(k - 1, g.Nz) : k - 1;
-        const long c000 = cidx(g, i, j, k), c100 = cidx(g, i - 1, j, k), c010 = cidx(g, i, j - 1, k),
-                   c110 = cidx(g, i - 1, j - 1, k), c001 = cidx(g, i, j, km), c101 = cidx(g, i - 1, j, km),
-                   c011 = cidx(g, i, j - 1, km), c111 = cidx(g, i - 1, j - 1, km);
+        // cells (i-1..i, j-1..j, k-1..k); the cell index of (i, j, k) is p
+        const int mx = wrapc(i - 1, g.Nx) - i, my = (wrapc(j - 1, g.Ny) - j) * g.Nx;
+        const int mz = ((g.pz ? wrapc(k - 1, g.Nz) : k - 1) - k) * (g.Nx * g.Ny);
+        const long c000 = p, c100 = p + mx, c010 = p + my, c110 = p + mx + my, c001 = p + mz, c101 = p + mx + mz,
+                   c011 = p + my + mz, c111 = p + mx + my + mz;
         auto DX = [&](const double *a) {
             return ix * (a[c000] - a[c100] + a[c010] - a[c110] + a[c001] - a[c101] + a[c011] - a[c111]);
         };
@@ -432,7 +433,8 @@ static __global__ __launch_bounds__(kT) void k_stress_update(Grid g, Mat M, doub
         const int j = (int)((c / g.Nx) % g.Ny);
         const int k = (int)(c / ((long)g.Nx * g.Ny));
         double G[3][3], v[3];
-        vel_grad(vel, g, i, j, k, G, v);
+        vel_grad_o(vel, g, c, wrapc(i + 1, g.Nx) - i, (wrapc(j + 1, g.Ny) - j) * g.Nx,
+                   (g.pz ? wrapc(k + 1, g.Nz) - k : 1) * (g.Nx * g.Ny), G, v);
         const double lam = M.lamb, mu = M.mu, L2 = lam + 2 * mu;
         const double d11 = dt * (lam * (G[2][2] + G[1][1]) + L2 * G[0][0]);
         const double d12 = dt * mu * (G[0][1] + G[1][0]);
@@ -789,10 +791,11 @@ static __global__ __launch_bounds__(kT) void k_rhs_trans(Grid g, Xform X, const 
     double o[3] = {0, 0, 0};
     if (g.pz || (k != 0 && k != g.Nz)) {
         const double ix = .25 * (1.0 / g.dx), iy = .25 * (1.0 / g.dy), iz = .25 * (1.0 / g.dz);
-        const int km = g.pz ? wrapc(k - 1, g.Nz) : k - 1;
-        const long c000 = cidx(g, i, j, k), c100 = cidx(g, i - 1, j, k), c010 = cidx(g, i, j - 1, k),
-                   c110 = cidx(g, i - 1, j - 1, k), c001 = cidx(g, i, j, km), c101 = cidx(g, i - 1, j, km),
-                   c011 = cidx(g, i, j - 1, km), c111 = cidx(g, i - 1, j - 1, km);
+        // cells (i-1..i, j-1..j, k-1..k); the cell index of (i, j, k) is p
+        const int mx = wrapc(i - 1, g.Nx) - i, my = (wrapc(j - 1, g.Ny) - j) * g.Nx;
+        const int mz = ((g.pz ? wrapc(k - 1, g.Nz) : k - 1) - k) * (g.Nx * g.Ny);
+        const long c000 = p, c100 = p + mx, c010 = p + my, c110 = p + mx + my, c001 = p + mz, c101 = p + mx + mz,
+                   c011 = p + my + mz, c111 = p + mx + my + mz;
         auto DX = [&](const double *a) {
             return ix * (a[c000] - a[c100] + a[c010] - a[c110] + a[c001] - a[c101] + a[c011] - a[c111]);
         };
@@ -825,7 +828,8 @@ static __global__ __launch_bounds__(kT) void k_stress_update_trans(Grid g, Mat M
     const int k = (int)(c / ((long)g.Nx * g.Ny));
     const double P0[3] = {B.ax + (double)i * g.dx, B.ay + (double)j * g.dy, B.az + (double)k * g.dz};
     double L[9], Lt[9], CD[9], A[9], Bm[9], Tti[9];
-    compute_L(vel, g, B, X, i, j, k, P0, L);
+    compute_L_o(vel, g, B, X, c, wrapc(i + 1, g.Nx) - i, (wrapc(j + 1, g.Ny) - j) * g.Nx,
+                (g.pz ? wrapc(k + 1, g.Nz) - k : 1) * (g.Nx * g.Ny), P0, L);
     m3t(L, Lt);
     m3t(X.Ti, Tti);
     const double tr = L[0] + L[4] + L[8];
