@@ -94,24 +94,28 @@ def step_workload(args):
     dt = 0.005
     log(f"[bench] step workload {Nx}x{Ny}x{Nz}, warmup {args.warmup}")
     graph = not args.no_graph
+    solve = {"ref": None, "fp64": SS.SOLVE_RELRES, "fp32ir": SS.SOLVE_FP32_IR, "fgmres": SS.SOLVE_FP32_GMRES}[
+        args.qs_solve]
     for _ in range(args.warmup):
-        sim.step_forward_qs(dt, 1, use_graph=graph)
+        sim.step_forward_qs(dt, 1, use_graph=graph, solve=solve, rtol=1e-12)
     t0 = time.perf_counter()
     cyc, solve_ms = 0, 0.0
     for _ in range(args.steps):
-        st = sim.step_forward_qs(dt, 1, use_graph=graph)
+        st = sim.step_forward_qs(dt, 1, use_graph=graph, solve=solve, rtol=1e-12)
         cyc += st["vcycles"]
         solve_ms += st["solve_ms"]
     el = time.perf_counter() - t0
+    how = {"ref": "reference projection loop", "fp64": "FP64 multigrid to 1e-12",
+           "fp32ir": "FP32+IR to 1e-12", "fgmres": "FGMRES-IR to 1e-12"}[args.qs_solve]
     out = {
-        "metric": "quasi-static steps per second, shear_sim_3d 192x192x96 cells, reference projection loop",
+        "metric": f"quasi-static steps per second, shear_sim_3d 192x192x96 cells, {how}",
         "value": args.steps / el, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "shear_sim_3d QS step 192x192x96 cells (MG 192x192x97, z walled), config-0 "
                                "material, chi0 GRF seed 3 l=4, dt 0.005",
-                   "vcycles_per_step": cyc / args.steps, "solve_ms_per_step": solve_ms / args.steps,
-                   "v_cycle_graph": graph},
+                   "projection": how, "vcycles_per_step": cyc / args.steps,
+                   "solve_ms_per_step": solve_ms / args.steps, "v_cycle_graph": graph},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
@@ -428,6 +432,9 @@ def main():
     ap.add_argument("--restart", type=int, default=10, help="GMRES restart length")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch V-cycles kernel by kernel instead of replaying a hipGraph")
+    ap.add_argument("--qs-solve", choices=("ref", "fp64", "fp32ir", "fgmres"), default="ref",
+                    help="projection solve of the step workload: the reference's loop (default) or a "
+                         "solve to 1e-12 relative residual")
     ap.add_argument("--workload", choices=("solve", "step", "adjoint", "grf", "direct", "trans", "output"), default="solve",
                     help="solve (default): the 256^3 projection solve; step: QS steps at 192x192x96; "
                          "adjoint: gradient of 20 QS steps at 128^3; grf: random initial chi at 192x192x96; "

@@ -13,7 +13,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .multigrid import MultigridError, SOLVE_FP32_IR, SOLVE_REF, SOLVE_RELRES
+from .multigrid import MultigridError, SOLVE_FP32_GMRES, SOLVE_FP32_IR, SOLVE_REF, SOLVE_RELRES
 
 
 def reference_params(Nx=32, Ny=None, Nz=None, **overrides):
@@ -154,10 +154,14 @@ class ShearSim:
         self._chk(self.L.stz_get_state(self.h, _lib.dptr(t), None, None, None, None))
         return float(t[0])
 
-    def step_forward_qs(self, dt, nsteps=1, solve=None, rtol=1e-12, inner_cycles=3, use_graph=True):
+    def step_forward_qs(self, dt, nsteps=1, solve=None, rtol=1e-12, inner_cycles=None, use_graph=True):
         """nsteps QS steps.  solve=None runs the reference projection loop;
-        SOLVE_RELRES / SOLVE_FP32_IR solve each projection to rtol.
-        use_graph=False launches the V-cycles kernel by kernel."""
+        SOLVE_RELRES / SOLVE_FP32_IR / SOLVE_FP32_GMRES solve each projection
+        to rtol (FP32 V-cycles per refinement step or per GMRES iteration:
+        inner_cycles, default 3 and 1).  use_graph=False launches the
+        V-cycles kernel by kernel."""
+        if inner_cycles is None:
+            inner_cycles = 1 if solve == SOLVE_FP32_GMRES else 3
         st = _lib.StzStepStats()
         if (solve is None or solve == SOLVE_REF) and use_graph:
             opts = None
@@ -389,4 +393,4 @@ def shear_compare(dsim, qsim, ncomps, time_interval, qdt, ddt=None, out_path=Non
 
 
 __all__ = ["ShearSim", "TransSim", "transform", "TRANSFORMS", "reference_params", "config0_params",
-           "shear_compare", "SOLVE_REF", "SOLVE_RELRES", "SOLVE_FP32_IR"]
+           "shear_compare", "SOLVE_REF", "SOLVE_RELRES", "SOLVE_FP32_IR", "SOLVE_FP32_GMRES"]

@@ -5,8 +5,10 @@ CPU oracle's (every projection solved to ||r|| <= 1e-13 ||r0||).
 Loss J = mean over cells of (ad_Dpl/dt)^2 after 3 steps of a 16x16x8 walled
 case whose pre-stress yields (plasticity, chi diffusion and the in-place
 ad_Dpl ordering all active).  Tolerance of the directional derivatives:
-1e-5 relative (central differences with step h: truncation O(h^2), solve
-noise ~1e-13 J / h).
+1e-6 relative for the parameters and for 15 of 16 seeded chi directions
+(central differences with step h: truncation O(h^2), solve noise
+~1e-13 J / h; the remaining direction crosses an adaptive sub-step limit,
+where J is only piecewise smooth).
 
 The transformed step (trans_sim_3d) is checked the same way for the pure
 shear transform (the reference's compiled-in family) and the simple shear;
@@ -97,15 +99,24 @@ def test_adjoint_loss_and_state(grad):
 
 @pytest.mark.parametrize("forward", ["gpu", "oracle"])
 def test_adjoint_chi0_matches_finite_differences(grad, forward):
+    # 16 seeded random directions, h = 1e-6: the directional derivatives
+    # agree to 1e-6 relative except where a direction crosses one of the
+    # adaptive sub-step limits of a yielding cell within +-h (J is only
+    # piecewise smooth there); measured on the GPU forward: 15 of 16
+    # directions within 2e-7, one at 8e-6.
     p, u, sig, chi, J, g, gp, _ = grad
     f = gpu_loss if forward == "gpu" else oracle_loss
     rng = np.random.default_rng(11)
-    for trial in range(2):
+    errs = []
+    for trial in range(16):
         d = rng.standard_normal(chi.size)
-        h = 1e-5
+        h = 1e-6
         fd = (f(p, u, sig, chi + h * d) - f(p, u, sig, chi - h * d)) / (2 * h)
         ad = float(g @ d)
-        assert abs(ad - fd) <= 1e-5 * abs(fd), (trial, ad, fd)
+        errs.append(abs(ad - fd) / abs(fd))
+    errs = np.array(errs)
+    assert np.sum(errs <= 1e-6) >= 15, errs
+    assert errs.max() <= 2e-5, errs
 
 
 @pytest.mark.parametrize("which,name", [(0, "sy"), (1, "eps0"), (2, "chi_inf")])
@@ -122,12 +133,15 @@ def test_adjoint_params_match_finite_differences(grad, which, name):
         p = dict(p)
         p["eps0"] = 0.1
         (J, g, gp), _ = adjoint(p, u, sig, chi)
-    h = 1e-6 * abs(p[name])
+    # eps0, chi_inf: h = 1e-5 relative (measured agreement 8e-8 and 2e-7;
+    # at 1e-6 the solve noise of J, ~1e-13 J / h, reaches 1e-6 of the
+    # derivative); s_y at eps0 = 0.1: h = 1e-6 relative
+    h = (1e-6 if name == "sy" else 1e-5) * abs(p[name])
     pp, pm = dict(p), dict(p)
     pp[name] += h
     pm[name] -= h
     fd = (gpu_loss(pp, u, sig, chi) - gpu_loss(pm, u, sig, chi)) / (2 * h)
-    assert abs(gp[which] - fd) <= 1e-5 * abs(fd) + 1e-12 * abs(J), (name, gp[which], fd)
+    assert abs(gp[which] - fd) <= 1e-6 * abs(fd) + 1e-12 * abs(J), (name, gp[which], fd)
 
 
 def test_adjoint_independent_of_checkpoint_spacing(grad):
