@@ -347,8 +347,13 @@ def test_gpu_solve_matches_oracle_solution(mode):
     assert np.linalg.norm(res) <= 1.01e-10 * np.linalg.norm(b)
 
 
-def test_gpu_fp32_ir_matches_fp64_at_size():
-    mg, b = make_c1(64)
+@pytest.mark.parametrize("n", [64, 128])
+def test_gpu_fp32_ir_matches_fp64_at_size(n):
+    # config 1 at its full size (n = 128: 128 x 128 x 129 vertices, uniform
+    # RHS seed 1; five levels by the reference's coarsening rule): FP32+IR
+    # and FGMRES-IR reach relative residual 1e-10 within 1e-9 of the FP64
+    # multigrid solution
+    mg, b = make_c1(n)
     N = b.shape[0]
     mg.setup_fields(np.zeros((N, 3)), b)
     s64 = mg.solve(mode=MG.SOLVE_RELRES, rtol=1e-10)
@@ -358,6 +363,10 @@ def test_gpu_fp32_ir_matches_fp64_at_size():
     x32 = mg.x(0)
     assert s64["res"] <= 1e-10 * s64["res0"] and s32["res"] <= 1e-10 * s32["res0"]
     assert relerr(x32, x64) < 1e-9
+    mg.setup_fields(np.zeros((N, 3)), b)
+    sg = mg.solve(mode=MG.SOLVE_FP32_GMRES, rtol=1e-10)
+    assert sg["res"] <= 1e-10 * sg["res0"]
+    assert relerr(mg.x(0), x64) < 1e-9
     # linearity: solve(2b) = 2 solve(b)
     mg.setup_fields(np.zeros((N, 3)), 2 * b)
     mg.solve(mode=MG.SOLVE_RELRES, rtol=1e-10)
