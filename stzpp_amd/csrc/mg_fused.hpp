@@ -106,6 +106,41 @@ struct LaneCoef<double> {
     }
 };
 
+__host__ __device__ constexpr int popc9(unsigned m) {
+    int n = 0;
+    for (int e = 0; e < 9; e++) n += (m >> e) & 1;
+    return n;
+}
+__host__ __device__ constexpr int nth_bit(unsigned m, int n) {
+    for (int e = 0; e < 9; e++)
+        if ((m >> e) & 1) {
+            if (n == 0) return e;
+            n--;
+        }
+    return 0;
+}
+
+// The broadcast coefficients of a block in groups of three: the empty asm
+// takes them as inputs and hands them back, so the three v_readlane of a
+// group issue before the first multiply-add that uses one of them.  A
+// v_readlane result read by the next VALU instruction needs two wait states
+// (an s_nop); three readlanes ahead of their uses cover them without one.
+// No instruction is emitted and the arithmetic is unchanged.
+template <unsigned M, int G, typename T>
+__device__ __forceinline__ void pin_group(T (&c)[9]) {
+    constexpr int n = popc9(M), i0 = 3 * G;
+    if constexpr (i0 + 2 < n)
+        asm volatile("" : "+s"(c[nth_bit(M, i0)]), "+s"(c[nth_bit(M, i0 + 1)]), "+s"(c[nth_bit(M, i0 + 2)]));
+    else if constexpr (i0 + 1 < n)
+        asm volatile("" : "+s"(c[nth_bit(M, i0)]), "+s"(c[nth_bit(M, i0 + 1)]));
+    else if constexpr (i0 < n)
+        asm volatile("" : "+s"(c[nth_bit(M, i0)]));
+}
+
+#ifndef STZ_FUSED_PIN
+#define STZ_FUSED_PIN 1
+#endif
+
 template <typename T, int PAT, int B, typename CF, typename LD>
 __device__ __forceinline__ void acc_block(const CF &cf, const LD &ld, T &a0, T &a1, T &a2) {
     constexpr unsigned m = pat_mask(PAT, B);
@@ -117,15 +152,30 @@ __device__ __forceinline__ void acc_block(const CF &cf, const LD &ld, T &a0, T &
         if constexpr (cols & 1) v0 = ld(0, di, dj, dk);
         if constexpr (cols & 2) v1 = ld(1, di, dj, dk);
         if constexpr (cols & 4) v2 = ld(2, di, dj, dk);
-        if constexpr (m & 0x001) a0 += cf.template get<q + 0>() * v0;
-        if constexpr (m & 0x002) a0 += cf.template get<q + 1>() * v1;
-        if constexpr (m & 0x004) a0 += cf.template get<q + 2>() * v2;
-        if constexpr (m & 0x008) a1 += cf.template get<q + 3>() * v0;
-        if constexpr (m & 0x010) a1 += cf.template get<q + 4>() * v1;
-        if constexpr (m & 0x020) a1 += cf.template get<q + 5>() * v2;
-        if constexpr (m & 0x040) a2 += cf.template get<q + 6>() * v0;
-        if constexpr (m & 0x080) a2 += cf.template get<q + 7>() * v1;
-        if constexpr (m & 0x100) a2 += cf.template get<q + 8>() * v2;
+        T c[9];
+        if constexpr (m & 0x001) c[0] = cf.template get<q + 0>();
+        if constexpr (m & 0x002) c[1] = cf.template get<q + 1>();
+        if constexpr (m & 0x004) c[2] = cf.template get<q + 2>();
+        if constexpr (m & 0x008) c[3] = cf.template get<q + 3>();
+        if constexpr (m & 0x010) c[4] = cf.template get<q + 4>();
+        if constexpr (m & 0x020) c[5] = cf.template get<q + 5>();
+        if constexpr (m & 0x040) c[6] = cf.template get<q + 6>();
+        if constexpr (m & 0x080) c[7] = cf.template get<q + 7>();
+        if constexpr (m & 0x100) c[8] = cf.template get<q + 8>();
+        if constexpr (STZ_FUSED_PIN) {
+            pin_group<m, 0>(c);
+            pin_group<m, 1>(c);
+            pin_group<m, 2>(c);
+        }
+        if constexpr (m & 0x001) a0 += c[0] * v0;
+        if constexpr (m & 0x002) a0 += c[1] * v1;
+        if constexpr (m & 0x004) a0 += c[2] * v2;
+        if constexpr (m & 0x008) a1 += c[3] * v0;
+        if constexpr (m & 0x010) a1 += c[4] * v1;
+        if constexpr (m & 0x020) a1 += c[5] * v2;
+        if constexpr (m & 0x040) a2 += c[6] * v0;
+        if constexpr (m & 0x080) a2 += c[7] * v1;
+        if constexpr (m & 0x100) a2 += c[8] * v2;
     }
 }
 
