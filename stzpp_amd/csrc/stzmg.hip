@@ -40,7 +40,8 @@ struct Level {
     float *x32b = nullptr;
     bool fused = false;   // fused two-launch sweeps on this level
     bool tiled = false;   // LDS plane-marching residual on this level
-    int ftile = 0;        // fused-smoother tile class (fused_tile)
+    int ftile = 0;        // fused-smoother tile class of FP32 sweeps (fused_tile)
+    int ftile64 = 0;      // and of FP64 sweeps
     int pat = 0;          // operator zero pattern (PAT_*)
 };
 
@@ -287,23 +288,24 @@ float *&xref_of<float>(Level &L) { return L.x32; }
 constexpr int kSmallTile = 16, kSmallThreads = 128;
 constexpr int kTinyTile = 8, kTinyThreads = 64;
 
-// Fused-smoother tile class of a level: 0 none, 1 the large tiles, 2 the
-// 16 x 16 tiles, 3 the 8 x 8 tiles.  The largest tile that divides the
+// Fused-smoother tile class of a level for sweeps in type T: 0 none, 1 the
+// large tiles of T (64 x 8 FP32, 32 x 16 FP64), 2 the 16 x 16 tiles, 3 the
+// 8 x 8 tiles.  The largest tile that divides the
 // plane is taken unless one launch would then have fewer than
 // STZ_FUSED_MIN_GROUPS workgroups (tiles times planes of one parity) and a
 // smaller tile divides the plane too.
 #ifndef STZ_FUSED_MIN_GROUPS
 #define STZ_FUSED_MIN_GROUPS 0
 #endif
+template <typename T>
 int fused_tile(const Level &L) {
     if (!(L.mode == STZMG_OP_PLANE && L.d.px && L.d.py && L.d.o >= 4 && L.d.N > 1024)) return 0;
     const long nP = (L.d.o + 1) / 2;
     auto groups = [&](int tx, int ty) { return (long)(L.d.m / tx) * (L.d.n / ty) * nP; };
     int best = 0;
-    if (L.d.m % FusedTile<float>::TX == 0 && L.d.n % FusedTile<float>::TY == 0 &&
-        L.d.m % FusedTile<double>::TX == 0 && L.d.n % FusedTile<double>::TY == 0) {
+    if (L.d.m % FusedTile<T>::TX == 0 && L.d.n % FusedTile<T>::TY == 0) {
         best = 1;
-        if (groups(FusedTile<float>::TX, FusedTile<float>::TY) >= STZ_FUSED_MIN_GROUPS) return best;
+        if (groups(FusedTile<T>::TX, FusedTile<T>::TY) >= STZ_FUSED_MIN_GROUPS) return best;
     }
     if (L.d.m % kSmallTile == 0 && L.d.n % kSmallTile == 0) {
         best = 2;
@@ -333,9 +335,10 @@ void launch_fused_tile(stzmg_t h, Level &L) {
 
 template <typename T, int PAT>
 void launch_fused_pat(stzmg_t h, Level &L) {
-    if (L.ftile == 1)
+    const int ft = sizeof(T) == 4 ? L.ftile : L.ftile64;
+    if (ft == 1)
         launch_fused_tile<T, PAT, FusedTile<T>::TX, FusedTile<T>::TY, FusedTile<T>::NT>(h, L);
-    else if (L.ftile == 2)
+    else if (ft == 2)
         launch_fused_tile<T, PAT, kSmallTile, kSmallTile, kSmallThreads>(h, L);
     else
         launch_fused_tile<T, PAT, kTinyTile, kTinyTile, kTinyThreads>(h, L);
@@ -672,8 +675,9 @@ int build_hierarchy_ops(stzmg_t h) {
             }
         }
         L.pat = pat;
-        L.ftile = fused_tile(L);
-        L.fused = h->fused_smoother && L.ftile && L.x64b;
+        L.ftile = fused_tile<float>(L);
+        L.ftile64 = fused_tile<double>(L);
+        L.fused = h->fused_smoother && L.ftile && L.ftile64 && L.x64b;
         L.tiled = h->tiled_resid && tiled_ok(L);
     }
     if (h->g64) { hipGraphExecDestroy(h->g64); h->g64 = nullptr; }
@@ -799,7 +803,7 @@ int stzmg_create(const stzmg_desc *d, stzmg_t *out) {
             alloc(&L.inv32, L.ncls * 9) || alloc(&L.x64, n3) || alloc(&L.r64, n3) || alloc(&L.t64, n3) ||
             alloc(&L.x32, n3) || alloc(&L.r32, n3) || alloc(&L.t32, n3))
             return bail(fail(h, STZMG_ERR_NOMEM, "device allocation failed at level %zu", l));
-        if (fused_tile(L) && (alloc(&L.x64b, n3) || alloc(&L.x32b, n3)))
+        if (fused_tile<float>(L) && fused_tile<double>(L) && (alloc(&L.x64b, n3) || alloc(&L.x32b, n3)))
             return bail(fail(h, STZMG_ERR_NOMEM, "device allocation failed at level %zu", l));
         hipMemsetAsync(L.ops64, 0, sizeof(double) * L.ncls * kBlk, h->stream);
         hipMemsetAsync(L.x64, 0, sizeof(double) * n3, h->stream);
