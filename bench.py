@@ -340,10 +340,10 @@ DIRECT_N = (192, 192, 96)
 # 192 x 192 x 96, from two rocprofv3 --pmc runs of
 # `python3 bench.py --workload direct --steps 20 --warmup 2 --no-cpu`
 # (FETCH_SIZE x 2.0 + WRITE_SIZE, the copy-kernel factors of the solve
-# profile): cell 448.5 MiB x 2 + 189.0 MiB, vertex 397.0 MiB x 2 + 163.7 MiB.
+# profile): cell 445.7 MiB x 2 + 189.0 MiB, vertex 372.3 MiB x 2 + 163.7 MiB.
 # Summary: profiles/direct_step_192x192x96_kernels.txt.
-TRAFFIC_DIRECT_BYTES_PER_LAUNCH = {"k_direct_cell": (448.5 * 2 + 189.0) * 2 ** 20,
-                                   "k_direct_vertex": (397.0 * 2 + 163.7) * 2 ** 20}
+TRAFFIC_DIRECT_BYTES_PER_LAUNCH = {"k_direct_cell": (445.7 * 2 + 189.0) * 2 ** 20,
+                                   "k_direct_vertex": (372.3 * 2 + 163.7) * 2 ** 20}
 
 
 def direct_workload(args):
