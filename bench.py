@@ -94,7 +94,8 @@ def step_workload(args):
     dt = 0.005
     log(f"[bench] step workload {Nx}x{Ny}x{Nz}, warmup {args.warmup}")
     graph = not args.no_graph
-    solve = {"ref": None, "fp64": SS.SOLVE_RELRES, "fp32ir": SS.SOLVE_FP32_IR, "fgmres": SS.SOLVE_FP32_GMRES}[
+    solve = {"ref": None, "fp64": SS.SOLVE_RELRES, "fp32ir": SS.SOLVE_FP32_IR, "fgmres": SS.SOLVE_FP32_GMRES,
+             "gmresir": SS.SOLVE_FP32_GMRES_IR}[
         args.qs_solve]
     for _ in range(args.warmup):
         sim.step_forward_qs(dt, 1, use_graph=graph, solve=solve, rtol=1e-12)
@@ -106,7 +107,8 @@ def step_workload(args):
         solve_ms += st["solve_ms"]
     el = time.perf_counter() - t0
     how = {"ref": "reference projection loop", "fp64": "FP64 multigrid to 1e-12",
-           "fp32ir": "FP32+IR to 1e-12", "fgmres": "FGMRES-IR to 1e-12"}[args.qs_solve]
+           "fp32ir": "FP32+IR to 1e-12", "fgmres": "FGMRES-IR to 1e-12",
+           "gmresir": "GMRES-IR to 1e-12"}[args.qs_solve]
     out = {
         "metric": f"quasi-static steps per second, shear_sim_3d 192x192x96 cells, {how}",
         "value": args.steps / el, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -430,9 +432,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--inner", type=int, default=3, help="FP32 V-cycles per stationary refinement step")
     ap.add_argument("--restart", type=int, default=10, help="GMRES restart length")
+    ap.add_argument("--inner-rtol", type=float, default=0.0,
+                    help="residual reduction of each FP32 GMRES solve of GMRES-IR (0: library default)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch V-cycles kernel by kernel instead of replaying a hipGraph")
-    ap.add_argument("--qs-solve", choices=("ref", "fp64", "fp32ir", "fgmres"), default="ref",
+    ap.add_argument("--qs-solve", choices=("ref", "fp64", "fp32ir", "fgmres", "gmresir"), default="ref",
                     help="projection solve of the step workload: the reference's loop (default) or a "
                          "solve to 1e-12 relative residual")
     ap.add_argument("--workload", choices=("solve", "step", "adjoint", "grf", "direct", "trans", "output"), default="solve",
@@ -492,13 +496,14 @@ def main():
         hip.hipDeviceSynchronize()
         t0 = time.perf_counter()
         s = mg.solve(mode=mode, rtol=1e-10, inner_cycles=inner, max_cycles=400,
-                     use_graph=not args.no_graph, restart=args.restart)
+                     use_graph=not args.no_graph, restart=args.restart, inner_rtol=args.inner_rtol)
         hip.hipDeviceSynchronize()
         return time.perf_counter() - t0, s
 
     for w in range(args.warmup):
         log(f"[bench] warmup {w}")
         solve(MG.SOLVE_FP32_GMRES)
+        solve(MG.SOLVE_FP32_GMRES_IR)
         solve(MG.SOLVE_FP32_IR, args.inner)
         solve(MG.SOLVE_RELRES)
 
@@ -522,11 +527,21 @@ def main():
     x32 = mg.x(0)
     log(f"[bench] fp32+ir (fgmres): {np.mean(t32s)*1e3:.1f} ms, {s32}")
 
+    tgis, sgi = [], None
+    for k in range(args.steps):
+        t, sgi = solve(MG.SOLVE_FP32_GMRES_IR)
+        tgis.append(t)
+    xgi = mg.x(0)
+    log(f"[bench] fp32+ir (gmres-ir): {np.mean(tgis)*1e3:.1f} ms, {sgi}")
+
     # error of the timed solution against the FP64 one (periodic system:
     # compare after removing the constant null-space component)
     d32 = x32 - x32.mean(axis=0)
     d64 = x64 - x64.mean(axis=0)
     err = float(np.linalg.norm(d32 - d64) / np.linalg.norm(d64))
+    dgi = xgi - xgi.mean(axis=0)
+    err_fg = err
+    err = float(np.linalg.norm(dgi - d64) / np.linalg.norm(d64))
 
     # Roofline of the dominant kernel, the fine-level FP32 fused smoother:
     # one sweep is two launches (one per z parity), each updating N/2 points
@@ -553,7 +568,8 @@ def main():
                       f"x {s64['cycles']} cycles",
         }
 
-    t32 = float(np.mean(t32s))
+    t32 = float(np.mean(tgis))
+    tfg = float(np.mean(t32s))
     t64 = float(np.mean(t64s))
     tir = float(np.mean(tirs))
     out = {
@@ -572,13 +588,18 @@ def main():
         "config": {
             "workload": f"trans-operator projection solve {n}^3 vec3, simple shear T(gamma=0.5), "
                         f"fully periodic, GRF rhs seed 2 l=8, V(2,2)+reference growth, rtol 1e-10",
-            "solver": f"FGMRES({args.restart}) in FP64, preconditioner one FP32 V-cycle",
+            "solver": f"GMRES-IR: FP64 refinement, each correction FP32 GMRES({args.restart}) to "
+                      f"{args.inner_rtol or 1e-4:g} preconditioned by one FP32 V-cycle",
             "fp64_s": t64,
             "fp64_cycles": s64["cycles"],
             "fp32_ir_stationary_s": tir,
             "fp32_ir_stationary_cycles": sir["cycles"],
+            "fp32_cycles": sgi["cycles"],
+            "refinements": sgi["outer"],
+            "fgmres_s": tfg,
             "fgmres_fp32_cycles": s32["cycles"],
             "fgmres_restarts": s32["outer"],
+            "fgmres_rel_err_vs_fp64": err_fg,
             "speedup_vs_fp64": t64 / t32,
             "rel_err_vs_fp64": err,
             "levels": mg.nlevels,

@@ -53,9 +53,15 @@ enum {
     STZMG_SOLVE_REF = 0,     /* reference projection loop (sum |r|^4 <= tol, + extra cycles) */
     STZMG_SOLVE_RELRES = 1,  /* FP64 V-cycles to ||r||_2 <= rtol ||b - A x0||_2 */
     STZMG_SOLVE_FP32_IR = 2, /* FP32 V-cycles on the defect inside FP64 refinement */
-    STZMG_SOLVE_FP32_GMRES = 3 /* flexible GMRES(restart) in FP64, right-preconditioned by
-                                  inner_cycles FP32 V-cycles from zero (GMRES-based
-                                  iterative refinement); stops on the true FP64 residual */
+    STZMG_SOLVE_FP32_GMRES = 3, /* flexible GMRES(restart) in FP64, right-preconditioned by
+                                   inner_cycles FP32 V-cycles from zero (GMRES-based
+                                   iterative refinement); stops on the true FP64 residual */
+    STZMG_SOLVE_FP32_GMRES_IR = 4 /* iterative refinement in FP64 whose correction is an
+                                     FP32 GMRES(restart) solve (FP32 Krylov basis and operator
+                                     applies, FP64 dot products), right-preconditioned by
+                                     inner_cycles FP32 V-cycles, each run until its residual
+                                     estimate fell by inner_rtol; stops on the true FP64
+                                     residual */
 };
 
 typedef struct {
@@ -78,7 +84,8 @@ typedef struct {
     int max_cycles;       /* cap on V-cycles (the reference loop has none) */
     int inner_cycles;     /* FP32_IR: FP32 V-cycles per refinement step */
     int use_graph;        /* replay V-cycles from a captured hipGraph */
-    int restart;          /* FP32_GMRES: Krylov vectors per restart (1..16) */
+    int restart;          /* FP32_GMRES(_IR): Krylov vectors per restart (1..16) */
+    double inner_rtol;    /* FP32_GMRES_IR: reduction of each inner solve (0: 1e-4) */
 } stzmg_solve_opts;
 
 typedef struct {

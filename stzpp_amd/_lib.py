@@ -39,6 +39,7 @@ class StzmgSolveOpts(ctypes.Structure):
         ("inner_cycles", ctypes.c_int),
         ("use_graph", ctypes.c_int),
         ("restart", ctypes.c_int),
+        ("inner_rtol", ctypes.c_double),
     ]
 
 

@@ -437,11 +437,12 @@ __global__ void k_soa_to_aos(const double *__restrict__ s, double *__restrict__ 
 }
 
 // FP32 defect d = res/s (s = ||res||_inf scale from the host) and e = 0.
-__global__ void k_defect_f32(const double *__restrict__ res, double inv_s, float *__restrict__ d,
+template <typename TI = double>
+__global__ void k_defect_f32(const TI *__restrict__ res, double inv_s, float *__restrict__ d,
                              float *__restrict__ e, long n) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    d[t] = (float)(res[t] * inv_s);
+    d[t] = (float)((double)res[t] * inv_s);
     e[t] = 0.0f;
 }
 

@@ -1,8 +1,10 @@
-// Vector kernels of the flexible GMRES solve (FP64 Krylov basis, FP32
-// preconditioned vectors).  Vectors are 3 N doubles (SoA vec3 fields of
-// level 0); a basis of nv vectors is stored contiguously, vector j at
-// V + j * n3.  Dot products are formed as per-block partials and summed by
-// k_reduce_cols in a fixed order, so results are reproducible.
+// Vector kernels of the GMRES solves: the flexible GMRES of
+// STZMG_SOLVE_FP32_GMRES (FP64 Krylov basis, FP32 preconditioned vectors)
+// and the FP32 inner GMRES of STZMG_SOLVE_FP32_GMRES_IR (FP32 basis).
+// Vectors are 3 N values of type TV (SoA vec3 fields of level 0); a basis
+// of nv vectors is stored contiguously, vector j at V + j * n3.  Sums are
+// formed in FP64 as per-block partials and summed by k_reduce_cols in a fixed
+// order, so results are reproducible.
 #pragma once
 
 #include "mg_kernels.hpp"
@@ -13,17 +15,17 @@ constexpr int kKrylovMax = 16;       // largest restart length
 constexpr int kDotBlocks = 1024;     // blocks of the grid-stride dot kernels
 
 // part[b * NV + j] = sum over this block's share of w . V_j, j < NV.
-template <int NV>
-__global__ __launch_bounds__(kThreads) void k_mdot(const double *__restrict__ w, const double *__restrict__ V,
-                                                   long n3, double *__restrict__ part) {
+template <int NV, typename TV = double>
+__global__ __launch_bounds__(kThreads) void k_mdot(const TV *__restrict__ w, const TV *__restrict__ V, long n3,
+                                                   double *__restrict__ part) {
     constexpr int nv = NV;
     double acc[NV];
 #pragma unroll
     for (int j = 0; j < NV; j++) acc[j] = 0;
     for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n3; i += (long)gridDim.x * kThreads) {
-        const double wi = w[i];
+        const double wi = (double)w[i];
 #pragma unroll
-        for (int j = 0; j < NV; j++) acc[j] += wi * V[(long)j * n3 + i];
+        for (int j = 0; j < NV; j++) acc[j] += wi * (double)V[(long)j * n3 + i];
     }
     __shared__ double red[kThreads / 64];
 #pragma unroll
@@ -42,9 +44,10 @@ __global__ __launch_bounds__(kThreads) void k_mdot(const double *__restrict__ w,
     }
 }
 
-// w -= sum_j coef[j] V_j (j < NV), then part[b] = block share of |w|^2.
-template <int NV>
-__global__ __launch_bounds__(kThreads) void k_maxpy_norm(double *__restrict__ w, const double *__restrict__ V,
+// w -= sum_j coef[j] V_j (j < NV) in FP64, stored as TV, then part[b] =
+// block share of |w|^2 (of the stored values).
+template <int NV, typename TV = double>
+__global__ __launch_bounds__(kThreads) void k_maxpy_norm(TV *__restrict__ w, const TV *__restrict__ V,
                                                          const double *__restrict__ coef, long n3,
                                                          double *__restrict__ part) {
     double c[NV];
@@ -52,11 +55,12 @@ __global__ __launch_bounds__(kThreads) void k_maxpy_norm(double *__restrict__ w,
     for (int j = 0; j < NV; j++) c[j] = coef[j];
     double acc = 0;
     for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n3; i += (long)gridDim.x * kThreads) {
-        double wi = w[i];
+        double wi = (double)w[i];
 #pragma unroll
-        for (int j = 0; j < NV; j++) wi -= c[j] * V[(long)j * n3 + i];
-        w[i] = wi;
-        acc += wi * wi;
+        for (int j = 0; j < NV; j++) wi -= c[j] * (double)V[(long)j * n3 + i];
+        const TV ws = (TV)wi;
+        w[i] = ws;
+        acc += (double)ws * (double)ws;
     }
     __shared__ double red[kThreads / 64];
 #pragma unroll
@@ -71,25 +75,25 @@ __global__ __launch_bounds__(kThreads) void k_maxpy_norm(double *__restrict__ w,
 }
 
 // Launch of the NV-vector kernels for a run-time count nv (1..kKrylovMax+1).
-template <int NV>
-void launch_mdot_n(hipStream_t st, const double *w, const double *V, int nv, long n3, double *part) {
+template <int NV, typename TV = double>
+void launch_mdot_n(hipStream_t st, const TV *w, const TV *V, int nv, long n3, double *part) {
     if constexpr (NV <= kKrylovMax + 1) {
         if (nv == NV) {
-            hipLaunchKernelGGL(k_mdot<NV>, dim3(kDotBlocks), dim3(kThreads), 0, st, w, V, n3, part);
+            hipLaunchKernelGGL((k_mdot<NV, TV>), dim3(kDotBlocks), dim3(kThreads), 0, st, w, V, n3, part);
             return;
         }
-        launch_mdot_n<NV + 1>(st, w, V, nv, n3, part);
+        launch_mdot_n<NV + 1, TV>(st, w, V, nv, n3, part);
     }
 }
-template <int NV>
-void launch_maxpy_n(hipStream_t st, double *w, const double *V, const double *coef, int nv, long n3,
-                    double *part) {
+template <int NV, typename TV = double>
+void launch_maxpy_n(hipStream_t st, TV *w, const TV *V, const double *coef, int nv, long n3, double *part) {
     if constexpr (NV <= kKrylovMax + 1) {
         if (nv == NV) {
-            hipLaunchKernelGGL(k_maxpy_norm<NV>, dim3(kDotBlocks), dim3(kThreads), 0, st, w, V, coef, n3, part);
+            hipLaunchKernelGGL((k_maxpy_norm<NV, TV>), dim3(kDotBlocks), dim3(kThreads), 0, st, w, V, coef, n3,
+                               part);
             return;
         }
-        launch_maxpy_n<NV + 1>(st, w, V, coef, nv, n3, part);
+        launch_maxpy_n<NV + 1, TV>(st, w, V, coef, nv, n3, part);
     }
 }
 
@@ -109,11 +113,12 @@ __global__ __launch_bounds__(kThreads) void k_reduce_cols(const double *__restri
     if (threadIdx.x == 0) out[j] = red[0];
 }
 
-// dst = a src.
-__global__ __launch_bounds__(kThreads) void k_scale(double *__restrict__ dst, const double *__restrict__ src,
-                                                    double a, long n3) {
+// dst = a src (in FP64, stored as TD).
+template <typename TD = double, typename TS = double>
+__global__ __launch_bounds__(kThreads) void k_scale(TD *__restrict__ dst, const TS *__restrict__ src, double a,
+                                                    long n3) {
     const long i = (long)blockIdx.x * kThreads + threadIdx.x;
-    if (i < n3) dst[i] = a * src[i];
+    if (i < n3) dst[i] = (TD)(a * (double)src[i]);
 }
 
 // x += sum_j y[j] Z_j with FP32 Z (j < nv).

@@ -74,6 +74,8 @@ struct stzmg_s {
     double *kpart = nullptr;  // dot partials
     double *kdev = nullptr;   // reduced dots / coefficients
     int kcap = 0;             // restart length the workspace holds
+    float *kV32 = nullptr;    // FP32 Krylov basis of the GMRES-IR solve
+    int kcap64 = 0, kcap32 = 0;  // restart lengths the two bases hold
     hipGraphExec_t g64 = nullptr, g32 = nullptr;
 };
 
@@ -562,18 +564,34 @@ int norms64(stzmg_t h, int l, double out[2]) {
     return 0;
 }
 
-// Workspace of the flexible GMRES solve for restart length m.
-int ensure_krylov(stzmg_t h, int m, long n3) {
-    if (h->kcap >= m) return 0;
-    hipFree(h->kV); hipFree(h->kZ); hipFree(h->ktmp); hipFree(h->kpart); hipFree(h->kdev);
-    h->kV = nullptr; h->kZ = nullptr; h->ktmp = nullptr; h->kpart = nullptr; h->kdev = nullptr;
-    h->kcap = 0;
-    HIPCHK(h, alloc(&h->kV, (long)(m + 1) * n3));
-    HIPCHK(h, alloc(&h->kZ, (long)m * n3));
-    HIPCHK(h, alloc(&h->ktmp, n3));
-    HIPCHK(h, alloc(&h->kpart, (long)kDotBlocks * (kKrylovMax + 1)));
-    HIPCHK(h, alloc(&h->kdev, 2L * (kKrylovMax + 1)));
-    h->kcap = m;
+// Workspace of the GMRES solves for restart length m: the FP32
+// preconditioned vectors and the reduction scratch, plus the FP64 Krylov
+// basis (flexible GMRES) or the FP32 one (GMRES-IR).
+int ensure_krylov(stzmg_t h, int m, long n3, bool basis64) {
+    if (h->kcap < m) {
+        hipFree(h->kZ); hipFree(h->kpart); hipFree(h->kdev);
+        h->kZ = nullptr; h->kpart = nullptr; h->kdev = nullptr;
+        h->kcap = 0;
+        HIPCHK(h, alloc(&h->kZ, (long)m * n3));
+        HIPCHK(h, alloc(&h->kpart, (long)kDotBlocks * (kKrylovMax + 1)));
+        HIPCHK(h, alloc(&h->kdev, 2L * (kKrylovMax + 1)));
+        h->kcap = m;
+    }
+    if (basis64 && h->kcap64 < m) {
+        hipFree(h->kV); hipFree(h->ktmp);
+        h->kV = nullptr; h->ktmp = nullptr;
+        h->kcap64 = 0;
+        HIPCHK(h, alloc(&h->kV, (long)(m + 1) * n3));
+        HIPCHK(h, alloc(&h->ktmp, n3));
+        h->kcap64 = m;
+    }
+    if (!basis64 && h->kcap32 < m) {
+        hipFree(h->kV32);
+        h->kV32 = nullptr;
+        h->kcap32 = 0;
+        HIPCHK(h, alloc(&h->kV32, (long)(m + 1) * n3));
+        h->kcap32 = m;
+    }
     return 0;
 }
 
@@ -827,6 +845,7 @@ int stzmg_destroy(stzmg_t h) {
     hipFree(h->dres);
     hipFree(h->dflag);
     hipFree(h->kV); hipFree(h->kZ); hipFree(h->ktmp); hipFree(h->kpart); hipFree(h->kdev);
+    hipFree(h->kV32);
     hipFree(h->qmap);
     hipStreamDestroy(h->stream);
     delete h;
@@ -1077,6 +1096,7 @@ void stzmg_solve_opts_default(stzmg_solve_opts *o) {
     o->inner_cycles = 3;
     o->use_graph = 1;
     o->restart = 10;
+    o->inner_rtol = 0;
 }
 
 int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) {
@@ -1151,7 +1171,7 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
         while (rc == 0 && s.res > target) {
             if (s.cycles >= opts->max_cycles) { rc = STZMG_ERR_NOT_CONVERGED; break; }
             const double scale = s.res / std::sqrt((double)n3);  // RMS of the defect
-            hipLaunchKernelGGL(k_defect_f32, dim3(nblk(n3)), dim3(kThreads), 0, h->stream, (const double *)L.t64,
+            hipLaunchKernelGGL((k_defect_f32<double>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream, (const double *)L.t64,
                                1.0 / scale, L.r32, L.x32, n3);
             if ((rc = run_vcycles<float>(h, inner, graph))) return rc;
             s.cycles += inner;
@@ -1175,7 +1195,7 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
         Level &L = h->lv[0];
         const long n3 = 3 * L.d.N;
         const int m = std::min(std::max(opts->restart, 1), kKrylovMax);
-        if ((rc = ensure_krylov(h, m, n3))) return rc;
+        if ((rc = ensure_krylov(h, m, n3, true))) return rc;
         if ((rc = norms64(h, 0, v))) return rc;  // t64 = r64 - A x64
         s.norms++;
         double beta = std::sqrt(v[1]);
@@ -1188,7 +1208,7 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
         auto Hc = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
         const unsigned nb3 = nblk(n3);
         while (rc == 0 && s.res > target && beta > 0) {
-            hipLaunchKernelGGL(k_scale, dim3(nb3), dim3(kThreads), 0, h->stream, h->kV, (const double *)L.t64,
+            hipLaunchKernelGGL((k_scale<double, double>), dim3(nb3), dim3(kThreads), 0, h->stream, h->kV, (const double *)L.t64,
                                1.0 / beta, n3);
             std::fill(g.begin(), g.end(), 0.0);
             g[0] = beta;
@@ -1198,7 +1218,7 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
                 double *vj = h->kV + (long)j * n3, *w = h->kV + (long)(j + 1) * n3;
                 float *zj = h->kZ + (long)j * n3;
                 // z_j = M^-1 (sigma v_j)
-                hipLaunchKernelGGL(k_defect_f32, dim3(nb3), dim3(kThreads), 0, h->stream, (const double *)vj, sigma,
+                hipLaunchKernelGGL((k_defect_f32<double>), dim3(nb3), dim3(kThreads), 0, h->stream, (const double *)vj, sigma,
                                    L.r32, L.x32, n3);
                 if ((rc = run_vcycles<float>(h, inner, graph))) return rc;
                 s.cycles += inner;
@@ -1224,7 +1244,7 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
                 Hc(j + 1, j) = wn / sigma;
                 if (!std::isfinite(wn)) { rc = STZMG_ERR_NONFINITE; break; }
                 if (wn > 0)
-                    hipLaunchKernelGGL(k_scale, dim3(nb3), dim3(kThreads), 0, h->stream, w, (const double *)w,
+                    hipLaunchKernelGGL((k_scale<double, double>), dim3(nb3), dim3(kThreads), 0, h->stream, w, (const double *)w,
                                        1.0 / wn, n3);
                 // Givens rotations
                 for (int i = 0; i < j; i++) {
@@ -1244,6 +1264,103 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
             }
             if (nj == 0) break;
             // y = H^-1 g, x += sum (y_i / sigma) z_i
+            for (int i = nj - 1; i >= 0; i--) {
+                double t = g[i];
+                for (int k2 = i + 1; k2 < nj; k2++) t -= Hc(i, k2) * y[k2];
+                y[i] = t / Hc(i, i);
+            }
+            for (int i = 0; i < nj; i++) hc[i] = y[i] / sigma;
+            HIPCHK(h, hipMemcpyAsync(h->kdev, hc.data(), sizeof(double) * nj, hipMemcpyHostToDevice, h->stream));
+            hipLaunchKernelGGL(k_xupdate, dim3(nb3), dim3(kThreads), 0, h->stream, L.x64, (const float *)h->kZ,
+                               (const double *)h->kdev, nj, n3);
+            LAUNCHCHK(h);
+            s.outer++;
+            if ((rc = norms64(h, 0, v))) return rc;
+            s.norms++;
+            beta = std::sqrt(v[1]);
+            s.res = beta;
+            if (!std::isfinite(s.res)) { rc = STZMG_ERR_NONFINITE; break; }
+        }
+    } else if (opts->mode == STZMG_SOLVE_FP32_GMRES_IR) {
+        // GMRES-based iterative refinement with an FP32 inner solve: each
+        // refinement step runs right-preconditioned GMRES(restart) on
+        // A e = r64 (the true FP64 residual) with an FP32 Krylov basis, FP32
+        // operator applies and inner_cycles FP32 V-cycles from zero as the
+        // preconditioner (dot products and the Hessenberg in FP64), until
+        // its residual estimate has fallen by inner_rtol; x64 += e in FP64,
+        // and the new FP64 residual decides.  FP32 rounding limits what one
+        // step can gain to about 1e-6; FP64 stays on the residual, the update,
+        // the coarse operators and the coarsest level.
+        Level &L = h->lv[0];
+        const long n3 = 3 * L.d.N;
+        const int m = std::min(std::max(opts->restart, 1), kKrylovMax);
+        if ((rc = ensure_krylov(h, m, n3, false))) return rc;
+        if ((rc = norms64(h, 0, v))) return rc;  // t64 = r64 - A x64
+        s.norms++;
+        double beta = std::sqrt(v[1]);
+        s.res0 = s.res = beta;
+        const double target = opts->rtol * beta;
+        const double inner_rtol = opts->inner_rtol > 0 ? opts->inner_rtol : 1e-4;
+        const int inner = opts->inner_cycles > 0 ? opts->inner_cycles : 1;
+        const double sigma = std::sqrt((double)n3);
+        if (!std::isfinite(beta)) rc = STZMG_ERR_NONFINITE;
+        std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), hc(m + 2);
+        auto Hc = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
+        const unsigned nb3 = nblk(n3);
+        while (rc == 0 && s.res > target && beta > 0) {
+            hipLaunchKernelGGL((k_scale<float, double>), dim3(nb3), dim3(kThreads), 0, h->stream, h->kV32,
+                               (const double *)L.t64, 1.0 / beta, n3);
+            std::fill(g.begin(), g.end(), 0.0);
+            g[0] = beta;
+            const double stop = std::max(target, inner_rtol * beta);
+            int nj = 0;
+            for (int j = 0; j < m; j++) {
+                if (s.cycles >= opts->max_cycles) { rc = STZMG_ERR_NOT_CONVERGED; break; }
+                const float *vj = h->kV32 + (long)j * n3;
+                float *w = h->kV32 + (long)(j + 1) * n3;
+                float *zj = h->kZ + (long)j * n3;
+                // z_j = M^-1 (sigma v_j)
+                hipLaunchKernelGGL((k_defect_f32<float>), dim3(nb3), dim3(kThreads), 0, h->stream, vj, sigma,
+                                   L.r32, L.x32, n3);
+                if ((rc = run_vcycles<float>(h, inner, graph))) return rc;
+                s.cycles += inner;
+                HIPCHK(h, hipMemcpyAsync(zj, L.x32, sizeof(float) * n3, hipMemcpyDeviceToDevice, h->stream));
+                // w' = A z_j in FP32 (the true w is w' / sigma)
+                launch_apply<float>(h, L, (const float *)zj, w);
+                launch_mdot_n<1, float>(h->stream, (const float *)w, (const float *)h->kV32, j + 1, n3, h->kpart);
+                hipLaunchKernelGGL(k_reduce_cols, dim3(j + 1), dim3(kThreads), 0, h->stream,
+                                   (const double *)h->kpart, kDotBlocks, j + 1, h->kdev);
+                launch_maxpy_n<1, float>(h->stream, w, (const float *)h->kV32, (const double *)h->kdev, j + 1, n3,
+                                         h->kpart);
+                hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(kThreads), 0, h->stream, (const double *)h->kpart,
+                                   kDotBlocks, 1, h->kdev + (j + 1));
+                LAUNCHCHK(h);
+                HIPCHK(h, hipMemcpyAsync(hc.data(), h->kdev, sizeof(double) * (j + 2), hipMemcpyDeviceToHost,
+                                         h->stream));
+                HIPCHK(h, hipStreamSynchronize(h->stream));
+                const double wn = std::sqrt(hc[j + 1]);
+                for (int i = 0; i <= j; i++) Hc(i, j) = hc[i] / sigma;
+                Hc(j + 1, j) = wn / sigma;
+                if (!std::isfinite(wn)) { rc = STZMG_ERR_NONFINITE; break; }
+                if (wn > 0)
+                    hipLaunchKernelGGL((k_scale<float, float>), dim3(nb3), dim3(kThreads), 0, h->stream, w,
+                                       (const float *)w, 1.0 / wn, n3);
+                for (int i = 0; i < j; i++) {
+                    const double a = Hc(i, j), b = Hc(i + 1, j);
+                    Hc(i, j) = cs[i] * a + sn[i] * b;
+                    Hc(i + 1, j) = -sn[i] * a + cs[i] * b;
+                }
+                const double a = Hc(j, j), b = Hc(j + 1, j), r = std::hypot(a, b);
+                cs[j] = r > 0 ? a / r : 1.0;
+                sn[j] = r > 0 ? b / r : 0.0;
+                Hc(j, j) = r;
+                Hc(j + 1, j) = 0;
+                g[j + 1] = -sn[j] * g[j];
+                g[j] = cs[j] * g[j];
+                nj = j + 1;
+                if (std::fabs(g[j + 1]) <= stop || wn == 0) break;
+            }
+            if (nj == 0) break;
             for (int i = nj - 1; i >= 0; i--) {
                 double t = g[i];
                 for (int k2 = i + 1; k2 < nj; k2++) t -= Hc(i, k2) * y[k2];

@@ -18,7 +18,7 @@ from . import _lib
 
 PLANE, POINT = 0, 1
 NORM_QUARTIC, NORM_L2SQ = 0, 1
-SOLVE_REF, SOLVE_RELRES, SOLVE_FP32_IR, SOLVE_FP32_GMRES = 0, 1, 2, 3
+SOLVE_REF, SOLVE_RELRES, SOLVE_FP32_IR, SOLVE_FP32_GMRES, SOLVE_FP32_GMRES_IR = 0, 1, 2, 3, 4
 OPT_COARSE_KERNEL, OPT_FUSED_SMOOTHER, OPT_TILED_RESIDUAL, OPT_BOTTOM_MAP = 0, 1, 2, 3
 
 
@@ -166,14 +166,16 @@ class Multigrid:
 
     # -- solve -------------------------------------------------------------------
     def solve(self, mode=SOLVE_RELRES, rtol=1e-10, tol=0.0, max_cycles=1000, inner_cycles=None,
-              extra_cycles=2, ref_norm_schedule=0, use_graph=True, restart=10):
+              extra_cycles=2, ref_norm_schedule=0, use_graph=True, restart=10, inner_rtol=0.0):
         """inner_cycles: FP32 V-cycles per refinement step (FP32_IR, default 3)
-        or per preconditioner application (FP32_GMRES, default 1)."""
+        or per preconditioner application (FP32_GMRES and FP32_GMRES_IR,
+        default 1).  inner_rtol: reduction of each FP32 GMRES solve of
+        FP32_GMRES_IR (0: the library's 1e-4)."""
         if inner_cycles is None:
-            inner_cycles = 1 if mode == SOLVE_FP32_GMRES else 3
+            inner_cycles = 1 if mode in (SOLVE_FP32_GMRES, SOLVE_FP32_GMRES_IR) else 3
         o = _lib.StzmgSolveOpts()
         self.L.stzmg_solve_opts_default(ctypes.byref(o))
-        o.restart = restart
+        o.restart, o.inner_rtol = restart, inner_rtol
         o.mode, o.rtol, o.tol = mode, rtol, tol
         o.max_cycles, o.inner_cycles, o.extra_cycles = max_cycles, inner_cycles, extra_cycles
         o.ref_norm_schedule, o.use_graph = ref_norm_schedule, int(bool(use_graph))

@@ -13,7 +13,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .multigrid import MultigridError, SOLVE_FP32_GMRES, SOLVE_FP32_IR, SOLVE_REF, SOLVE_RELRES
+from .multigrid import (MultigridError, SOLVE_FP32_GMRES, SOLVE_FP32_GMRES_IR, SOLVE_FP32_IR, SOLVE_REF,
+                        SOLVE_RELRES)
 
 
 def reference_params(Nx=32, Ny=None, Nz=None, **overrides):
@@ -156,12 +157,12 @@ class ShearSim:
 
     def step_forward_qs(self, dt, nsteps=1, solve=None, rtol=1e-12, inner_cycles=None, use_graph=True):
         """nsteps QS steps.  solve=None runs the reference projection loop;
-        SOLVE_RELRES / SOLVE_FP32_IR / SOLVE_FP32_GMRES solve each projection
-        to rtol (FP32 V-cycles per refinement step or per GMRES iteration:
-        inner_cycles, default 3 and 1).  use_graph=False launches the
+        SOLVE_RELRES / SOLVE_FP32_IR / SOLVE_FP32_GMRES / SOLVE_FP32_GMRES_IR
+        solve each projection to rtol (FP32 V-cycles per refinement step or
+        per GMRES iteration: inner_cycles, default 3 and 1).  use_graph=False launches the
         V-cycles kernel by kernel."""
         if inner_cycles is None:
-            inner_cycles = 1 if solve == SOLVE_FP32_GMRES else 3
+            inner_cycles = 1 if solve in (SOLVE_FP32_GMRES, SOLVE_FP32_GMRES_IR) else 3
         st = _lib.StzStepStats()
         if (solve is None or solve == SOLVE_REF) and use_graph:
             opts = None
@@ -393,4 +394,5 @@ def shear_compare(dsim, qsim, ncomps, time_interval, qdt, ddt=None, out_path=Non
 
 
 __all__ = ["ShearSim", "TransSim", "transform", "TRANSFORMS", "reference_params", "config0_params",
-           "shear_compare", "SOLVE_REF", "SOLVE_RELRES", "SOLVE_FP32_IR", "SOLVE_FP32_GMRES"]
+           "shear_compare", "SOLVE_REF", "SOLVE_RELRES", "SOLVE_FP32_IR", "SOLVE_FP32_GMRES",
+           "SOLVE_FP32_GMRES_IR"]

@@ -1,7 +1,7 @@
 """The quasi-static time loop at full size (192 x 192 x 96 cells, MG
 192 x 192 x 97, z walled, config-0 material, chi0 = 0.09 + 0.01 GRF seed 3
-l = 4, dt = 0.005, 100 steps): the FP32-smoothed iterative refinement and
-the FGMRES projection solves follow the plain FP64 multigrid trajectory
+l = 4, dt = 0.005, 100 steps): the FP32-smoothed iterative refinement, the
+FGMRES and the GMRES-IR projection solves follow the plain FP64 multigrid trajectory
 (every projection solved to 1e-12 relative residual) to 1e-12 of each
 field's magnitude after 100 steps; the reference's own loop (sum |r|^4
 stopping rule) stays within its loose stopping tolerance of it (1e-6).
@@ -37,7 +37,7 @@ def rel(a, b):
     return float(np.abs(a - b).max() / np.abs(b).max())
 
 
-@pytest.mark.parametrize("solve", [SS.SOLVE_FP32_IR, SS.SOLVE_FP32_GMRES])
+@pytest.mark.parametrize("solve", [SS.SOLVE_FP32_IR, SS.SOLVE_FP32_GMRES, SS.SOLVE_FP32_GMRES_IR])
 def test_fp32_solves_follow_fp64_for_100_steps(fp64, solve):
     st = run(solve)
     assert st["t"] == pytest.approx(fp64["t"], rel=1e-15)

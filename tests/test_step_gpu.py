@@ -68,7 +68,7 @@ def test_gpu_step_matches_oracle_same_smoother(golden, case):
 
 
 @pytest.mark.parametrize("case", ["step_plastic.npz", "step_c0.npz"])
-@pytest.mark.parametrize("solve", [SS.SOLVE_RELRES, SS.SOLVE_FP32_IR])
+@pytest.mark.parametrize("solve", [SS.SOLVE_RELRES, SS.SOLVE_FP32_IR, SS.SOLVE_FP32_GMRES_IR])
 def test_gpu_step_converged_matches_oracle(golden, case, solve):
     g = golden(case)
     a, _ = gpu_run(g, solve=solve, rtol=1e-13)
@@ -170,6 +170,7 @@ def test_gpu_trans_step_matches_oracle_same_smoother(golden, case):
 @pytest.mark.parametrize("case,solve,smoother", [
     ("trans_step_walled.npz", SS.SOLVE_RELRES, 0),
     ("trans_step_walled.npz", SS.SOLVE_FP32_IR, 0),
+    ("trans_step_walled.npz", SS.SOLVE_FP32_GMRES_IR, 0),
     # fully periodic: the projection operator is singular (constant vectors),
     # and the constant the solve leaves in U depends on the iterates and is
     # advected in the next step; the oracle takes the GPU's 8-colour iterates
