@@ -395,12 +395,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(STZ_FUSED_WA
 // once instead of through 27 cached neighbour loads per point.  Each thread
 // owns R = TX*TY/NT vertically adjacent points of a plane, so the LDS rows
 // they share are read once.  With NORM, one pair of partials (sum |r|^4,
-// sum |r|^2) per workgroup goes to part[blockIdx linear].
+// sum |r|^2) per workgroup goes to part[blockIdx linear].  With APPLY and
+// xcopy given, x is also copied to xcopy (the centre values it stages).
 // Grid: (m/TX, n/TY, ceil(o/kchunk)).
 template <typename T, int PAT, bool NORM, bool APPLY, int TX, int TY, int NT>
 __global__ __launch_bounds__(NT) void k_resid_planes(Dims d, const T *__restrict__ x, const T *__restrict__ r,
                                                      const T *__restrict__ ops, T *__restrict__ out,
-                                                     double *__restrict__ part, int kchunk) {
+                                                     double *__restrict__ part, int kchunk,
+                                                     T *__restrict__ xcopy) {
     static_assert(NT == kThreads, "block_sum2 assumes kThreads threads");
     static_assert(NT % TX == 0 && (TX * TY) % NT == 0, "thread layout");
     constexpr int WX = TX + 2, WY = TY + 2, CSZ = WX * WY, PSZ = 3 * CSZ;
@@ -472,14 +474,21 @@ __global__ __launch_bounds__(NT) void k_resid_planes(Dims d, const T *__restrict
             T a0 = 0, a1 = 0, a2 = 0;
             acc_blocks<T, PAT>(cf, ld, a0, a1, a2, std::make_integer_sequence<int, 27>{});
             // centre block (acc_blocks leaves it out: the smoother inverts it)
+            const T v0 = ld(0, 0, 0, 0), v1 = ld(1, 0, 0, 0), v2 = ld(2, 0, 0, 0);
             {
                 constexpr int C = 9 * 13;
-                const T v0 = ld(0, 0, 0, 0), v1 = ld(1, 0, 0, 0), v2 = ld(2, 0, 0, 0);
                 a0 += cf.template get<C + 0>() * v0 + cf.template get<C + 1>() * v1 + cf.template get<C + 2>() * v2;
                 a1 += cf.template get<C + 3>() * v0 + cf.template get<C + 4>() * v1 + cf.template get<C + 5>() * v2;
                 a2 += cf.template get<C + 6>() * v0 + cf.template get<C + 7>() * v1 + cf.template get<C + 8>() * v2;
             }
             const long p = base + (x0 + xx) + (long)d.m * (y0 + yy);
+            if constexpr (APPLY) {
+                if (xcopy) {
+                    xcopy[p] = v0;
+                    xcopy[p + d.N] = v1;
+                    xcopy[p + 2 * d.N] = v2;
+                }
+            }
             T o0 = a0, o1 = a1, o2 = a2;
             if constexpr (!APPLY) {
                 o0 = r[p] - a0;

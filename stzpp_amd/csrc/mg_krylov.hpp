@@ -97,9 +97,11 @@ void launch_maxpy_n(hipStream_t st, TV *w, const TV *V, const double *coef, int 
     }
 }
 
-// out[j] = sum_b part[b * nv + j] (one block per column, fixed order).
+// out[j] = sum_b part[b * nv + j] (one block per column, fixed order),
+// times scale[j] when scale is given.
 __global__ __launch_bounds__(kThreads) void k_reduce_cols(const double *__restrict__ part, int nblocks, int nv,
-                                                          double *__restrict__ out) {
+                                                          double *__restrict__ out,
+                                                          const double *__restrict__ scale) {
     const int j = blockIdx.x;
     double t = 0;
     for (int b = threadIdx.x; b < nblocks; b += kThreads) t += part[(long)b * nv + j];
@@ -110,7 +112,7 @@ __global__ __launch_bounds__(kThreads) void k_reduce_cols(const double *__restri
         if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[j] = red[0];
+    if (threadIdx.x == 0) out[j] = scale ? red[0] * scale[j] : red[0];
 }
 
 // dst = a src (in FP64, stored as TD).

@@ -196,26 +196,28 @@ long resid_parts(const Level &L) {
 }
 
 template <typename T, bool NORM, bool APPLY, int PAT>
-void launch_resid_tiled(stzmg_t h, Level &L, const T *x, const T *r, T *out) {
+void launch_resid_tiled(stzmg_t h, Level &L, const T *x, const T *r, T *out, T *xcopy) {
     int kc;
     const dim3 g = tiled_grid<T>(L, &kc);
     hipLaunchKernelGGL((k_resid_planes<T, PAT, NORM, APPLY, ResidTile<T>::TX, ResidTile<T>::TY, kThreads>), g,
-                       dim3(kThreads), 0, h->stream, L.d, x, r, ops_of<T>(L), out, h->part, kc);
+                       dim3(kThreads), 0, h->stream, L.d, x, r, ops_of<T>(L), out, h->part, kc, xcopy);
 }
 
 // out = r - A x (APPLY: out = A x) on level L, optionally with norm partials
-// into h->part.
+// into h->part.  With APPLY and xcopy given, x is also copied to xcopy.
 template <typename T, bool NORM, bool APPLY>
-void launch_resid_any(stzmg_t h, Level &L, const T *x, const T *r, T *out) {
+void launch_resid_any(stzmg_t h, Level &L, const T *x, const T *r, T *out, T *xcopy = nullptr) {
     if (L.tiled) {
         switch (L.pat) {
-            case PAT_SHEAR: launch_resid_tiled<T, NORM, APPLY, PAT_SHEAR>(h, L, x, r, out); break;
-            case PAT_TRANS_XZ: launch_resid_tiled<T, NORM, APPLY, PAT_TRANS_XZ>(h, L, x, r, out); break;
-            case PAT_NOCORNER: launch_resid_tiled<T, NORM, APPLY, PAT_NOCORNER>(h, L, x, r, out); break;
-            default: launch_resid_tiled<T, NORM, APPLY, PAT_DENSE>(h, L, x, r, out); break;
+            case PAT_SHEAR: launch_resid_tiled<T, NORM, APPLY, PAT_SHEAR>(h, L, x, r, out, xcopy); break;
+            case PAT_TRANS_XZ: launch_resid_tiled<T, NORM, APPLY, PAT_TRANS_XZ>(h, L, x, r, out, xcopy); break;
+            case PAT_NOCORNER: launch_resid_tiled<T, NORM, APPLY, PAT_NOCORNER>(h, L, x, r, out, xcopy); break;
+            default: launch_resid_tiled<T, NORM, APPLY, PAT_DENSE>(h, L, x, r, out, xcopy); break;
         }
         return;
     }
+    if (APPLY && xcopy)
+        (void)hipMemcpyAsync(xcopy, x, sizeof(T) * 3 * L.d.N, hipMemcpyDeviceToDevice, h->stream);
     dim3 g(nblk((long)L.d.m * L.d.n), L.d.o);
     if (L.mode == STZMG_OP_PLANE)
         hipLaunchKernelGGL((k_resid<T, 0, NORM, APPLY>), g, dim3(kThreads), 0, h->stream, L.d, x, r,
@@ -231,8 +233,8 @@ void launch_resid(stzmg_t h, Level &L, const T *x, const T *r, T *out) {
 }
 
 template <typename T>
-void launch_apply(stzmg_t h, Level &L, const T *x, T *out) {
-    launch_resid_any<T, false, true>(h, L, x, (const T *)nullptr, out);
+void launch_apply(stzmg_t h, Level &L, const T *x, T *out, T *xcopy = nullptr) {
+    launch_resid_any<T, false, true>(h, L, x, (const T *)nullptr, out, xcopy);
 }
 
 int ncol(int n, int prd) { return (prd && (n & 1)) ? 3 : 2; }
@@ -1230,11 +1232,11 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
                 // d_i = (w', v_i); w' -= sum d_i v_i; |w'|^2
                 launch_mdot_n<1>(h->stream, (const double *)w, (const double *)h->kV, j + 1, n3, h->kpart);
                 hipLaunchKernelGGL(k_reduce_cols, dim3(j + 1), dim3(kThreads), 0, h->stream,
-                                   (const double *)h->kpart, kDotBlocks, j + 1, h->kdev);
+                                   (const double *)h->kpart, kDotBlocks, j + 1, h->kdev, (const double *)nullptr);
                 launch_maxpy_n<1>(h->stream, w, (const double *)h->kV, (const double *)h->kdev, j + 1, n3,
                                   h->kpart);
                 hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(kThreads), 0, h->stream, (const double *)h->kpart,
-                                   kDotBlocks, 1, h->kdev + (j + 1));
+                                   kDotBlocks, 1, h->kdev + (j + 1), (const double *)nullptr);
                 LAUNCHCHK(h);
                 HIPCHK(h, hipMemcpyAsync(hc.data(), h->kdev, sizeof(double) * (j + 2), hipMemcpyDeviceToHost,
                                          h->stream));
@@ -1291,6 +1293,12 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
         // and the new FP64 residual decides.  FP32 rounding limits what one
         // step can gain to about 1e-6; FP64 stays on the residual, the update,
         // the coarse operators and the coarsest level.
+        //
+        // Basis vectors are kept unnormalised: u_j = nrm_j v_j.  The
+        // reduction of the dot products scales (w, u_i) by 1/nrm_i^2, which
+        // gives the Gram-Schmidt coefficient of u_i directly, and the next
+        // defect scales u_j by sigma/nrm_j, so no separate normalisation pass
+        // runs.
         Level &L = h->lv[0];
         const long n3 = 3 * L.d.N;
         const int m = std::min(std::max(opts->restart, 1), kKrylovMax);
@@ -1304,47 +1312,52 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
         const int inner = opts->inner_cycles > 0 ? opts->inner_cycles : 1;
         const double sigma = std::sqrt((double)n3);
         if (!std::isfinite(beta)) rc = STZMG_ERR_NONFINITE;
-        std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), hc(m + 2);
+        std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), hc(m + 2), nrm(m + 1),
+            isq(kKrylovMax + 1);
         auto Hc = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
         const unsigned nb3 = nblk(n3);
+        double *dscale = h->kdev + (kKrylovMax + 1);  // 1/nrm_i^2 on the device
         while (rc == 0 && s.res > target && beta > 0) {
             hipLaunchKernelGGL((k_scale<float, double>), dim3(nb3), dim3(kThreads), 0, h->stream, h->kV32,
                                (const double *)L.t64, 1.0 / beta, n3);
             std::fill(g.begin(), g.end(), 0.0);
             g[0] = beta;
+            nrm[0] = 1.0;
+            isq[0] = 1.0;
             const double stop = std::max(target, inner_rtol * beta);
             int nj = 0;
             for (int j = 0; j < m; j++) {
                 if (s.cycles >= opts->max_cycles) { rc = STZMG_ERR_NOT_CONVERGED; break; }
-                const float *vj = h->kV32 + (long)j * n3;
+                const float *uj = h->kV32 + (long)j * n3;
                 float *w = h->kV32 + (long)(j + 1) * n3;
                 float *zj = h->kZ + (long)j * n3;
-                // z_j = M^-1 (sigma v_j)
-                hipLaunchKernelGGL((k_defect_f32<float>), dim3(nb3), dim3(kThreads), 0, h->stream, vj, sigma,
-                                   L.r32, L.x32, n3);
+                // z_j = M^-1 (sigma v_j) = M^-1 (sigma / nrm_j u_j)
+                hipLaunchKernelGGL((k_defect_f32<float>), dim3(nb3), dim3(kThreads), 0, h->stream, uj,
+                                   sigma / nrm[j], L.r32, L.x32, n3);
                 if ((rc = run_vcycles<float>(h, inner, graph))) return rc;
                 s.cycles += inner;
-                HIPCHK(h, hipMemcpyAsync(zj, L.x32, sizeof(float) * n3, hipMemcpyDeviceToDevice, h->stream));
-                // w' = A z_j in FP32 (the true w is w' / sigma)
-                launch_apply<float>(h, L, (const float *)zj, w);
+                // w' = A z_j in FP32 (the true w is w' / sigma); the same
+                // pass keeps z_j (x32 is overwritten by the next V-cycles)
+                launch_apply<float>(h, L, (const float *)L.x32, w, zj);
+                // c_i = (w', u_i) / nrm_i^2; w' -= sum c_i u_i; |w'|^2
+                HIPCHK(h, hipMemcpyAsync(dscale + j, &isq[j], sizeof(double), hipMemcpyHostToDevice, h->stream));
                 launch_mdot_n<1, float>(h->stream, (const float *)w, (const float *)h->kV32, j + 1, n3, h->kpart);
                 hipLaunchKernelGGL(k_reduce_cols, dim3(j + 1), dim3(kThreads), 0, h->stream,
-                                   (const double *)h->kpart, kDotBlocks, j + 1, h->kdev);
+                                   (const double *)h->kpart, kDotBlocks, j + 1, h->kdev, (const double *)dscale);
                 launch_maxpy_n<1, float>(h->stream, w, (const float *)h->kV32, (const double *)h->kdev, j + 1, n3,
                                          h->kpart);
                 hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(kThreads), 0, h->stream, (const double *)h->kpart,
-                                   kDotBlocks, 1, h->kdev + (j + 1));
+                                   kDotBlocks, 1, h->kdev + (j + 1), (const double *)nullptr);
                 LAUNCHCHK(h);
                 HIPCHK(h, hipMemcpyAsync(hc.data(), h->kdev, sizeof(double) * (j + 2), hipMemcpyDeviceToHost,
                                          h->stream));
                 HIPCHK(h, hipStreamSynchronize(h->stream));
                 const double wn = std::sqrt(hc[j + 1]);
-                for (int i = 0; i <= j; i++) Hc(i, j) = hc[i] / sigma;
+                for (int i = 0; i <= j; i++) Hc(i, j) = hc[i] * nrm[i] / sigma;
                 Hc(j + 1, j) = wn / sigma;
                 if (!std::isfinite(wn)) { rc = STZMG_ERR_NONFINITE; break; }
-                if (wn > 0)
-                    hipLaunchKernelGGL((k_scale<float, float>), dim3(nb3), dim3(kThreads), 0, h->stream, w,
-                                       (const float *)w, 1.0 / wn, n3);
+                nrm[j + 1] = wn;
+                isq[j + 1] = wn > 0 ? 1.0 / (wn * wn) : 0.0;
                 for (int i = 0; i < j; i++) {
                     const double a = Hc(i, j), b = Hc(i + 1, j);
                     Hc(i, j) = cs[i] * a + sn[i] * b;
