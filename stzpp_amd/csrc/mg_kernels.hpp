@@ -283,14 +283,65 @@ __device__ __forceinline__ void interp_point(const Dims &F, const Dims &C, const
     xf[p + 2 * F.N] += s2;
 }
 
-// Grid: (ceil(mf*nf/256), of).
+// The same x_f += P x_c, one thread per coarse vertex (I, J, K) and the up
+// to eight fine vertices 2(I, J, K) + (0|1, 0|1, 0|1) whose parents lie in
+// {I, I+1} x {J, J+1} x {K, K+1}: the eight coarse values are loaded once
+// instead of once per fine vertex.  Each fine sum runs over the same parents
+// in the same order with the same weights as interp_point, so the results
+// are identical.  Requires f = 2c (periodic) or f = 2c - 1 (walled) per
+// dimension, the only sizes the hierarchy builds.
+// Grid: (ceil(mc*nc/256), oc).
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_interp(Dims F, Dims C, const T *__restrict__ xc,
-                                                     T *__restrict__ xf) {
+__global__ __launch_bounds__(kThreads) void k_interp_cells(Dims F, Dims C, const T *__restrict__ xc,
+                                                           T *__restrict__ xf) {
     const long pl = (long)blockIdx.x * kThreads + threadIdx.x;
-    if (pl >= (long)F.m * F.n) return;
-    const int j = (int)(pl / F.m), i = (int)(pl - (long)j * F.m);
-    interp_point<T>(F, C, xc, xf, i, j, blockIdx.y);
+    if (pl >= (long)C.m * C.n) return;
+    const int J = (int)(pl / C.m), I = (int)(pl - (long)J * C.m), K = blockIdx.y;
+    // second parent along each dimension (used only when the odd fine
+    // vertex exists: then I + 1 < C.m unless periodic, where it wraps)
+    const int I1 = I + 1 < C.m ? I + 1 : 0, J1 = J + 1 < C.n ? J + 1 : 0, K1 = K + 1 < C.o ? K + 1 : 0;
+    const bool ox = 2 * I + 1 < F.m, oy = 2 * J + 1 < F.n, oz = 2 * K + 1 < F.o;
+    const int Ii[2] = {I, I1}, Ji[2] = {J, J1}, Ki[2] = {K, K1};
+    T cv[2][2][2][3];
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                const bool need = (c == 0 || oz) && (b == 0 || oy) && (a == 0 || ox);
+                const long q = Ii[a] + (long)C.m * (Ji[b] + (long)C.n * Ki[c]);
+#pragma unroll
+                for (int e = 0; e < 3; e++) cv[c][b][a][e] = need ? xc[q + e * C.N] : T(0);
+            }
+#pragma unroll
+    for (int fk = 0; fk < 2; fk++)
+#pragma unroll
+        for (int fj = 0; fj < 2; fj++)
+#pragma unroll
+            for (int fi = 0; fi < 2; fi++) {
+                if ((fk && !oz) || (fj && !oy) || (fi && !ox)) continue;
+                // parents of this fine vertex: 1 (even) or 2 (odd) per dimension
+                const int nk = fk + 1, nj = fj + 1, ni = fi + 1;
+                const T wk = fk ? T(0.5) : T(1), wj = fj ? T(0.5) : T(1), wi = fi ? T(0.5) : T(1);
+                T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+                for (int c = 0; c < 2; c++)
+#pragma unroll
+                    for (int b = 0; b < 2; b++)
+#pragma unroll
+                        for (int a = 0; a < 2; a++) {
+                            if (c >= nk || b >= nj || a >= ni) continue;
+                            const T w = wi * wj * wk;
+                            s0 += w * cv[c][b][a][0];
+                            s1 += w * cv[c][b][a][1];
+                            s2 += w * cv[c][b][a][2];
+                        }
+                const long p = (2 * I + fi) + (long)F.m * ((2 * J + fj) + (long)F.n * (2 * K + fk));
+                xf[p] += s0;
+                xf[p + F.N] += s1;
+                xf[p + 2 * F.N] += s2;
+            }
 }
 
 // Galerkin coarse operator A_c = (1/8) P^T A_f P (region::compute_rats,

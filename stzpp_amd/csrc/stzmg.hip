@@ -415,8 +415,8 @@ void launch_restrict(stzmg_t h, int l) {
 template <typename T>
 void launch_interp(stzmg_t h, int l) {
     Level &F = h->lv[l - 1], &C = h->lv[l];
-    dim3 g(nblk((long)F.d.m * F.d.n), F.d.o);
-    hipLaunchKernelGGL((k_interp<T>), g, dim3(kThreads), 0, h->stream, F.d, C.d,
+    dim3 g(nblk((long)C.d.m * C.d.n), C.d.o);
+    hipLaunchKernelGGL((k_interp_cells<T>), g, dim3(kThreads), 0, h->stream, F.d, C.d,
                        (const T *)x_of<T>(C), x_of<T>(F));
 }
 
