@@ -314,13 +314,20 @@ __global__ __launch_bounds__(kThreads) void k_interp_cells(Dims F, Dims C, const
 #pragma unroll
                 for (int e = 0; e < 3; e++) cv[c][b][a][e] = need ? xc[q + e * C.N] : T(0);
             }
+    // x-periodic fine rows have even length, so the pair 2I, 2I+1 is one
+    // aligned two-value access per component
+    struct alignas(2 * sizeof(T)) Pair {
+        T v[2];
+    };
+    const bool pair = ox && !(F.m & 1);
 #pragma unroll
     for (int fk = 0; fk < 2; fk++)
 #pragma unroll
-        for (int fj = 0; fj < 2; fj++)
+        for (int fj = 0; fj < 2; fj++) {
+            if ((fk && !oz) || (fj && !oy)) continue;
+            T sv[2][3];
 #pragma unroll
             for (int fi = 0; fi < 2; fi++) {
-                if ((fk && !oz) || (fj && !oy) || (fi && !ox)) continue;
                 // parents of this fine vertex: 1 (even) or 2 (odd) per dimension
                 const int nk = fk + 1, nj = fj + 1, ni = fi + 1;
                 const T wk = fk ? T(0.5) : T(1), wj = fj ? T(0.5) : T(1), wi = fi ? T(0.5) : T(1);
@@ -337,11 +344,28 @@ __global__ __launch_bounds__(kThreads) void k_interp_cells(Dims F, Dims C, const
                             s1 += w * cv[c][b][a][1];
                             s2 += w * cv[c][b][a][2];
                         }
-                const long p = (2 * I + fi) + (long)F.m * ((2 * J + fj) + (long)F.n * (2 * K + fk));
-                xf[p] += s0;
-                xf[p + F.N] += s1;
-                xf[p + 2 * F.N] += s2;
+                sv[fi][0] = s0;
+                sv[fi][1] = s1;
+                sv[fi][2] = s2;
             }
+            const long p = 2 * I + (long)F.m * ((2 * J + fj) + (long)F.n * (2 * K + fk));
+            if (pair) {
+#pragma unroll
+                for (int e = 0; e < 3; e++) {
+                    Pair *q = reinterpret_cast<Pair *>(xf + p + e * F.N);
+                    Pair u = *q;
+                    u.v[0] += sv[0][e];
+                    u.v[1] += sv[1][e];
+                    *q = u;
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 3; e++) {
+                    xf[p + e * F.N] += sv[0][e];
+                    if (ox) xf[p + 1 + e * F.N] += sv[1][e];
+                }
+            }
+        }
 }
 
 // Galerkin coarse operator A_c = (1/8) P^T A_f P (region::compute_rats,
