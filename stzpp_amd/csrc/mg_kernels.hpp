@@ -231,14 +231,74 @@ __device__ __forceinline__ void restrict_point(const Dims &F, const Dims &C, con
     xc[p + 2 * C.N] = 0;
 }
 
-// Grid: (ceil(mc*nc/256), oc).
+// The same restriction, one thread per coarse column (I, J) marching over
+// kchunk coarse planes.  Fine plane 2K+1 is the +z plane of coarse plane K
+// and the -z plane of K+1 with the same weights, so its nine-point partial
+// sum -- formed from zero in restrict_point's order -- is carried in
+// registers instead of being fetched again.  Each coarse sum is the carried
+// partial plus the 0 and +z planes' terms in restrict_point's order, so the
+// results are identical.
+// Grid: (ceil(mc*nc/256), ceil(oc/kchunk)).
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_restrict(Dims F, Dims C, const T *__restrict__ res,
-                                                       T *__restrict__ rc, T *__restrict__ xc) {
+__global__ __launch_bounds__(kThreads) void k_restrict_march(Dims F, Dims C, const T *__restrict__ res,
+                                                             T *__restrict__ rc, T *__restrict__ xc, int kchunk) {
     const long pl = (long)blockIdx.x * kThreads + threadIdx.x;
     if (pl >= (long)C.m * C.n) return;
     const int J = (int)(pl / C.m), I = (int)(pl - (long)J * C.m);
-    restrict_point<T>(F, C, res, rc, xc, I, J, blockIdx.y);
+    const int K0 = blockIdx.y * kchunk;
+    const int K1 = K0 + kchunk < C.o ? K0 + kchunk : C.o;
+    int fj[3], fi[3];
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
+        fj[e] = fine_of(J, e - 1, F.n, F.py);
+        fi[e] = fine_of(I, e - 1, F.m, F.px);
+    }
+    // terms of fine plane fk with z weight wk added to (s0, s1, s2), and
+    // with PART also to the partial (q0, q1, q2) that starts from zero
+    auto plane = [&](int fk, T wk, T &s0, T &s1, T &s2, bool part, T &q0, T &q1, T &q2) {
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            if (fj[b] < 0) continue;
+            const T wj = b == 1 ? wk : T(0.5) * wk;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                if (fi[a] < 0) continue;
+                const T w = T(0.125) * (a == 1 ? wj : T(0.5) * wj);
+                const long q = fi[a] + (long)F.m * (fj[b] + (long)F.n * fk);
+                const T v0 = res[q], v1 = res[q + F.N], v2 = res[q + 2 * F.N];
+                s0 += w * v0;
+                s1 += w * v1;
+                s2 += w * v2;
+                if (part) {
+                    q0 += w * v0;
+                    q1 += w * v1;
+                    q2 += w * v2;
+                }
+            }
+        }
+    };
+    T c0 = 0, c1 = 0, c2 = 0, d0 = 0, d1 = 0, d2 = 0;  // carried partial of the -z plane
+    {
+        const int fk = fine_of(K0, -1, F.o, F.pz);
+        if (fk >= 0) plane(fk, T(0.5), c0, c1, c2, false, d0, d1, d2);
+    }
+    for (int K = K0; K < K1; K++) {
+        T s0 = c0, s1 = c1, s2 = c2;
+        plane(2 * K, T(1), s0, s1, s2, false, d0, d1, d2);
+        const int fk = fine_of(K, 1, F.o, F.pz);
+        T q0 = 0, q1 = 0, q2 = 0;
+        if (fk >= 0) plane(fk, T(0.5), s0, s1, s2, true, q0, q1, q2);
+        c0 = q0;
+        c1 = q1;
+        c2 = q2;
+        const long p = I + (long)C.m * (J + (long)C.n * K);
+        rc[p] = s0;
+        rc[p + C.N] = s1;
+        rc[p + 2 * C.N] = s2;
+        xc[p] = 0;
+        xc[p + C.N] = 0;
+        xc[p + 2 * C.N] = 0;
+    }
 }
 
 // Coarse parents of fine vertex f along one dimension: f even sits on coarse

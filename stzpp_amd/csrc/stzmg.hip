@@ -407,9 +407,13 @@ template <typename T>
 void launch_restrict(stzmg_t h, int l) {
     Level &F = h->lv[l - 1], &C = h->lv[l];
     launch_resid<T, false>(h, F, x_of<T>(F), r_of<T>(F), t_of<T>(F));
-    dim3 g(nblk((long)C.d.m * C.d.n), C.d.o);
-    hipLaunchKernelGGL((k_restrict<T>), g, dim3(kThreads), 0, h->stream, F.d, C.d,
-                       (const T *)t_of<T>(F), r_of<T>(C), x_of<T>(C));
+    // coarse planes per thread: as many as keep about 1024 workgroups
+    const unsigned bx = nblk((long)C.d.m * C.d.n);
+    int kc = 1;
+    while (kc < 16 && (long)bx * ((C.d.o + 2 * kc - 1) / (2 * kc)) >= 1024) kc *= 2;
+    dim3 g(bx, (C.d.o + kc - 1) / kc);
+    hipLaunchKernelGGL((k_restrict_march<T>), g, dim3(kThreads), 0, h->stream, F.d, C.d,
+                       (const T *)t_of<T>(F), r_of<T>(C), x_of<T>(C), kc);
 }
 
 template <typename T>
