@@ -319,16 +319,39 @@ int fused_tile(const Level &L) {
     return best;
 }
 
+// Workgroups of k_gs_planes<T, PAT, TX, TY, NT> resident on the device at
+// once (CUs x the occupancy its LDS and registers allow), queried once.
+template <typename T, int PAT, int TX, int TY, int NT>
+long fused_resident() {
+    static const long r = [] {
+        int dev = 0, ncu = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+            ncu = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per, reinterpret_cast<const void *>(&k_gs_planes<T, PAT, TX, TY, NT>), NT, 0) != hipSuccess ||
+            per < 1)
+            per = 1;
+        return (long)per * ncu;
+    }();
+    return r;
+}
+
 template <typename T, int PAT, int TX, int TY, int NT>
 void launch_fused_tile(stzmg_t h, Level &L) {
     const Dims &d = L.d;
     T *x = xref_of<T>(L), *xb = xb_of<T>(L);
+    const long tiles = (long)(d.m / TX) * (d.n / TY);
+    const long resident = fused_resident<T, PAT, TX, TY, NT>();
     for (int P = 0; P < 2; P++) {
         const int nP = (d.o - P + 1) / 2;
-        // planes per workgroup: enough workgroups to fill the chip
-        const long tiles = (long)(d.m / TX) * (d.n / TY);
-        int kchunk = 1;
-        while (kchunk < 8 && tiles * ((nP + 2 * kchunk - 1) / (2 * kchunk)) >= 2048) kchunk *= 2;
+        // Planes per workgroup: the z range is cut into as many chunks as
+        // let every tile's chunks be resident at once, so the launch runs as
+        // one round of workgroups and each stages its first planes once
+        // (same-parity planes do not interact: the chunking leaves the
+        // results unchanged).
+        const long nz = std::min<long>(nP, std::max<long>(1, resident / tiles));
+        const int kchunk = (int)((nP + nz - 1) / nz);
         dim3 g(d.m / TX, d.n / TY, (nP + kchunk - 1) / kchunk);
         hipLaunchKernelGGL((k_gs_planes<T, PAT, TX, TY, NT>), g, dim3(NT), 0, h->stream, d, (const T *)x,
                            (const T *)(P == 0 ? x : xb), xb, (const T *)r_of<T>(L), ops_of<T>(L), inv_of<T>(L),
