@@ -172,14 +172,36 @@ bool tiled_ok(const Level &L) {
            L.d.n % ResidTile<double>::TY == 0;
 }
 
-// Grid of the plane-marching residual: planes per workgroup doubled (up to
-// 16, so the two extra staged planes cost at most 1/8 of the reads) while at
-// least 1024 workgroups remain.
+// Workgroups of the plane-marching residual in type T resident on the
+// device at once (CUs x the occupancy its LDS and registers allow; every
+// variant stages the same planes), queried once.
+template <typename T>
+long resid_resident() {
+    static const long r = [] {
+        int dev = 0, ncu = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+            ncu = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per,
+                reinterpret_cast<const void *>(
+                    &k_resid_planes<T, PAT_DENSE, true, false, ResidTile<T>::TX, ResidTile<T>::TY, kThreads>),
+                kThreads, 0) != hipSuccess ||
+            per < 1)
+            per = 1;
+        return (long)per * ncu;
+    }();
+    return r;
+}
+
+// Grid of the plane-marching residual: the z range is cut into as many
+// chunks as let every tile's chunks be resident at once, so a launch runs
+// as one round of workgroups, each staging its first planes once.
 template <typename T>
 dim3 tiled_grid(const Level &L, int *kchunk) {
     const long tiles = (long)(L.d.m / ResidTile<T>::TX) * (L.d.n / ResidTile<T>::TY);
-    int kc = 1;
-    while (kc < 16 && tiles * ((L.d.o + 2 * kc - 1) / (2 * kc)) >= 1024) kc *= 2;
+    const long nz = std::min<long>(L.d.o, std::max<long>(1, resid_resident<T>() / tiles));
+    const int kc = (int)((L.d.o + nz - 1) / nz);
     *kchunk = kc;
     return dim3(L.d.m / ResidTile<T>::TX, L.d.n / ResidTile<T>::TY, (L.d.o + kc - 1) / kc);
 }
