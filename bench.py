@@ -34,12 +34,12 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernel (fine-level FP32 fused
 # smoother, one z parity of the 256^3 workload), from two rocprofv3 --pmc
 # runs of `python3 bench.py --steps 1 --warmup 0 --no-cpu --no-graph`:
-# FETCH_SIZE 156.3 MiB x 2.0 + WRITE_SIZE 96.0 MiB x 1.0 per dispatch.  The
+# FETCH_SIZE 154.5 MiB x 2.0 + WRITE_SIZE 96.0 MiB x 1.0 per dispatch.  The
 # factors come from the plain copy kernel k_convert<double,double> of the
 # same run (3 x 256^3 doubles = 384 MiB each way), which reads FETCH_SIZE
 # 192.0 MiB and WRITE_SIZE 384.0 MiB.  Summary:
 # profiles/mg_solve_256_trans_kernels.txt.
-TRAFFIC_256_BYTES_PER_LAUNCH = (156.3 * 2.0 + 96.0) * 2 ** 20
+TRAFFIC_256_BYTES_PER_LAUNCH = (154.5 * 2.0 + 96.0) * 2 ** 20
 
 
 def log(msg):
