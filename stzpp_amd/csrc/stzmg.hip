@@ -264,7 +264,10 @@ int ncol(int n, int prd) { return (prd && (n & 1)) ? 3 : 2; }
 // Tile and workgroup size of the fused smoother: 64 x 8 in FP32 and
 // 32 x 16 in FP64.  The three staged planes (3 comps, 4-point halo) take
 // 41 KB and 55 KB of LDS.  Measured on the 256^3 FP32 sweep (MI355X): 64 x 8
-// 0.42 ms, 64 x 16 0.44 ms, 32 x 8 (128 threads) 0.53 ms, 64 x 4 0.66 ms.
+// 0.42 ms, 64 x 16 0.44 ms, 32 x 8 (128 threads) 0.53 ms, 64 x 4 0.66 ms;
+// with launches of one resident round: 64 x 8 0.37 ms, 64 x 8 with 512
+// threads 0.44 ms, 64 x 16 with 512 threads 0.51 ms (and the FP64 32 x 16
+// sweep 0.58 ms with 256 threads, 0.96 ms with 512).
 #ifndef STZ_F32_TX
 #define STZ_F32_TX 64
 #endif
