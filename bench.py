@@ -446,8 +446,8 @@ def main():
                          "output: the reference's output cubes at 192x192x96")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(one process drives one GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.workload == "step":
         return step_workload(args)
