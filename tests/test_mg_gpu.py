@@ -411,6 +411,27 @@ def test_gpu_fp32_gmres_matches_fp64(kind, mode):
     assert cyc < 40
 
 
+@pytest.mark.parametrize("restart,inner_rtol,inner", [(3, 1e-6, 1), (2, 1e-5, 2)])
+def test_gpu_gmres_ir_short_cycles(restart, inner_rtol, inner):
+    # a short restart length ends each FP32 GMRES cycle before inner_rtol is
+    # reached, so the solve takes several refinement steps, each starting a new
+    # unnormalised FP32 basis: it still reaches 1e-11 and the FP64 solution
+    n = 32
+    mg, b = make_c1(n)
+    N = b.shape[0]
+    mg.setup_fields(np.zeros((N, 3)), b)
+    s64 = mg.solve(mode=MG.SOLVE_RELRES, rtol=1e-11)
+    x64 = mg.x(0)
+    mg.setup_fields(np.zeros((N, 3)), b)
+    s = mg.solve(mode=MG.SOLVE_FP32_GMRES_IR, rtol=1e-11, restart=restart, inner_rtol=inner_rtol,
+                 inner_cycles=inner, max_cycles=400)
+    assert s["res"] <= 1e-11 * s["res0"]
+    assert s64["res"] <= 1e-11 * s64["res0"]
+    # every refinement step ran at most restart GMRES iterations
+    assert s["outer"] >= 3 and s["cycles"] <= s["outer"] * restart * inner
+    assert relerr(mg.x(0), x64) < 1e-9
+
+
 def test_gpu_solve_ref_mode_reference_rule():
     n = 32
     mg, b = make_c1(n)
