@@ -150,6 +150,12 @@ int stzmg_norm(stzmg_t h, int level, int kind, double *out);
 int stzmg_vcycle(stzmg_t h, int n);
 /* One FP32 V-cycle on level-0 FP32 fields (x32, r32), for measurement. */
 int stzmg_vcycle_fp32(stzmg_t h, int n);
+/* The FP32 smoother and residual of the iterative-refinement inner solve on
+ * a level (no reference counterpart; for parity tests): the level's FP64 x
+ * and r rounded to FP32, then nsweeps FP32 sweeps with x widened back into
+ * the FP64 x; or r - A x in FP32, widened and copied to host AoS. */
+int stzmg_gauss_seidel_fp32(stzmg_t h, int level, int nsweeps);
+int stzmg_residual_fp32(stzmg_t h, int level, double *out);
 /* The projection solve loop (src/shear_sim_3d.cpp:567-600) and the added
  * FP32 + FP64 iterative-refinement mode. */
 int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *stats);
@@ -161,7 +167,7 @@ void stzmg_solve_opts_default(stzmg_solve_opts *o);
  *                            the built-in threshold, 0 launches every pass
  *                            separately. */
 enum { STZMG_OPT_COARSE_KERNEL = 0, STZMG_OPT_FUSED_SMOOTHER = 1, STZMG_OPT_TILED_RESIDUAL = 2,
-       STZMG_OPT_BOTTOM_MAP = 3 };
+       STZMG_OPT_BOTTOM_MAP = 3, STZMG_OPT_CHUNK_CAP = 4, STZMG_OPT_RESTRICT_PLANES = 5 };
 /*   STZMG_OPT_FUSED_SMOOTHER 1 (default): per-plane levels with periodic x, y
  *                            (x and y sizes multiples of 8, more than 1024
  *                            points) sweep with two fused launches (one per z
@@ -181,7 +187,17 @@ enum { STZMG_OPT_COARSE_KERNEL = 0, STZMG_OPT_FUSED_SMOOTHER = 1, STZMG_OPT_TILE
  *                            workgroup.  Building Q costs a few milliseconds
  *                            per operator; 0 keeps the sweeps (for operators
  *                            that change every solve).  Results differ in the
- *                            last bits (another summation order). */
+ *                            last bits (another summation order).
+ *   STZMG_OPT_CHUNK_CAP      0 (default): the fused sweeps and the
+ *                            plane-marching residual cut z into as many
+ *                            chunks as the device keeps resident at once;
+ *                            value > 0 caps that count at `value` workgroups,
+ *                            so small grids also march several planes per
+ *                            workgroup (sweeps unchanged; residual norms may
+ *                            differ in the last bits).
+ *   STZMG_OPT_RESTRICT_PLANES 0 (default): coarse planes per restriction
+ *                            thread chosen from the grid; value > 0 fixes it
+ *                            (results unchanged). */
 int stzmg_set_option(stzmg_t h, int option, int value);
 
 /* The HIP stream (hipStream_t) the handle launches on. */

@@ -137,6 +137,8 @@ SIGNATURES = {
     "stzmg_norm": ([_vp, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
     "stzmg_vcycle": ([_vp, ctypes.c_int], ctypes.c_int),
     "stzmg_vcycle_fp32": ([_vp, ctypes.c_int], ctypes.c_int),
+    "stzmg_gauss_seidel_fp32": ([_vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "stzmg_residual_fp32": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
     "stzmg_solve": ([_vp, ctypes.POINTER(StzmgSolveOpts), ctypes.POINTER(StzmgSolveStats)], ctypes.c_int),
     "stzmg_solve_opts_default": ([ctypes.POINTER(StzmgSolveOpts)], None),
     "stzmg_stream": ([_vp], _vp),

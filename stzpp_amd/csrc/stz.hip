@@ -329,10 +329,10 @@ int stz_get_state(stz_t s, double *t, double *u, double *sig, double *chi, doubl
 
 namespace stz_detail {
 
-void step_options(stz_t s, const stzmg_solve_opts *opts, stzmg_solve_opts &o) {
+bool step_options(stz_t s, const stzmg_solve_opts *opts, stzmg_solve_opts &o) {
     if (opts && !(opts->mode == STZMG_SOLVE_REF && !(opts->tol > 0))) {
         o = *opts;
-        return;
+        return false;
     }
     // the reference's loop; with opts (mode STZMG_SOLVE_REF, tol 0) its
     // use_graph is kept
@@ -347,6 +347,7 @@ void step_options(stz_t s, const stzmg_solve_opts *opts, stzmg_solve_opts &o) {
     // (src/trans_sim_3d.cpp:1020-1040)
     o.ref_norm_schedule = s->p.transform != STZ_T_NONE ? 1 : 0;
     o.max_cycles = 100000;
+    return true;
 }
 
 int step_once(stz_t s, double dt, stzmg_solve_opts &o, bool ref_tol, stz_step_stats &S, double *sigstar) {
@@ -437,9 +438,11 @@ int stz_step_qs(stz_t s, double dt, int nsteps, const stzmg_solve_opts *opts, st
     stz_step_stats S{};
     auto t0 = std::chrono::steady_clock::now();
     stzmg_solve_opts o;
-    stz_detail::step_options(s, opts, o);
+    // the reference's tolerance follows the transformed operator, which is
+    // rebuilt every step (the untransformed one is fixed by dt)
+    const bool ref_loop = stz_detail::step_options(s, opts, o);
     for (int q = 0; q < nsteps; q++)
-        if (int rc = stz_detail::step_once(s, dt, o, trans && !opts, S, nullptr)) return rc;
+        if (int rc = stz_detail::step_once(s, dt, o, trans && ref_loop, S, nullptr)) return rc;
     S.step_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (st) *st = S;
     return 0;

@@ -44,8 +44,10 @@ stz::Xform xform(const stz_t s, double t);
 int transposed_trans_operator(stz_t s, double dt, double t);
 // Projection operator of the untransformed step for step size dt.
 int ensure_operator(stz_t s, double dt);
-// Solve options of a step call: opts, or the reference's loop when null.
-void step_options(stz_t s, const stzmg_solve_opts *opts, stzmg_solve_opts &o);
+// Solve options of a step call: opts, or the reference's loop when opts is
+// null or asks for it (mode STZMG_SOLVE_REF with tol 0).  Returns true when
+// the reference's loop was chosen: its tolerance then follows the operator.
+bool step_options(stz_t s, const stzmg_solve_opts *opts, stzmg_solve_opts &o);
 // One quasi-static step on the handle's stream.  ref_tol: recompute the
 // reference tolerance from the current operator (transformed step).  When
 // sigstar is not null it receives the advected stress sigma* (6 Nc doubles,

@@ -64,6 +64,8 @@ struct stzmg_s {
     int fused_smoother = 1;   // two-launch fused sweeps on eligible levels
     int tiled_resid = 1;      // plane-marching residual on eligible levels
     int bottom_map = 1;       // coarsest solve as x = Q r (mg_coarse.hpp)
+    long chunk_cap = 0;       // cap on the workgroups the z-chunked launches assume resident (0: none)
+    int restrict_planes = 0;  // coarse planes per restriction thread (0: chosen from the grid)
     double *qmap = nullptr;   // Q, row-major, qmap_n x qmap_n
     long qmap_n = 0;
     bool qmap_ok = false;     // Q belongs to the current operator
@@ -198,9 +200,10 @@ long resid_resident() {
 // chunks as let every tile's chunks be resident at once, so a launch runs
 // as one round of workgroups, each staging its first planes once.
 template <typename T>
-dim3 tiled_grid(const Level &L, int *kchunk) {
+dim3 tiled_grid(const Level &L, long cap, int *kchunk) {
     const long tiles = (long)(L.d.m / ResidTile<T>::TX) * (L.d.n / ResidTile<T>::TY);
-    const long nz = std::min<long>(L.d.o, std::max<long>(1, resid_resident<T>() / tiles));
+    const long resident = cap > 0 ? std::min(cap, resid_resident<T>()) : resid_resident<T>();
+    const long nz = std::min<long>(L.d.o, std::max<long>(1, resident / tiles));
     const int kc = (int)((L.d.o + nz - 1) / nz);
     *kchunk = kc;
     return dim3(L.d.m / ResidTile<T>::TX, L.d.n / ResidTile<T>::TY, (L.d.o + kc - 1) / kc);
@@ -208,10 +211,10 @@ dim3 tiled_grid(const Level &L, int *kchunk) {
 
 // Number of norm partials the residual of level L writes.
 template <typename T>
-long resid_parts(const Level &L) {
+long resid_parts(stzmg_t h, const Level &L) {
     if (L.tiled) {
         int kc;
-        const dim3 g = tiled_grid<T>(L, &kc);
+        const dim3 g = tiled_grid<T>(L, h->chunk_cap, &kc);
         return (long)g.x * g.y * g.z;
     }
     return (long)nblk((long)L.d.m * L.d.n) * L.d.o;
@@ -220,7 +223,7 @@ long resid_parts(const Level &L) {
 template <typename T, bool NORM, bool APPLY, int PAT>
 void launch_resid_tiled(stzmg_t h, Level &L, const T *x, const T *r, T *out, T *xcopy) {
     int kc;
-    const dim3 g = tiled_grid<T>(L, &kc);
+    const dim3 g = tiled_grid<T>(L, h->chunk_cap, &kc);
     hipLaunchKernelGGL((k_resid_planes<T, PAT, NORM, APPLY, ResidTile<T>::TX, ResidTile<T>::TY, kThreads>), g,
                        dim3(kThreads), 0, h->stream, L.d, x, r, ops_of<T>(L), out, h->part, kc, xcopy);
 }
@@ -263,7 +266,8 @@ int ncol(int n, int prd) { return (prd && (n & 1)) ? 3 : 2; }
 
 // Tile and workgroup size of the fused smoother: 64 x 8 in FP32 and
 // 32 x 16 in FP64.  The three staged planes (3 comps, 4-point halo) take
-// 41 KB and 55 KB of LDS.  Measured on the 256^3 FP32 sweep (MI355X): 64 x 8
+// 3 x 3 x 72 x 16 x 4 B = 40.5 KiB and 3 x 3 x 40 x 24 x 8 B = 67.5 KiB of
+// LDS (three and two workgroups per CU).  Measured on the 256^3 FP32 sweep (MI355X): 64 x 8
 // 0.42 ms, 64 x 16 0.44 ms, 32 x 8 (128 threads) 0.53 ms, 64 x 4 0.66 ms;
 // with launches of one resident round: 64 x 8 0.37 ms, 64 x 8 with 512
 // threads 0.44 ms, 64 x 16 with 512 threads 0.51 ms (and the FP64 32 x 16
@@ -367,7 +371,8 @@ void launch_fused_tile(stzmg_t h, Level &L) {
     const Dims &d = L.d;
     T *x = xref_of<T>(L), *xb = xb_of<T>(L);
     const long tiles = (long)(d.m / TX) * (d.n / TY);
-    const long resident = fused_resident<T, PAT, TX, TY, NT>();
+    const long dev_resident = fused_resident<T, PAT, TX, TY, NT>();
+    const long resident = h->chunk_cap > 0 ? std::min(h->chunk_cap, dev_resident) : dev_resident;
     for (int P = 0; P < 2; P++) {
         const int nP = (d.o - P + 1) / 2;
         // Planes per workgroup: the z range is cut into as many chunks as
@@ -459,6 +464,7 @@ void launch_restrict(stzmg_t h, int l) {
     const unsigned bx = nblk((long)C.d.m * C.d.n);
     int kc = 1;
     while (kc < 16 && (long)bx * ((C.d.o + 2 * kc - 1) / (2 * kc)) >= 1024) kc *= 2;
+    if (h->restrict_planes > 0) kc = h->restrict_planes;
     dim3 g(bx, (C.d.o + kc - 1) / kc);
     hipLaunchKernelGGL((k_restrict_march<T>), g, dim3(kThreads), 0, h->stream, F.d, C.d,
                        (const T *)t_of<T>(F), r_of<T>(C), x_of<T>(C), kc);
@@ -607,7 +613,7 @@ int ensure_part(stzmg_t h, long nb) {
 // The residual is left in t64 of that level.
 int norms64(stzmg_t h, int l, double out[2]) {
     Level &L = h->lv[l];
-    const long nb = resid_parts<double>(L);
+    const long nb = resid_parts<double>(h, L);
     if (ensure_part(h, nb)) return STZMG_ERR_NOMEM;
     launch_resid<double, true>(h, L, L.x64, L.r64, L.t64);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, h->stream, (const double *)h->part, nb,
@@ -942,6 +948,12 @@ int stzmg_set_option(stzmg_t h, int option, int value) {
     } else if (option == STZMG_OPT_BOTTOM_MAP) {
         h->bottom_map = value != 0;
         h->qmap_ok = false;  // (re)built at the next V-cycle when on
+    } else if (option == STZMG_OPT_CHUNK_CAP) {
+        if (value < 0) return fail(h, STZMG_ERR_ARG, "chunk cap must be >= 0");
+        h->chunk_cap = value;
+    } else if (option == STZMG_OPT_RESTRICT_PLANES) {
+        if (value < 0) return fail(h, STZMG_ERR_ARG, "restriction planes must be >= 0");
+        h->restrict_planes = value;
     } else {
         return fail(h, STZMG_ERR_ARG, "unknown option %d", option);
     }
@@ -1067,6 +1079,56 @@ int stzmg_gauss_seidel(stzmg_t h, int level, int nsweeps) {
     restore_level<double>(h, h->lv[level], home);
     LAUNCHCHK(h);
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+namespace {
+// Level fields rounded to FP32 (x32 <- x64, r32 <- r64).
+void level_to_fp32(stzmg_t h, Level &L) {
+    const long n3 = 3 * L.d.N;
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream,
+                       (const double *)L.x64, L.x32, n3);
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream,
+                       (const double *)L.r64, L.r32, n3);
+}
+}  // namespace
+
+int stzmg_gauss_seidel_fp32(stzmg_t h, int level, int nsweeps) {
+    if (int rc = check_level(h, level)) return rc;
+    if (int rc = require_ops(h)) return rc;
+    if (nsweeps < 0) return fail(h, STZMG_ERR_ARG, "nsweeps must be >= 0");
+    Level &L = h->lv[level];
+    float *home = L.x32;
+    level_to_fp32(h, L);
+    for (int q = 0; q < nsweeps; q++) launch_gs_sweep<float>(h, L);
+    restore_level<float>(h, L, home);
+    const long n3 = 3 * L.d.N;
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream,
+                       (const float *)L.x32, L.x64, n3);
+    LAUNCHCHK(h);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int stzmg_residual_fp32(stzmg_t h, int level, double *out) {
+    if (int rc = check_level(h, level)) return rc;
+    if (int rc = require_ops(h)) return rc;
+    if (!out) return STZMG_ERR_ARG;
+    Level &L = h->lv[level];
+    const long N = L.d.N;
+    level_to_fp32(h, L);
+    launch_resid<float, false>(h, L, L.x32, L.r32, L.t32);
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(3 * N)), dim3(kThreads), 0, h->stream,
+                       (const float *)L.t32, L.t64, 3 * N);
+    LAUNCHCHK(h);
+    std::vector<double> soa((size_t)3 * N);
+    HIPCHK(h, hipMemcpyAsync(soa.data(), L.t64, sizeof(double) * soa.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (long p = 0; p < N; p++) {
+        out[3 * p] = soa[p];
+        out[3 * p + 1] = soa[p + N];
+        out[3 * p + 2] = soa[p + 2 * N];
+    }
     return 0;
 }
 

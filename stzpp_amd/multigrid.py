@@ -20,6 +20,7 @@ PLANE, POINT = 0, 1
 NORM_QUARTIC, NORM_L2SQ = 0, 1
 SOLVE_REF, SOLVE_RELRES, SOLVE_FP32_IR, SOLVE_FP32_GMRES, SOLVE_FP32_GMRES_IR = 0, 1, 2, 3, 4
 OPT_COARSE_KERNEL, OPT_FUSED_SMOOTHER, OPT_TILED_RESIDUAL, OPT_BOTTOM_MAP = 0, 1, 2, 3
+OPT_CHUNK_CAP, OPT_RESTRICT_PLANES = 4, 5
 
 
 class MultigridError(RuntimeError):
@@ -134,6 +135,17 @@ class Multigrid:
     # -- level operations (multigrid.cc:412-458) -------------------------------
     def gauss_seidel(self, level=0, nsweeps=1):
         self._chk(self.L.stzmg_gauss_seidel(self.h, level, nsweeps))
+
+    def gauss_seidel_fp32(self, level=0, nsweeps=1):
+        """nsweeps sweeps of the FP32 smoother of the refinement's inner solve
+        on the level's fields rounded to FP32; x is widened back to FP64."""
+        self._chk(self.L.stzmg_gauss_seidel_fp32(self.h, level, nsweeps))
+
+    def residual_fp32(self, level=0):
+        """r - A x in FP32 on the level's fields rounded to FP32."""
+        out = np.zeros((self.npts(level), 3))
+        self._chk(self.L.stzmg_residual_fp32(self.h, level, _lib.dptr(out)))
+        return out
 
     def restriction(self, level):
         self._chk(self.L.stzmg_restriction(self.h, level))

@@ -198,6 +198,27 @@ def test_gpu_trans_step_within_reference_solve_error(golden):
         assert relerr(a[key], g[key]) < 1e-4, key
 
 
+def test_gpu_trans_step_graph_and_launches_agree(golden):
+    # the reference loop launched kernel by kernel (explicit options, mode
+    # REF with tol 0) and replayed from the captured graph: the stopping
+    # tolerance follows each step's operator in both, from the first step of
+    # a fresh simulation on, so the V-cycle counts and the fields agree
+    g = golden("trans_step_walled.npz")
+    N = tuple(int(v) for v in g["N"])
+    prd = tuple(int(v) for v in g["prd"])
+    out = []
+    for use_graph in (True, False):
+        sim = SS.TransSim(*N, float(g["Ut"]), family="pure_shear", params=params_of(g), prd=prd)
+        sim.set_initial_conditions(g["chi0"], u=g["u0"], sig=g["sig0"])
+        cyc = [sim.step_forward_qs(float(g["dt"]), 1, use_graph=use_graph)["vcycles"]
+               for _ in range(int(g["nsteps"]))]
+        out.append((sim.state(), cyc))
+    (a, ca), (b, cb) = out
+    assert ca == cb
+    for key in KEYS:
+        assert relerr(a[key], b[key]) < 1e-12, key
+
+
 @pytest.mark.parametrize("family", ["simple_shear", "homogeneous", "identity"])
 def test_gpu_trans_families_match_oracle(golden, family):
     # the other transform families: the simple shear couples x and z in the
