@@ -1,0 +1,487 @@
+// Z-marching fused 8-colour block Gauss-Seidel for per-plane operators on
+// levels periodic in x and y, with x a multiple of the tile width (32 or 64).
+//
+// Same colour order and the same halo recomputation as k_gs_planes
+// (mg_fused.hpp): one launch updates every plane of one z parity P, a
+// workgroup owns a TX x TY tile and recomputes a shrinking halo (3, 2, 1, 0)
+// over the four in-plane colours, reading x_cur / x_new and writing x_new.
+// What differs is how the work reaches the hardware:
+//
+// * The planes k-1 and k+1 (the other parity) do not change during the
+//   launch, so their part of the stencil sum is taken once per point before
+//   the in-plane sub-passes ("phase Z"); the sub-passes then only add the
+//   eight in-plane blocks.
+// * Coefficients live in scalar registers: each phase loads its nonzero
+//   coefficients in groups of at most kGroupRegs registers through the
+//   scalar cache, one wait per group, and the multiply-adds take them as
+//   scalar operands (no per-point broadcast instruction).
+// * Staging reads one window row per wave instruction: row addresses are
+//   scalar, the per-lane column offsets (with the periodic wrap) are computed
+//   once per workgroup, so a staged value costs one load and one LDS store.
+// * The global loads of the next two planes are issued before the
+//   arithmetic of the current plane and stored to LDS after it.
+#pragma once
+
+#include <type_traits>
+#include <utility>
+
+#include "mg_fused.hpp"
+
+namespace stzmg {
+
+template <int I>
+using IC = std::integral_constant<int, I>;
+
+template <typename F, int... Is>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, Is...>) {
+    (f(IC<Is>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// Blocks of the two phases: Z = the 18 blocks with dk != 0, then the eight
+// in-plane blocks and the inverse centre block (index 27, 9 entries).
+__host__ __device__ constexpr int march_block(int phase, int i) {
+    return phase == 0 ? (i < 9 ? i : 9 + i) : (i < 4 ? 9 + i : (i < 8 ? 10 + i : 27));
+}
+__host__ __device__ constexpr int march_len(int phase) { return phase == 0 ? 18 : 9; }
+__host__ __device__ constexpr unsigned march_mask(int pat, int b) { return b == 27 ? 0x1FFu : pat_mask(pat, b); }
+
+// Greedy grouping of a phase's blocks: a new group starts where adding the
+// block would exceed G coefficients.
+__host__ __device__ constexpr int march_group(int pat, int phase, int G, int i) {
+    int g = 0, run = 0;
+    for (int q = 0; q <= i; q++) {
+        const int nz = popc9(march_mask(pat, march_block(phase, q)));
+        if (run > 0 && run + nz > G) {
+            g++;
+            run = 0;
+        }
+        run += nz;
+    }
+    return g;
+}
+__host__ __device__ constexpr int march_ngroups(int pat, int phase, int G) {
+    return march_group(pat, phase, G, march_len(phase) - 1) + 1;
+}
+// Coefficients in group g, and position of block i's first coefficient in it.
+__host__ __device__ constexpr int march_gsize(int pat, int phase, int G, int g) {
+    int n = 0;
+    for (int q = 0; q < march_len(phase); q++)
+        if (march_group(pat, phase, G, q) == g) n += popc9(march_mask(pat, march_block(phase, q)));
+    return n;
+}
+__host__ __device__ constexpr int march_pos(int pat, int phase, int G, int i) {
+    const int g = march_group(pat, phase, G, i);
+    int n = 0;
+    for (int q = 0; q < i; q++)
+        if (march_group(pat, phase, G, q) == g) n += popc9(march_mask(pat, march_block(phase, q)));
+    return n;
+}
+// rank of entry e among the set bits of m
+__host__ __device__ constexpr int rank9(unsigned m, int e) {
+    int n = 0;
+    for (int q = 0; q < e; q++) n += (m >> q) & 1;
+    return n;
+}
+
+// Packed coefficient table of a plane: the nonzero entries of pattern PAT in
+// phase order (phase Z blocks, then the in-plane blocks, then the inverse
+// centre block), so every group is a contiguous run (wide scalar loads).
+__host__ __device__ constexpr int march_count(int pat, int phase) {
+    int n = 0;
+    for (int q = 0; q < march_len(phase); q++) n += popc9(march_mask(pat, march_block(phase, q)));
+    return n;
+}
+__host__ __device__ constexpr int march_stride(int pat) { return march_count(pat, 0) + march_count(pat, 1); }
+// Source of packed entry j: 9 b + e for operator entries (b < 27), 243 + e
+// for the inverse block.
+__host__ __device__ constexpr int march_src(int pat, int j) {
+    for (int ph = 0; ph < 2; ph++)
+        for (int q = 0; q < march_len(ph); q++) {
+            const int b = march_block(ph, q);
+            const unsigned m = march_mask(pat, b);
+            const int n = popc9(m);
+            if (j < n) return b == 27 ? 243 + nth_bit(m, j) : 9 * b + nth_bit(m, j);
+            j -= n;
+        }
+    return 0;
+}
+// Offset of group g of a phase in the packed table.
+__host__ __device__ constexpr int march_goff(int pat, int phase, int G, int g) {
+    int n = phase == 0 ? 0 : march_count(pat, 0);
+    for (int q = 0; q < g; q++) n += march_gsize(pat, phase, G, q);
+    return n;
+}
+
+// pk[c * stride + j] = entry march_src(j) of class c (operator or inverse).
+template <typename T>
+__global__ void k_pack_march(const T *__restrict__ ops, const T *__restrict__ inv, const int *__restrict__ src,
+                             int stride, long ncls, T *__restrict__ pk) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncls * stride) return;
+    const long c = t / stride;
+    const int j = (int)(t - c * stride), q = src[j];
+    pk[t] = q < kBlk ? ops[c * kBlk + q] : inv[c * 9 + (q - kBlk)];
+}
+
+// Scalar registers per coefficient group (a double takes two).
+template <typename T>
+struct MarchGroup {
+    static constexpr int regs = 48;
+    static constexpr int G = regs / (int)(sizeof(T) / 4);
+};
+
+// Loads group g of a phase into scalar registers: the addresses are uniform
+// and the operator read-only, so the loads go through the scalar cache, and
+// every value passes through an empty asm that takes it in a scalar
+// register, so the group is complete and held there from this point on
+// (one wait for the group).
+template <typename T, int PAT, int PHASE, int GRP, int N>
+__device__ __forceinline__ void march_load(const T *__restrict__ pk, T (&c)[N]) {
+    constexpr int G = MarchGroup<T>::G;
+    constexpr int off = march_goff(PAT, PHASE, G, GRP);
+    static_for<N>([&](auto q) { c[q] = pk[off + decltype(q)::value]; });
+#pragma unroll
+    for (int q = 0; q < N; q++) asm volatile("" : "+s"(c[q]));
+}
+
+// Adds block (phase, i) of group GRP times the neighbour values of one point.
+// ld(c, di, dj, dk) reads component c of the neighbour.
+template <typename T, int PAT, int PHASE, int I, int N, typename LD>
+__device__ __forceinline__ void march_block_acc(const T (&c)[N], const LD &ld, T &a0, T &a1, T &a2) {
+    constexpr int G = MarchGroup<T>::G;
+    constexpr int b = march_block(PHASE, I);
+    constexpr unsigned m = march_mask(PAT, b);
+    if constexpr (b != 27 && m != 0) {
+        constexpr int di = b % 3 - 1, dj = (b / 3) % 3 - 1, dk = b / 9 - 1;
+        constexpr unsigned cols = pat_cols(m);
+        constexpr int q0 = march_pos(PAT, PHASE, G, I);
+        T v0 = 0, v1 = 0, v2 = 0;
+        if constexpr (cols & 1) v0 = ld(0, di, dj, dk);
+        if constexpr (cols & 2) v1 = ld(1, di, dj, dk);
+        if constexpr (cols & 4) v2 = ld(2, di, dj, dk);
+        if constexpr (m & 0x001) a0 += c[q0 + rank9(m, 0)] * v0;
+        if constexpr (m & 0x002) a0 += c[q0 + rank9(m, 1)] * v1;
+        if constexpr (m & 0x004) a0 += c[q0 + rank9(m, 2)] * v2;
+        if constexpr (m & 0x008) a1 += c[q0 + rank9(m, 3)] * v0;
+        if constexpr (m & 0x010) a1 += c[q0 + rank9(m, 4)] * v1;
+        if constexpr (m & 0x020) a1 += c[q0 + rank9(m, 5)] * v2;
+        if constexpr (m & 0x040) a2 += c[q0 + rank9(m, 6)] * v0;
+        if constexpr (m & 0x080) a2 += c[q0 + rank9(m, 7)] * v1;
+        if constexpr (m & 0x100) a2 += c[q0 + rank9(m, 8)] * v2;
+    }
+}
+
+// Waves per SIMD that the LDS of one workgroup (three staged window planes)
+// allows on a CU of 160 KiB: the register allocation is held to that
+// occupancy.
+template <typename T, int TX, int TY, int NT>
+constexpr int march_waves() {
+    constexpr int lds = 3 * 3 * (TX + 8) * (TY + 8) * (int)sizeof(T);
+    constexpr int wgs = (160 * 1024) / lds;
+    constexpr int w = wgs * NT / 64 / 4;
+    return w < 1 ? 1 : (w > 8 ? 8 : w);
+}
+
+// Grid: (m/TX, n/TY, ceil(planes of parity P / kchunk)).  Requires a PLANE
+// operator, x and y periodic, m % TX == 0, n % TY == 0, TX in {32, 64} and
+// every sub-pass of the tile at most NT points.  pk: the packed table of the
+// level (k_pack_march), one row of march_stride(PAT) values per plane.
+template <typename T, int PAT, int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<T, TX, TY, NT>()))) void k_gs_march(Dims d, const T *__restrict__ xs, const T *__restrict__ xo,
+                                                 T *__restrict__ xd, const T *__restrict__ r,
+                                                 const T *__restrict__ pk, int P, int kchunk) {
+    constexpr int H = 4;                   // 3 recomputed halo points + the stencil reach
+    constexpr int WX = TX + 2 * H, WY = TY + 2 * H;
+    constexpr int HX = WX / 2;             // x is stored split by parity
+    constexpr int CSZ = WX * WY;           // one component of one plane
+    constexpr int PSZ = 3 * CSZ;           // one plane slot
+    constexpr int NW = NT / 64;            // waves
+    constexpr int RPI = 64 / TX;           // window rows per chunk-A instruction
+    static_assert(TX == 32 || TX == 64, "tile width");
+    static_assert(TY % 2 == 0 && NT % 64 == 0, "tile shape");
+    // chunk A: window columns [0, TX), RPI rows per instruction; chunk B:
+    // the last 2H columns, one element per thread and instruction
+    constexpr int QA = 3 * WY / RPI, NA = QA / NW;
+    constexpr int EB = 3 * WY * 2 * H, NB = (EB + NT - 1) / NT;
+    // write-back of the TX x TY tile: RPI rows per instruction
+    constexpr int QW = 3 * TY / RPI, NWB = (QW + NW - 1) / NW;
+    using SP = SubPass<TX, TY, NT>;
+    static_assert(SP::npts(0) <= NT, "one point per thread and sub-pass");
+    constexpr int G = MarchGroup<T>::G;
+    __shared__ T lds[3 * PSZ];
+
+    int tx, ty, tz;
+    xcd_tile(tx, ty, tz);
+    const int x0 = tx * TX, y0 = ty * TY;
+    const int nP = (d.o - P + 1) / 2;
+    const int kk0 = tz * kchunk;
+    const int kk1 = kk0 + kchunk < nP ? kk0 + kchunk : nP;
+    if (kk0 >= kk1) return;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const long mn = (long)d.m * d.n;
+
+    // ---- per-thread geometry, once per workgroup ----
+    // chunk A: column of this lane and its row within an instruction.  Wave
+    // wv reads the window rows q = wv + NW i (RPI rows each); rows are
+    // contiguous in LDS (row q at q RPI WX), and their global offsets take
+    // YA distinct values per component, kept per lane in ga[].
+    static_assert(WY % (NW * RPI) == 0, "window rows per wave");
+    constexpr int YA = WY / (NW * RPI);
+    const int ca = lane % TX, rsub = lane / TX;
+    unsigned ga[YA];
+#pragma unroll
+    for (int j = 0; j < YA; j++)
+        ga[j] = (unsigned)(wrap1(x0 - H + ca, d.m) +
+                           (long)d.m * wrap1(y0 - H + (wv + NW * j) * RPI + rsub, d.n));
+    const int la_off = (wv * RPI + rsub) * WX + (ca & 1) * HX + (ca >> 1);
+    // chunk B: up to NB elements of the right strip
+    unsigned vb_off[NB];
+    int lb_off[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+        const int e = tid + NT * j < EB ? tid + NT * j : EB - 1;
+        const int row = e / (2 * H), col = TX + e % (2 * H);
+        const int c = row / WY, yy = row - c * WY;
+        vb_off[j] = (unsigned)(c * d.N + (long)d.m * wrap1(y0 - H + yy, d.n) + wrap1(x0 - H + col, d.m));
+        lb_off[j] = c * CSZ + yy * WX + (col & 1) * HX + (col >> 1);
+    }
+    // sub-pass points: LDS index of the point (component 0, slot 0), its
+    // offset in a plane of r, and whether this thread owns one
+    int pb[4];
+    unsigned ro[4];
+    static_for<4>([&](auto s_) {
+        constexpr int s = decltype(s_)::value;
+        const int t = tid < SP::npts(s) ? tid : SP::npts(s) - 1;
+        const int tyy = t / SP::nx(s), txx = t - tyy * SP::nx(s);
+        const int xx = SP::xf(s) + 2 * txx + H, yy = SP::yf(s) + 2 * tyy + H;
+        pb[s] = yy * WX + (xx & 1) * HX + (xx >> 1);
+        ro[s] = (unsigned)(wrap1(x0 + xx - H, d.m) + (long)d.m * wrap1(y0 + yy - H, d.n));
+    });
+
+    // ---- staging ----
+    struct Stage {
+        T a[NA];
+        T b[NB];
+    };
+    auto fetch = [&](const T *__restrict__ src, int kg, Stage &v) {
+        bool valid = true;
+        int kw = kg;
+        if (kg < 0 || kg >= d.o) {
+            if (d.pz) kw = kg < 0 ? kg + d.o : kg - d.o;
+            else valid = false;
+        }
+        if (valid) {
+            const T *base = src + mn * kw;
+#pragma unroll
+            for (int i = 0; i < NA; i++) v.a[i] = (base + (i / YA) * d.N)[ga[i % YA]];
+#pragma unroll
+            for (int j = 0; j < NB; j++) v.b[j] = base[vb_off[j]];
+        } else {
+#pragma unroll
+            for (int i = 0; i < NA; i++) v.a[i] = 0;
+#pragma unroll
+            for (int j = 0; j < NB; j++) v.b[j] = 0;
+        }
+    };
+    auto store = [&](int slot, const Stage &v) {
+        T *sl = lds + slot * PSZ;
+#pragma unroll
+        for (int i = 0; i < NA; i++) sl[la_off + i * NW * RPI * WX] = v.a[i];
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+            if (EB % NT == 0 || tid + NT * j < EB) sl[lb_off[j]] = v.b[j];
+    };
+
+    int sm = 0, sc = 1, sp = 2;  // slots of planes k-1, k, k+1
+    {
+        Stage v;
+        const int k = P + 2 * kk0;
+        fetch(xo, k - 1, v);
+        store(sm, v);
+        fetch(xs, k, v);
+        store(sc, v);
+        fetch(xo, k + 1, v);
+        store(sp, v);
+    }
+    __syncthreads();
+
+    // right-hand side of the four points of plane kr
+    auto load_r = [&](int kr, T (&v)[4][3]) {
+        const T *rk = r + mn * kr;
+        static_for<4>([&](auto s) {
+            v[s][0] = rk[ro[s]];
+            v[s][1] = (rk + d.N)[ro[s]];
+            v[s][2] = (rk + 2 * d.N)[ro[s]];
+        });
+    };
+    T rr[4][3];
+    load_r(P + 2 * kk0, rr);
+    // complete here: the wait the compiler places before the first use of rr
+    // inside the loop would otherwise count the prefetches issued there too
+#pragma unroll
+    for (int s = 0; s < 4; s++) asm volatile("" : "+v"(rr[s][0]), "+v"(rr[s][1]), "+v"(rr[s][2]));
+
+    for (int kk = kk0; kk < kk1; kk++) {
+        const int k = P + 2 * kk;
+        const bool more = kk + 1 < kk1;
+        // the next plane's right-hand side, then its two staged planes, in
+        // flight during this plane's arithmetic (issued in that order: the
+        // loads complete in order, and nothing here waits for them)
+        T rn[4][3];
+        Stage n1, n2;
+        if (more) {
+            load_r(k + 2, rn);
+            fetch(xs, k + 2, n1);
+            fetch(xo, k + 3, n2);
+        }
+        const T *A = pk + (long)k * march_stride(PAT);
+
+        // phase Z: planes k-1 and k+1
+        T acc[4][3];
+        static_for<4>([&](auto s) { acc[s][0] = acc[s][1] = acc[s][2] = 0; });
+        {
+            int bm[4], bp[4];
+            static_for<4>([&](auto s) {
+                bm[s] = pb[s] + sm * PSZ;
+                bp[s] = pb[s] + sp * PSZ;
+            });
+            static_for<march_ngroups(PAT, 0, G)>([&](auto g_) {
+                constexpr int GR = decltype(g_)::value;
+                constexpr int NG = march_gsize(PAT, 0, G, GR);
+                if constexpr (NG > 0) {
+                    T c[NG];
+                    march_load<T, PAT, 0, GR>(A, c);
+                    static_for<4>([&](auto s) {
+                        auto ld = [&](int cc, int di, int dj, int dk) -> T {
+                            const int b0 = dk < 0 ? bm[s] : bp[s];
+                            const int xp = (SP::xf(decltype(s)::value) + H) & 1;
+                            const int off = dj * WX + (((xp + di) & 1) - xp) * HX + ((xp + di) >> 1);
+                            return lds[b0 + cc * CSZ + off];
+                        };
+                        static_for<march_len(0)>([&](auto i) {
+                            if constexpr (march_group(PAT, 0, G, decltype(i)::value) == GR)
+                                march_block_acc<T, PAT, 0, decltype(i)::value>(c, ld, acc[s][0], acc[s][1],
+                                                                                acc[s][2]);
+                        });
+                    });
+                }
+            });
+        }
+
+        // the phase-Z sums are complete here: the arithmetic is not sunk
+        // into the sub-passes, where its coefficients would stay live
+#pragma unroll
+        for (int s = 0; s < 4; s++) asm volatile("" : "+v"(acc[s][0]), "+v"(acc[s][1]), "+v"(acc[s][2]));
+
+        // in-plane sub-passes on the centre slot
+        constexpr int NG1 = march_ngroups(PAT, 1, G);
+        auto inplane = [&](auto s, auto &&coef) {
+            constexpr int S = decltype(s)::value;
+            const int bc = pb[S] + sc * PSZ;
+            constexpr int xp = (SP::xf(S) + H) & 1;
+            auto ld = [&](int cc, int di, int dj, int) -> T {
+                const int off = dj * WX + (((xp + di) & 1) - xp) * HX + ((xp + di) >> 1);
+                return lds[bc + cc * CSZ + off];
+            };
+            coef(ld);
+            (void)bc;
+        };
+        if constexpr (NG1 == 1) {
+            // one group: loaded once for the four sub-passes
+            constexpr int NC = march_gsize(PAT, 1, G, 0);
+            T c[NC];
+            march_load<T, PAT, 1, 0>(A, c);
+            static_for<4>([&](auto s) {
+                constexpr int S = decltype(s)::value;
+                if (tid < SP::npts(S)) {
+                    inplane(s, [&](auto &ld) {
+                        static_for<8>([&](auto i) {
+                            march_block_acc<T, PAT, 1, decltype(i)::value>(c, ld, acc[S][0], acc[S][1], acc[S][2]);
+                        });
+                    });
+                    const T e0 = rr[S][0] - acc[S][0], e1 = rr[S][1] - acc[S][1], e2 = rr[S][2] - acc[S][2];
+                    constexpr int qi = march_pos(PAT, 1, G, 8);
+                    const int bc = pb[S] + sc * PSZ;
+                    lds[bc] = c[qi + 0] * e0 + c[qi + 1] * e1 + c[qi + 2] * e2;
+                    lds[bc + CSZ] = c[qi + 3] * e0 + c[qi + 4] * e1 + c[qi + 5] * e2;
+                    lds[bc + 2 * CSZ] = c[qi + 6] * e0 + c[qi + 7] * e1 + c[qi + 8] * e2;
+                }
+                __syncthreads();
+            });
+        } else {
+            static_for<4>([&](auto s) {
+                constexpr int S = decltype(s)::value;
+                if (tid < SP::npts(S)) {
+                    T e0 = 0, e1 = 0, e2 = 0;
+                    static_for<NG1>([&](auto g_) {
+                        constexpr int GR = decltype(g_)::value;
+                        constexpr int NC = march_gsize(PAT, 1, G, GR);
+                        T c[NC];
+                        march_load<T, PAT, 1, GR>(A, c);
+                        inplane(s, [&](auto &ld) {
+                            static_for<8>([&](auto i) {
+                                if constexpr (march_group(PAT, 1, G, decltype(i)::value) == GR)
+                                    march_block_acc<T, PAT, 1, decltype(i)::value>(c, ld, acc[S][0], acc[S][1],
+                                                                                    acc[S][2]);
+                            });
+                        });
+                        if constexpr (GR == NG1 - 1) {
+                            // the group that holds the inverse block is the last
+                            constexpr int qi = march_pos(PAT, 1, G, 8);
+                            const T f0 = rr[S][0] - acc[S][0], f1 = rr[S][1] - acc[S][1],
+                                    f2 = rr[S][2] - acc[S][2];
+                            e0 = c[qi + 0] * f0 + c[qi + 1] * f1 + c[qi + 2] * f2;
+                            e1 = c[qi + 3] * f0 + c[qi + 4] * f1 + c[qi + 5] * f2;
+                            e2 = c[qi + 6] * f0 + c[qi + 7] * f1 + c[qi + 8] * f2;
+                        }
+                    });
+                    const int bc = pb[S] + sc * PSZ;
+                    lds[bc] = e0;
+                    lds[bc + CSZ] = e1;
+                    lds[bc + 2 * CSZ] = e2;
+                }
+                __syncthreads();
+            });
+        }
+
+        // write-back of the tile
+        {
+            const T *sl = lds + sc * PSZ;
+            T *xk = xd + mn * k;
+            const unsigned vw = (unsigned)(x0 + ca + rsub * d.m);
+            const int lw = rsub * WX + ((H + ca) & 1) * HX + ((H + ca) >> 1);
+#pragma unroll
+            for (int i = 0; i < NWB; i++) {
+                const int q = wv + NW * i;
+                if (QW % NW == 0 || q < QW) {
+                    const int r0 = q * RPI;
+                    const int c = r0 / TY, yy0 = r0 - c * TY;
+                    T *rowp = xk + c * d.N + (long)d.m * (y0 + yy0);
+                    rowp[vw] = sl[c * CSZ + (yy0 + H) * WX + lw];
+                }
+            }
+        }
+        if (more) {
+            __syncthreads();
+            const int nsm = sp, nsc = sm, nsp = sc;
+            store(nsc, n1);
+            store(nsp, n2);
+#pragma unroll
+            for (int s = 0; s < 4; s++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) rr[s][c] = rn[s][c];
+            sm = nsm;
+            sc = nsc;
+            sp = nsp;
+            __syncthreads();
+        }
+    }
+}
+
+}  // namespace stzmg

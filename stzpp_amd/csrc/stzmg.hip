@@ -20,6 +20,7 @@
 #include "mg_internal.hpp"
 #include "mg_fused.hpp"
 #include "mg_kernels.hpp"
+#include "mg_march.hpp"
 #include "mg_krylov.hpp"
 
 using namespace stzmg;
@@ -43,6 +44,9 @@ struct Level {
     int ftile = 0;        // fused-smoother tile class of FP32 sweeps (fused_tile)
     int ftile64 = 0;      // and of FP64 sweeps
     int pat = 0;          // operator zero pattern (PAT_*)
+    // packed per-plane coefficients of the z-marching sweep (k_pack_march)
+    double *pk64 = nullptr;
+    float *pk32 = nullptr;
 };
 
 inline unsigned nblk(long n, int t = kThreads) { return (unsigned)((n + t - 1) / t); }
@@ -79,6 +83,7 @@ struct stzmg_s {
     float *kV32 = nullptr;    // FP32 Krylov basis of the GMRES-IR solve
     int kcap64 = 0, kcap32 = 0;  // restart lengths the two bases hold
     hipGraphExec_t g64 = nullptr, g32 = nullptr;
+    int *pkmap = nullptr;       // k_pack_march source maps, kPackMax ints per pattern
 };
 
 namespace {
@@ -114,6 +119,7 @@ void free_level(Level &L) {
     hipFree(L.x64); hipFree(L.r64); hipFree(L.t64);
     hipFree(L.x32); hipFree(L.r32); hipFree(L.t32);
     hipFree(L.x64b); hipFree(L.x32b);
+    hipFree(L.pk64); hipFree(L.pk32);
     L = Level{};
 }
 
@@ -221,23 +227,26 @@ long resid_parts(stzmg_t h, const Level &L) {
 }
 
 template <typename T, bool NORM, bool APPLY, int PAT>
-void launch_resid_tiled(stzmg_t h, Level &L, const T *x, const T *r, T *out, T *xcopy) {
+void launch_resid_tiled(stzmg_t h, Level &L, const T *x, const T *r, T *out, T *xcopy, double *part) {
     int kc;
     const dim3 g = tiled_grid<T>(L, h->chunk_cap, &kc);
     hipLaunchKernelGGL((k_resid_planes<T, PAT, NORM, APPLY, ResidTile<T>::TX, ResidTile<T>::TY, kThreads>), g,
-                       dim3(kThreads), 0, h->stream, L.d, x, r, ops_of<T>(L), out, h->part, kc, xcopy);
+                       dim3(kThreads), 0, h->stream, L.d, x, r, ops_of<T>(L), out, part, kc, xcopy);
 }
 
 // out = r - A x (APPLY: out = A x) on level L, optionally with norm partials
-// into h->part.  With APPLY and xcopy given, x is also copied to xcopy.
+// into part (h->part when null).  With APPLY and xcopy given, x is also
+// copied to xcopy.
 template <typename T, bool NORM, bool APPLY>
-void launch_resid_any(stzmg_t h, Level &L, const T *x, const T *r, T *out, T *xcopy = nullptr) {
+void launch_resid_any(stzmg_t h, Level &L, const T *x, const T *r, T *out, T *xcopy = nullptr,
+                      double *part = nullptr) {
+    if (!part) part = h->part;
     if (L.tiled) {
         switch (L.pat) {
-            case PAT_SHEAR: launch_resid_tiled<T, NORM, APPLY, PAT_SHEAR>(h, L, x, r, out, xcopy); break;
-            case PAT_TRANS_XZ: launch_resid_tiled<T, NORM, APPLY, PAT_TRANS_XZ>(h, L, x, r, out, xcopy); break;
-            case PAT_NOCORNER: launch_resid_tiled<T, NORM, APPLY, PAT_NOCORNER>(h, L, x, r, out, xcopy); break;
-            default: launch_resid_tiled<T, NORM, APPLY, PAT_DENSE>(h, L, x, r, out, xcopy); break;
+            case PAT_SHEAR: launch_resid_tiled<T, NORM, APPLY, PAT_SHEAR>(h, L, x, r, out, xcopy, part); break;
+            case PAT_TRANS_XZ: launch_resid_tiled<T, NORM, APPLY, PAT_TRANS_XZ>(h, L, x, r, out, xcopy, part); break;
+            case PAT_NOCORNER: launch_resid_tiled<T, NORM, APPLY, PAT_NOCORNER>(h, L, x, r, out, xcopy, part); break;
+            default: launch_resid_tiled<T, NORM, APPLY, PAT_DENSE>(h, L, x, r, out, xcopy, part); break;
         }
         return;
     }
@@ -246,15 +255,15 @@ void launch_resid_any(stzmg_t h, Level &L, const T *x, const T *r, T *out, T *xc
     dim3 g(nblk((long)L.d.m * L.d.n), L.d.o);
     if (L.mode == STZMG_OP_PLANE)
         hipLaunchKernelGGL((k_resid<T, 0, NORM, APPLY>), g, dim3(kThreads), 0, h->stream, L.d, x, r,
-                           ops_of<T>(L), out, h->part);
+                           ops_of<T>(L), out, part);
     else
         hipLaunchKernelGGL((k_resid<T, 1, NORM, APPLY>), g, dim3(kThreads), 0, h->stream, L.d, x, r,
-                           ops_of<T>(L), out, h->part);
+                           ops_of<T>(L), out, part);
 }
 
 template <typename T, bool NORM>
-void launch_resid(stzmg_t h, Level &L, const T *x, const T *r, T *out) {
-    launch_resid_any<T, NORM, false>(h, L, x, r, out);
+void launch_resid(stzmg_t h, Level &L, const T *x, const T *r, T *out, double *part = nullptr) {
+    launch_resid_any<T, NORM, false>(h, L, x, r, out, nullptr, part);
 }
 
 template <typename T>
@@ -348,17 +357,38 @@ int fused_tile(const Level &L) {
     return best;
 }
 
-// Workgroups of k_gs_planes<T, PAT, TX, TY, NT> resident on the device at
-// once (CUs x the occupancy its LDS and registers allow), queried once.
-template <typename T, int PAT, int TX, int TY, int NT>
+// The large tile class sweeps with the z-marching kernel k_gs_march
+// (mg_march.hpp); STZ_MARCH=0 builds the staged-broadcast k_gs_planes for it
+// instead (for comparison).  The small tile classes use k_gs_planes.
+#ifndef STZ_MARCH
+#define STZ_MARCH 1
+#endif
+template <typename T, int PAT, int TX, int TY, int NT, bool MARCH>
+const void *sweep_kernel() {
+    if constexpr (MARCH)
+        return reinterpret_cast<const void *>(&k_gs_march<T, PAT, TX, TY, NT>);
+    else
+        return reinterpret_cast<const void *>(&k_gs_planes<T, PAT, TX, TY, NT>);
+}
+
+template <typename T>
+T *pk_of(Level &L);
+template <>
+double *pk_of<double>(Level &L) { return L.pk64; }
+template <>
+float *pk_of<float>(Level &L) { return L.pk32; }
+
+// Workgroups of a sweep kernel resident on the device at once (CUs x the
+// occupancy its LDS and registers allow), queried once per kernel.
+template <typename T, int PAT, int TX, int TY, int NT, bool MARCH>
 long fused_resident() {
     static const long r = [] {
         int dev = 0, ncu = 0, per = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
             ncu = 1;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per, reinterpret_cast<const void *>(&k_gs_planes<T, PAT, TX, TY, NT>), NT, 0) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, sweep_kernel<T, PAT, TX, TY, NT, MARCH>(), NT, 0) !=
+                hipSuccess ||
             per < 1)
             per = 1;
         return (long)per * ncu;
@@ -366,12 +396,12 @@ long fused_resident() {
     return r;
 }
 
-template <typename T, int PAT, int TX, int TY, int NT>
+template <typename T, int PAT, int TX, int TY, int NT, bool MARCH = false>
 void launch_fused_tile(stzmg_t h, Level &L) {
     const Dims &d = L.d;
     T *x = xref_of<T>(L), *xb = xb_of<T>(L);
     const long tiles = (long)(d.m / TX) * (d.n / TY);
-    const long dev_resident = fused_resident<T, PAT, TX, TY, NT>();
+    const long dev_resident = fused_resident<T, PAT, TX, TY, NT, MARCH>();
     const long resident = h->chunk_cap > 0 ? std::min(h->chunk_cap, dev_resident) : dev_resident;
     for (int P = 0; P < 2; P++) {
         const int nP = (d.o - P + 1) / 2;
@@ -383,9 +413,13 @@ void launch_fused_tile(stzmg_t h, Level &L) {
         const long nz = std::min<long>(nP, std::max<long>(1, resident / tiles));
         const int kchunk = (int)((nP + nz - 1) / nz);
         dim3 g(d.m / TX, d.n / TY, (nP + kchunk - 1) / kchunk);
-        hipLaunchKernelGGL((k_gs_planes<T, PAT, TX, TY, NT>), g, dim3(NT), 0, h->stream, d, (const T *)x,
-                           (const T *)(P == 0 ? x : xb), xb, (const T *)r_of<T>(L), ops_of<T>(L), inv_of<T>(L),
-                           P, kchunk);
+        const T *xo = P == 0 ? x : xb;
+        if constexpr (MARCH)
+            hipLaunchKernelGGL((k_gs_march<T, PAT, TX, TY, NT>), g, dim3(NT), 0, h->stream, d, (const T *)x, xo, xb,
+                               (const T *)r_of<T>(L), (const T *)pk_of<T>(L), P, kchunk);
+        else
+            hipLaunchKernelGGL((k_gs_planes<T, PAT, TX, TY, NT>), g, dim3(NT), 0, h->stream, d, (const T *)x, xo,
+                               xb, (const T *)r_of<T>(L), ops_of<T>(L), inv_of<T>(L), P, kchunk);
     }
     std::swap(xref_of<T>(L), xb_of<T>(L));
 }
@@ -394,7 +428,7 @@ template <typename T, int PAT>
 void launch_fused_pat(stzmg_t h, Level &L) {
     const int ft = sizeof(T) == 4 ? L.ftile : L.ftile64;
     if (ft == 1)
-        launch_fused_tile<T, PAT, FusedTile<T>::TX, FusedTile<T>::TY, FusedTile<T>::NT>(h, L);
+        launch_fused_tile<T, PAT, FusedTile<T>::TX, FusedTile<T>::TY, FusedTile<T>::NT, STZ_MARCH != 0>(h, L);
     else if (ft == 2)
         launch_fused_tile<T, PAT, kSmallTile, kSmallTile, kSmallThreads>(h, L);
     else
@@ -699,6 +733,42 @@ int build_bottom_map(stzmg_t h) {
     return 0;
 }
 
+// Packed coefficient tables of the levels that sweep with k_gs_march (the
+// large tile class): the nonzero entries of each plane's stencil in the
+// kernel's load order, then the inverse centre block.
+constexpr int kPackMax = kBlk + 9;
+constexpr int kPats = 4;
+
+int pack_march(stzmg_t h) {
+    if (!STZ_MARCH) return 0;
+    if (!h->pkmap) {
+        std::vector<int> src((size_t)kPats * kPackMax, 0);
+        for (int pat = 0; pat < kPats; pat++)
+            for (int j = 0; j < march_stride(pat); j++) src[(size_t)pat * kPackMax + j] = march_src(pat, j);
+        HIPCHK(h, alloc(&h->pkmap, (long)src.size()));
+        HIPCHK(h, hipMemcpy(h->pkmap, src.data(), sizeof(int) * src.size(), hipMemcpyHostToDevice));
+    }
+    for (auto &L : h->lv) {
+        if (!L.fused) continue;
+        const int stride = march_stride(L.pat);
+        const long n = L.ncls * stride;
+        const int *map = h->pkmap + (long)L.pat * kPackMax;
+        if (L.ftile == 1) {
+            if (!L.pk32) HIPCHK(h, alloc(&L.pk32, L.ncls * kPackMax));
+            hipLaunchKernelGGL(k_pack_march<float>, dim3(nblk(n)), dim3(kThreads), 0, h->stream,
+                               (const float *)L.ops32, (const float *)L.inv32, map, stride, L.ncls, L.pk32);
+        }
+        if (L.ftile64 == 1) {
+            if (!L.pk64) HIPCHK(h, alloc(&L.pk64, L.ncls * kPackMax));
+            hipLaunchKernelGGL(k_pack_march<double>, dim3(nblk(n)), dim3(kThreads), 0, h->stream,
+                               (const double *)L.ops64, (const double *)L.inv64, map, stride, L.ncls, L.pk64);
+        }
+        LAUNCHCHK(h);
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
 // Galerkin hierarchy and inverse tables after the fine operator changed.
 int build_hierarchy_ops(stzmg_t h) {
     if (int rc = coarse_kernel_attrs(h)) return rc;
@@ -758,6 +828,7 @@ int build_hierarchy_ops(stzmg_t h) {
         L.fused = h->fused_smoother && L.ftile && L.ftile64 && L.x64b;
         L.tiled = h->tiled_resid && tiled_ok(L);
     }
+    if (int rc = pack_march(h)) return rc;
     if (h->g64) { hipGraphExecDestroy(h->g64); h->g64 = nullptr; }
     if (h->g32) { hipGraphExecDestroy(h->g32); h->g32 = nullptr; }
     h->ops_set = true;
@@ -907,6 +978,7 @@ int stzmg_destroy(stzmg_t h) {
     hipFree(h->kV); hipFree(h->kZ); hipFree(h->ktmp); hipFree(h->kpart); hipFree(h->kdev);
     hipFree(h->kV32);
     hipFree(h->qmap);
+    hipFree(h->pkmap);
     hipStreamDestroy(h->stream);
     delete h;
     return 0;
