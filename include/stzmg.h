@@ -146,6 +146,17 @@ int stzmg_residual(stzmg_t h, int level, double *out);
 /* STZMG_NORM_QUARTIC = sum |r|^4 (l2_error_all, region.cc:3031-3057),
  * STZMG_NORM_L2SQ = sum |r|^2. */
 int stzmg_norm(stzmg_t h, int level, int kind, double *out);
+/* stzmg_norm without any device allocation (no reference counterpart; for
+ * callers that cannot allocate: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same kernels and summation order
+ * as stzmg_norm, so the same value; the norm partials go to a fixed device
+ * array of the library, one per GPU (1 MiB, held until the process ends)
+ * instead of the handle's lazily allocated buffer.  A level whose residual
+ * needs more than 65536 partials (per-point residual kernel above 256 x 256
+ * x 256 points) is rejected with STZMG_ERR_UNSUPPORTED.  All handles on one
+ * GPU share that array: calls that overlap on one GPU (from several threads
+ * or streams) race. */
+int stzmg_norm_noalloc(stzmg_t h, int level, int kind, double *out);
 /* multigrid::v_cycle (multigrid.cc:253-297), FP64. */
 int stzmg_vcycle(stzmg_t h, int n);
 /* One FP32 V-cycle on level-0 FP32 fields (x32, r32), for measurement. */

@@ -135,6 +135,7 @@ SIGNATURES = {
     "stzmg_interpolation": ([_vp, ctypes.c_int], ctypes.c_int),
     "stzmg_residual": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
     "stzmg_norm": ([_vp, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
+    "stzmg_norm_noalloc": ([_vp, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
     "stzmg_vcycle": ([_vp, ctypes.c_int], ctypes.c_int),
     "stzmg_vcycle_fp32": ([_vp, ctypes.c_int], ctypes.c_int),
     "stzmg_gauss_seidel_fp32": ([_vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),

@@ -8,6 +8,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <mutex>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -51,6 +52,36 @@ struct Level {
 
 inline unsigned nblk(long n, int t = kThreads) { return (unsigned)((n + t - 1) / t); }
 
+// Device memory of stzmg_norm_noalloc: the per-workgroup norm partials
+// (pairs sum |r|^4, sum |r|^2) followed by the reduced pair.  Each GPU the
+// library runs on holds its own copy (1 MiB) until the process ends; host
+// code reaches it only through the address hipGetSymbolAddress returns for
+// the current device (scratch_on_device).
+constexpr long kNoallocParts = 65536;
+static __device__ __align__(256) double scratch[2 * kNoallocParts + 2];
+
+constexpr int kMaxDevices = 64;
+
+// Address of this device's scratch, looked up once per device with that
+// device current; null on failure (the HIP error in *err).
+double *scratch_on_device(int dev, hipError_t *err) {
+    static std::mutex mu;
+    static double *addr[kMaxDevices] = {};
+    *err = hipSuccess;
+    if (dev < 0 || dev >= kMaxDevices) {
+        *err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        *err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (*err != hipSuccess) return nullptr;
+        addr[dev] = static_cast<double *>(p);
+    }
+    return addr[dev];
+}
+
 }  // namespace
 
 struct stzmg_s {
@@ -83,6 +114,7 @@ struct stzmg_s {
     float *kV32 = nullptr;    // FP32 Krylov basis of the GMRES-IR solve
     int kcap64 = 0, kcap32 = 0;  // restart lengths the two bases hold
     hipGraphExec_t g64 = nullptr, g32 = nullptr;
+    double *scratch = nullptr;  // this device's scratch of stzmg_norm_noalloc
     int *pkmap = nullptr;       // k_pack_march source maps, kPackMax ints per pattern
 };
 
@@ -941,6 +973,13 @@ int stzmg_create(const stzmg_desc *d, stzmg_t *out) {
         cur = nx;
     }
     if (hipStreamCreate(&h->stream) != hipSuccess) return bail(fail(h, STZMG_ERR_HIP, "hipStreamCreate failed"));
+    {
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) h->scratch = scratch_on_device(dev, &e);
+        if (e != hipSuccess)
+            return bail(fail(h, STZMG_ERR_HIP, "scratch address lookup failed: %s", hipGetErrorString(e)));
+    }
     h->lv.resize(dims.size());
     for (size_t l = 0; l < dims.size(); l++) {
         Level &L = h->lv[l];
@@ -1253,6 +1292,30 @@ int stzmg_norm(stzmg_t h, int level, int kind, double *out) {
     if (!out || (kind != STZMG_NORM_QUARTIC && kind != STZMG_NORM_L2SQ)) return STZMG_ERR_ARG;
     double v[2];
     if (int rc = norms64(h, level, v)) return rc;
+    *out = kind == STZMG_NORM_QUARTIC ? v[0] : v[1];
+    return 0;
+}
+
+int stzmg_norm_noalloc(stzmg_t h, int level, int kind, double *out) {
+    if (int rc = check_level(h, level)) return rc;
+    if (int rc = require_ops(h)) return rc;
+    if (!out || (kind != STZMG_NORM_QUARTIC && kind != STZMG_NORM_L2SQ)) return STZMG_ERR_ARG;
+    Level &L = h->lv[level];
+    const long nb = resid_parts<double>(h, L);
+    if (nb > kNoallocParts)
+        return fail(h, STZMG_ERR_UNSUPPORTED,
+                    "stzmg_norm_noalloc: level %d needs %ld norm partials, the scratch holds %ld "
+                    "(use stzmg_norm)",
+                    level, nb, kNoallocParts);
+    // every workgroup writes its own pair and the reduction overwrites the
+    // result, so nothing needs clearing between calls
+    double *part = h->scratch, *res = h->scratch + 2 * kNoallocParts;
+    launch_resid<double, true>(h, L, L.x64, L.r64, L.t64, part);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, h->stream, (const double *)part, nb, res);
+    LAUNCHCHK(h);
+    double v[2];
+    HIPCHK(h, hipMemcpyAsync(v, res, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     *out = kind == STZMG_NORM_QUARTIC ? v[0] : v[1];
     return 0;
 }
