@@ -12,14 +12,16 @@
 //   the in-plane sub-passes ("phase Z"); the sub-passes then only add the
 //   eight in-plane blocks.
 // * Coefficients live in scalar registers: each phase loads its nonzero
-//   coefficients in groups of at most kGroupRegs registers through the
+//   coefficients in groups of at most MarchGroup<T>::regs registers through the
 //   scalar cache, one wait per group, and the multiply-adds take them as
 //   scalar operands (no per-point broadcast instruction).
 // * Staging reads one window row per wave instruction: row addresses are
 //   scalar, the per-lane column offsets (with the periodic wrap) are computed
 //   once per workgroup, so a staged value costs one load and one LDS store.
-// * The global loads of the next two planes are issued before the
-//   arithmetic of the current plane and stored to LDS after it.
+// * The global loads of the next plane's right-hand side and of the next two
+//   staged planes are issued before the arithmetic of the current plane and
+//   stored to LDS after it; the new values of the tile's own points go from
+//   registers to x_new in the sub-pass that computes them.
 #pragma once
 
 #include <type_traits>
@@ -41,20 +43,35 @@ __device__ __forceinline__ void static_for(F &&f) {
     static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
-// Blocks of the two phases: Z = the 18 blocks with dk != 0, then the eight
-// in-plane blocks and the inverse centre block (index 27, 9 entries).
-__host__ __device__ constexpr int march_block(int phase, int i) {
+// Pattern keys of the marching kernel: a zero pattern PAT_* (0..3), plus
+// kMarchSym when every plane's stencil is centrosymmetric (block(-o) equal
+// to block(o) bit for bit, as the shear and transformed stencils and their
+// per-plane wall rows are).  Such a stencil sums each opposite pair of
+// neighbours first, x(p+o) + x(p-o), and multiplies once: half the
+// multiply-adds and half the coefficients.
+constexpr int kMarchSym = 4;
+__host__ __device__ constexpr bool march_sym(int key) { return key >= kMarchSym; }
+
+// Blocks of the two phases: Z = the 18 blocks with dk != 0 (centrosymmetric:
+// the 9 with dk = -1), then the eight in-plane blocks (centrosymmetric: the 4
+// before the centre) and the inverse centre block (index 27, 9 entries).
+__host__ __device__ constexpr int march_block(int key, int phase, int i) {
+    if (march_sym(key)) return phase == 0 ? i : (i < 4 ? 9 + i : 27);
     return phase == 0 ? (i < 9 ? i : 9 + i) : (i < 4 ? 9 + i : (i < 8 ? 10 + i : 27));
 }
-__host__ __device__ constexpr int march_len(int phase) { return phase == 0 ? 18 : 9; }
-__host__ __device__ constexpr unsigned march_mask(int pat, int b) { return b == 27 ? 0x1FFu : pat_mask(pat, b); }
+__host__ __device__ constexpr int march_len(int key, int phase) {
+    return march_sym(key) ? (phase == 0 ? 9 : 5) : (phase == 0 ? 18 : 9);
+}
+__host__ __device__ constexpr unsigned march_mask(int key, int b) {
+    return b == 27 ? 0x1FFu : pat_mask(key % kMarchSym, b);
+}
 
 // Greedy grouping of a phase's blocks: a new group starts where adding the
 // block would exceed G coefficients.
 __host__ __device__ constexpr int march_group(int pat, int phase, int G, int i) {
     int g = 0, run = 0;
     for (int q = 0; q <= i; q++) {
-        const int nz = popc9(march_mask(pat, march_block(phase, q)));
+        const int nz = popc9(march_mask(pat, march_block(pat, phase, q)));
         if (run > 0 && run + nz > G) {
             g++;
             run = 0;
@@ -64,20 +81,20 @@ __host__ __device__ constexpr int march_group(int pat, int phase, int G, int i) 
     return g;
 }
 __host__ __device__ constexpr int march_ngroups(int pat, int phase, int G) {
-    return march_group(pat, phase, G, march_len(phase) - 1) + 1;
+    return march_group(pat, phase, G, march_len(pat, phase) - 1) + 1;
 }
 // Coefficients in group g, and position of block i's first coefficient in it.
 __host__ __device__ constexpr int march_gsize(int pat, int phase, int G, int g) {
     int n = 0;
-    for (int q = 0; q < march_len(phase); q++)
-        if (march_group(pat, phase, G, q) == g) n += popc9(march_mask(pat, march_block(phase, q)));
+    for (int q = 0; q < march_len(pat, phase); q++)
+        if (march_group(pat, phase, G, q) == g) n += popc9(march_mask(pat, march_block(pat, phase, q)));
     return n;
 }
 __host__ __device__ constexpr int march_pos(int pat, int phase, int G, int i) {
     const int g = march_group(pat, phase, G, i);
     int n = 0;
     for (int q = 0; q < i; q++)
-        if (march_group(pat, phase, G, q) == g) n += popc9(march_mask(pat, march_block(phase, q)));
+        if (march_group(pat, phase, G, q) == g) n += popc9(march_mask(pat, march_block(pat, phase, q)));
     return n;
 }
 // rank of entry e among the set bits of m
@@ -92,7 +109,7 @@ __host__ __device__ constexpr int rank9(unsigned m, int e) {
 // centre block), so every group is a contiguous run (wide scalar loads).
 __host__ __device__ constexpr int march_count(int pat, int phase) {
     int n = 0;
-    for (int q = 0; q < march_len(phase); q++) n += popc9(march_mask(pat, march_block(phase, q)));
+    for (int q = 0; q < march_len(pat, phase); q++) n += popc9(march_mask(pat, march_block(pat, phase, q)));
     return n;
 }
 __host__ __device__ constexpr int march_stride(int pat) { return march_count(pat, 0) + march_count(pat, 1); }
@@ -100,8 +117,8 @@ __host__ __device__ constexpr int march_stride(int pat) { return march_count(pat
 // for the inverse block.
 __host__ __device__ constexpr int march_src(int pat, int j) {
     for (int ph = 0; ph < 2; ph++)
-        for (int q = 0; q < march_len(ph); q++) {
-            const int b = march_block(ph, q);
+        for (int q = 0; q < march_len(pat, ph); q++) {
+            const int b = march_block(pat, ph, q);
             const unsigned m = march_mask(pat, b);
             const int n = popc9(m);
             if (j < n) return b == 27 ? 243 + nth_bit(m, j) : 9 * b + nth_bit(m, j);
@@ -125,6 +142,18 @@ __global__ void k_pack_march(const T *__restrict__ ops, const T *__restrict__ in
     const long c = t / stride;
     const int j = (int)(t - c * stride), q = src[j];
     pk[t] = q < kBlk ? ops[c * kBlk + q] : inv[c * 9 + (q - kBlk)];
+}
+
+// Element at a 32-bit byte offset from a base pointer: with the base
+// uniform, the access takes the scalar-base plus vector-offset form (no
+// 64-bit address arithmetic per access).
+template <typename T>
+__device__ __forceinline__ const T &at_bytes(const T *base, unsigned off) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+template <typename T>
+__device__ __forceinline__ T &at_bytes(T *base, unsigned off) {
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off);
 }
 
 // Scalar registers per coefficient group (a double takes two).
@@ -153,16 +182,23 @@ __device__ __forceinline__ void march_load(const T *__restrict__ pk, T (&c)[N]) 
 template <typename T, int PAT, int PHASE, int I, int N, typename LD>
 __device__ __forceinline__ void march_block_acc(const T (&c)[N], const LD &ld, T &a0, T &a1, T &a2) {
     constexpr int G = MarchGroup<T>::G;
-    constexpr int b = march_block(PHASE, I);
+    constexpr int b = march_block(PAT, PHASE, I);
     constexpr unsigned m = march_mask(PAT, b);
     if constexpr (b != 27 && m != 0) {
         constexpr int di = b % 3 - 1, dj = (b / 3) % 3 - 1, dk = b / 9 - 1;
         constexpr unsigned cols = pat_cols(m);
         constexpr int q0 = march_pos(PAT, PHASE, G, I);
         T v0 = 0, v1 = 0, v2 = 0;
-        if constexpr (cols & 1) v0 = ld(0, di, dj, dk);
-        if constexpr (cols & 2) v1 = ld(1, di, dj, dk);
-        if constexpr (cols & 4) v2 = ld(2, di, dj, dk);
+        if constexpr (march_sym(PAT)) {
+            // the opposite neighbour has the same block
+            if constexpr (cols & 1) v0 = ld(0, di, dj, dk) + ld(0, -di, -dj, -dk);
+            if constexpr (cols & 2) v1 = ld(1, di, dj, dk) + ld(1, -di, -dj, -dk);
+            if constexpr (cols & 4) v2 = ld(2, di, dj, dk) + ld(2, -di, -dj, -dk);
+        } else {
+            if constexpr (cols & 1) v0 = ld(0, di, dj, dk);
+            if constexpr (cols & 2) v1 = ld(1, di, dj, dk);
+            if constexpr (cols & 4) v2 = ld(2, di, dj, dk);
+        }
         if constexpr (m & 0x001) a0 += c[q0 + rank9(m, 0)] * v0;
         if constexpr (m & 0x002) a0 += c[q0 + rank9(m, 1)] * v1;
         if constexpr (m & 0x004) a0 += c[q0 + rank9(m, 2)] * v2;
@@ -175,12 +211,22 @@ __device__ __forceinline__ void march_block_acc(const T (&c)[N], const LD &ld, T
     }
 }
 
+// Threads that own a point of sub-pass s under the lane mapping of
+// k_gs_march (a prefix of the workgroup).
+template <int TX, int TY, int NT>
+constexpr int march_lanes(int s) {
+    using SP = SubPass<TX, TY, NT>;
+    if (TX != 64) return SP::npts(s);
+    const int ny = SP::ny(s), extra = (SP::nx(s) - 32) * ny;
+    return 32 * (ny + (extra + 31) / 32);
+}
+
 // Waves per SIMD that the LDS of one workgroup (three staged window planes)
 // allows on a CU of 160 KiB: the register allocation is held to that
 // occupancy.
 template <typename T, int TX, int TY, int NT>
 constexpr int march_waves() {
-    constexpr int lds = 3 * 3 * (TX + 8) * (TY + 8) * (int)sizeof(T);
+    constexpr int lds = 3 * 3 * (TX + 7) * (TY + 8) * (int)sizeof(T);
     constexpr int wgs = (160 * 1024) / lds;
     constexpr int w = wgs * NT / 64 / 4;
     return w < 1 ? 1 : (w > 8 ? 8 : w);
@@ -194,9 +240,15 @@ template <typename T, int PAT, int TX, int TY, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<T, TX, TY, NT>()))) void k_gs_march(Dims d, const T *__restrict__ xs, const T *__restrict__ xo,
                                                  T *__restrict__ xd, const T *__restrict__ r,
                                                  const T *__restrict__ pk, int P, int kchunk) {
-    constexpr int H = 4;                   // 3 recomputed halo points + the stencil reach
-    constexpr int WX = TX + 2 * H, WY = TY + 2 * H;
-    constexpr int HX = WX / 2;             // x is stored split by parity
+    // The window holds what the four sub-passes read: their points reach
+    // at most 3 (x) beyond the tile and their stencils one more, i.e. x in
+    // [-3, TX + 3] (origin HXL = 3 points left of the tile), and y in
+    // [-4, TY + 4) (origin H = 4; an even number of rows keeps the row
+    // staging regular).  At 64 x 8 in FP32 the three slots then take
+    // 40,896 bytes: four workgroups per CU.
+    constexpr int H = 4, HXL = 3;
+    constexpr int WX = TX + 2 * HXL + 1, WY = TY + 2 * H;
+    constexpr int HX = (WX + 1) / 2;       // x is stored split by parity
     constexpr int CSZ = WX * WY;           // one component of one plane
     constexpr int PSZ = 3 * CSZ;           // one plane slot
     constexpr int NW = NT / 64;            // waves
@@ -204,13 +256,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
     static_assert(TX == 32 || TX == 64, "tile width");
     static_assert(TY % 2 == 0 && NT % 64 == 0, "tile shape");
     // chunk A: window columns [0, TX), RPI rows per instruction; chunk B:
-    // the last 2H columns, one element per thread and instruction
+    // the last WX - TX columns, one element per thread and instruction
+    constexpr int CB = WX - TX;
     constexpr int QA = 3 * WY / RPI, NA = QA / NW;
-    constexpr int EB = 3 * WY * 2 * H, NB = (EB + NT - 1) / NT;
-    // write-back of the TX x TY tile: RPI rows per instruction
-    constexpr int QW = 3 * TY / RPI, NWB = (QW + NW - 1) / NW;
+    constexpr int EB = 3 * WY * CB, NB = (EB + NT - 1) / NT;
     using SP = SubPass<TX, TY, NT>;
-    static_assert(SP::npts(0) <= NT, "one point per thread and sub-pass");
+    static_assert(march_lanes<TX, TY, NT>(0) <= NT && march_lanes<TX, TY, NT>(1) <= NT &&
+                      march_lanes<TX, TY, NT>(2) <= NT && march_lanes<TX, TY, NT>(3) <= NT,
+                  "one point per thread and sub-pass");
     constexpr int G = MarchGroup<T>::G;
     __shared__ T lds[3 * PSZ];
 
@@ -237,8 +290,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
     unsigned ga[YA];
 #pragma unroll
     for (int j = 0; j < YA; j++)
-        ga[j] = (unsigned)(wrap1(x0 - H + ca, d.m) +
-                           (long)d.m * wrap1(y0 - H + (wv + NW * j) * RPI + rsub, d.n));
+        ga[j] = (unsigned)(sizeof(T) * (wrap1(x0 - HXL + ca, d.m) +
+                                        (long)d.m * wrap1(y0 - H + (wv + NW * j) * RPI + rsub, d.n)));
     const int la_off = (wv * RPI + rsub) * WX + (ca & 1) * HX + (ca >> 1);
     // chunk B: up to NB elements of the right strip
     unsigned vb_off[NB];
@@ -246,22 +299,54 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
 #pragma unroll
     for (int j = 0; j < NB; j++) {
         const int e = tid + NT * j < EB ? tid + NT * j : EB - 1;
-        const int row = e / (2 * H), col = TX + e % (2 * H);
+        const int row = e / CB, col = TX + e % CB;
         const int c = row / WY, yy = row - c * WY;
-        vb_off[j] = (unsigned)(c * d.N + (long)d.m * wrap1(y0 - H + yy, d.n) + wrap1(x0 - H + col, d.m));
+        vb_off[j] = (unsigned)(sizeof(T) *
+                               (c * d.N + (long)d.m * wrap1(y0 - H + yy, d.n) + wrap1(x0 - HXL + col, d.m)));
         lb_off[j] = c * CSZ + yy * WX + (col & 1) * HX + (col >> 1);
     }
     // sub-pass points: LDS index of the point (component 0, slot 0), its
-    // offset in a plane of r, and whether this thread owns one
+    // offset in a plane of r and x, whether this thread owns one and whether
+    // it lies in the tile itself.  Lanes map to points so that a half-wave
+    // reads one LDS row segment: with TX = 64 (32..35 points per sub-pass
+    // row) half-wave h < ny takes the first 32 points of row h and the
+    // columns beyond go to the half-waves after; otherwise points are dealt
+    // in order.
     int pb[4];
-    unsigned ro[4];
+    unsigned ro[4], wo[4];
+    bool own[4], inside[4];
     static_for<4>([&](auto s_) {
         constexpr int s = decltype(s_)::value;
-        const int t = tid < SP::npts(s) ? tid : SP::npts(s) - 1;
-        const int tyy = t / SP::nx(s), txx = t - tyy * SP::nx(s);
-        const int xx = SP::xf(s) + 2 * txx + H, yy = SP::yf(s) + 2 * tyy + H;
+        constexpr int nx = SP::nx(s), ny = SP::ny(s);
+        int txx, tyy;
+        bool ok;
+        if constexpr (TX == 64) {
+            constexpr int ex = nx - 32;
+            const int hw = tid >> 5, l = tid & 31;
+            if (hw < ny) {
+                ok = true;
+                txx = l;
+                tyy = hw;
+            } else {
+                const int e = l + 32 * (hw - ny);
+                ok = e < ex * ny;
+                const int ee = ok ? e : 0;
+                txx = 32 + (ex > 0 ? ee % (ex > 0 ? ex : 1) : 0);
+                tyy = ex > 0 ? ee / (ex > 0 ? ex : 1) : 0;
+                if (!ok) txx = 0;
+            }
+        } else {
+            ok = tid < SP::npts(s);
+            const int t = ok ? tid : 0;
+            tyy = t / nx;
+            txx = t - tyy * nx;
+        }
+        const int xx = SP::xf(s) + 2 * txx + HXL, yy = SP::yf(s) + 2 * tyy + H;
         pb[s] = yy * WX + (xx & 1) * HX + (xx >> 1);
-        ro[s] = (unsigned)(wrap1(x0 + xx - H, d.m) + (long)d.m * wrap1(y0 + yy - H, d.n));
+        ro[s] = (unsigned)(sizeof(T) * (wrap1(x0 + xx - HXL, d.m) + (long)d.m * wrap1(y0 + yy - H, d.n)));
+        own[s] = ok;
+        inside[s] = ok && xx >= HXL && xx < HXL + TX && yy >= H && yy < H + TY;
+        wo[s] = (unsigned)(sizeof(T) * ((x0 + xx - HXL) + (long)d.m * (y0 + yy - H)));
     });
 
     // ---- staging ----
@@ -279,9 +364,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
         if (valid) {
             const T *base = src + mn * kw;
 #pragma unroll
-            for (int i = 0; i < NA; i++) v.a[i] = (base + (i / YA) * d.N)[ga[i % YA]];
+            for (int i = 0; i < NA; i++) v.a[i] = at_bytes(base + (i / YA) * d.N, ga[i % YA]);
 #pragma unroll
-            for (int j = 0; j < NB; j++) v.b[j] = base[vb_off[j]];
+            for (int j = 0; j < NB; j++) v.b[j] = at_bytes(base, vb_off[j]);
         } else {
 #pragma unroll
             for (int i = 0; i < NA; i++) v.a[i] = 0;
@@ -315,9 +400,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
     auto load_r = [&](int kr, T (&v)[4][3]) {
         const T *rk = r + mn * kr;
         static_for<4>([&](auto s) {
-            v[s][0] = rk[ro[s]];
-            v[s][1] = (rk + d.N)[ro[s]];
-            v[s][2] = (rk + 2 * d.N)[ro[s]];
+            v[s][0] = at_bytes(rk, ro[s]);
+            v[s][1] = at_bytes(rk + d.N, ro[s]);
+            v[s][2] = at_bytes(rk + 2 * d.N, ro[s]);
         });
     };
     T rr[4][3];
@@ -358,13 +443,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
                     T c[NG];
                     march_load<T, PAT, 0, GR>(A, c);
                     static_for<4>([&](auto s) {
+                        // waves without a point of this sub-pass skip it
+                        // (a wave-uniform branch)
+                        if (wv * 64 >= march_lanes<TX, TY, NT>(decltype(s)::value)) return;
                         auto ld = [&](int cc, int di, int dj, int dk) -> T {
                             const int b0 = dk < 0 ? bm[s] : bp[s];
-                            const int xp = (SP::xf(decltype(s)::value) + H) & 1;
+                            const int xp = (SP::xf(decltype(s)::value) + HXL) & 1;
                             const int off = dj * WX + (((xp + di) & 1) - xp) * HX + ((xp + di) >> 1);
                             return lds[b0 + cc * CSZ + off];
                         };
-                        static_for<march_len(0)>([&](auto i) {
+                        static_for<march_len(PAT, 0)>([&](auto i) {
                             if constexpr (march_group(PAT, 0, G, decltype(i)::value) == GR)
                                 march_block_acc<T, PAT, 0, decltype(i)::value>(c, ld, acc[s][0], acc[s][1],
                                                                                 acc[s][2]);
@@ -379,12 +467,26 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
 #pragma unroll
         for (int s = 0; s < 4; s++) asm volatile("" : "+v"(acc[s][0]), "+v"(acc[s][1]), "+v"(acc[s][2]));
 
-        // in-plane sub-passes on the centre slot
+        // in-plane sub-passes on the centre slot.  A new value goes to LDS
+        // (the later sub-passes read it) and, for a point of the tile
+        // itself, straight to x_new.
+        T *xk = xd + mn * k;
+        auto put = [&](int S, T v0, T v1, T v2) {
+            const int bc = pb[S] + sc * PSZ;
+            lds[bc] = v0;
+            lds[bc + CSZ] = v1;
+            lds[bc + 2 * CSZ] = v2;
+            if (inside[S]) {
+                at_bytes(xk, wo[S]) = v0;
+                at_bytes(xk + d.N, wo[S]) = v1;
+                at_bytes(xk + 2 * d.N, wo[S]) = v2;
+            }
+        };
         constexpr int NG1 = march_ngroups(PAT, 1, G);
         auto inplane = [&](auto s, auto &&coef) {
             constexpr int S = decltype(s)::value;
             const int bc = pb[S] + sc * PSZ;
-            constexpr int xp = (SP::xf(S) + H) & 1;
+            constexpr int xp = (SP::xf(S) + HXL) & 1;
             auto ld = [&](int cc, int di, int dj, int) -> T {
                 const int off = dj * WX + (((xp + di) & 1) - xp) * HX + ((xp + di) >> 1);
                 return lds[bc + cc * CSZ + off];
@@ -399,25 +501,24 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
             march_load<T, PAT, 1, 0>(A, c);
             static_for<4>([&](auto s) {
                 constexpr int S = decltype(s)::value;
-                if (tid < SP::npts(S)) {
+                if (own[S]) {
                     inplane(s, [&](auto &ld) {
-                        static_for<8>([&](auto i) {
+                        static_for<march_len(PAT, 1) - 1>([&](auto i) {
                             march_block_acc<T, PAT, 1, decltype(i)::value>(c, ld, acc[S][0], acc[S][1], acc[S][2]);
                         });
                     });
                     const T e0 = rr[S][0] - acc[S][0], e1 = rr[S][1] - acc[S][1], e2 = rr[S][2] - acc[S][2];
-                    constexpr int qi = march_pos(PAT, 1, G, 8);
-                    const int bc = pb[S] + sc * PSZ;
-                    lds[bc] = c[qi + 0] * e0 + c[qi + 1] * e1 + c[qi + 2] * e2;
-                    lds[bc + CSZ] = c[qi + 3] * e0 + c[qi + 4] * e1 + c[qi + 5] * e2;
-                    lds[bc + 2 * CSZ] = c[qi + 6] * e0 + c[qi + 7] * e1 + c[qi + 8] * e2;
+                    constexpr int qi = march_pos(PAT, 1, G, march_len(PAT, 1) - 1);
+                    put(S, c[qi + 0] * e0 + c[qi + 1] * e1 + c[qi + 2] * e2,
+                        c[qi + 3] * e0 + c[qi + 4] * e1 + c[qi + 5] * e2,
+                        c[qi + 6] * e0 + c[qi + 7] * e1 + c[qi + 8] * e2);
                 }
                 __syncthreads();
             });
         } else {
             static_for<4>([&](auto s) {
                 constexpr int S = decltype(s)::value;
-                if (tid < SP::npts(S)) {
+                if (own[S]) {
                     T e0 = 0, e1 = 0, e2 = 0;
                     static_for<NG1>([&](auto g_) {
                         constexpr int GR = decltype(g_)::value;
@@ -425,7 +526,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
                         T c[NC];
                         march_load<T, PAT, 1, GR>(A, c);
                         inplane(s, [&](auto &ld) {
-                            static_for<8>([&](auto i) {
+                            static_for<march_len(PAT, 1) - 1>([&](auto i) {
                                 if constexpr (march_group(PAT, 1, G, decltype(i)::value) == GR)
                                     march_block_acc<T, PAT, 1, decltype(i)::value>(c, ld, acc[S][0], acc[S][1],
                                                                                     acc[S][2]);
@@ -433,7 +534,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
                         });
                         if constexpr (GR == NG1 - 1) {
                             // the group that holds the inverse block is the last
-                            constexpr int qi = march_pos(PAT, 1, G, 8);
+                            constexpr int qi = march_pos(PAT, 1, G, march_len(PAT, 1) - 1);
                             const T f0 = rr[S][0] - acc[S][0], f1 = rr[S][1] - acc[S][1],
                                     f2 = rr[S][2] - acc[S][2];
                             e0 = c[qi + 0] * f0 + c[qi + 1] * f1 + c[qi + 2] * f2;
@@ -441,34 +542,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
                             e2 = c[qi + 6] * f0 + c[qi + 7] * f1 + c[qi + 8] * f2;
                         }
                     });
-                    const int bc = pb[S] + sc * PSZ;
-                    lds[bc] = e0;
-                    lds[bc + CSZ] = e1;
-                    lds[bc + 2 * CSZ] = e2;
+                    put(S, e0, e1, e2);
                 }
                 __syncthreads();
             });
         }
 
-        // write-back of the tile
-        {
-            const T *sl = lds + sc * PSZ;
-            T *xk = xd + mn * k;
-            const unsigned vw = (unsigned)(x0 + ca + rsub * d.m);
-            const int lw = rsub * WX + ((H + ca) & 1) * HX + ((H + ca) >> 1);
-#pragma unroll
-            for (int i = 0; i < NWB; i++) {
-                const int q = wv + NW * i;
-                if (QW % NW == 0 || q < QW) {
-                    const int r0 = q * RPI;
-                    const int c = r0 / TY, yy0 = r0 - c * TY;
-                    T *rowp = xk + c * d.N + (long)d.m * (y0 + yy0);
-                    rowp[vw] = sl[c * CSZ + (yy0 + H) * WX + lw];
-                }
-            }
-        }
         if (more) {
-            __syncthreads();
+            // planes k-1 and k are free after the last sub-pass's barrier
             const int nsm = sp, nsc = sm, nsp = sc;
             store(nsc, n1);
             store(nsp, n2);

@@ -45,6 +45,7 @@ struct Level {
     int ftile = 0;        // fused-smoother tile class of FP32 sweeps (fused_tile)
     int ftile64 = 0;      // and of FP64 sweeps
     int pat = 0;          // operator zero pattern (PAT_*)
+    bool sym = false;     // every plane's stencil centrosymmetric (k_gs_march pairs neighbours)
     // packed per-plane coefficients of the z-marching sweep (k_pack_march)
     double *pk64 = nullptr;
     float *pk32 = nullptr;
@@ -305,22 +306,21 @@ void launch_apply(stzmg_t h, Level &L, const T *x, T *out, T *xcopy = nullptr) {
 
 int ncol(int n, int prd) { return (prd && (n & 1)) ? 3 : 2; }
 
-// Tile and workgroup size of the fused smoother: 64 x 8 in FP32 and
-// 32 x 16 in FP64.  The three staged planes (3 comps, 4-point halo) take
-// 3 x 3 x 72 x 16 x 4 B = 40.5 KiB and 3 x 3 x 40 x 24 x 8 B = 67.5 KiB of
-// LDS (three and two workgroups per CU).  Measured on the 256^3 FP32 sweep (MI355X): 64 x 8
-// 0.42 ms, 64 x 16 0.44 ms, 32 x 8 (128 threads) 0.53 ms, 64 x 4 0.66 ms;
-// with launches of one resident round: 64 x 8 0.37 ms, 64 x 8 with 512
-// threads 0.44 ms, 64 x 16 with 512 threads 0.51 ms (and the FP64 32 x 16
-// sweep 0.58 ms with 256 threads, 0.96 ms with 512).
+// Tile and workgroup size of the large fused-smoother class (k_gs_march):
+// 64 x 16 with 512 threads in FP32 (three staged planes of 71 x 24 points,
+// 61,344 B of LDS: two workgroups per CU) and 32 x 16 with 256 threads in
+// FP64 (39 x 24 points, 67,392 B: two per CU).  Measured on the 256^3
+// transformed-operator sweep (MI355X): FP32 64 x 16 / 512 threads 0.256 ms,
+// 64 x 8 / 256 0.272 ms, 32 x 16 / 256 0.291 ms; FP64 32 x 16 0.519 ms,
+// 32 x 8 0.666 ms.
 #ifndef STZ_F32_TX
 #define STZ_F32_TX 64
 #endif
 #ifndef STZ_F32_TY
-#define STZ_F32_TY 8
+#define STZ_F32_TY 16
 #endif
 #ifndef STZ_F32_NT
-#define STZ_F32_NT 256
+#define STZ_F32_NT 512
 #endif
 #ifndef STZ_F64_TX
 #define STZ_F64_TX 32
@@ -459,8 +459,17 @@ void launch_fused_tile(stzmg_t h, Level &L) {
 template <typename T, int PAT>
 void launch_fused_pat(stzmg_t h, Level &L) {
     const int ft = sizeof(T) == 4 ? L.ftile : L.ftile64;
-    if (ft == 1)
-        launch_fused_tile<T, PAT, FusedTile<T>::TX, FusedTile<T>::TY, FusedTile<T>::NT, STZ_MARCH != 0>(h, L);
+    if (ft == 1) {
+        constexpr int TX = FusedTile<T>::TX, TY = FusedTile<T>::TY, NT = FusedTile<T>::NT;
+        if constexpr (STZ_MARCH != 0) {
+            if (L.sym)
+                launch_fused_tile<T, PAT + kMarchSym, TX, TY, NT, true>(h, L);
+            else
+                launch_fused_tile<T, PAT, TX, TY, NT, true>(h, L);
+        } else {
+            launch_fused_tile<T, PAT, TX, TY, NT, false>(h, L);
+        }
+    }
     else if (ft == 2)
         launch_fused_tile<T, PAT, kSmallTile, kSmallTile, kSmallThreads>(h, L);
     else
@@ -769,7 +778,7 @@ int build_bottom_map(stzmg_t h) {
 // large tile class): the nonzero entries of each plane's stencil in the
 // kernel's load order, then the inverse centre block.
 constexpr int kPackMax = kBlk + 9;
-constexpr int kPats = 4;
+constexpr int kPats = 2 * kMarchSym;  // zero patterns, plain and centrosymmetric
 
 int pack_march(stzmg_t h) {
     if (!STZ_MARCH) return 0;
@@ -782,9 +791,10 @@ int pack_march(stzmg_t h) {
     }
     for (auto &L : h->lv) {
         if (!L.fused) continue;
-        const int stride = march_stride(L.pat);
+        const int key = L.pat + (L.sym ? kMarchSym : 0);
+        const int stride = march_stride(key);
         const long n = L.ncls * stride;
-        const int *map = h->pkmap + (long)L.pat * kPackMax;
+        const int *map = h->pkmap + (long)key * kPackMax;
         if (L.ftile == 1) {
             if (!L.pk32) HIPCHK(h, alloc(&L.pk32, L.ncls * kPackMax));
             hipLaunchKernelGGL(k_pack_march<float>, dim3(nblk(n)), dim3(kThreads), 0, h->stream,
@@ -855,6 +865,15 @@ int build_hierarchy_ops(stzmg_t h) {
             }
         }
         L.pat = pat;
+        // centrosymmetric: block(-o) equal to block(o) in every plane
+        L.sym = true;
+        for (long c = 0; c < L.ncls && L.sym; c++)
+            for (int b = 0; b < 13 && L.sym; b++)
+                for (int e = 0; e < 9; e++)
+                    if (tab[(size_t)c * kBlk + 9 * b + e] != tab[(size_t)c * kBlk + 9 * (26 - b) + e]) {
+                        L.sym = false;
+                        break;
+                    }
         L.ftile = fused_tile<float>(L);
         L.ftile64 = fused_tile<double>(L);
         L.fused = h->fused_smoother && L.ftile && L.ftile64 && L.x64b;
