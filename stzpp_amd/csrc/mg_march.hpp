@@ -252,14 +252,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
     constexpr int CSZ = WX * WY;           // one component of one plane
     constexpr int PSZ = 3 * CSZ;           // one plane slot
     constexpr int NW = NT / 64;            // waves
-    constexpr int RPI = 64 / TX;           // window rows per chunk-A instruction
     static_assert(TX == 32 || TX == 64, "tile width");
     static_assert(TY % 2 == 0 && NT % 64 == 0, "tile shape");
-    // chunk A: window columns [0, TX), RPI rows per instruction; chunk B:
-    // the last WX - TX columns, one element per thread and instruction
+    // chunk A: the tile's columns of every window row in 16-byte loads;
+    // chunk B: the WX - TX halo columns, one element per thread and load
+    constexpr int VW = 16 / (int)sizeof(T);  // values per 16-byte load
+    constexpr int LPR = TX / VW;             // lanes per window row
+    constexpr int RPI = 64 / LPR;            // window rows per wave instruction
+    static_assert(TX % VW == 0 && 64 % LPR == 0, "tile width");
     constexpr int CB = WX - TX;
-    constexpr int QA = 3 * WY / RPI, NA = QA / NW;
+    constexpr int NA = (3 * WY + NW * RPI - 1) / (NW * RPI);
     constexpr int EB = 3 * WY * CB, NB = (EB + NT - 1) / NT;
+    struct alignas(16) Vec {
+        T v[VW];
+    };
     using SP = SubPass<TX, TY, NT>;
     static_assert(march_lanes<TX, TY, NT>(0) <= NT && march_lanes<TX, TY, NT>(1) <= NT &&
                       march_lanes<TX, TY, NT>(2) <= NT && march_lanes<TX, TY, NT>(3) <= NT,
@@ -280,26 +286,31 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
     const long mn = (long)d.m * d.n;
 
     // ---- per-thread geometry, once per workgroup ----
-    // chunk A: column of this lane and its row within an instruction.  Wave
-    // wv reads the window rows q = wv + NW i (RPI rows each); rows are
-    // contiguous in LDS (row q at q RPI WX), and their global offsets take
-    // YA distinct values per component, kept per lane in ga[].
-    static_assert(WY % (NW * RPI) == 0, "window rows per wave");
-    constexpr int YA = WY / (NW * RPI);
-    const int ca = lane % TX, rsub = lane / TX;
-    unsigned ga[YA];
+    // chunk A: the tile's own TX columns of each window row, VW consecutive
+    // values per lane (one 16-byte load: the row segment starts at the tile's
+    // aligned x0), LPR lanes per row and RPI rows per wave instruction.
+    // Instruction i of wave wv covers rows (wv + NW i) RPI + lane / LPR
+    // (component-major, 3 WY rows); per lane the global and LDS offsets of
+    // each instruction are computed once.
+    unsigned ga[NA];
+    int la[NA];
+    const int col4 = lane % LPR;
 #pragma unroll
-    for (int j = 0; j < YA; j++)
-        ga[j] = (unsigned)(sizeof(T) * (wrap1(x0 - HXL + ca, d.m) +
-                                        (long)d.m * wrap1(y0 - H + (wv + NW * j) * RPI + rsub, d.n)));
-    const int la_off = (wv * RPI + rsub) * WX + (ca & 1) * HX + (ca >> 1);
-    // chunk B: up to NB elements of the right strip
+    for (int i = 0; i < NA; i++) {
+        const int row0 = (wv + NW * i) * RPI + lane / LPR;
+        const int row = row0 < 3 * WY ? row0 : 3 * WY - 1;
+        const int c = row / WY, yy = row - c * WY;
+        ga[i] = (unsigned)(sizeof(T) * (c * d.N + (long)d.m * wrap1(y0 - H + yy, d.n) + x0 + col4 * VW));
+        la[i] = c * CSZ + yy * WX + ((HXL + col4 * VW) >> 1);
+    }
+    // chunk B: the halo columns [0, HXL) and [HXL + TX, WX), up to NB
+    // elements per thread
     unsigned vb_off[NB];
     int lb_off[NB];
 #pragma unroll
     for (int j = 0; j < NB; j++) {
         const int e = tid + NT * j < EB ? tid + NT * j : EB - 1;
-        const int row = e / CB, col = TX + e % CB;
+        const int row = e / CB, cb = e % CB, col = cb < HXL ? cb : TX + cb;
         const int c = row / WY, yy = row - c * WY;
         vb_off[j] = (unsigned)(sizeof(T) *
                                (c * d.N + (long)d.m * wrap1(y0 - H + yy, d.n) + wrap1(x0 - HXL + col, d.m)));
@@ -351,7 +362,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
 
     // ---- staging ----
     struct Stage {
-        T a[NA];
+        Vec a[NA];
         T b[NB];
     };
     auto fetch = [&](const T *__restrict__ src, int kg, Stage &v) {
@@ -364,20 +375,29 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
         if (valid) {
             const T *base = src + mn * kw;
 #pragma unroll
-            for (int i = 0; i < NA; i++) v.a[i] = at_bytes(base + (i / YA) * d.N, ga[i % YA]);
+            for (int i = 0; i < NA; i++) v.a[i] = at_bytes(reinterpret_cast<const Vec *>(base), ga[i]);
 #pragma unroll
             for (int j = 0; j < NB; j++) v.b[j] = at_bytes(base, vb_off[j]);
         } else {
 #pragma unroll
-            for (int i = 0; i < NA; i++) v.a[i] = 0;
+            for (int i = 0; i < NA; i++)
+#pragma unroll
+                for (int e = 0; e < VW; e++) v.a[i].v[e] = 0;
 #pragma unroll
             for (int j = 0; j < NB; j++) v.b[j] = 0;
         }
     };
+    // element e of a chunk-A load sits at window x = HXL + col4 VW + e,
+    // parity-split: half (HXL + e) & 1, index la + ((HXL & 1) + e) / 2
     auto store = [&](int slot, const Stage &v) {
         T *sl = lds + slot * PSZ;
 #pragma unroll
-        for (int i = 0; i < NA; i++) sl[la_off + i * NW * RPI * WX] = v.a[i];
+        for (int i = 0; i < NA; i++)
+            if ((3 * WY) % (NW * RPI) == 0 || (wv + NW * i) * RPI + lane / LPR < 3 * WY) {
+#pragma unroll
+                for (int e = 0; e < VW; e++)
+                    sl[la[i] + ((HXL + e) & 1) * HX + ((HXL & 1) + e) / 2] = v.a[i].v[e];
+            }
 #pragma unroll
         for (int j = 0; j < NB; j++)
             if (EB % NT == 0 || tid + NT * j < EB) sl[lb_off[j]] = v.b[j];
@@ -412,14 +432,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
 #pragma unroll
     for (int s = 0; s < 4; s++) asm volatile("" : "+v"(rr[s][0]), "+v"(rr[s][1]), "+v"(rr[s][2]));
 
-    for (int kk = kk0; kk < kk1; kk++) {
+    // One plane.  The next plane's right-hand side and the next two staged
+    // planes (k + 2, k + 3) are issued first and stay in flight during this
+    // plane's arithmetic; the planes go to LDS at the end.  The loads
+    // complete in order, so each wait covers only what is due.  (Issuing the
+    // planes two steps ahead into a second register set measured the same.)
+    auto step = [&](int kk, Stage &n1, Stage &n2) {
         const int k = P + 2 * kk;
         const bool more = kk + 1 < kk1;
-        // the next plane's right-hand side, then its two staged planes, in
-        // flight during this plane's arithmetic (issued in that order: the
-        // loads complete in order, and nothing here waits for them)
         T rn[4][3];
-        Stage n1, n2;
         if (more) {
             load_r(k + 2, rn);
             fetch(xs, k + 2, n1);
@@ -562,7 +583,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
             sp = nsp;
             __syncthreads();
         }
-    }
+    };
+
+    Stage n1, n2;
+    for (int kk = kk0; kk < kk1; kk++) step(kk, n1, n2);
 }
 
 }  // namespace stzmg
