@@ -56,12 +56,17 @@ enum {
     STZMG_SOLVE_FP32_GMRES = 3, /* flexible GMRES(restart) in FP64, right-preconditioned by
                                    inner_cycles FP32 V-cycles from zero (GMRES-based
                                    iterative refinement); stops on the true FP64 residual */
-    STZMG_SOLVE_FP32_GMRES_IR = 4 /* iterative refinement in FP64 whose correction is an
+    STZMG_SOLVE_FP32_GMRES_IR = 4, /* iterative refinement in FP64 whose correction is an
                                      FP32 GMRES(restart) solve (FP32 Krylov basis and operator
                                      applies, FP64 dot products), right-preconditioned by
                                      inner_cycles FP32 V-cycles, each run until its residual
                                      estimate fell by inner_rtol; stops on the true FP64
                                      residual */
+    STZMG_SOLVE_GMRES64 = 5       /* flexible GMRES(restart) entirely in FP64, right-
+                                     preconditioned by inner_cycles FP64 V-cycles from zero
+                                     (the FP64 counterpart of STZMG_SOLVE_FP32_GMRES, for
+                                     measuring what the FP32 precision alone gains); stops on
+                                     the true FP64 residual */
 };
 
 typedef struct {

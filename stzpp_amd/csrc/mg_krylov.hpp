@@ -124,7 +124,8 @@ __global__ __launch_bounds__(kThreads) void k_scale(TD *__restrict__ dst, const 
 }
 
 // x += sum_j y[j] Z_j with FP32 Z (j < nv).
-__global__ __launch_bounds__(kThreads) void k_xupdate(double *__restrict__ x, const float *__restrict__ Z,
+template <typename TZ = float>
+__global__ __launch_bounds__(kThreads) void k_xupdate(double *__restrict__ x, const TZ *__restrict__ Z,
                                                       const double *__restrict__ y, int nv, long n3) {
     const long i = (long)blockIdx.x * kThreads + threadIdx.x;
     if (i >= n3) return;

@@ -113,6 +113,9 @@ struct stzmg_s {
     double *kdev = nullptr;   // reduced dots / coefficients
     int kcap = 0;             // restart length the workspace holds
     float *kV32 = nullptr;    // FP32 Krylov basis of the GMRES-IR solve
+    double *kZ64 = nullptr;   // FP64 preconditioned vectors of the FP64 GMRES solve
+    double *kX64 = nullptr, *kB64 = nullptr;  // its solution and right-hand side
+    int kcapz64 = 0;
     int kcap64 = 0, kcap32 = 0;  // restart lengths the two bases hold
     hipGraphExec_t g64 = nullptr, g32 = nullptr;
     double *scratch = nullptr;  // this device's scratch of stzmg_norm_noalloc
@@ -699,6 +702,37 @@ int norms64(stzmg_t h, int l, double out[2]) {
     return 0;
 }
 
+// Norms of r - A x on level l for explicit fields x, r (the residual left in
+// t64 of that level).
+int norms64_of(stzmg_t h, int l, const double *x, const double *r, double out[2]) {
+    Level &L = h->lv[l];
+    const long nb = resid_parts<double>(h, L);
+    if (ensure_part(h, nb)) return STZMG_ERR_NOMEM;
+    launch_resid<double, true>(h, L, x, r, L.t64);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, h->stream, (const double *)h->part, nb,
+                       h->dres);
+    LAUNCHCHK(h);
+    HIPCHK(h, hipMemcpyAsync(out, h->dres, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+// Workspace of the FP64 GMRES solve: m FP64 preconditioned vectors and the
+// solution and right-hand side while the level-0 fields serve the
+// preconditioner.
+int ensure_krylov64(stzmg_t h, int m, long n3) {
+    if (h->kcapz64 < m) {
+        hipFree(h->kZ64); hipFree(h->kX64); hipFree(h->kB64);
+        h->kZ64 = h->kX64 = h->kB64 = nullptr;
+        h->kcapz64 = 0;
+        HIPCHK(h, alloc(&h->kZ64, (long)m * n3));
+        HIPCHK(h, alloc(&h->kX64, n3));
+        HIPCHK(h, alloc(&h->kB64, n3));
+        h->kcapz64 = m;
+    }
+    return 0;
+}
+
 // Workspace of the GMRES solves for restart length m: the FP32
 // preconditioned vectors and the reduction scratch, plus the FP64 Krylov
 // basis (flexible GMRES) or the FP32 one (GMRES-IR).
@@ -1035,6 +1069,7 @@ int stzmg_destroy(stzmg_t h) {
     hipFree(h->dflag);
     hipFree(h->kV); hipFree(h->kZ); hipFree(h->ktmp); hipFree(h->kpart); hipFree(h->kdev);
     hipFree(h->kV32);
+    hipFree(h->kZ64); hipFree(h->kX64); hipFree(h->kB64);
     hipFree(h->qmap);
     hipFree(h->pkmap);
     hipStreamDestroy(h->stream);
@@ -1541,7 +1576,7 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
             }
             for (int i = 0; i < nj; i++) hc[i] = y[i] / sigma;
             HIPCHK(h, hipMemcpyAsync(h->kdev, hc.data(), sizeof(double) * nj, hipMemcpyHostToDevice, h->stream));
-            hipLaunchKernelGGL(k_xupdate, dim3(nb3), dim3(kThreads), 0, h->stream, L.x64, (const float *)h->kZ,
+            hipLaunchKernelGGL(k_xupdate<float>, dim3(nb3), dim3(kThreads), 0, h->stream, L.x64, (const float *)h->kZ,
                                (const double *)h->kdev, nj, n3);
             LAUNCHCHK(h);
             s.outer++;
@@ -1649,7 +1684,7 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
             }
             for (int i = 0; i < nj; i++) hc[i] = y[i] / sigma;
             HIPCHK(h, hipMemcpyAsync(h->kdev, hc.data(), sizeof(double) * nj, hipMemcpyHostToDevice, h->stream));
-            hipLaunchKernelGGL(k_xupdate, dim3(nb3), dim3(kThreads), 0, h->stream, L.x64, (const float *)h->kZ,
+            hipLaunchKernelGGL(k_xupdate<float>, dim3(nb3), dim3(kThreads), 0, h->stream, L.x64, (const float *)h->kZ,
                                (const double *)h->kdev, nj, n3);
             LAUNCHCHK(h);
             s.outer++;
@@ -1659,6 +1694,99 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
             s.res = beta;
             if (!std::isfinite(s.res)) { rc = STZMG_ERR_NONFINITE; break; }
         }
+    } else if (opts->mode == STZMG_SOLVE_GMRES64) {
+        // Flexible GMRES(m) in FP64 right-preconditioned by inner_cycles FP64
+        // V-cycles from zero: z_j = M^-1 v_j with the level-0 FP64 fields
+        // as the V-cycle's workspace, so the solution and right-hand side
+        // live in kX64 / kB64 during the solve.  The FP64 counterpart of
+        // STZMG_SOLVE_FP32_GMRES (same Gram-Schmidt, Givens rotations and
+        // stopping on the true residual at each restart).
+        Level &L = h->lv[0];
+        const long n3 = 3 * L.d.N;
+        const int m = std::min(std::max(opts->restart, 1), kKrylovMax);
+        if ((rc = ensure_krylov(h, m, n3, true))) return rc;
+        if ((rc = ensure_krylov64(h, m, n3))) return rc;
+        HIPCHK(h, hipMemcpyAsync(h->kX64, L.x64, sizeof(double) * n3, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->kB64, L.r64, sizeof(double) * n3, hipMemcpyDeviceToDevice, h->stream));
+        if ((rc = norms64_of(h, 0, h->kX64, h->kB64, v))) return rc;  // t64 = b - A x
+        s.norms++;
+        double beta = std::sqrt(v[1]);
+        s.res0 = s.res = beta;
+        const double target = opts->rtol * beta;
+        const int inner = opts->inner_cycles > 0 ? opts->inner_cycles : 1;
+        if (!std::isfinite(beta)) rc = STZMG_ERR_NONFINITE;
+        std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m), hc(m + 2);
+        auto Hc = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
+        const unsigned nb3 = nblk(n3);
+        while (rc == 0 && s.res > target && beta > 0) {
+            hipLaunchKernelGGL((k_scale<double, double>), dim3(nb3), dim3(kThreads), 0, h->stream, h->kV,
+                               (const double *)L.t64, 1.0 / beta, n3);
+            std::fill(g.begin(), g.end(), 0.0);
+            g[0] = beta;
+            int nj = 0;
+            for (int j = 0; j < m; j++) {
+                if (s.cycles >= opts->max_cycles) { rc = STZMG_ERR_NOT_CONVERGED; break; }
+                double *vj = h->kV + (long)j * n3, *w = h->kV + (long)(j + 1) * n3;
+                double *zj = h->kZ64 + (long)j * n3;
+                // z_j = M^-1 v_j
+                HIPCHK(h, hipMemcpyAsync(L.r64, vj, sizeof(double) * n3, hipMemcpyDeviceToDevice, h->stream));
+                hipLaunchKernelGGL((k_fill<double>), dim3(nb3), dim3(kThreads), 0, h->stream, L.x64, 0.0, n3);
+                if ((rc = run_vcycles<double>(h, inner, graph))) return rc;
+                s.cycles += inner;
+                // w = A z_j; the same pass keeps z_j
+                launch_apply<double>(h, L, (const double *)L.x64, w, zj);
+                launch_mdot_n<1>(h->stream, (const double *)w, (const double *)h->kV, j + 1, n3, h->kpart);
+                hipLaunchKernelGGL(k_reduce_cols, dim3(j + 1), dim3(kThreads), 0, h->stream,
+                                   (const double *)h->kpart, kDotBlocks, j + 1, h->kdev, (const double *)nullptr);
+                launch_maxpy_n<1>(h->stream, w, (const double *)h->kV, (const double *)h->kdev, j + 1, n3,
+                                  h->kpart);
+                hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(kThreads), 0, h->stream, (const double *)h->kpart,
+                                   kDotBlocks, 1, h->kdev + (j + 1), (const double *)nullptr);
+                LAUNCHCHK(h);
+                HIPCHK(h, hipMemcpyAsync(hc.data(), h->kdev, sizeof(double) * (j + 2), hipMemcpyDeviceToHost,
+                                         h->stream));
+                HIPCHK(h, hipStreamSynchronize(h->stream));
+                const double wn = std::sqrt(hc[j + 1]);
+                for (int i = 0; i <= j; i++) Hc(i, j) = hc[i];
+                Hc(j + 1, j) = wn;
+                if (!std::isfinite(wn)) { rc = STZMG_ERR_NONFINITE; break; }
+                if (wn > 0)
+                    hipLaunchKernelGGL((k_scale<double, double>), dim3(nb3), dim3(kThreads), 0, h->stream, w,
+                                       (const double *)w, 1.0 / wn, n3);
+                for (int i = 0; i < j; i++) {
+                    const double a = Hc(i, j), b = Hc(i + 1, j);
+                    Hc(i, j) = cs[i] * a + sn[i] * b;
+                    Hc(i + 1, j) = -sn[i] * a + cs[i] * b;
+                }
+                const double a = Hc(j, j), b = Hc(j + 1, j), r = std::hypot(a, b);
+                cs[j] = r > 0 ? a / r : 1.0;
+                sn[j] = r > 0 ? b / r : 0.0;
+                Hc(j, j) = r;
+                Hc(j + 1, j) = 0;
+                g[j + 1] = -sn[j] * g[j];
+                g[j] = cs[j] * g[j];
+                nj = j + 1;
+                if (std::fabs(g[j + 1]) <= target || wn == 0) break;
+            }
+            if (nj == 0) break;
+            for (int i = nj - 1; i >= 0; i--) {
+                double t = g[i];
+                for (int k2 = i + 1; k2 < nj; k2++) t -= Hc(i, k2) * y[k2];
+                y[i] = t / Hc(i, i);
+            }
+            HIPCHK(h, hipMemcpyAsync(h->kdev, y.data(), sizeof(double) * nj, hipMemcpyHostToDevice, h->stream));
+            hipLaunchKernelGGL(k_xupdate<double>, dim3(nb3), dim3(kThreads), 0, h->stream, h->kX64,
+                               (const double *)h->kZ64, (const double *)h->kdev, nj, n3);
+            LAUNCHCHK(h);
+            s.outer++;
+            if ((rc = norms64_of(h, 0, h->kX64, h->kB64, v))) return rc;
+            s.norms++;
+            beta = std::sqrt(v[1]);
+            s.res = beta;
+            if (!std::isfinite(s.res)) { rc = STZMG_ERR_NONFINITE; break; }
+        }
+        HIPCHK(h, hipMemcpyAsync(L.x64, h->kX64, sizeof(double) * n3, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(L.r64, h->kB64, sizeof(double) * n3, hipMemcpyDeviceToDevice, h->stream));
     } else {
         return fail(h, STZMG_ERR_ARG, "unknown solve mode %d", opts->mode);
     }

@@ -18,7 +18,7 @@ from . import _lib
 
 PLANE, POINT = 0, 1
 NORM_QUARTIC, NORM_L2SQ = 0, 1
-SOLVE_REF, SOLVE_RELRES, SOLVE_FP32_IR, SOLVE_FP32_GMRES, SOLVE_FP32_GMRES_IR = 0, 1, 2, 3, 4
+SOLVE_REF, SOLVE_RELRES, SOLVE_FP32_IR, SOLVE_FP32_GMRES, SOLVE_FP32_GMRES_IR, SOLVE_GMRES64 = 0, 1, 2, 3, 4, 5
 OPT_COARSE_KERNEL, OPT_FUSED_SMOOTHER, OPT_TILED_RESIDUAL, OPT_BOTTOM_MAP = 0, 1, 2, 3
 OPT_CHUNK_CAP, OPT_RESTRICT_PLANES = 4, 5
 
@@ -184,7 +184,7 @@ class Multigrid:
         default 1).  inner_rtol: reduction of each FP32 GMRES solve of
         FP32_GMRES_IR (0: the library's 1e-4)."""
         if inner_cycles is None:
-            inner_cycles = 1 if mode in (SOLVE_FP32_GMRES, SOLVE_FP32_GMRES_IR) else 3
+            inner_cycles = 1 if mode in (SOLVE_FP32_GMRES, SOLVE_FP32_GMRES_IR, SOLVE_GMRES64) else 3
         o = _lib.StzmgSolveOpts()
         self.L.stzmg_solve_opts_default(ctypes.byref(o))
         o.restart, o.inner_rtol = restart, inner_rtol

@@ -99,6 +99,19 @@ def trans_case():
                         blocks=np.array(outs), mu=1.3, lamb=1.7, h=np.array([0.11, 0.13, 0.17]))
 
 
+def config0_params(Nx):
+    """Config-0 parameters (mu = 1, K = 2.2, s_y = 1, eps0 = 10, chi_inf = 0.15,
+    c0 = 0.4, U = 0.01); the remaining constants are the reference driver's,
+    src/shear_sim.cpp:18-86, for Nx cells across x in [-1, 1)."""
+    TZ, kb = 21000.0, 1.3806503e-23
+    E, nu, sy = 101e9, 0.35, 0.85e9
+    mu_r, rho = E / (2 * (1 + nu)), 6125.0
+    ts = 1e-2 / np.sqrt(mu_r / rho)
+    return dict(mu=1.0, lamb=2.2 - 2.0 / 3.0, sy=1.0, tau0=1e-13 / ts, c0=0.4, eps0=10.0,
+                delta=8000 / TZ, omega=300 * 26.0 / 17.0 * 1e-30 * sy / (TZ * kb), bath_temp=400 / TZ,
+                chi_inf=0.15, ez=1.0, diff_l=1.5 * 2.0 / Nx, U=0.01)
+
+
 def step_case(name):
     """QS steps of the reference shear_sim_3d<sym_mat3>.
 
@@ -113,16 +126,8 @@ def step_case(name):
 
     from stzpp_amd import fields
 
-    # config-0 parameters (see stzpp_amd.shear_sim.config0_params); the
-    # remaining constants are the reference driver's, src/shear_sim.cpp:18-86
-    TZ, kb = 21000.0, 1.3806503e-23
-    E, nu, sy = 101e9, 0.35, 0.85e9
-    mu_r, rho = E / (2 * (1 + nu)), 6125.0
-    ts = 1e-2 / np.sqrt(mu_r / rho)
     N = (16, 16, 8) if name == "step_plastic" else (32, 32, 32)
-    p = dict(mu=1.0, lamb=2.2 - 2.0 / 3.0, sy=1.0, tau0=1e-13 / ts, c0=0.4, eps0=10.0,
-             delta=8000 / TZ, omega=300 * 26.0 / 17.0 * 1e-30 * sy / (TZ * kb), bath_temp=400 / TZ,
-             chi_inf=0.15, ez=1.0, diff_l=1.5 * 2.0 / N[0], U=0.01)
+    p = config0_params(N[0])
     Nx, Ny, Nz = N
     nc, nv = Nx * Ny * Nz, Nx * Ny * (Nz + 1)
     chi0 = 0.09 + 0.01 * fields.grf((Nz, Ny, Nx), 3.0, 0).ravel()
@@ -424,6 +429,62 @@ def l2_case(name):
     np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
 
 
+def adj_case(name):
+    """Central finite differences of the reference's QS step
+    (step_forward_qs, src/shear_sim_3d.cpp:381-389) for the adjoint's loss
+    J = mean over cells of (ad_Dpl/dt)^2 (ad_Dpl as set at :431) after 3
+    steps: 32 x 32 x 16 cells, z walled, config-0 material with eps0 = 0.3,
+    a yielding band of two planes (sigma_12 = 1.12 against 0.5 elsewhere,
+    with noise), t0 = 0.3, dt = 0.01.  Directions: 16 seeded random values
+    on 4 x 4 x 4 patches of cells that overlap the band.  For each the
+    reference's J at chi0 +- h d for h = 1e-6 and h = 3e-7, so a test can
+    tell where J is smooth at that scale (the adaptive plastic sub-steps
+    make it piecewise smooth)."""
+    import tempfile
+    N = (32, 32, 16)
+    Nx, Ny, Nz = N
+    nc, nv = Nx * Ny * Nz, Nx * Ny * (Nz + 1)
+    P = config0_params(Nx)
+    P["eps0"] = 0.3
+    rng = np.random.default_rng(7)
+    sig0 = np.zeros((nc, 6))
+    k = np.arange(nc) // (Nx * Ny)
+    sig0[:, 1] = np.where((k >= 7) & (k <= 8), 1.12, 0.5) + 0.02 * rng.standard_normal(nc)
+    sig0[:, 0] = 0.05 * rng.standard_normal(nc)
+    chi0 = 0.09 + 0.005 * rng.standard_normal(nc)
+    u0 = 1e-3 * rng.standard_normal((nv, 3))
+    t0, dt, nsteps = 0.3, 0.01, 3
+    outdir = tempfile.mkdtemp()
+
+    def loss(chi):
+        ref = O.RefShear(N, P, outdir)
+        ref.setup(dt)
+        ref.set_state(t0, u0, sig0, chi)
+        ref.step(dt, nsteps)
+        d = ref.state()["dpl"]
+        ref.close()
+        return float(np.mean((d / dt) ** 2))
+
+    drng = np.random.default_rng(12)
+    idx, val, jp, jm = [], [], [], []
+    for _ in range(16):
+        ci, cj, ck = drng.integers(0, Nx), drng.integers(0, Ny), drng.integers(5, 9)
+        cells = np.array([((ci + a) % Nx) + Nx * (((cj + b) % Ny) + Ny * (ck + c))
+                          for c in range(4) for b in range(4) for a in range(4)])
+        v = drng.standard_normal(cells.size)
+        d = np.zeros(nc)
+        d[cells] = v
+        idx.append(cells)
+        val.append(v)
+        jp.append([loss(chi0 + h * d) for h in (1e-6, 3e-7)])
+        jm.append([loss(chi0 - h * d) for h in (1e-6, 3e-7)])
+    out = dict(N=np.array(N), t0=np.array(t0), dt=np.array(dt), nsteps=np.array(nsteps), u0=u0, sig0=sig0,
+               chi0=chi0, J0=np.array(loss(chi0)), h=np.array([1e-6, 3e-7]), dir_cells=np.array(idx),
+               dir_vals=np.array(val), J_plus=np.array(jp), J_minus=np.array(jm),
+               **{f"p_{k}": np.array(v) for k, v in P.items()})
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+
+
 CASES = {
     "shear_16x16x17": (("shear_16x16x17", (16, 16, 17), (1, 1, 0), "shear", 11),),
     "trans_16x16x16": (("trans_16x16x16", (16, 16, 16), (1, 1, 1), "trans", 12),),
@@ -449,12 +510,14 @@ if __name__ == "__main__":
             direct_case(which)
         elif which.startswith("l2_"):
             l2_case(which)
+        elif which.startswith("adj_"):
+            adj_case(which)
         else:
             mg_case(*CASES[which][0])
         sys.exit(0)
     for key in ["trans", *CASES, "step_plastic", "step_c0", "trans_step_walled", "trans_step_periodic",
                 *GRF_CASES, "output_walled", "output_periodic", "output_trans", "direct_walled",
-                "direct_periodic", "l2_walled", "l2_periodic"]:
+                "direct_periodic", "l2_walled", "l2_periodic", "adj_ref_32"]:
         subprocess.run([sys.executable, __file__, key], check=True, cwd=ROOT,
                        stdout=subprocess.DEVNULL)
         print("wrote", key)

@@ -375,7 +375,7 @@ def test_gpu_fp32_ir_matches_fp64_at_size(n):
     assert relerr(mg.x(0), 2 * x64) < 1e-9
 
 
-@pytest.mark.parametrize("mode", [MG.SOLVE_FP32_GMRES, MG.SOLVE_FP32_GMRES_IR])
+@pytest.mark.parametrize("mode", [MG.SOLVE_FP32_GMRES, MG.SOLVE_FP32_GMRES_IR, MG.SOLVE_GMRES64])
 @pytest.mark.parametrize("kind", ["shear", "trans"])
 def test_gpu_fp32_gmres_matches_fp64(kind, mode):
     # FGMRES in FP64 preconditioned by FP32 V-cycles, and GMRES-IR (FP32
