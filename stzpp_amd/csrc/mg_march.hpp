@@ -1,5 +1,5 @@
 // Z-marching fused 8-colour block Gauss-Seidel for per-plane operators on
-// levels periodic in x and y, with x a multiple of the tile width (32 or 64).
+// levels periodic in x and y, with x a multiple of the tile width (16, 32 or 64).
 //
 // Same colour order and the same halo recomputation as k_gs_planes
 // (mg_fused.hpp): one launch updates every plane of one z parity P, a
@@ -224,16 +224,20 @@ constexpr int march_lanes(int s) {
 // Waves per SIMD that the LDS of one workgroup (three staged window planes)
 // allows on a CU of 160 KiB: the register allocation is held to that
 // occupancy.
+// The 16-wide tiles serve small levels (16^3 to 32^3 points), whose launches
+// are latency-bound: they take the registers of two waves per SIMD rather
+// than spill to reach the occupancy their LDS would allow.
 template <typename T, int TX, int TY, int NT>
 constexpr int march_waves() {
     constexpr int lds = 3 * 3 * (TX + 7) * (TY + 8) * (int)sizeof(T);
     constexpr int wgs = (160 * 1024) / lds;
     constexpr int w = wgs * NT / 64 / 4;
-    return w < 1 ? 1 : (w > 8 ? 8 : w);
+    constexpr int cap = TX == 16 ? 2 : 8;
+    return w < 1 ? 1 : (w > cap ? cap : w);
 }
 
 // Grid: (m/TX, n/TY, ceil(planes of parity P / kchunk)).  Requires a PLANE
-// operator, x and y periodic, m % TX == 0, n % TY == 0, TX in {32, 64} and
+// operator, x and y periodic, m % TX == 0, n % TY == 0, TX in {16, 32, 64} and
 // every sub-pass of the tile at most NT points.  pk: the packed table of the
 // level (k_pack_march), one row of march_stride(PAT) values per plane.
 template <typename T, int PAT, int TX, int TY, int NT>
@@ -252,7 +256,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
     constexpr int CSZ = WX * WY;           // one component of one plane
     constexpr int PSZ = 3 * CSZ;           // one plane slot
     constexpr int NW = NT / 64;            // waves
-    static_assert(TX == 32 || TX == 64, "tile width");
+    static_assert(TX == 16 || TX == 32 || TX == 64, "tile width");
     static_assert(TY % 2 == 0 && NT % 64 == 0, "tile shape");
     // chunk A: the tile's columns of every window row in 16-byte loads;
     // chunk B: the WX - TX halo columns, one element per thread and load

@@ -473,8 +473,16 @@ void launch_fused_pat(stzmg_t h, Level &L) {
             launch_fused_tile<T, PAT, TX, TY, NT, false>(h, L);
         }
     }
-    else if (ft == 2)
-        launch_fused_tile<T, PAT, kSmallTile, kSmallTile, kSmallThreads>(h, L);
+    else if (ft == 2) {
+        if constexpr (STZ_MARCH != 0) {
+            if (L.sym)
+                launch_fused_tile<T, PAT + kMarchSym, kSmallTile, kSmallTile, kSmallThreads, true>(h, L);
+            else
+                launch_fused_tile<T, PAT, kSmallTile, kSmallTile, kSmallThreads, true>(h, L);
+        } else {
+            launch_fused_tile<T, PAT, kSmallTile, kSmallTile, kSmallThreads>(h, L);
+        }
+    }
     else
         launch_fused_tile<T, PAT, kTinyTile, kTinyTile, kTinyThreads>(h, L);
 }
@@ -829,12 +837,12 @@ int pack_march(stzmg_t h) {
         const int stride = march_stride(key);
         const long n = L.ncls * stride;
         const int *map = h->pkmap + (long)key * kPackMax;
-        if (L.ftile == 1) {
+        if (L.ftile == 1 || L.ftile == 2) {
             if (!L.pk32) HIPCHK(h, alloc(&L.pk32, L.ncls * kPackMax));
             hipLaunchKernelGGL(k_pack_march<float>, dim3(nblk(n)), dim3(kThreads), 0, h->stream,
                                (const float *)L.ops32, (const float *)L.inv32, map, stride, L.ncls, L.pk32);
         }
-        if (L.ftile64 == 1) {
+        if (L.ftile64 == 1 || L.ftile64 == 2) {
             if (!L.pk64) HIPCHK(h, alloc(&L.pk64, L.ncls * kPackMax));
             hipLaunchKernelGGL(k_pack_march<double>, dim3(nblk(n)), dim3(kThreads), 0, h->stream,
                                (const double *)L.ops64, (const double *)L.inv64, map, stride, L.ncls, L.pk64);
