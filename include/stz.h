@@ -52,7 +52,8 @@ typedef struct {
  * T = diag(e^{Ut t}, 1, e^{-Ut t}) is the one the reference compiles in;
  * simple shear T = I + (Ut t / bz) e_x e_z^T, homogeneous T = e^{Ut t} I and
  * the identity are its commented alternatives. */
-enum { STZ_T_NONE = 0, STZ_T_PURE_SHEAR = 1, STZ_T_SIMPLE_SHEAR = 2, STZ_T_HOMOGENEOUS = 3, STZ_T_IDENTITY = 4 };
+enum { STZ_T_NONE = 0, STZ_T_PURE_SHEAR = 1, STZ_T_SIMPLE_SHEAR = 2, STZ_T_HOMOGENEOUS = 3, STZ_T_IDENTITY = 4,
+       STZ_T_USER = 5 /* set by stz_set_transform */ };
 
 typedef struct {
     int steps;             /* steps taken */
@@ -113,6 +114,18 @@ int stz_adjoint(stz_t s, double dt, int nsteps, const stz_adjoint_opts *o, doubl
 /* T, T^-1 and dT/dt (row-major 3x3) of a transform family at time t
  * (compute_T, compute_Tinv, compute_dTdt, src/trans_sim_3d.hh:57-125). */
 int stz_transform(int family, double Ut, double bz, double t, double *T, double *Tinv, double *dTdt);
+/* An arbitrary transform for a transformed-coordinate handle (one created
+ * with a transform family), in place of the families of
+ * src/trans_sim_3d.hh:57-140: from the handle's current time t0 on,
+ * T(t) = T + (t - t0) dTdt with dT/dt = dTdt (row-major 3x3), T^-1 from
+ * cofactors.  The step uses T(t) for its advection and T(t + dt) for its
+ * projection, so a transform affine in time (simple shear, any constant
+ * velocity gradient) is exact, and a caller that sets T(t) and dT/dt(t)
+ * before every step gets T(t + dt) to first order in dt.  As in the
+ * reference's generated stencil (src/stenc_funcs.hh), the projection
+ * operator reads the upper triangle of T only.  STZMG_ERR_STATE for an
+ * untransformed handle, STZMG_ERR_SINGULAR for det T = 0. */
+int stz_set_transform(stz_t s, const double *T, const double *dTdt);
 
 /* ---- random initial chi (Gaussian random field) -------------------------
  *

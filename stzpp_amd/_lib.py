@@ -99,6 +99,7 @@ SIGNATURES = {
     "stz_grf_field_device": ([_ip, _ip, ctypes.POINTER(StzGrfParams), _vp, _vp, _vp], ctypes.c_int),
     "stz_grf_time_kernels": ([_ip, _ip, ctypes.POINTER(StzGrfParams), ctypes.c_int, _dp], ctypes.c_int),
     "stz_grf_last_error": ([], ctypes.c_char_p),
+    "stz_set_transform": ([_vp, _dp, _dp], ctypes.c_int),
     "stz_set_random_initial_conditions": ([_vp, ctypes.POINTER(StzGrfParams), _dp, ctypes.c_char_p], ctypes.c_int),
     "stz_cube_size": ([_vp, ctypes.c_int], ctypes.c_longlong),
     "stz_output_time_kernel": ([_vp, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),

@@ -58,8 +58,35 @@ void m3set(double *a, double a11, double a12, double a13, double a21, double a22
     a[0] = a11; a[1] = a12; a[2] = a13; a[3] = a21; a[4] = a22; a[5] = a23; a[6] = a31; a[7] = a32; a[8] = a33;
 }
 
+// Inverse of a 3x3 matrix by cofactors; false when singular.
+bool m3inv(const double *a, double *o) {
+    const double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8], c02 = a[3] * a[7] - a[4] * a[6];
+    const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    if (!(det != 0.0) || !std::isfinite(det)) return false;
+    const double id = 1.0 / det;
+    o[0] = c00 * id;
+    o[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+    o[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+    o[3] = c01 * id;
+    o[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+    o[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+    o[6] = c02 * id;
+    o[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+    o[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+    return true;
+}
+
 Xform xform_at(const stz_t s, double t) {
     Xform X;
+    if (s->p.transform == STZ_T_USER) {
+        for (int e = 0; e < 9; e++) {
+            X.T[e] = s->user_T[e] + (t - s->user_t0) * s->user_dT[e];
+            X.dT[e] = s->user_dT[e];
+        }
+        if (!m3inv(X.T, X.Ti))
+            for (int e = 0; e < 9; e++) X.Ti[e] = std::nan("");
+        return X;
+    }
     stz_transform(s->p.transform, s->p.Ut, s->p.bz, t, X.T, X.Ti, X.dT);
     return X;
 }
@@ -203,6 +230,23 @@ int stz_transform(int fam, double Ut, double bz, double t, double *T, double *Ti
 }
 
 static thread_local std::string g_stz_err;
+
+int stz_set_transform(stz_t s, const double *T, const double *dTdt) {
+    if (!s || !T || !dTdt) return STZMG_ERR_ARG;
+    if (s->p.transform == STZ_T_NONE)
+        return sfail(s, STZMG_ERR_STATE, "stz_set_transform: the handle runs the untransformed step");
+    double Ti[9];
+    if (!m3inv(T, Ti)) return sfail(s, STZMG_ERR_SINGULAR, "stz_set_transform: T is singular");
+    for (int e = 0; e < 9; e++) {
+        if (!std::isfinite(T[e]) || !std::isfinite(dTdt[e]))
+            return sfail(s, STZMG_ERR_ARG, "stz_set_transform: non-finite entry");
+        s->user_T[e] = T[e];
+        s->user_dT[e] = dTdt[e];
+    }
+    s->user_t0 = s->t;
+    s->p.transform = STZ_T_USER;
+    return 0;
+}
 
 int stz_create(const stz_params *p, stz_t *out) {
     if (!p || !out) return STZMG_ERR_ARG;

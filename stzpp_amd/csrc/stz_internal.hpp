@@ -30,6 +30,8 @@ struct stz_s {
     std::string out_dir;        // per-step output (stz_set_output)
     int out_flags = 0;
     int pm_writes = 0;          // proj_mag.dat lines written by this handle
+    double user_T[9] = {0}, user_dT[9] = {0};  // stz_set_transform: T(t) = user_T + (t - user_t0) user_dT
+    double user_t0 = 0;
     std::string err;
 };
 

@@ -29,7 +29,7 @@ def grf(shape, ell, seed, unit_variance=True):
         sh = [1] * len(shape)
         sh[ax] = len(k)
         filt = filt * g.reshape(sh)
-    f = np.fft.irfftn(spec * filt, s=shape)
+    f = np.fft.irfftn(spec * filt, s=shape, axes=list(range(len(shape))))
     f -= f.mean()
     if unit_variance:
         f /= f.std()

@@ -43,7 +43,7 @@ class StepError(MultigridError):
 
 
 # transform families of trans_sim_3d (src/trans_sim_3d.hh:57-125); include/stz.h STZ_T_*
-TRANSFORMS = {"none": 0, "pure_shear": 1, "simple_shear": 2, "homogeneous": 3, "identity": 4}
+TRANSFORMS = {"none": 0, "pure_shear": 1, "simple_shear": 2, "homogeneous": 3, "identity": 4, "user": 5}
 
 
 def transform(family, Ut, t, bz=0.5):
@@ -350,7 +350,19 @@ class TransSim(ShearSim):
     def matrices(self, t=None):
         """T, T^-1, dT/dt at time t (default: the current simulation time)."""
         t = self.state_time() if t is None else t
+        if self.family == "user":
+            T = self.user_T + (t - self.user_t0) * self.user_dT
+            return T, np.linalg.inv(T), self.user_dT.copy()
         return transform(self.family, self.Ut, t, self.params.get("bz", 0.5))
+
+    def set_transform(self, T, dTdt):
+        """An arbitrary transform from the current time t0 on:
+        T(t) = T + (t - t0) dTdt (stz_set_transform)."""
+        Ta = np.ascontiguousarray(T, dtype=np.float64).reshape(9)
+        Da = np.ascontiguousarray(dTdt, dtype=np.float64).reshape(9)
+        self._chk(self.L.stz_set_transform(self.h, _lib.dptr(Ta), _lib.dptr(Da)))
+        self.family = "user"
+        self.user_T, self.user_dT, self.user_t0 = Ta.reshape(3, 3).copy(), Da.reshape(3, 3).copy(), self.state_time()
 
 
 def shear_compare(dsim, qsim, ncomps, time_interval, qdt, ddt=None, out_path=None, time_scale=1.0, **qs_kw):

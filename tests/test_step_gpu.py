@@ -238,3 +238,66 @@ def test_gpu_transform_matches_oracle():
             B = O.transform_at(fam, 0.8, 0.5, t)
             for x, y in zip(A, B):
                 assert np.array_equal(x, y), fam
+
+
+def test_gpu_user_transform_reproduces_simple_shear(golden):
+    # stz_set_transform with the simple shear's T(t0) and dT/dt: the family
+    # is affine in t, so the user transform gives the same steps
+    g = golden("trans_step_walled.npz")
+    N = tuple(int(v) for v in g["N"])
+    prd = tuple(int(v) for v in g["prd"])
+    Ut, dt = float(g["Ut"]), float(g["dt"])
+    out = []
+    for user in (False, True):
+        sim = SS.TransSim(*N, Ut, family="identity" if user else "simple_shear", params=params_of(g), prd=prd)
+        sim.set_initial_conditions(g["chi0"], u=g["u0"], sig=g["sig0"])
+        if user:
+            T, _, dT = SS.transform("simple_shear", Ut, 0.0)
+            sim.set_transform(T, dT)
+        for _ in range(int(g["nsteps"])):
+            sim.step_forward_qs(dt, 1, solve=SS.SOLVE_RELRES, rtol=1e-13)
+        out.append(sim.state())
+    a, b = out
+    for key in KEYS:
+        assert relerr(a[key], b[key]) < 1e-10, key
+
+
+def test_gpu_user_transform_operator_and_errors(golden):
+    # a general (non-family) constant-rate transform: the step's projection
+    # operator is the transformed stencil at T(t + dt); an untransformed
+    # handle and a singular T are refused
+    import ctypes
+
+    from stzpp_amd import _lib
+    from stzpp_amd import multigrid as MG
+
+    g = golden("trans_step_walled.npz")
+    N = tuple(int(v) for v in g["N"])
+    prd = tuple(int(v) for v in g["prd"])
+    p = params_of(g)
+    dt = float(g["dt"])
+    sim = SS.TransSim(*N, 0.1, family="identity", params=p, prd=prd)
+    sim.set_initial_conditions(g["chi0"], u=g["u0"], sig=g["sig0"])
+    T0 = np.array([[1.05, 0.1, 0.2], [0.0, 0.97, 0.05], [0.0, 0.0, 1.02]])
+    dT = np.array([[0.1, 0.0, 0.3], [0.0, -0.05, 0.0], [0.0, 0.0, 0.02]])
+    sim.set_transform(T0, dT)
+    st = sim.step_forward_qs(dt, 1, solve=SS.SOLVE_RELRES, rtol=1e-12)
+    assert st["steps"] == 1
+    s = sim.state()
+    assert all(np.isfinite(s[k]).all() for k in KEYS)
+    L = _lib.load()
+    mg = L.stz_multigrid(sim.h)
+    n = ctypes.c_long()
+    assert L.stzmg_get_level_operator(mg, 0, None, ctypes.byref(n)) == 0
+    ops = np.zeros((n.value, 27, 3, 3))
+    assert L.stzmg_get_level_operator(mg, 0, _lib.dptr(ops), None) == 0
+    q = sim.params
+    h = [(q["bx"] - q["ax"]) / N[0], (q["by"] - q["ay"]) / N[1], (q["bz"] - q["az"]) / N[2]]
+    ref = MG.trans_stencil(T0 + dt * dT, q["mu"], q["lamb"], *h, dt)
+    assert relerr(ops[n.value // 2], ref) < 1e-14
+    # refusals
+    plain = SS.ShearSim(*N, params=p)
+    Ta = np.eye(3).ravel().copy()
+    assert L.stz_set_transform(plain.h, _lib.dptr(Ta), _lib.dptr(np.zeros(9))) == -5
+    sing = np.zeros(9)
+    assert L.stz_set_transform(sim.h, _lib.dptr(sing), _lib.dptr(np.zeros(9))) == -8
