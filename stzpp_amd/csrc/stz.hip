@@ -1,7 +1,7 @@
 // Host side of the quasi-static STZ step and its C ABI (include/stz.h).
 //
 // One step follows shear_sim_3d::step_forward_qs (src/shear_sim_3d.cpp:
-// 381-389): plasticity + advection (two passes, k_plastic then k_advect, so
+// 381-389): plasticity + advection (two passes, k_plastic then k_advect_march, so
 // that the reference's in-place ad_Dpl ordering is reproduced without a
 // sequential loop), time advance, projection (right-hand side, multigrid
 // solve, copy-out with plate values, stress update).  The transformed step
@@ -25,6 +25,7 @@
 #include "../../include/stz.h"
 #include "mg_internal.hpp"
 #include "stz_internal.hpp"
+#include "stz_advect.hpp"
 #include "stz_kernels.hpp"
 
 using namespace stz;
@@ -394,6 +395,40 @@ bool step_options(stz_t s, const stzmg_solve_opts *opts, stzmg_solve_opts &o) {
     return true;
 }
 
+template <bool TRANS, bool PZ>
+static void advect_variant(stz_t s, double dt, const XformD &X, const Box &box) {
+    const Grid &g = s->g;
+    int &r = s->adv_resident[(TRANS ? 2 : 0) + (PZ ? 1 : 0)];
+    if (r <= 0) {
+        // workgroups the device keeps resident at once: one round of the launch
+        int dev = 0, ncu = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+            ncu = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_advect_march<TRANS, PZ>, kT, 0) != hipSuccess ||
+            per < 1)
+            per = 1;
+        r = ncu * per;
+    }
+    const AdvTiles at = adv_tiles(g, r);
+    hipLaunchKernelGGL((k_advect_march<TRANS, PZ>), dim3((unsigned)((long)at.ntx * at.nty * at.nzc)), dim3(kT), 0,
+                       s->stream, g, s->M, dt, X, box, at, (const double *)s->vel, (const double *)s->sig,
+                       (const double *)s->chi, (const double *)s->adapt, (const double *)s->dchi,
+                       (const double *)s->dpl2, (const double *)s->dpl, s->sig2, s->chi2);
+}
+
+// Pass 2 of the step: k_advect_march for the plain or transformed step,
+// periodic or walled in z.
+static void launch_advect(stz_t s, bool trans, double dt, const XformD &X, const Box &box) {
+    if (trans) {
+        if (s->g.pz) advect_variant<true, true>(s, dt, X, box);
+        else advect_variant<true, false>(s, dt, X, box);
+    } else {
+        if (s->g.pz) advect_variant<false, true>(s, dt, X, box);
+        else advect_variant<false, false>(s, dt, X, box);
+    }
+}
+
 int step_once(stz_t s, double dt, stzmg_solve_opts &o, bool ref_tol, stz_step_stats &S, double *sigstar) {
     const bool trans = s->p.transform != STZ_T_NONE;
     const Grid &g = s->g;
@@ -404,18 +439,12 @@ int step_once(stz_t s, double dt, stzmg_solve_opts &o, bool ref_tol, stz_step_st
         hipLaunchKernelGGL(k_plastic_trans, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt, Xn,
                            (const double *)s->sig, (const double *)s->chi, s->adapt, s->dchi, s->dpl2,
                            s->counters);
-        hipLaunchKernelGGL(k_advect_trans, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt, Xn, box,
-                           (const double *)s->vel, (const double *)s->sig, (const double *)s->chi,
-                           (const double *)s->adapt, (const double *)s->dchi, (const double *)s->dpl2,
-                           (const double *)s->dpl, s->sig2, s->chi2);
+        launch_advect(s, true, dt, xform_products(Xn), box);
     } else {
         hipLaunchKernelGGL(k_plastic, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt,
                            (const double *)s->sig, (const double *)s->chi, s->adapt, s->dchi, s->dpl2,
                            s->counters);
-        hipLaunchKernelGGL(k_advect, dim3(nb(g.Nc)), dim3(kT), 0, s->stream, g, s->M, dt,
-                           (const double *)s->vel, (const double *)s->sig, (const double *)s->chi,
-                           (const double *)s->adapt, (const double *)s->dchi, (const double *)s->dpl2,
-                           (const double *)s->dpl, s->sig2, s->chi2);
+        launch_advect(s, false, dt, XformD{}, box);
     }
     SHIP(s, hipGetLastError());
     std::swap(s->sig, s->sig2);

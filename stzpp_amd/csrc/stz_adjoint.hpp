@@ -279,7 +279,7 @@ static __global__ __launch_bounds__(kT) void k_adj_div(Grid g, const double *__r
     ssb[5 * g.Nc + c] = sb[5 * g.Nc + c] + a33;
 }
 
-// Transpose of k_advect.  In: the pre-step state (vel, sig, chi, the old
+// Transpose of pass 2 of the plain step (k_advect_march<false, PZ>).  In: the pre-step state (vel, sig, chi, the old
 // ad_Dpl dold), this step's plastic quantities (adapt, dnew) and the adjoint
 // of sigma* (ssb) and chi* (xsb).  Out (accumulated, the identity parts
 // already in sb_out / xb_out): sigma-bar, chi-bar, u-bar, dnew-bar, dold-bar;
@@ -593,7 +593,8 @@ static __global__ __launch_bounds__(kT) void k_adj_div_trans(Grid g, Xform X, co
     ssb[5 * g.Nc + c] = sb[5 * g.Nc + c] + a33;
 }
 
-// Transpose of k_advect_trans (see there for the forward terms).  Writes
+// Transpose of pass 2 of the transformed step (k_advect_march<true, PZ>,
+// see there for the forward terms).  Writes
 // adapt-bar and dchi-bar of the cell; accumulates sigma'-bar (including the
 // physical-stress part of the plastic update), chi-bar, U-bar, dnew-bar and
 // dold-bar.

@@ -32,6 +32,7 @@ struct stz_s {
     int pm_writes = 0;          // proj_mag.dat lines written by this handle
     double user_T[9] = {0}, user_dT[9] = {0};  // stz_set_transform: T(t) = user_T + (t - user_t0) user_dT
     double user_t0 = 0;
+    int adv_resident[4] = {0, 0, 0, 0};  // resident workgroups of the k_advect_march variants
     std::string err;
 };
 

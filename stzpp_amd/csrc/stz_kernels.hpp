@@ -265,99 +265,6 @@ static __global__ __launch_bounds__(kT) void k_plastic(Grid g, Mat M, double dt,
     dpl_new[c] = a / 2. / M.mu * sb;
 }
 
-// Pass 2: advection of stress and chi with ENO2, Truesdell terms and chi
-// diffusion (advection_grid_step, src/shear_sim_3d.cpp:436-525), merged
-// into new arrays (merge_updates, :154-162).  ad_Dpl follows the reference's
-// in-place lexicographic loop: this step's value at the cell and at its
-// -x/-y/-z neighbours inside the domain, last step's value at +x/+y/+z and
-// at periodic images, zero across a wall.
-static __global__ __launch_bounds__(kT) void k_advect(Grid g, Mat M, double dt, const double *__restrict__ vel,
-                                               const double *__restrict__ sig, const double *__restrict__ chi,
-                                               const double *__restrict__ adapt, const double *__restrict__ dchi,
-                                               const double *__restrict__ dnew, const double *__restrict__ dold,
-                                               double *__restrict__ sig_out, double *__restrict__ chi_out) {
-    const long c = xcd_block() * kT + threadIdx.x;
-    if (c >= g.Nc) return;
-    const int i = (int)(c % g.Nx);
-    const int j = (int)((c / g.Nx) % g.Ny);
-    const int k = (int)(c / ((long)g.Nx * g.Ny));
-    const Stencil5 S = stencil5(g, i, j, k, g.Nz);
-    const bool zin = g.pz || (k >= 2 && k <= g.Nz - 3);  // no wall ghost in the z stencil
-    double G[3][3], v[3];
-    vel_grad_o(vel, g, c, S.ox[3], S.oy[3], S.oz[3], G, v);
-    const double hx = g.dx / dt, hy = g.dy / dt, hz = g.dz / dt;
-    double ug[7];
-#pragma unroll
-    for (int q = 0; q < 7; q++) {
-        const double *f = q < 6 ? sig + q * g.Nc : chi;
-        const double *fc = f + c;
-        const double f0 = fc[0];
-        const double ddx = eno2(hx, v[0], fc[S.ox[4]], fc[S.ox[3]], f0, fc[S.ox[1]], fc[S.ox[0]]);
-        const double ddy = eno2(hy, v[1], fc[S.oy[4]], fc[S.oy[3]], f0, fc[S.oy[1]], fc[S.oy[0]]);
-        double z2p, z1p, z1m, z2m;
-        if (zin) {
-            z2p = fc[S.oz[4]];
-            z1p = fc[S.oz[3]];
-            z1m = fc[S.oz[1]];
-            z2m = fc[S.oz[0]];
-        } else {
-            z2p = cellv(f, g, i, j, k + 2);
-            z1p = cellv(f, g, i, j, k + 1);
-            z1m = cellv(f, g, i, j, k - 1);
-            z2m = cellv(f, g, i, j, k - 2);
-        }
-        const double ddz = eno2(hz, v[2], z2p, z1p, f0, z1m, z2m);
-        ug[q] = v[0] * ddx + v[1] * ddy + v[2] * ddz;
-    }
-    const double s11 = sig[c], s12 = sig[g.Nc + c], s13 = sig[2 * g.Nc + c];
-    const double s22 = sig[3 * g.Nc + c], s23 = sig[4 * g.Nc + c], s33 = sig[5 * g.Nc + c];
-    const double du_dx = G[0][0], du_dy = G[0][1], du_dz = G[0][2];
-    const double dv_dx = G[1][0], dv_dy = G[1][1], dv_dz = G[1][2];
-    const double dw_dx = G[2][0], dw_dy = G[2][1], dw_dz = G[2][2];
-    const double t11 = s11 * (du_dx - dv_dy - dw_dz) + 2 * s13 * du_dz + 2 * s12 * du_dy;
-    const double t12 = s23 * du_dz + s13 * dv_dz - s12 * dw_dz + s22 * du_dy + s11 * dv_dx;
-    const double t13 = s33 * du_dz + s11 * dw_dx + s23 * du_dy + s12 * dw_dy - s13 * dv_dy;
-    const double t22 = 2 * s23 * dv_dz + s22 * (dv_dy - du_dx - dw_dz) + 2 * s12 * dv_dx;
-    const double t23 = s33 * dv_dz + s22 * dw_dy + s13 * dv_dx - s23 * du_dx + s12 * dw_dx;
-    const double t33 = s33 * (dw_dz - dv_dy - du_dx) + 2 * s23 * dw_dy + 2 * s13 * dw_dx;
-    const double a = adapt[c], third = 1. / 3.;
-    sig_out[c] = s11 + (a * third * (s33 + s22 - 2 * s11) - ug[0] + dt * t11);
-    sig_out[g.Nc + c] = s12 + (-a * s12 - ug[1] + dt * t12);
-    sig_out[2 * g.Nc + c] = s13 + (-a * s13 - ug[2] + dt * t13);
-    sig_out[3 * g.Nc + c] = s22 + (a * third * (s11 - 2 * s22 + s33) - ug[3] + dt * t22);
-    sig_out[4 * g.Nc + c] = s23 + (-a * s23 - ug[4] + dt * t23);
-    sig_out[5 * g.Nc + c] = s33 + (a * third * (s11 + s22 - 2 * s33) - ug[5] + dt * t33);
-    // chi diffusion (chi_diffusion, src/shear_sim_3d.cpp:1217-1237)
-    const double Dc = dnew[c];
-    const double Dxm = i > 0 ? dnew[c - 1] : dold[c + S.ox[1]];
-    const double Dxp = dold[c + S.ox[3]];
-    const double Dym = j > 0 ? dnew[c - g.Nx] : dold[c + S.oy[1]];
-    const double Dyp = dold[c + S.oy[3]];
-    double Dzm, Dzp;
-    if (g.pz) {
-        Dzm = k > 0 ? dnew[c - (long)g.Nx * g.Ny] : dold[c + S.oz[1]];
-        Dzp = dold[c + S.oz[3]];
-    } else {
-        Dzm = k > 0 ? dnew[c - (long)g.Nx * g.Ny] : 0.0;
-        Dzp = k < g.Nz - 1 ? dold[c + (long)g.Nx * g.Ny] : 0.0;
-    }
-    const double *xc = chi + c;
-    const double x0 = xc[0];
-    const double xxm = xc[S.ox[1]], xxp = xc[S.ox[3]];
-    const double xym = xc[S.oy[1]], xyp = xc[S.oy[3]];
-    const double xzm = zin ? xc[S.oz[1]] : cellv(chi, g, i, j, k - 1);
-    const double xzp = zin ? xc[S.oz[3]] : cellv(chi, g, i, j, k + 1);
-    const double ix = 1 / g.dx, iy = 1 / g.dy, iz = 1 / g.dz;
-    const double ddx = .5 * ix * (Dxp - Dxm) * (.5 * ix * (xxp - xxm)) + Dc * (ix * ix * (xxp - 2 * x0 + xxm));
-    const double ddy = .5 * iy * (Dyp - Dym) * (.5 * iy * (xyp - xym)) + Dc * (iy * iy * (xyp - 2 * x0 + xym));
-    const double ddz = .5 * iz * (Dzp - Dzm) * (.5 * iz * (xzp - xzm)) + Dc * (iz * iz * (xzp - 2 * x0 + xzm));
-    const double diff = M.diff_l * M.diff_l * (ddx + ddy + ddz);
-    double cchi = dchi[c];
-    cchi -= ug[6];
-    cchi += diff / M.c0;
-    chi_out[c] = x0 + cchi;
-}
-
 // Right-hand side and warm start of the projection solve (setup_fields with
 // x_field / r_field, src/shear_sim_3d.hh:276-385): x = current velocity;
 // r = div sigma* at interior vertices (backward 8-cell differences,
@@ -629,151 +536,6 @@ static __global__ __launch_bounds__(kT) void k_plastic_trans(Grid g, Mat M, doub
     adapt[c] = a;
     dchi[c] = dc;
     dpl_new[c] = D;
-}
-
-// Pass 2 of the transformed step (advection_grid_step,
-// src/trans_sim_3d.cpp:880-992): L and T terms, ENO2 advection of sigma'
-// and chi (the x derivative of chi takes f(+1) as its f(+2), :985), the
-// plastic update T^-1 adapt dev(T sigma' T^t) T^-t, and chi diffusion with
-// the metric J = T^-1 T^-t (chi_diffusion, :779-812, including the mixed
-// y-z difference of :802 whose last two terms are the same cell).  ad_Dpl
-// follows the in-place order as in k_advect.
-static __global__ __launch_bounds__(kT) void k_advect_trans(Grid g, Mat M, double dt, Xform X, Box B,
-                                                     const double *__restrict__ vel, const double *__restrict__ sig,
-                                                     const double *__restrict__ chi,
-                                                     const double *__restrict__ adapt,
-                                                     const double *__restrict__ dchi, const double *__restrict__ dnew,
-                                                     const double *__restrict__ dold, double *__restrict__ sig_out,
-                                                     double *__restrict__ chi_out) {
-    const long c = xcd_block() * kT + threadIdx.x;
-    if (c >= g.Nc) return;
-    const int i = (int)(c % g.Nx);
-    const int j = (int)((c / g.Nx) % g.Ny);
-    const int k = (int)(c / ((long)g.Nx * g.Ny));
-    const double P0[3] = {B.ax + ((double)i + .5) * g.dx, B.ay + ((double)j + .5) * g.dy,
-                          B.az + ((double)k + .5) * g.dz};
-    const Stencil5 S = stencil5(g, i, j, k, g.Nz);
-    const bool zin = g.pz || (k >= 2 && k <= g.Nz - 3);  // no wall ghost in the z stencil
-    const int o1x = S.ox[3], o1y = S.oy[3], o1z = S.oz[3];
-    double L[9];
-    compute_L_o(vel, g, B, X, c, o1x, o1y, o1z, P0, L);
-    double v[3];
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-        const double *u = vel + q * g.Nv + c;
-        v[q] = 0.125 * (u[0] + u[o1x] + u[o1z] + u[o1x + o1z] + u[o1y] + u[o1x + o1y] + u[o1y + o1z] +
-                        u[o1x + o1y + o1z]);
-    }
-    double sp[9], os[9], Tti[9], tmp[9], tmp2[9];
-    old_sig(sig, g.Nc, c, X, sp, os);
-    m3t(X.Ti, Tti);
-    // L term: T^-1 (-tr(L) sigma + sigma L + L^t sigma) T^-t
-    double Lt[9], A[9], Bm[9], Lterm[9];
-    const double trL = L[0] + L[4] + L[8];
-    m3t(L, Lt);
-    m3mul(os, L, A);
-    m3mul(Lt, os, Bm);
-#pragma unroll
-    for (int e = 0; e < 9; e++) tmp[e] = (os[e] * -(trL) + A[e]) + Bm[e];
-    m3mul(X.Ti, tmp, tmp2);
-    m3mul(tmp2, Tti, Lterm);
-    // T term: -(T^-1 dT sigma' + sigma' (T^-1 dT)^t)
-    double TdT[9], TdTt[9], Tterm[9];
-    m3mul(X.Ti, X.dT, TdT);
-    m3t(TdT, TdTt);
-    m3mul(TdT, sp, A);
-    m3mul(sp, TdTt, Bm);
-#pragma unroll
-    for (int e = 0; e < 9; e++) Tterm[e] = (A[e] + Bm[e]) * -1.;
-    const double hx = g.dx / dt, hy = g.dy / dt, hz = g.dz / dt;
-    const int comp[6] = {0, 1, 2, 4, 5, 8};
-    double cs[6];
-#pragma unroll
-    for (int q = 0; q < 6; q++) {
-        const double *f = sig + q * g.Nc;
-        const double *fc = f + c;
-        const double f0 = fc[0];
-        const double ddx = eno2(hx, v[0], fc[S.ox[4]], fc[S.ox[3]], f0, fc[S.ox[1]], fc[S.ox[0]]);
-        const double ddy = eno2(hy, v[1], fc[S.oy[4]], fc[S.oy[3]], f0, fc[S.oy[1]], fc[S.oy[0]]);
-        const double ddz = zin ? eno2(hz, v[2], fc[S.oz[4]], fc[S.oz[3]], f0, fc[S.oz[1]], fc[S.oz[0]])
-                               : eno2(hz, v[2], cellv(f, g, i, j, k + 2), cellv(f, g, i, j, k + 1), f0,
-                                      cellv(f, g, i, j, k - 1), cellv(f, g, i, j, k - 2));
-        const double ug = v[0] * ddx + v[1] * ddy + v[2] * ddz;
-        cs[q] = 0.0 + (dt * (Lterm[comp[q]] + Tterm[comp[q]]) - ug);
-    }
-    const double a = adapt[c];
-    if (a != 0.0) {
-        const double third = 1. / 3., tr = os[0] + os[4] + os[8];
-        const double dv[9] = {os[0] - third * tr, os[1], os[2], os[1], os[4] - third * tr, os[5],
-                              os[2], os[5], os[8] - third * tr};
-        double Ta[9], P[9];
-#pragma unroll
-        for (int e = 0; e < 9; e++) Ta[e] = X.Ti[e] * a;
-        m3mul(Ta, dv, tmp);
-        m3mul(tmp, Tti, P);
-#pragma unroll
-        for (int q = 0; q < 6; q++) cs[q] -= P[comp[q]];
-    }
-#pragma unroll
-    for (int q = 0; q < 6; q++) sig_out[q * g.Nc + c] = sig[q * g.Nc + c] + cs[q];
-    // chi: advection (x quirk) and metric diffusion
-    const double *xc = chi + c;
-    const double x0 = xc[0];
-    const double cx = eno2(hx, v[0], xc[S.ox[3]], xc[S.ox[3]], x0, xc[S.ox[1]], xc[S.ox[0]]);
-    const double cy = eno2(hy, v[1], xc[S.oy[4]], xc[S.oy[3]], x0, xc[S.oy[1]], xc[S.oy[0]]);
-    const double cz = zin ? eno2(hz, v[2], xc[S.oz[4]], xc[S.oz[3]], x0, xc[S.oz[1]], xc[S.oz[0]])
-                          : eno2(hz, v[2], cellv(chi, g, i, j, k + 2), cellv(chi, g, i, j, k + 1), x0,
-                                 cellv(chi, g, i, j, k - 1), cellv(chi, g, i, j, k - 2));
-    const double Dc = dnew[c];
-    const double Dxm = i > 0 ? dnew[c - 1] : dold[c + S.ox[1]];
-    const double Dxp = dold[c + S.ox[3]];
-    const double Dym = j > 0 ? dnew[c - g.Nx] : dold[c + S.oy[1]];
-    const double Dyp = dold[c + S.oy[3]];
-    double Dzm, Dzp;
-    if (g.pz) {
-        Dzm = k > 0 ? dnew[c - (long)g.Nx * g.Ny] : dold[c + S.oz[1]];
-        Dzp = dold[c + S.oz[3]];
-    } else {
-        Dzm = k > 0 ? dnew[c - (long)g.Nx * g.Ny] : 0.0;
-        Dzp = k < g.Nz - 1 ? dold[c + (long)g.Nx * g.Ny] : 0.0;
-    }
-    const double ix = 1. / g.dx, iy = 1. / g.dy, iz = 1. / g.dz;
-    const double xxm = xc[S.ox[1]], xxp = xc[S.ox[3]];
-    const double xym = xc[S.oy[1]], xyp = xc[S.oy[3]];
-    const double xzm = zin ? xc[S.oz[1]] : cellv(chi, g, i, j, k - 1);
-    const double xzp = zin ? xc[S.oz[3]] : cellv(chi, g, i, j, k + 1);
-    const double dDx = .5 * ix * (Dxp - Dxm), dDy = .5 * iy * (Dyp - Dym), dDz = .5 * iz * (Dzp - Dzm);
-    const double gc[3] = {.5 * ix * (xxp - xxm), .5 * iy * (xyp - xym), .5 * iz * (xzp - xzm)};
-    double J[9], Jg[3];
-    m3mul(X.Ti, Tti, J);
-    m3vec(J, gc, Jg);
-    const double term1 = dDx * Jg[0] + dDy * Jg[1] + dDz * Jg[2];
-    const double dxx = ix * ix * (xxp - 2 * x0 + xxm);
-    const double dyy = iy * iy * (xyp - 2 * x0 + xym);
-    const double dzz = iz * iz * (xzp - 2 * x0 + xzm);
-    const double dxy = .25 * ix * iy *
-                       (xc[S.ox[3] + S.oy[3]] + xc[S.ox[1] + S.oy[1]] - xc[S.ox[3] + S.oy[1]] -
-                        xc[S.ox[1] + S.oy[3]]);
-    const double dxz = zin ? .25 * ix * iz *
-                                 (xc[S.ox[3] + S.oz[3]] + xc[S.ox[1] + S.oz[1]] - xc[S.ox[3] + S.oz[1]] -
-                                  xc[S.ox[1] + S.oz[3]])
-                           : .25 * ix * iz *
-                                 (cellv(chi, g, i + 1, j, k + 1) + cellv(chi, g, i - 1, j, k - 1) -
-                                  cellv(chi, g, i + 1, j, k - 1) - cellv(chi, g, i - 1, j, k + 1));
-    const double ypzm = zin ? xc[S.oy[3] + S.oz[1]] : cellv(chi, g, i, j + 1, k - 1);
-    const double yzpp = zin ? xc[S.oy[3] + S.oz[3]] : cellv(chi, g, i, j + 1, k + 1);
-    const double yzmm = zin ? xc[S.oy[1] + S.oz[1]] : cellv(chi, g, i, j - 1, k - 1);
-    const double dyz = .25 * iy * iz * (yzpp + yzmm - ypzm + ypzm);
-    double term2 = 0;
-    term2 += J[0] * dxx + J[1] * dxy + J[2] * dxz;
-    term2 += J[3] * dxy + J[4] * dyy + J[5] * dyz;
-    term2 += J[6] * dxz + J[7] * dyz + J[8] * dzz;
-    term2 *= Dc;
-    const double diff = M.diff_l * M.diff_l * (term1 + term2);
-    double cchi = dchi[c];
-    cchi -= v[0] * cx + v[1] * cy + v[2] * cz;
-    cchi += diff / M.c0;
-    chi_out[c] = x0 + cchi;
 }
 
 // Right-hand side T div sigma' at interior vertices, zero on the walls, and
