@@ -41,12 +41,12 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernel (fine-level FP32 fused
 # smoother, one z parity of the 256^3 workload), from two rocprofv3 --pmc
 # runs of `python3 bench.py --steps 1 --warmup 0 --no-cpu --no-graph`:
-# FETCH_SIZE 154.5 MiB x 2.0 + WRITE_SIZE 96.0 MiB x 1.0 per dispatch.  The
-# factors come from the plain copy kernel k_convert<double,double> of the
-# same run (3 x 256^3 doubles = 384 MiB each way), which reads FETCH_SIZE
-# 192.0 MiB and WRITE_SIZE 384.0 MiB.  Summary:
+# FETCH_SIZE 159.09 MiB x 2.0 + WRITE_SIZE 102.52 MiB x 1.0 per dispatch
+# (average of 658 dispatches).  The factors come from the plain copy kernel
+# k_convert<double,double> of the same run (3 x 256^3 doubles = 384 MiB each
+# way), which reads FETCH_SIZE 192.0 MiB and WRITE_SIZE 384.0 MiB.  Summary:
 # profiles/mg_solve_256_trans_kernels.txt.
-TRAFFIC_256_BYTES_PER_LAUNCH = (154.5 * 2.0 + 96.0) * 2 ** 20
+TRAFFIC_256_BYTES_PER_LAUNCH = (159.09 * 2.0 + 102.52) * 2 ** 20
 
 
 def log(msg):
@@ -67,23 +67,160 @@ def make_problem(n):
     return mg, b, st
 
 
-def cpu_baseline(n, st, b, budget_s=30.0):
-    """Reference algorithm (lexicographic block GS, reference schedule) in the
-    C oracle on one host core: one V-cycle at the bench size, scaled by the
-    number of FP64 cycles the GPU solve needs."""
+def host_cpu():
+    """(CPU model, nproc, threads for the CPU baseline): the OpenMP thread
+    count this process may use (OMP_NUM_THREADS within the affinity mask),
+    at most 16."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = nproc
+    try:
+        want = int(os.environ.get("OMP_NUM_THREADS", "") or aff)
+    except ValueError:
+        want = aff
+    return model, nproc, max(1, min(want, aff, 16))
+
+
+def rank_grid(threads, dims, prd):
+    """The reference's subdomain grid for the largest rank count <= threads
+    that calc_processor_division accepts (src/common.cpp:7-60)."""
     import oracle as O
 
+    for P in range(threads, 0, -1):
+        r = O.processor_division(P, *dims, prd)
+        if r:
+            return P, r
+    return 1, (1, 1, 1)
+
+
+def calibration():
+    """Timings of the compiled reference beside the oracle on one machine
+    (written by oracle/ref/time_reference.py), or None."""
+    path = os.path.join(ROOT, "oracle", "ref", "reference_timings.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(n, st, b, gpu_fp64_cycles):
+    """The reference's algorithm (lexicographic block GS on the subdomain grid
+    of P ranks, reference schedule) in the C oracle on P OpenMP threads of
+    this host: one timed V-cycle x the oracle's V-cycle count to 1e-10 for
+    that grid (measured once by oracle/ref/time_reference.py; the GPU's FP64
+    count where none is recorded).  The reference-equivalent time scales it by
+    the reference/oracle V-cycle ratio measured on one machine at equal
+    ranks."""
+    import oracle as O
+
+    model, nproc, threads = host_cpu()
+    P, ranks = rank_grid(threads, (n, n, n), (1, 1, 1))
+    O.set_threads(P)
     orc = O.OracleMG(n, n, n, (1, 1, 1), mode=O.PLANE)
     orc.set_ops(0, O.plane_operator(n, 1, st, None))
     orc.rat()
     orc.params(smoother=0)
-    N = n ** 3
-    orc.set_fields(0, np.zeros((N, 3)), b)
+    orc.set_ranks(ranks)
+    orc.set_fields(0, np.zeros((n ** 3, 3)), b)
     t0 = time.perf_counter()
     orc.vcycle(1)
-    dt = time.perf_counter() - t0
+    cyc_s = time.perf_counter() - t0
     orc.close()
-    return dt
+    cal = calibration()
+    cycles, src = gpu_fp64_cycles, "the GPU FP64 multigrid's count"
+    ratio = None
+    if cal and n == 256:
+        rec = cal.get("trans_256_cycles_to_1e-10", {}).get(str(P))
+        if rec and rec.get("cycles"):
+            cycles, src = rec["cycles"], f"the oracle's count for ranks {tuple(rec['ranks'])}"
+        t = cal.get("trans_256", {})
+        ratio = t.get("calibration_8" if P > 1 else "calibration_1")
+    out = {
+        "value": cyc_s * cycles,
+        "unit": "s (time to 1e-10: one measured V-cycle x cycle count)",
+        "cores": P,
+        "kind": "port",
+        "sample": f"one lexicographic V-cycle of the C oracle at {n}^3 on {P} threads as the reference's "
+                  f"{ranks[0]}x{ranks[1]}x{ranks[2]} rank grid took {cyc_s:.2f} s; x {cycles} cycles ({src})",
+        "cpu_model": model,
+        "nproc": nproc,
+        "ranks": list(ranks),
+        "vcycle_s": cyc_s,
+        "cycles": cycles,
+    }
+    if ratio:
+        out["reference_equivalent_s"] = cyc_s * cycles * ratio
+        out["calibration"] = {
+            "reference_over_oracle": ratio,
+            "measured_on": cal.get("cpu_model"),
+            "ranks": 8 if P > 1 else 1,
+            "source": "oracle/ref/reference_timings.json (reference trans_sim_3d V-cycle at 256^3 vs the oracle's)",
+        }
+    return out
+
+
+def step_roofline(sim, n_points):
+    """Roofline of the step's dominant kernel, the fine-level FP64 fused
+    Gauss-Seidel sweep of its projection solver (two launches, one per z
+    parity, 72 algorithmic bytes per point per sweep), timed with HIP events
+    on the solver's stream."""
+    import ctypes
+
+    from stzpp_amd import _lib
+
+    L = _lib.load()
+    h = L.stz_multigrid(sim.h)
+    v = ctypes.c_double()
+    rc = L.stzmg_time_kernel(h, 0, 0, 20, ctypes.byref(v))
+    if rc:
+        return None
+    sweep_s = v.value * 1e-3
+    alg = 36.0 * n_points
+    achieved = alg / (sweep_s / 2) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "k_gs_march<double, ...> (fine-level fused GS of the projection, one z parity)",
+            "alg_bytes_per_launch": alg, "launch_us": sweep_s / 2 * 1e6}
+
+
+def oracle_step_baseline(N, prd, params, state0, dt, family=None, Ut=0.0):
+    """One step of the oracle (the reference's step and projection loop with
+    the lexicographic sweep on the subdomain grid of P ranks, P OpenMP
+    threads) from the workload's initial state.  The advection pass of the
+    oracle is sequential (it keeps the reference's in-place ad_Dpl order)."""
+    import oracle as O
+
+    model, nproc, threads = host_cpu()
+    nzv = N[2] + (0 if prd[2] else 1)
+    P, ranks = rank_grid(threads, (N[0], N[1], nzv), prd)
+    O.set_threads(P)
+    box = (params["ax"], params["bx"], params["ay"], params["by"], params["az"], params["bz"])
+    orc = O.OracleShear(N, params, box=box, prd=prd)
+    if family:
+        orc.set_transform(family, Ut)
+    orc.set_ranks(ranks)
+    orc.set_state(state0["t"], state0["u"], state0["sig"], state0["chi"])
+    t0 = time.perf_counter()
+    orc.step(dt, 1)
+    el = time.perf_counter() - t0
+    cyc = orc.vcycles
+    orc.close()
+    return {"value": 1.0 / el, "unit": "steps/s", "cores": P, "kind": "port",
+            "sample": f"the first step of the workload by the C oracle on {P} threads (rank grid "
+                      f"{ranks[0]}x{ranks[1]}x{ranks[2]}): {el:.2f} s, {cyc} V-cycles",
+            "cpu_model": model, "nproc": nproc, "ranks": list(ranks), "step_s": el, "vcycles": cyc}
 
 
 def step_workload(args):
@@ -98,6 +235,7 @@ def step_workload(args):
     # chi0 from the reference's random initial condition (cube-cutoff
     # Gaussian, l = 4 cells, cut 20) on the GPU, noise seed 3
     sim.set_random_initial_conditions(gauss_fac=5, conv_l=4.0, chi_avg=0.09, sigma=0.01, seed=3)
+    state0 = sim.state()
     dt = 0.005
     log(f"[bench] step workload {Nx}x{Ny}x{Nz}, warmup {args.warmup}")
     graph = not args.no_graph
@@ -125,8 +263,11 @@ def step_workload(args):
                                "material, chi0 GRF seed 3 l=4, dt 0.005",
                    "projection": how, "vcycles_per_step": cyc / args.steps,
                    "solve_ms_per_step": solve_ms / args.steps, "v_cycle_graph": graph},
-        "roofline": None, "cpu_baseline": None,
+        "roofline": step_roofline(sim, Nx * Ny * (Nz + 1)), "cpu_baseline": None,
     }
+    if not args.no_cpu:
+        log("[bench] cpu baseline (oracle step, all cores)")
+        out["cpu_baseline"] = oracle_step_baseline((Nx, Ny, Nz), (1, 1, 0), p, state0, dt)
     print(json.dumps(out), flush=True)
 
 
@@ -256,6 +397,7 @@ def trans_workload(args):
     p["eps0"] = 0.3
     sim = SS.TransSim(*N, 0.5, family="pure_shear", params=p, prd=(1, 1, 1))
     sim.set_random_initial_conditions(gauss_fac=5, conv_l=4.0, chi_avg=0.09, sigma=0.01, seed=3)
+    state0 = sim.state()
     dt = 0.005
     graph = not args.no_graph
     log(f"[bench] trans workload {N}, warmup {args.warmup}")
@@ -278,8 +420,11 @@ def trans_workload(args):
                                "Ut 0.5, config-0 material eps0 0.3, chi0 GRF seed 3 l=4, dt 0.005",
                    "vcycles_per_step": cyc / args.steps, "solve_ms_per_step": solve_ms / args.steps,
                    "v_cycle_graph": graph},
-        "roofline": None, "cpu_baseline": None,
+        "roofline": step_roofline(sim, N[0] * N[1] * N[2]), "cpu_baseline": None,
     }
+    if not args.no_cpu:
+        log("[bench] cpu baseline (oracle step, all cores)")
+        out["cpu_baseline"] = oracle_step_baseline(N, (1, 1, 1), p, state0, dt, family="pure_shear", Ut=0.5)
     print(json.dumps(out), flush=True)
 
 
@@ -585,16 +730,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log("[bench] cpu baseline (oracle, 1 core, one V-cycle)")
-        cyc_s = cpu_baseline(n, st, b)
-        cpu = {
-            "value": cyc_s * s64["cycles"],
-            "unit": "s (estimated time to 1e-10: one measured V-cycle x GPU FP64 cycle count)",
-            "cores": 1,
-            "kind": "port",
-            "sample": f"one lexicographic V-cycle of the C oracle at {n}^3 took {cyc_s:.2f} s; "
-                      f"x {s64['cycles']} cycles",
-        }
+        log("[bench] cpu baseline (oracle, all cores, one V-cycle)")
+        cpu = cpu_baseline(n, st, b, s64["cycles"])
 
     t32 = float(np.mean(tgis))
     tfg = float(np.mean(t32s))

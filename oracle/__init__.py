@@ -57,6 +57,9 @@ def lib():
             ("mgo_set_fields", [vp, ctypes.c_int, _dp, _dp], None),
             ("mgo_get_fields", [vp, ctypes.c_int, _dp, _dp], None),
             ("mgo_params", [vp] + [ctypes.c_int] * 5, None),
+            ("mgo_set_ranks", [vp] + [ctypes.c_int] * 3, None),
+            ("mgo_set_threads", [ctypes.c_int], None),
+            ("mgo_get_threads", [], ctypes.c_int),
             ("mgo_do_rat", [vp], None),
             ("mgo_do_residual", [vp, ctypes.c_int, _dp], None),
             ("mgo_do_apply", [vp, ctypes.c_int, _dp, _dp], None),
@@ -75,6 +78,38 @@ def lib():
 
 
 PLANE, POINT = 0, 1
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's parallel loops (0 leaves the default)."""
+    lib().mgo_set_threads(int(n))
+
+
+def get_threads():
+    return lib().mgo_get_threads()
+
+
+def processor_division(procs, m, n, o, prd):
+    """Subdomain grid of the reference's calc_processor_division
+    (src/common.cpp:7-60): among i*j*k = procs splits that divide m, n, o
+    evenly, the first (k outermost, then j) with the smallest cut surface
+    m*n*(k or k-1) + m*o*(j or j-1) + n*o*(i or i-1), the -1 where that
+    dimension is not periodic.  None when no split divides evenly."""
+    best, dims = m * n * o, None
+    for k in range(1, procs + 1):
+        if procs % k or o % k:
+            continue
+        p2 = procs // k
+        for j in range(1, p2 + 1):
+            if p2 % j or n % j:
+                continue
+            i = p2 // j
+            if m % i:
+                continue
+            t = m * n * (k if prd[2] else k - 1) + m * o * (j if prd[1] else j - 1) + n * o * (i if prd[0] else i - 1)
+            if t < best:
+                best, dims = t, (i, j, k)
+    return dims
 
 
 class OracleMG:
@@ -136,6 +171,12 @@ class OracleMG:
 
     def params(self, v_down=2, v_up=2, bottom=24, growth=1, smoother=0):
         self.L.mgo_params(self.h, v_down, v_up, bottom, growth, smoother)
+
+    def set_ranks(self, ranks):
+        """Subdomain grid of the lexicographic sweep: the reference's sweep on
+        ranks[0] x ranks[1] x ranks[2] MPI ranks (subdomains swept in parallel
+        on OpenMP threads)."""
+        self.L.mgo_set_ranks(self.h, *(int(r) for r in ranks))
 
     def rat(self):
         self.L.mgo_do_rat(self.h)
@@ -479,6 +520,7 @@ def _step_lib():
         L.sto_vcycles.argtypes = [vp]
         L.sto_vcycles.restype = ctypes.c_int
         L.sto_set_solver.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.sto_set_ranks.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.sto_set_transform.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L.sto_set_eta.argtypes = [vp, ctypes.c_double]
         L.sto_proj_mag.argtypes = [vp]
@@ -561,6 +603,12 @@ class OracleShear:
         """smoother 0 = lexicographic (reference), 1 = 8-colour; rtol 0 = the
         reference stopping rule, else a relative-residual solve."""
         self.L.sto_set_solver(self.h, smoother, rtol)
+
+    def set_ranks(self, ranks):
+        """The projection's lexicographic sweep as the reference's on
+        ranks[0] x ranks[1] x ranks[2] MPI ranks (subdomains on OpenMP
+        threads)."""
+        self.L.sto_set_ranks(self.h, *(int(r) for r in ranks))
 
     def set_transform(self, family, Ut):
         """Run the transformed-coordinate step of trans_sim_3d with transform

@@ -24,6 +24,12 @@
  * The 8-colour smoother (mgo_gs_color) has no reference counterpart: it is
  * the ordering the GPU kernels use, kept here so their sweeps can be
  * checked one for one.
+ *
+ * Loops whose iterations write disjoint points (operator application,
+ * restriction, interpolation, the Galerkin product, the sweeps of one
+ * colour and the subdomains of a lexicographic sweep) run on OpenMP
+ * threads; every result is independent of the thread count.  Sums (the
+ * norms) stay serial.
  */
 #include "mg_oracle.h"
 
@@ -51,6 +57,7 @@ int mgo_build(mgo_hier *h, int m, int n, int o, const int prd[3], int mode,
     h->bottom_sweeps = 24;
     h->growth = 1;
     h->smoother = 0;
+    h->ranks[0] = h->ranks[1] = h->ranks[2] = 1;
     int l = 0;
     for (;;) {
         mgo_level *L = &h->lv[l];
@@ -136,6 +143,7 @@ static long nbr(const mgo_level *L, int i, int j, int k, int *wrapped) {
 }
 
 void mgo_apply(const mgo_level *L, const double *x, double *out) {
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < L->o; k++)
         for (int j = 0; j < L->n; j++)
             for (int i = 0; i < L->m; i++) {
@@ -178,7 +186,12 @@ void mgo_block_inverse(const double *a, double *inv) {
     inv[6] = c20 * id; inv[7] = c21 * id; inv[8] = c22 * id;
 }
 
-static void gs_point(mgo_level *L, int i, int j, int k, const double *snap) {
+/* One point of block Gauss-Seidel.  With a snapshot, every neighbour outside
+ * the sweeping block [lo, hi) or reached through a periodic wrap is read from
+ * it: the ghost layer the reference refreshes once per sweep
+ * (region.cc:1601, :2760-2847).  Without one (the colour sweeps) all
+ * neighbours are read live. */
+static void gs_point(mgo_level *L, int i, int j, int k, const double *snap, const int *lo, const int *hi) {
     const double *A = mgo_op(L, i, j, k);
     double acc[3] = {0, 0, 0};
     for (int b = 0; b < 27; b++) {
@@ -187,7 +200,11 @@ static void gs_point(mgo_level *L, int i, int j, int k, const double *snap) {
         int w = 0;
         long q = nbr(L, i + di, j + dj, k + dk, &w);
         if (q < 0) continue;
-        mv_acc(A + 9 * b, (w && snap ? snap : L->x) + 3 * q, acc);
+        const double *src = L->x;
+        if (snap && (w || i + di < lo[0] || i + di >= hi[0] || j + dj < lo[1] || j + dj >= hi[1] ||
+                     k + dk < lo[2] || k + dk >= hi[2]))
+            src = snap;
+        mv_acc(A + 9 * b, src + 3 * q, acc);
     }
     double inv[9], d[3];
     mgo_block_inverse(A + 9 * 13, inv);
@@ -200,17 +217,42 @@ static void gs_point(mgo_level *L, int i, int j, int k, const double *snap) {
     L->x[3 * p + 2] = inv[6] * d[0] + inv[7] * d[1] + inv[8] * d[2];
 }
 
-/* Lexicographic in-place block Gauss-Seidel.  Periodic images are read from
- * a snapshot taken at the start of the sweep, as the reference reads its
- * ghost layer refreshed once per sweep (region.cc:1601). */
-void mgo_gs_lex(mgo_level *L) {
+static int imin(int a, int b) { return a < b ? a : b; }
+
+/* Lexicographic in-place block Gauss-Seidel of the reference on a grid of
+ * ranks[0] x ranks[1] x ranks[2] subdomains: each subdomain
+ * [m*ip/mp, m*(ip+1)/mp) x ... (the split of calc_processor_division's
+ * equal blocks, src/common.cpp:7-60) is swept lexicographically, and
+ * everything outside it, and every periodic image, is read from a snapshot
+ * taken at the start of the sweep, as each rank of the reference reads its
+ * ghost layer.  The subdomains are independent within a sweep and run on
+ * OpenMP threads; the result does not depend on the thread count.  A 1x1x1
+ * grid is the single-rank sweep; a grid of one point per subdomain is block
+ * Jacobi.  Levels with fewer points than subdomains along a dimension use
+ * one subdomain per point there. */
+void mgo_gs_lex_blocks(mgo_level *L, const int ranks[3]) {
     long N = npts(L);
     double *snap = (double *)malloc(sizeof(double) * 3 * N);
     memcpy(snap, L->x, sizeof(double) * 3 * N);
-    for (int k = 0; k < L->o; k++)
-        for (int j = 0; j < L->n; j++)
-            for (int i = 0; i < L->m; i++) gs_point(L, i, j, k, snap);
+    const int R[3] = {imin(ranks[0], L->m), imin(ranks[1], L->n), imin(ranks[2], L->o)};
+    const int nb = R[0] * R[1] * R[2];
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < nb; b++) {
+        const int bi = b % R[0], bj = (b / R[0]) % R[1], bk = b / (R[0] * R[1]);
+        const int lo[3] = {L->m * bi / R[0], L->n * bj / R[1], L->o * bk / R[2]};
+        const int hi[3] = {L->m * (bi + 1) / R[0], L->n * (bj + 1) / R[1], L->o * (bk + 1) / R[2]};
+        for (int k = lo[2]; k < hi[2]; k++)
+            for (int j = lo[1]; j < hi[1]; j++)
+                for (int i = lo[0]; i < hi[0]; i++) gs_point(L, i, j, k, snap, lo, hi);
+    }
     free(snap);
+}
+
+/* The single-rank lexicographic sweep: periodic images are read from a
+ * snapshot taken at the start of the sweep (region.cc:1601). */
+void mgo_gs_lex(mgo_level *L) {
+    const int one[3] = {1, 1, 1};
+    mgo_gs_lex_blocks(L, one);
 }
 
 static int col1(int i, int n, int prd) { return (prd && (n & 1) && i == n - 1) ? 2 : (i & 1); }
@@ -229,19 +271,20 @@ void mgo_gs_color(mgo_level *L) {
     for (int cz = 0; cz < ncz; cz++)
         for (int cy = 0; cy < ncy; cy++)
             for (int cx = 0; cx < ncx; cx++)
+#pragma omp parallel for schedule(static)
                 for (int k = 0; k < L->o; k++) {
                     if (col1(k, L->o, L->prd[2]) != cz) continue;
                     for (int j = 0; j < L->n; j++) {
                         if (col1(j, L->n, L->prd[1]) != cy) continue;
                         for (int i = 0; i < L->m; i++)
-                            if (col1(i, L->m, L->prd[0]) == cx) gs_point(L, i, j, k, NULL);
+                            if (col1(i, L->m, L->prd[0]) == cx) gs_point(L, i, j, k, NULL, NULL, NULL);
                     }
                 }
 }
 
 void mgo_gs(mgo_hier *h, int l) {
     if (h->smoother == 1) mgo_gs_color(&h->lv[l]);
-    else mgo_gs_lex(&h->lv[l]);
+    else mgo_gs_lex_blocks(&h->lv[l], h->ranks);
 }
 
 /* Candidate fine vertices of coarse vertex I along one dimension. */
@@ -261,6 +304,7 @@ void mgo_restrict(mgo_hier *h, int l) {
     long NF = npts(F), NC = npts(C);
     double *res = (double *)malloc(sizeof(double) * 3 * NF);
     mgo_residual(F, res);
+#pragma omp parallel for schedule(static)
     for (int K = 0; K < C->o; K++)
         for (int J = 0; J < C->n; J++)
             for (int I = 0; I < C->m; I++) {
@@ -298,6 +342,7 @@ void mgo_restrict(mgo_hier *h, int l) {
 /* x_f += P x_c (multigrid.cc:420-430). */
 void mgo_interp(mgo_hier *h, int l) {
     mgo_level *F = &h->lv[l - 1], *C = &h->lv[l];
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < F->o; k++) {
         int Jk[2]; double wk[2];
         int nk = parents1(k, F->o, C->o, F->prd[2], Jk, wk);
@@ -385,8 +430,10 @@ static void rat_point(const mgo_level *F, const mgo_level *C, int I, int J, int 
 
 void mgo_rat_level(const mgo_level *F, mgo_level *C) {
     if (C->mode == MGO_PLANE) {
+#pragma omp parallel for schedule(dynamic, 1)
         for (int K = 0; K < C->o; K++) rat_point(F, C, 0, 0, K, C->ops + (long)K * MGO_BLK);
     } else {
+#pragma omp parallel for schedule(dynamic, 1)
         for (int K = 0; K < C->o; K++)
             for (int J = 0; J < C->n; J++)
                 for (int I = 0; I < C->m; I++)

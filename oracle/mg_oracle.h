@@ -39,6 +39,7 @@ typedef struct {
     mgo_level lv[MGO_MAX_LEVELS];
     int v_down, v_up, bottom_sweeps, growth;
     int smoother; /* 0 = lexicographic with per-sweep ghost snapshot, 1 = 8-colour */
+    int ranks[3]; /* subdomain grid of the lexicographic sweep (1,1,1 = one rank) */
 } mgo_hier;
 
 int mgo_coarse_size(int nf, int prd);
@@ -52,6 +53,7 @@ void mgo_rat_level(const mgo_level *F, mgo_level *C);
 void mgo_apply(const mgo_level *L, const double *x, double *out);
 void mgo_residual(const mgo_level *L, double *out);
 void mgo_gs_lex(mgo_level *L);
+void mgo_gs_lex_blocks(mgo_level *L, const int ranks[3]);
 void mgo_gs_color(mgo_level *L);
 void mgo_gs(mgo_hier *h, int l);
 void mgo_restrict(mgo_hier *h, int l);

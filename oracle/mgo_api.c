@@ -3,6 +3,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <omp.h>
+
 #include "mg_oracle.h"
 
 void *mgo_new(int m, int n, int o, const int *prd, int mode, long coarsest) {
@@ -42,6 +44,16 @@ void mgo_params(void *hp, int v_down, int v_up, int bottom, int growth, int smoo
     h->v_down = v_down; h->v_up = v_up; h->bottom_sweeps = bottom; h->growth = growth;
     h->smoother = smoother;
 }
+void mgo_set_ranks(void *hp, int px, int py, int pz) {
+    mgo_hier *h = (mgo_hier *)hp;
+    h->ranks[0] = px < 1 ? 1 : px;
+    h->ranks[1] = py < 1 ? 1 : py;
+    h->ranks[2] = pz < 1 ? 1 : pz;
+}
+void mgo_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+}
+int mgo_get_threads(void) { return omp_get_max_threads(); }
 void mgo_do_rat(void *h) { mgo_rat((mgo_hier *)h); }
 void mgo_do_residual(void *hp, int l, double *out) { mgo_residual(&((mgo_hier *)hp)->lv[l], out); }
 void mgo_do_apply(void *hp, int l, const double *x, double *out) { mgo_apply(&((mgo_hier *)hp)->lv[l], x, out); }

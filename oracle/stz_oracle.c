@@ -57,6 +57,7 @@ typedef struct {
     double T[9], Ti[9], dT[9];
     int textbook;          /* test switch: 1 = textbook x-ENO for chi instead of :985 */
     double rho, kappa;     /* density and viscous damping of the direct step */
+    int ranks[3];          /* subdomain grid of the projection's lexicographic sweep */
 } sto_sim;
 
 static long gi(const sto_sim *S, int i, int j, int k) {
@@ -82,6 +83,7 @@ void *sto_new(int Nx, int Ny, int Nz, const int *prd, const double *box, const d
     long n = S->nxyg * S->nzg;
     for (int q = 0; q < F_NF; q++) S->f[q] = (double *)calloc((size_t)n, sizeof(double));
     S->std_iters = 16;
+    S->ranks[0] = S->ranks[1] = S->ranks[2] = 1;
     return S;
 }
 
@@ -571,6 +573,7 @@ static int mg_o(const sto_sim *S) { return S->Nz + (S->prd[2] ? 0 : 1); }
 static void setup_mg(sto_sim *S, double dt) {
     int o = mg_o(S);
     if (!S->mg_ready) mgo_build(&S->mg, S->Nx, S->Ny, o, S->prd, MGO_PLANE, 1024);
+    memcpy(S->mg.ranks, S->ranks, sizeof(S->ranks));
     /* stencil = -dt*stencil_base, fix (src/shear_sim_3d.hh:45-55) */
     double L2 = S->lamb + 2 * S->mu, c = (S->lamb + S->mu) / 4, mu = S->mu;
     double dx = S->dx, dy = S->dy, dz = S->dz;
@@ -833,6 +836,16 @@ void sto_get_state(void *hp, double *t, double *u, double *sig, double *chi, dou
 }
 
 int sto_vcycles(void *hp) { return ((sto_sim *)hp)->vcycles; }
+
+/* The projection's lexicographic sweep as the reference's on a px x py x pz
+ * grid of ranks (mgo_gs_lex_blocks); 1 x 1 x 1 is the single-rank sweep. */
+void sto_set_ranks(void *hp, int px, int py, int pz) {
+    sto_sim *S = (sto_sim *)hp;
+    S->ranks[0] = px < 1 ? 1 : px;
+    S->ranks[1] = py < 1 ? 1 : py;
+    S->ranks[2] = pz < 1 ? 1 : pz;
+    memcpy(S->mg.ranks, S->ranks, sizeof(S->ranks));
+}
 
 void sto_set_solver(void *hp, int smoother, double rtol) {
     sto_sim *S = (sto_sim *)hp;

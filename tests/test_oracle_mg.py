@@ -249,3 +249,107 @@ def test_colour_order_sweep_is_order_independent_within_colour():
                             rhs = r.ravel()[rows] - A[rows] @ xv + Ad @ xv[rows]
                             xv[rows] = np.linalg.solve(Ad, rhs)
     assert relerr(xa.ravel(), xv) < 1e-12
+
+
+# ---- the reference's sweep on several ranks (subdomain-blocked, OpenMP) ----
+
+def walled_case(dims=(8, 8, 9)):
+    h = 1.0 / dims[0]
+    st, fix = O.shear_operator(1.0, 1.5, h, h, h, 1.0)
+    orc = O.OracleMG(*dims, (1, 1, 0), mode=O.PLANE)
+    orc.set_ops(0, O.plane_operator(dims[2], 0, st, fix))
+    N = orc.npts(0)
+    return orc, O.det_field(N, 21), O.det_field(N, 22)
+
+
+def test_one_point_subdomains_give_block_jacobi():
+    # every neighbour is read from the start-of-sweep snapshot: block Jacobi
+    # x' = x + D^-1 (r - A x), with D the 3x3 diagonal blocks
+    orc, x, r = walled_case()
+    m, n, o = 8, 8, 9
+    orc.set_ranks((m, n, o))
+    orc.set_fields(0, x, r)
+    orc.gs(0, 1)
+    A = dense(orc, 0)
+    res = (r.ravel() - A @ x.ravel()).reshape(-1, 3)
+    want = x.copy()
+    for p in range(x.shape[0]):
+        D = A[3 * p:3 * p + 3, 3 * p:3 * p + 3]
+        want[p] += np.linalg.solve(D, res[p])
+    assert relerr(orc.fields(0)[0], want) < 1e-12
+
+
+@pytest.mark.parametrize("ranks", [(2, 2, 1), (2, 2, 3), (4, 2, 2)])
+def test_subdomain_sweep_is_forward_substitution_per_block(ranks):
+    # the sweep as a matrix splitting: within a subdomain, points earlier in
+    # lexicographic order contribute their new values (a block-triangular
+    # solve), everything else its old value.  Walled in z and subdomains of
+    # at most half a periodic dimension, so no periodic image lies inside
+    # the sweeping subdomain and "outside the subdomain" is the whole
+    # snapshot rule.  One subdomain across a periodic dimension reads its
+    # images from the snapshot too; that case (one rank) is pinned by the
+    # reference's golden sweep above.
+    orc, x, r = walled_case((8, 8, 9))
+    m, n, o = 8, 8, 9
+    orc.set_ranks(ranks)
+    orc.set_fields(0, x, r)
+    orc.gs(0, 1)
+    A = dense(orc, 0)
+    R = ranks
+
+    def part(i, n_, r_):
+        # subdomain b holds [n*b/r, n*(b+1)/r)
+        return next(b for b in range(r_) if n_ * b // r_ <= i < n_ * (b + 1) // r_)
+
+    def block(i, j, k):
+        return (part(i, m, R[0]), part(j, n, R[1]), part(k, o, R[2]))
+
+    N = m * n * o
+    owner = [block(p % m, (p // m) % n, p // (m * n)) for p in range(N)]
+    xv = x.ravel().copy()
+    xo = x.ravel().copy()
+    for b in sorted(set(owner), key=lambda t: (t[2], t[1], t[0])):
+        pts = [p for p in range(N) if owner[p] == b]  # lexicographic order
+        inb = np.zeros(N, bool)
+        inb[pts] = True
+        done = np.zeros(N, bool)
+        for p in pts:
+            rows = slice(3 * p, 3 * p + 3)
+            cols_new = np.repeat(done & inb, 3)
+            rhs = r.ravel()[rows] - A[rows][:, cols_new] @ xv[cols_new] - A[rows][:, ~cols_new] @ xo[~cols_new]
+            rhs += A[rows, rows] @ xo[rows]
+            xv[rows] = np.linalg.solve(A[rows, rows], rhs)
+            done[p] = True
+    assert relerr(orc.fields(0)[0].ravel(), xv) < 1e-12
+
+
+def test_subdomain_vcycle_independent_of_thread_count():
+    n = 16
+    h = 1.0 / n
+    st = -O.trans_stencil_blocks(np.array([[1, 0, 0.5], [0, 1, 0], [0, 0, 1.0]]), 1.0, 1.5, h, h, h)
+    out = []
+    nt = O.get_threads()
+    try:
+        for threads in (1, 8):
+            O.set_threads(threads)
+            orc = O.OracleMG(n, n, n, (1, 1, 1), mode=O.PLANE)
+            orc.set_ops(0, O.plane_operator(n, 1, st, None))
+            orc.rat()
+            orc.set_ranks((2, 2, 2))
+            orc.set_fields(0, np.zeros((n ** 3, 3)), O.det_field(n ** 3, 5))
+            orc.vcycle(2)
+            out.append(orc.fields(0)[0])
+    finally:
+        O.set_threads(nt)
+    assert np.array_equal(out[0], out[1])
+
+
+def test_processor_division_matches_reference_rule():
+    # src/common.cpp:7-60: minimal cut surface among even splits, first
+    # found on ties, non-periodic dimensions cut one face fewer
+    assert O.processor_division(8, 256, 256, 256, (1, 1, 1)) == (2, 2, 2)
+    assert O.processor_division(8, 128, 128, 128, (1, 1, 0)) == (2, 2, 2)
+    assert O.processor_division(2, 16, 16, 16, (1, 1, 0)) == (2, 1, 1)
+    assert O.processor_division(4, 256, 256, 16, (1, 1, 0)) == (2, 2, 1)
+    assert O.processor_division(16, 192, 192, 96, (1, 1, 0)) == (4, 4, 1)
+    assert O.processor_division(7, 8, 8, 8, (1, 1, 1)) is None
