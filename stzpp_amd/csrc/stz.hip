@@ -268,6 +268,12 @@ int stz_create(const stz_params *p, stz_t *out) {
     g.Nzv = g.Nz + (g.pz ? 0 : 1);
     g.Nc = (long)g.Nx * g.Ny * g.Nz;
     g.Nv = (long)g.Nx * g.Ny * g.Nzv;
+    if (g.Nv > (long)std::numeric_limits<int>::max()) {
+        // the step kernels index cells and vertices with 32-bit integers
+        g_stz_err = "grid too large: at most 2^31 - 1 vertices";
+        delete s;
+        return STZMG_ERR_UNSUPPORTED;
+    }
     g.dx = (p->bx - p->ax) / g.Nx;
     g.dy = (p->by - p->ay) / g.Ny;
     g.dz = (p->bz - p->az) / g.Nz;

@@ -146,7 +146,7 @@ static __global__ __launch_bounds__(kT) void k_advect_march(
     const int nx = g.Nx, ny = g.Ny, nz = g.Nz;
     const int nxy = nx * ny;
     const int col = wrapm(i, nx) + nx * wrapm(j, ny);  // load position of this lane
-    const int nc = (int)g.Nc, nv = (int)g.Nv;
+    const long nc = g.Nc, nv = g.Nv;  // component strides (cell and vertex indices fit in int)
 
     auto stage_vertices = [&](int kv) {
         const int kw = PZ ? wrapc(kv, nz) : kv;
