@@ -238,13 +238,22 @@ long resid_resident() {
     return r;
 }
 
+// Workgroups of the residual with norm partials: the resident count of a
+// 256-CU MI355X (3 FP32 / 6 FP64 workgroups per CU), fixed so that the
+// partial sums are grouped the same way on every device and a stopping
+// decision taken on the norm does not depend on the device's CU count.
+template <typename T>
+constexpr long kNormResident = sizeof(T) == 4 ? 768 : 1536;
+
 // Grid of the plane-marching residual: the z range is cut into as many
 // chunks as let every tile's chunks be resident at once, so a launch runs
-// as one round of workgroups, each staging its first planes once.
+// as one round of workgroups, each staging its first planes once.  With
+// norm partials the chunking follows kNormResident instead of the device.
 template <typename T>
-dim3 tiled_grid(const Level &L, long cap, int *kchunk) {
+dim3 tiled_grid(const Level &L, long cap, int *kchunk, bool norm) {
     const long tiles = (long)(L.d.m / ResidTile<T>::TX) * (L.d.n / ResidTile<T>::TY);
-    const long resident = cap > 0 ? std::min(cap, resid_resident<T>()) : resid_resident<T>();
+    const long dev = norm ? kNormResident<T> : resid_resident<T>();
+    const long resident = cap > 0 ? std::min(cap, dev) : dev;
     const long nz = std::min<long>(L.d.o, std::max<long>(1, resident / tiles));
     const int kc = (int)((L.d.o + nz - 1) / nz);
     *kchunk = kc;
@@ -256,7 +265,7 @@ template <typename T>
 long resid_parts(stzmg_t h, const Level &L) {
     if (L.tiled) {
         int kc;
-        const dim3 g = tiled_grid<T>(L, h->chunk_cap, &kc);
+        const dim3 g = tiled_grid<T>(L, h->chunk_cap, &kc, true);
         return (long)g.x * g.y * g.z;
     }
     return (long)nblk((long)L.d.m * L.d.n) * L.d.o;
@@ -265,7 +274,7 @@ long resid_parts(stzmg_t h, const Level &L) {
 template <typename T, bool NORM, bool APPLY, int PAT>
 void launch_resid_tiled(stzmg_t h, Level &L, const T *x, const T *r, T *out, T *xcopy, double *part) {
     int kc;
-    const dim3 g = tiled_grid<T>(L, h->chunk_cap, &kc);
+    const dim3 g = tiled_grid<T>(L, h->chunk_cap, &kc, NORM);
     hipLaunchKernelGGL((k_resid_planes<T, PAT, NORM, APPLY, ResidTile<T>::TX, ResidTile<T>::TY, kThreads>), g,
                        dim3(kThreads), 0, h->stream, L.d, x, r, ops_of<T>(L), out, part, kc, xcopy);
 }
