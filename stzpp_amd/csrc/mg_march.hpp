@@ -1,5 +1,6 @@
 // Z-marching fused 8-colour block Gauss-Seidel for per-plane operators on
-// levels periodic in x and y, with x a multiple of the tile width (16, 32 or 64).
+// levels periodic in x and y, with x a multiple of the tile width (8, 16, 32
+// or 64).
 //
 // Same colour order and the same halo recomputation as k_gs_planes
 // (mg_fused.hpp): one launch updates every plane of one z parity P, a
@@ -256,7 +257,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
     constexpr int CSZ = WX * WY;           // one component of one plane
     constexpr int PSZ = 3 * CSZ;           // one plane slot
     constexpr int NW = NT / 64;            // waves
-    static_assert(TX == 16 || TX == 32 || TX == 64, "tile width");
+    static_assert(TX == 8 || TX == 16 || TX == 32 || TX == 64, "tile width");
     static_assert(TY % 2 == 0 && NT % 64 == 0, "tile shape");
     // chunk A: the tile's columns of every window row in 16-byte loads;
     // chunk B: the WX - TX halo columns, one element per thread and load

@@ -492,8 +492,16 @@ void launch_fused_pat(stzmg_t h, Level &L) {
             launch_fused_tile<T, PAT, kSmallTile, kSmallTile, kSmallThreads>(h, L);
         }
     }
-    else
-        launch_fused_tile<T, PAT, kTinyTile, kTinyTile, kTinyThreads>(h, L);
+    else {
+        if constexpr (STZ_MARCH != 0) {
+            if (L.sym)
+                launch_fused_tile<T, PAT + kMarchSym, kTinyTile, kTinyTile, kTinyThreads, true>(h, L);
+            else
+                launch_fused_tile<T, PAT, kTinyTile, kTinyTile, kTinyThreads, true>(h, L);
+        } else {
+            launch_fused_tile<T, PAT, kTinyTile, kTinyTile, kTinyThreads>(h, L);
+        }
+    }
 }
 
 template <typename T>
@@ -846,17 +854,21 @@ int pack_march(stzmg_t h) {
         const int stride = march_stride(key);
         const long n = L.ncls * stride;
         const int *map = h->pkmap + (long)key * kPackMax;
-        if (L.ftile == 1 || L.ftile == 2) {
+        // every fused tile class sweeps with k_gs_march, which reads the
+        // packed table (a level without one would hand it a null pointer)
+        if (L.ftile >= 1) {
             if (!L.pk32) HIPCHK(h, alloc(&L.pk32, L.ncls * kPackMax));
             hipLaunchKernelGGL(k_pack_march<float>, dim3(nblk(n)), dim3(kThreads), 0, h->stream,
                                (const float *)L.ops32, (const float *)L.inv32, map, stride, L.ncls, L.pk32);
         }
-        if (L.ftile64 == 1 || L.ftile64 == 2) {
+        if (L.ftile64 >= 1) {
             if (!L.pk64) HIPCHK(h, alloc(&L.pk64, L.ncls * kPackMax));
             hipLaunchKernelGGL(k_pack_march<double>, dim3(nblk(n)), dim3(kThreads), 0, h->stream,
                                (const double *)L.ops64, (const double *)L.inv64, map, stride, L.ncls, L.pk64);
         }
         LAUNCHCHK(h);
+        if ((L.ftile >= 1 && !L.pk32) || (L.ftile64 >= 1 && !L.pk64))
+            return fail(h, STZMG_ERR_STATE, "packed coefficient table missing on a fused level");
     }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return 0;

@@ -1,7 +1,8 @@
 """Parity of the z-chunked paths of the hot multigrid kernels with the oracle.
 
-The fused smoother and the plane-marching residual cut z into chunks sized
-to the workgroups the device keeps resident; the restriction marches a run
+The fused smoother (every tile class: 64-, 32-, 16- and 8-wide tiles) and
+the plane-marching residual cut z into chunks sized to the workgroups the
+device keeps resident; the restriction marches a run
 of coarse planes per thread.  At the sizes an oracle finishes in seconds the
 device's resident count gives one plane per chunk, so these tests cap it
 (STZMG_OPT_CHUNK_CAP) and fix the restriction's run (STZMG_OPT_RESTRICT_PLANES)
@@ -59,7 +60,9 @@ def f32(a):
 
 @pytest.mark.parametrize("cap", CAPS)
 @pytest.mark.parametrize("kind,prd,n", [("trans", (1, 1, 1), 64), ("shear", (1, 1, 0), 64),
-                                        ("rat", (1, 1, 0), 64), ("trans", (1, 1, 0), 32)])
+                                        ("rat", (1, 1, 0), 64), ("trans", (1, 1, 0), 32),
+                                        # 24 x 24 planes: the 8 x 8 tile class
+                                        ("trans", (1, 1, 1), 24), ("shear", (1, 1, 0), 24)])
 def test_gpu_fused_sweep_chunked_matches_oracle(kind, prd, n, cap):
     o, planes = operator(kind, n, prd)
     N = n * n * o
