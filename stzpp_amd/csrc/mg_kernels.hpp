@@ -436,13 +436,18 @@ __global__ __launch_bounds__(kThreads) void k_interp_cells(Dims F, Dims C, const
 // 1/2 if |g - 2(I+D)| = 1.  PLANE levels use representatives (0,0,K).
 __device__ __forceinline__ double pw(int s) { return s == 0 ? 1.0 : ((s == 1 || s == -1) ? 0.5 : 0.0); }
 
+// One entry of one coarse block per thread (thread t: class t / 243, block
+// (t % 243) / 9, entry t % 9).  Each entry is summed over the fine parents
+// and fine blocks in one fixed order, so the result does not depend on how
+// the entries are spread over threads.
 template <int MODE_F, int MODE_C>
 __global__ void k_rat(Dims F, Dims C, const double *__restrict__ opsF, double *__restrict__ opsC,
                       long nrep) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nrep * 27) return;
-    const long rep = t / 27;
-    const int D = (int)(t - rep * 27);
+    if (t >= nrep * kBlk) return;
+    const long rep = t / kBlk;
+    const int DE = (int)(t - rep * kBlk);
+    const int D = DE / 9, e = DE - 9 * D;
     const int Di = D % 3 - 1, Dj = (D / 3) % 3 - 1, Dk = D / 9 - 1;
     int I, J, K;
     if (MODE_C == 0) {
@@ -457,9 +462,7 @@ __global__ void k_rat(Dims F, Dims C, const double *__restrict__ opsF, double *_
     const bool okc = (C.px || (I + Di >= 0 && I + Di < C.m)) &&
                      (C.py || (J + Dj >= 0 && J + Dj < C.n)) &&
                      (C.pz || (K + Dk >= 0 && K + Dk < C.o));
-    double acc[9];
-#pragma unroll
-    for (int e = 0; e < 9; e++) acc[e] = 0;
+    double acc = 0;
     if (okc) {
         for (int ek = -1; ek <= 1; ek++) {
             const int fk = fine_of(K, ek, F.o, F.pz);
@@ -472,7 +475,7 @@ __global__ void k_rat(Dims F, Dims C, const double *__restrict__ opsF, double *_
                     if (fi < 0) continue;
                     const double wr = 0.125 * pw(ei) * pw(ej) * pw(ek);
                     const long pf = fi + (long)F.m * (fj + (long)F.n * fk);
-                    const double *A = opsF + cls_of<MODE_F>(pf, fk) * kBlk;
+                    const double *A = opsF + cls_of<MODE_F>(pf, fk) * kBlk + e;
                     for (int dk = -1; dk <= 1; dk++) {
                         if (!F.pz && (fk + dk < 0 || fk + dk >= F.o)) continue;
                         const double w3 = pw(ek + dk - 2 * Dk);
@@ -485,9 +488,7 @@ __global__ void k_rat(Dims F, Dims C, const double *__restrict__ opsF, double *_
                                 if (!F.px && (fi + di < 0 || fi + di >= F.m)) continue;
                                 const double w = wr * w2 * pw(ei + di - 2 * Di);
                                 if (w == 0) continue;
-                                const double *a = A + 9 * ((di + 1) + 3 * (dj + 1) + 9 * (dk + 1));
-#pragma unroll
-                                for (int e = 0; e < 9; e++) acc[e] += w * a[e];
+                                acc += w * A[9 * ((di + 1) + 3 * (dj + 1) + 9 * (dk + 1))];
                             }
                         }
                     }
@@ -495,9 +496,7 @@ __global__ void k_rat(Dims F, Dims C, const double *__restrict__ opsF, double *_
             }
         }
     }
-    double *o = opsC + rep * kBlk + 9 * D;
-#pragma unroll
-    for (int e = 0; e < 9; e++) o[e] = acc[e];
+    opsC[rep * kBlk + DE] = acc;
 }
 
 // Centre-block inverse by cofactors over the determinant (src/mat3.hh:177-183)

@@ -882,7 +882,7 @@ int build_hierarchy_ops(stzmg_t h) {
         Level &L = h->lv[l];
         if (l > 0) {
             Level &F = h->lv[l - 1];
-            const long nthr = L.ncls * 27;
+            const long nthr = L.ncls * kBlk;
             if (F.mode == STZMG_OP_PLANE && L.mode == STZMG_OP_PLANE)
                 hipLaunchKernelGGL((k_rat<0, 0>), dim3(nblk(nthr, 64)), dim3(64), 0, h->stream, F.d, L.d,
                                    (const double *)F.ops64, L.ops64, L.ncls);
