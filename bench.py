@@ -174,8 +174,8 @@ def cpu_baseline(n, st, b, gpu_fp64_cycles):
 def step_roofline(sim, n_points):
     """Roofline of the step's dominant kernel, the fine-level FP64 fused
     Gauss-Seidel sweep of its projection solver (two launches, one per z
-    parity, 72 algorithmic bytes per point per sweep), timed with HIP events
-    on the solver's stream."""
+    parity, 72 algorithmic bytes per updated point), each launch timed with a
+    HIP event pair on the solver's stream."""
     import ctypes
 
     from stzpp_amd import _lib
@@ -183,16 +183,16 @@ def step_roofline(sim, n_points):
     L = _lib.load()
     h = L.stz_multigrid(sim.h)
     v = ctypes.c_double()
-    rc = L.stzmg_time_kernel(h, 0, 0, 20, ctypes.byref(v))
+    rc = L.stzmg_time_kernel(h, 4, 0, 20, ctypes.byref(v))
     if rc:
         return None
-    sweep_s = v.value * 1e-3
+    launch_s = v.value * 1e-3
     alg = 36.0 * n_points
-    achieved = alg / (sweep_s / 2) / 1e9
+    achieved = alg / launch_s / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
             "kernel": "k_gs_march<double, ...> (fine-level fused GS of the projection, one z parity)",
-            "alg_bytes_per_launch": alg, "launch_us": sweep_s / 2 * 1e6}
+            "alg_bytes_per_launch": alg, "launch_us": launch_s * 1e6}
 
 
 def oracle_step_baseline(N, prd, params, state0, dt, family=None, Ut=0.0):
@@ -732,10 +732,12 @@ def main():
     # Roofline of the dominant kernel, the fine-level FP32 fused smoother:
     # one sweep is two launches (one per z parity), each updating N/2 points
     # at 36 algorithmic bytes per point (read x, read r, write x, FP32 vec3),
-    # so 18 N bytes per launch.  The sweep is timed with HIP events on the
-    # solver's stream over 20 back-to-back sweeps.
+    # so 18 N bytes per launch.  Each launch of 20 back-to-back sweeps is
+    # timed with a HIP event pair on the solver's stream (the mean kernel
+    # duration; the sweep time beside it also holds the gaps between
+    # launches).
     sweep_s = mg.time_kernel(2, 0, 20) * 1e-3
-    launch_s = sweep_s / 2
+    launch_s = mg.time_kernel(5, 0, 20) * 1e-3
     alg_launch = 18.0 * N
     achieved = alg_launch / launch_s / 1e9
     sweep64_s = mg.time_kernel(0, 0, 10) * 1e-3

@@ -221,8 +221,12 @@ void *stzmg_stream(stzmg_t h);
 /* Mean time of `reps` back-to-back repetitions of one operation on a level,
  * timed with HIP events on the handle's stream, for roofline accounting
  * (no reference counterpart): kind 0 = one FP64 Gauss-Seidel sweep,
- * 1 = one FP64 residual r - A x, 2 = one FP32 sweep, 3 = one FP32 residual.
- * The level's solution is left as it was. */
+ * 1 = one FP64 residual r - A x, 2 = one FP32 sweep, 3 = one FP32 residual;
+ * kind 4 (FP64) and 5 (FP32) = the mean duration of one launch of the fused
+ * sweep (two per sweep), from an event pair around each launch, without the
+ * gaps between launches (STZMG_ERR_UNSUPPORTED on a level without the fused
+ * sweep).  The operations run on the level's fields, which are overwritten;
+ * the level stays usable (set its fields again before a solve). */
 int stzmg_time_kernel(stzmg_t h, int kind, int level, int reps, double *ms_per_unit);
 
 /* Operator builders (host side, FP64).

@@ -101,6 +101,10 @@ struct stzmg_s {
     int tiled_resid = 1;      // plane-marching residual on eligible levels
     int bottom_map = 1;       // coarsest solve as x = Q r (mg_coarse.hpp)
     long chunk_cap = 0;       // cap on the workgroups the z-chunked launches assume resident (0: none)
+    // stzmg_time_kernel kinds 4, 5: event pairs recorded around each fused
+    // sweep launch (null: no recording)
+    std::vector<hipEvent_t> *launch_ev = nullptr;
+    size_t launch_ev_next = 0;
     int restrict_planes = 0;  // coarse planes per restriction thread (0: chosen from the grid)
     double *qmap = nullptr;   // Q, row-major, qmap_n x qmap_n
     long qmap_n = 0;
@@ -458,12 +462,18 @@ void launch_fused_tile(stzmg_t h, Level &L) {
         const int kchunk = (int)((nP + nz - 1) / nz);
         dim3 g(d.m / TX, d.n / TY, (nP + kchunk - 1) / kchunk);
         const T *xo = P == 0 ? x : xb;
+        const bool rec = h->launch_ev && h->launch_ev_next + 2 <= h->launch_ev->size();
+        if (rec) (void)hipEventRecord((*h->launch_ev)[h->launch_ev_next], h->stream);
         if constexpr (MARCH)
             hipLaunchKernelGGL((k_gs_march<T, PAT, TX, TY, NT>), g, dim3(NT), 0, h->stream, d, (const T *)x, xo, xb,
                                (const T *)r_of<T>(L), (const T *)pk_of<T>(L), P, kchunk);
         else
             hipLaunchKernelGGL((k_gs_planes<T, PAT, TX, TY, NT>), g, dim3(NT), 0, h->stream, d, (const T *)x, xo,
                                xb, (const T *)r_of<T>(L), ops_of<T>(L), inv_of<T>(L), P, kchunk);
+        if (rec) {
+            (void)hipEventRecord((*h->launch_ev)[h->launch_ev_next + 1], h->stream);
+            h->launch_ev_next += 2;
+        }
     }
     std::swap(xref_of<T>(L), xb_of<T>(L));
 }
@@ -1830,32 +1840,71 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
 int stzmg_time_kernel(stzmg_t h, int kind, int level, int reps, double *ms_per_unit) {
     if (int rc = check_level(h, level)) return rc;
     if (int rc = require_ops(h)) return rc;
-    if (reps < 1 || !ms_per_unit || kind < 0 || kind > 3) return STZMG_ERR_ARG;
+    if (reps < 1 || !ms_per_unit || kind < 0 || kind > 5) return STZMG_ERR_ARG;
     Level &L = h->lv[level];
+    if (kind >= 4 && !(kind == 4 ? L.fused : L.fused && L.ftile >= 1))
+        return fail(h, STZMG_ERR_UNSUPPORTED, "time_kernel: level %d has no fused sweep", level);
     double *home64 = L.x64;
     float *home32 = L.x32;
     hipEvent_t a, b;
     HIPCHK(h, hipEventCreate(&a));
     HIPCHK(h, hipEventCreate(&b));
     auto body = [&]() {
-        if (kind == 0) launch_gs_sweep<double>(h, L);
+        if (kind == 0 || kind == 4) launch_gs_sweep<double>(h, L);
         else if (kind == 1) launch_resid<double, false>(h, L, L.x64, L.r64, L.t64);
-        else if (kind == 2) launch_gs_sweep<float>(h, L);
+        else if (kind == 2 || kind == 5) launch_gs_sweep<float>(h, L);
         else launch_resid<float, false>(h, L, L.x32, L.r32, L.t32);
     };
     body();
-    HIPCHK(h, hipEventRecord(a, h->stream));
-    for (int q = 0; q < reps; q++) body();
-    HIPCHK(h, hipEventRecord(b, h->stream));
-    HIPCHK(h, hipEventSynchronize(b));
     float ms = 0;
-    HIPCHK(h, hipEventElapsedTime(&ms, a, b));
+    if (kind < 4) {
+        HIPCHK(h, hipEventRecord(a, h->stream));
+        for (int q = 0; q < reps; q++) body();
+        HIPCHK(h, hipEventRecord(b, h->stream));
+        HIPCHK(h, hipEventSynchronize(b));
+        HIPCHK(h, hipEventElapsedTime(&ms, a, b));
+        ms /= reps;
+    } else {
+        // one event pair around each launch of the fused sweep (two per
+        // sweep): the mean kernel duration, without the gaps between launches
+        std::vector<hipEvent_t> ev(4 * (size_t)reps);
+        hipError_t e = hipSuccess;
+        size_t made = 0;
+        for (; made < ev.size() && e == hipSuccess; made++) e = hipEventCreate(&ev[made]);
+        if (e == hipSuccess) {
+            h->launch_ev = &ev;
+            h->launch_ev_next = 0;
+            for (int q = 0; q < reps; q++) body();
+            h->launch_ev = nullptr;
+            e = hipStreamSynchronize(h->stream);
+        }
+        double tot = 0;
+        const size_t used = h->launch_ev_next;
+        for (size_t q = 0; e == hipSuccess && q + 1 < used; q += 2) {
+            float d = 0;
+            e = hipEventElapsedTime(&d, ev[q], ev[q + 1]);
+            tot += d;
+        }
+        h->launch_ev = nullptr;
+        for (size_t q = 0; q < made; q++) (void)hipEventDestroy(ev[q]);
+        if (e != hipSuccess) {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            return fail(h, STZMG_ERR_HIP, "time_kernel: %s", hipGetErrorString(e));
+        }
+        if (used < 2) {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            return fail(h, STZMG_ERR_UNSUPPORTED, "time_kernel: level %d has no fused sweep", level);
+        }
+        ms = (float)(tot / (double)(used / 2));
+    }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     restore_level<double>(h, L, home64);
     restore_level<float>(h, L, home32);
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    *ms_per_unit = ms / reps;
+    *ms_per_unit = ms;
     return 0;
 }
 

@@ -172,3 +172,30 @@ def test_gpu_c1_solution_checked_by_per_point_residual():
         mg.set_option(MG.OPT_TILED_RESIDUAL, 0)
         res = mg.residual(0)
         assert np.linalg.norm(res) <= 1.01e-10 * np.linalg.norm(b), mode
+
+
+def test_gpu_time_kernel_leaves_level_and_reports_launches():
+    # stzmg_time_kernel (no reference counterpart): the timings are positive,
+    # the per-launch duration of the fused sweep (kinds 4, 5) is at most
+    # about half a sweep, the level's buffers stay consistent (a sweep after
+    # it still equals the oracle's), and a level without the fused sweep is
+    # refused for kinds 4, 5
+    n = 64
+    o, planes = operator("trans", n, (1, 1, 1))
+    N = n * n * o
+    x, r = O.det_field(N, 81), O.det_field(N, 82)
+    mg = MG.Multigrid(n, n, o, (1, 1, 1))
+    mg.setup_matrices_planes(planes)
+    mg.setup_fields(x, r)
+    t = {k: mg.time_kernel(k, 0, 5) for k in range(6)}
+    assert all(np.isfinite(v) and v > 0 for v in t.values()), t
+    assert 2 * t[5] <= 1.5 * t[2] and 2 * t[4] <= 1.5 * t[0], t
+    mg.setup_fields(x, r)
+    mg.gauss_seidel(0, 1)
+    orc = oracle_for(n, o, (1, 1, 1), planes, x, r)
+    orc.gs(0, 1)
+    assert relerr(mg.x(0), orc.fields(0)[0]) < 1e-12
+    # the coarsest levels have no fused sweep (at most 1024 points)
+    last = mg.nlevels - 1
+    with pytest.raises(MG.MultigridError):
+        mg.time_kernel(5, last, 2)
