@@ -321,18 +321,37 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
                                (c * d.N + (long)d.m * wrap1(y0 - H + yy, d.n) + wrap1(x0 - HXL + col, d.m)));
         lb_off[j] = c * CSZ + yy * WX + (col & 1) * HX + (col >> 1);
     }
-    // sub-pass points: LDS index of the point (component 0, slot 0), its
-    // offset in a plane of r and x, whether this thread owns one and whether
-    // it lies in the tile itself.  Lanes map to points so that a half-wave
-    // reads one LDS row segment: with TX = 64 (32..35 points per sub-pass
-    // row) half-wave h < ny takes the first 32 points of row h and the
-    // columns beyond go to the half-waves after; otherwise points are dealt
-    // in order.
+    // sub-pass points, dealt in pairs: a thread that owns point (x, y) of
+    // sub-pass 2g (x even) also owns (x + 1, y) of sub-pass 2g + 1 where that
+    // point exists (sub-passes 1 and 3 start one x after 0 and 2, with no
+    // more columns and no more rows beyond the shared ones).  The pair's
+    // right-hand sides are one aligned two-value load per component and its
+    // new values one two-value store, once the second is computed: full cache
+    // lines, where points of one colour alone fill every other value.  A
+    // pair lies inside the tile or outside it as a whole (x and the tile
+    // edges are even), and a pair inside has its second point.  Per thread:
+    // LDS index of each point (component 0, slot 0), whether it owns it, and
+    // per pair the offset in a plane of r and x and whether it is inside.
+    // Lanes map to pairs so that a half-wave reads one LDS row segment: with
+    // TX = 64 (32..35 pairs per row) half-wave h < ny takes the first 32
+    // pairs of row h and the columns beyond go to the half-waves after;
+    // otherwise pairs are dealt in order.
+    static_assert(SP::xf(1) == SP::xf(0) + 1 && SP::yf(1) == SP::yf(0) && SP::nx(1) <= SP::nx(0) &&
+                      SP::ny(1) <= SP::ny(0),
+                  "sub-pass 1 pairs with sub-pass 0");
+    static_assert(SP::xf(3) == SP::xf(2) + 1 && SP::yf(3) == SP::yf(2) + 2 && SP::nx(3) <= SP::nx(2) &&
+                      SP::ny(3) + 1 <= SP::ny(2),
+                  "sub-pass 3 pairs with sub-pass 2");
+    struct alignas(2 * sizeof(T)) Pair {
+        T v[2];
+    };
     int pb[4];
-    unsigned ro[4], wo[4];
-    bool own[4], inside[4];
-    static_for<4>([&](auto s_) {
-        constexpr int s = decltype(s_)::value;
+    unsigned ro[2], wo[2];
+    bool own[4], inside[2];
+    static_for<2>([&](auto g_) {
+        constexpr int g = decltype(g_)::value;
+        constexpr int s = 2 * g;                                // first sub-pass of the pair
+        constexpr int dy = (SP::yf(s + 1) - SP::yf(s)) / 2;   // row offset of the second
         constexpr int nx = SP::nx(s), ny = SP::ny(s);
         int txx, tyy;
         bool ok;
@@ -359,10 +378,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
         }
         const int xx = SP::xf(s) + 2 * txx + HXL, yy = SP::yf(s) + 2 * tyy + H;
         pb[s] = yy * WX + (xx & 1) * HX + (xx >> 1);
-        ro[s] = (unsigned)(sizeof(T) * (wrap1(x0 + xx - HXL, d.m) + (long)d.m * wrap1(y0 + yy - H, d.n)));
+        pb[s + 1] = yy * WX + ((xx + 1) & 1) * HX + ((xx + 1) >> 1);
         own[s] = ok;
-        inside[s] = ok && xx >= HXL && xx < HXL + TX && yy >= H && yy < H + TY;
-        wo[s] = (unsigned)(sizeof(T) * ((x0 + xx - HXL) + (long)d.m * (y0 + yy - H)));
+        own[s + 1] = ok && txx < SP::nx(s + 1) && tyy >= dy && tyy - dy < SP::ny(s + 1);
+        ro[g] = (unsigned)(sizeof(T) * (wrap1(x0 + xx - HXL, d.m) + (long)d.m * wrap1(y0 + yy - H, d.n)));
+        inside[g] = ok && xx >= HXL && xx < HXL + TX && yy >= H && yy < H + TY;
+        wo[g] = (unsigned)(sizeof(T) * ((x0 + xx - HXL) + (long)d.m * (y0 + yy - H)));
     });
 
     // ---- staging ----
@@ -421,13 +442,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
     }
     __syncthreads();
 
-    // right-hand side of the four points of plane kr
+    // right-hand side of the four points of plane kr (two pairs)
     auto load_r = [&](int kr, T (&v)[4][3]) {
         const T *rk = r + mn * kr;
-        static_for<4>([&](auto s) {
-            v[s][0] = at_bytes(rk, ro[s]);
-            v[s][1] = at_bytes(rk + d.N, ro[s]);
-            v[s][2] = at_bytes(rk + 2 * d.N, ro[s]);
+        static_for<2>([&](auto g) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const Pair q = at_bytes(reinterpret_cast<const Pair *>(rk + c * d.N), ro[g]);
+                v[2 * g][c] = q.v[0];
+                v[2 * g + 1][c] = q.v[1];
+            }
         });
     };
     T rr[4][3];
@@ -471,7 +495,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
                     static_for<4>([&](auto s) {
                         // waves without a point of this sub-pass skip it
                         // (a wave-uniform branch)
-                        if (wv * 64 >= march_lanes<TX, TY, NT>(decltype(s)::value)) return;
+                        if (wv * 64 >= march_lanes<TX, TY, NT>(decltype(s)::value & 2)) return;
                         auto ld = [&](int cc, int di, int dj, int dk) -> T {
                             const int b0 = dk < 0 ? bm[s] : bp[s];
                             const int xp = (SP::xf(decltype(s)::value) + HXL) & 1;
@@ -494,18 +518,25 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
         for (int s = 0; s < 4; s++) asm volatile("" : "+v"(acc[s][0]), "+v"(acc[s][1]), "+v"(acc[s][2]));
 
         // in-plane sub-passes on the centre slot.  A new value goes to LDS
-        // (the later sub-passes read it) and, for a point of the tile
-        // itself, straight to x_new.
+        // (the later sub-passes read it); for a pair of the tile itself the
+        // first value waits in registers and the second sub-pass stores both
+        // to x_new.
         T *xk = xd + mn * k;
-        auto put = [&](int S, T v0, T v1, T v2) {
+        T hold[3] = {0, 0, 0};
+        auto put = [&](auto s_, T v0, T v1, T v2) {
+            constexpr int S = decltype(s_)::value;
             const int bc = pb[S] + sc * PSZ;
             lds[bc] = v0;
             lds[bc + CSZ] = v1;
             lds[bc + 2 * CSZ] = v2;
-            if (inside[S]) {
-                at_bytes(xk, wo[S]) = v0;
-                at_bytes(xk + d.N, wo[S]) = v1;
-                at_bytes(xk + 2 * d.N, wo[S]) = v2;
+            if constexpr (S % 2 == 0) {
+                hold[0] = v0;
+                hold[1] = v1;
+                hold[2] = v2;
+            } else if (inside[S / 2]) {
+                at_bytes(reinterpret_cast<Pair *>(xk), wo[S / 2]) = Pair{{hold[0], v0}};
+                at_bytes(reinterpret_cast<Pair *>(xk + d.N), wo[S / 2]) = Pair{{hold[1], v1}};
+                at_bytes(reinterpret_cast<Pair *>(xk + 2 * d.N), wo[S / 2]) = Pair{{hold[2], v2}};
             }
         };
         constexpr int NG1 = march_ngroups(PAT, 1, G);
@@ -535,7 +566,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
                     });
                     const T e0 = rr[S][0] - acc[S][0], e1 = rr[S][1] - acc[S][1], e2 = rr[S][2] - acc[S][2];
                     constexpr int qi = march_pos(PAT, 1, G, march_len(PAT, 1) - 1);
-                    put(S, c[qi + 0] * e0 + c[qi + 1] * e1 + c[qi + 2] * e2,
+                    put(s, c[qi + 0] * e0 + c[qi + 1] * e1 + c[qi + 2] * e2,
                         c[qi + 3] * e0 + c[qi + 4] * e1 + c[qi + 5] * e2,
                         c[qi + 6] * e0 + c[qi + 7] * e1 + c[qi + 8] * e2);
                 }
@@ -568,7 +599,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
                             e2 = c[qi + 6] * f0 + c[qi + 7] * f1 + c[qi + 8] * f2;
                         }
                     });
-                    put(S, e0, e1, e2);
+                    put(s, e0, e1, e2);
                 }
                 __syncthreads();
             });
