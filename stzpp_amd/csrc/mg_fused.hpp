@@ -389,133 +389,4 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(STZ_FUSED_WA
     }
 }
 
-// out = r - A x (APPLY: out = A x) for a PLANE operator on a level periodic
-// in x and y, marching a TX x TY tile through kchunk planes with the planes
-// k-1, k, k+1 staged in LDS (one-point halo), so x is read from HBM about
-// once instead of through 27 cached neighbour loads per point.  Each thread
-// owns R = TX*TY/NT vertically adjacent points of a plane, so the LDS rows
-// they share are read once.  With NORM, one pair of partials (sum |r|^4,
-// sum |r|^2) per workgroup goes to part[blockIdx linear].  With APPLY and
-// xcopy given, x is also copied to xcopy (the centre values it stages).
-// Grid: (m/TX, n/TY, ceil(o/kchunk)).
-template <typename T, int PAT, bool NORM, bool APPLY, int TX, int TY, int NT>
-__global__ __launch_bounds__(NT) void k_resid_planes(Dims d, const T *__restrict__ x, const T *__restrict__ r,
-                                                     const T *__restrict__ ops, T *__restrict__ out,
-                                                     double *__restrict__ part, int kchunk,
-                                                     T *__restrict__ xcopy) {
-    static_assert(NT == kThreads, "block_sum2 assumes kThreads threads");
-    static_assert(NT % TX == 0 && (TX * TY) % NT == 0, "thread layout");
-    constexpr int WX = TX + 2, WY = TY + 2, CSZ = WX * WY, PSZ = 3 * CSZ;
-    constexpr int R = TX * TY / NT;  // rows per thread
-    __shared__ T lds[3 * PSZ];
-
-    int tx, ty, tz;
-    xcd_tile(tx, ty, tz);
-    const int x0 = tx * TX, y0 = ty * TY;
-    const int k0 = tz * kchunk;
-    const int k1 = k0 + kchunk < d.o ? k0 + kchunk : d.o;
-    const int tid = threadIdx.x;
-    const int xx = tid % TX, yb = (tid / TX) * R;
-
-    // plane kg of x (zero outside a walled z range) into registers, then LDS
-    constexpr int NPER = (PSZ + NT - 1) / NT;
-    auto fetch = [&](int kg, T (&v)[NPER]) {
-        bool valid = true;
-        if (kg < 0 || kg >= d.o) {
-            if (d.pz) kg = kg < 0 ? kg + d.o : kg - d.o;
-            else valid = false;
-        }
-        const long base = (long)d.m * d.n * kg;
-#pragma unroll
-        for (int u = 0; u < NPER; u++) {
-            const int q = tid + u * NT;
-            T val = 0;
-            if (valid && q < PSZ) {
-                const int c = q / CSZ, rem = q - c * CSZ, yy = rem / WX, xq = rem - yy * WX;
-                const int gi = wrap1(x0 + xq - 1, d.m), gj = wrap1(y0 + yy - 1, d.n);
-                val = x[c * d.N + base + gi + (long)d.m * gj];
-            }
-            v[u] = val;
-        }
-    };
-    auto store = [&](int slot, const T (&v)[NPER]) {
-#pragma unroll
-        for (int u = 0; u < NPER; u++) {
-            const int q = tid + u * NT;
-            if (q < PSZ) lds[slot * PSZ + q] = v[u];
-        }
-    };
-
-    double q4 = 0, q2 = 0;
-    int sm = 0, sc = 1, sp = 2;
-    T va[NPER];
-    if (k0 < k1) {
-        fetch(k0 - 1, va);
-        store(sm, va);
-        fetch(k0, va);
-        store(sc, va);
-        fetch(k0 + 1, va);
-        store(sp, va);
-    }
-    __syncthreads();
-    for (int k = k0; k < k1; k++) {
-        // the plane after next is in flight while this one is computed
-        if (k + 1 < k1) fetch(k + 2, va);
-        const long base = (long)d.m * d.n * k;
-        LaneCoef<T> cf;
-        cf.load(ops + (long)k * kBlk, nullptr);
-#pragma unroll
-        for (int u = 0; u < R; u++) {
-            const int yy = yb + u;
-            auto ld = [&](int c, int di, int dj, int dk) -> T {
-                const int sl = dk < 0 ? sm : (dk > 0 ? sp : sc);
-                return lds[sl * PSZ + c * CSZ + (yy + 1 + dj) * WX + (xx + 1 + di)];
-            };
-            T a0 = 0, a1 = 0, a2 = 0;
-            acc_blocks<T, PAT>(cf, ld, a0, a1, a2, std::make_integer_sequence<int, 27>{});
-            // centre block (acc_blocks leaves it out: the smoother inverts it)
-            const T v0 = ld(0, 0, 0, 0), v1 = ld(1, 0, 0, 0), v2 = ld(2, 0, 0, 0);
-            {
-                constexpr int C = 9 * 13;
-                a0 += cf.template get<C + 0>() * v0 + cf.template get<C + 1>() * v1 + cf.template get<C + 2>() * v2;
-                a1 += cf.template get<C + 3>() * v0 + cf.template get<C + 4>() * v1 + cf.template get<C + 5>() * v2;
-                a2 += cf.template get<C + 6>() * v0 + cf.template get<C + 7>() * v1 + cf.template get<C + 8>() * v2;
-            }
-            const long p = base + (x0 + xx) + (long)d.m * (y0 + yy);
-            if constexpr (APPLY) {
-                if (xcopy) {
-                    xcopy[p] = v0;
-                    xcopy[p + d.N] = v1;
-                    xcopy[p + 2 * d.N] = v2;
-                }
-            }
-            T o0 = a0, o1 = a1, o2 = a2;
-            if constexpr (!APPLY) {
-                o0 = r[p] - a0;
-                o1 = r[p + d.N] - a1;
-                o2 = r[p + 2 * d.N] - a2;
-            }
-            out[p] = o0;
-            out[p + d.N] = o1;
-            out[p + 2 * d.N] = o2;
-            if constexpr (NORM) {
-                const double s = (double)o0 * o0 + (double)o1 * o1 + (double)o2 * o2;
-                q2 += s;
-                q4 += s * s;
-            }
-        }
-        if (k + 1 < k1) {
-            __syncthreads();
-            store(sm, va);
-            const int t = sm;
-            sm = sc;
-            sc = sp;
-            sp = t;
-            __syncthreads();
-        }
-    }
-    if constexpr (NORM)
-        block_sum2(q4, q2, part, (long)blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z));
-}
-
 }  // namespace stzmg

@@ -22,6 +22,7 @@
 #include "mg_fused.hpp"
 #include "mg_kernels.hpp"
 #include "mg_march.hpp"
+#include "mg_resid.hpp"
 #include "mg_krylov.hpp"
 
 using namespace stzmg;
@@ -49,6 +50,9 @@ struct Level {
     // packed per-plane coefficients of the z-marching sweep (k_pack_march)
     double *pk64 = nullptr;
     float *pk32 = nullptr;
+    // and of the marching residual (the centre block in place of its inverse)
+    double *pkr64 = nullptr;
+    float *pkr32 = nullptr;
 };
 
 inline unsigned nblk(long n, int t = kThreads) { return (unsigned)((n + t - 1) / t); }
@@ -124,6 +128,7 @@ struct stzmg_s {
     hipGraphExec_t g64 = nullptr, g32 = nullptr;
     double *scratch = nullptr;  // this device's scratch of stzmg_norm_noalloc
     int *pkmap = nullptr;       // k_pack_march source maps, kPackMax ints per pattern
+    int *pkrmap = nullptr;      // and those of the residual tables
 };
 
 namespace {
@@ -160,6 +165,7 @@ void free_level(Level &L) {
     hipFree(L.x32); hipFree(L.r32); hipFree(L.t32);
     hipFree(L.x64b); hipFree(L.x32b);
     hipFree(L.pk64); hipFree(L.pk32);
+    hipFree(L.pkr64); hipFree(L.pkr32);
     L = Level{};
 }
 
@@ -201,7 +207,14 @@ double *t_of<double>(Level &L) { return L.t64; }
 template <>
 float *t_of<float>(Level &L) { return L.t32; }
 
-// Tiles of the plane-marching residual (k_resid_planes): 64 x 16 in FP32,
+template <typename T>
+T *pkr_of(Level &L);
+template <>
+double *pkr_of<double>(Level &L) { return L.pkr64; }
+template <>
+float *pkr_of<float>(Level &L) { return L.pkr32; }
+
+// Tiles of the plane-marching residual (k_resid_march): 64 x 16 in FP32,
 // 32 x 16 in FP64, 256 threads.
 template <typename T>
 struct ResidTile;
@@ -233,7 +246,7 @@ long resid_resident() {
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
                 &per,
                 reinterpret_cast<const void *>(
-                    &k_resid_planes<T, PAT_DENSE, true, false, ResidTile<T>::TX, ResidTile<T>::TY, kThreads>),
+                    &k_resid_march<T, PAT_DENSE, true, false, ResidTile<T>::TX, ResidTile<T>::TY, kThreads>),
                 kThreads, 0) != hipSuccess ||
             per < 1)
             per = 1;
@@ -242,10 +255,12 @@ long resid_resident() {
     return r;
 }
 
-// Workgroups of the residual with norm partials: the resident count of a
-// 256-CU MI355X (3 FP32 / 6 FP64 workgroups per CU), fixed so that the
-// partial sums are grouped the same way on every device and a stopping
-// decision taken on the norm does not depend on the device's CU count.
+// Workgroups of the residual with norm partials: a fixed count (768 FP32,
+// 1536 FP64; k_resid_march keeps three workgroups per CU of a 256-CU
+// MI355X in either type, so the FP32 grid is one round and the FP64 grid
+// two), so that the partial sums are grouped the same way on every device
+// and a stopping decision taken on the norm does not depend on the
+// device's CU count.
 template <typename T>
 constexpr long kNormResident = sizeof(T) == 4 ? 768 : 1536;
 
@@ -279,8 +294,13 @@ template <typename T, bool NORM, bool APPLY, int PAT>
 void launch_resid_tiled(stzmg_t h, Level &L, const T *x, const T *r, T *out, T *xcopy, double *part) {
     int kc;
     const dim3 g = tiled_grid<T>(L, h->chunk_cap, &kc, NORM);
-    hipLaunchKernelGGL((k_resid_planes<T, PAT, NORM, APPLY, ResidTile<T>::TX, ResidTile<T>::TY, kThreads>), g,
-                       dim3(kThreads), 0, h->stream, L.d, x, r, ops_of<T>(L), out, part, kc, xcopy);
+    constexpr int TX = ResidTile<T>::TX, TY = ResidTile<T>::TY;
+    if (L.sym)
+        hipLaunchKernelGGL((k_resid_march<T, PAT + kMarchSym, NORM, APPLY, TX, TY, kThreads>), g, dim3(kThreads), 0,
+                           h->stream, L.d, x, r, (const T *)pkr_of<T>(L), out, part, kc, xcopy);
+    else
+        hipLaunchKernelGGL((k_resid_march<T, PAT, NORM, APPLY, TX, TY, kThreads>), g, dim3(kThreads), 0, h->stream,
+                           L.d, x, r, (const T *)pkr_of<T>(L), out, part, kc, xcopy);
 }
 
 // out = r - A x (APPLY: out = A x) on level L, optionally with norm partials
@@ -850,19 +870,39 @@ constexpr int kPackMax = kBlk + 9;
 constexpr int kPats = 2 * kMarchSym;  // zero patterns, plain and centrosymmetric
 
 int pack_march(stzmg_t h) {
-    if (!STZ_MARCH) return 0;
     if (!h->pkmap) {
-        std::vector<int> src((size_t)kPats * kPackMax, 0);
+        // the sweep's tables end with the inverse centre block (source 243 +
+        // e), the residual's with the centre block itself (9 * 13 + e)
+        std::vector<int> src((size_t)kPats * kPackMax, 0), srcr(src.size(), 0);
         for (int pat = 0; pat < kPats; pat++)
-            for (int j = 0; j < march_stride(pat); j++) src[(size_t)pat * kPackMax + j] = march_src(pat, j);
+            for (int j = 0; j < march_stride(pat); j++) {
+                const int q = march_src(pat, j);
+                src[(size_t)pat * kPackMax + j] = q;
+                srcr[(size_t)pat * kPackMax + j] = q < kBlk ? q : 9 * 13 + (q - kBlk);
+            }
         HIPCHK(h, alloc(&h->pkmap, (long)src.size()));
         HIPCHK(h, hipMemcpy(h->pkmap, src.data(), sizeof(int) * src.size(), hipMemcpyHostToDevice));
+        HIPCHK(h, alloc(&h->pkrmap, (long)srcr.size()));
+        HIPCHK(h, hipMemcpy(h->pkrmap, srcr.data(), sizeof(int) * srcr.size(), hipMemcpyHostToDevice));
     }
     for (auto &L : h->lv) {
-        if (!L.fused) continue;
+        if (L.mode != STZMG_OP_PLANE) continue;
         const int key = L.pat + (L.sym ? kMarchSym : 0);
         const int stride = march_stride(key);
         const long n = L.ncls * stride;
+        // the marching residual's tables, wherever it may run (the tiled
+        // residual can be switched on after the operator is set)
+        if (tiled_ok(L)) {
+            const int *mapr = h->pkrmap + (long)key * kPackMax;
+            if (!L.pkr32) HIPCHK(h, alloc(&L.pkr32, L.ncls * kPackMax));
+            if (!L.pkr64) HIPCHK(h, alloc(&L.pkr64, L.ncls * kPackMax));
+            hipLaunchKernelGGL(k_pack_march<float>, dim3(nblk(n)), dim3(kThreads), 0, h->stream,
+                               (const float *)L.ops32, (const float *)L.inv32, mapr, stride, L.ncls, L.pkr32);
+            hipLaunchKernelGGL(k_pack_march<double>, dim3(nblk(n)), dim3(kThreads), 0, h->stream,
+                               (const double *)L.ops64, (const double *)L.inv64, mapr, stride, L.ncls, L.pkr64);
+            LAUNCHCHK(h);
+        }
+        if (!STZ_MARCH || !L.fused) continue;
         const int *map = h->pkmap + (long)key * kPackMax;
         // every fused tile class sweeps with k_gs_march, which reads the
         // packed table (a level without one would hand it a null pointer)
@@ -1111,6 +1151,7 @@ int stzmg_destroy(stzmg_t h) {
     hipFree(h->kZ64); hipFree(h->kX64); hipFree(h->kB64);
     hipFree(h->qmap);
     hipFree(h->pkmap);
+    hipFree(h->pkrmap);
     hipStreamDestroy(h->stream);
     delete h;
     return 0;
