@@ -41,12 +41,12 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernel (fine-level FP32 fused
 # smoother, one z parity of the 256^3 workload), from two rocprofv3 --pmc
 # runs of `python3 bench.py --steps 1 --warmup 0 --no-cpu --no-graph`:
-# FETCH_SIZE 159.09 MiB x 2.0 + WRITE_SIZE 102.52 MiB x 1.0 per dispatch
-# (average of 658 dispatches).  The factors come from the plain copy kernel
+# FETCH_SIZE 177.50 MiB x 2.0 + WRITE_SIZE 96.05 MiB x 1.0 per dispatch
+# (average of 844 dispatches).  The factors come from the plain copy kernel
 # k_convert<double,double> of the same run (3 x 256^3 doubles = 384 MiB each
 # way), which reads FETCH_SIZE 192.0 MiB and WRITE_SIZE 384.0 MiB.  Summary:
 # profiles/mg_solve_256_trans_kernels.txt.
-TRAFFIC_256_BYTES_PER_LAUNCH = (159.09 * 2.0 + 102.52) * 2 ** 20
+TRAFFIC_256_BYTES_PER_LAUNCH = (177.50 * 2.0 + 96.05) * 2 ** 20
 
 
 def log(msg):
@@ -692,6 +692,21 @@ def main():
         t, sgi = solve(MG.SOLVE_FP32_GMRES_IR)
         tgis.append(t)
     xgi = mg.x(0)
+
+    # Roofline of the dominant kernel, the fine-level FP32 fused smoother:
+    # one sweep is two launches (one per z parity), each updating N/2 points
+    # at 36 algorithmic bytes per point (read x, read r, write x, FP32 vec3),
+    # so 18 N bytes per launch.  Timed right after the timed solves, on the
+    # level's fields (which it overwrites; the solution was copied out above),
+    # after five unrecorded sweeps that bring the device back to the busy
+    # state of the solves: each launch of 20 back-to-back sweeps with a HIP
+    # event pair on the solver's stream (the mean kernel duration plus the
+    # dispatch inside the pair; the sweep time beside it also holds the gaps
+    # between launches).
+    mg.time_kernel(2, 0, 5)
+    launch_s = mg.time_kernel(5, 0, 20) * 1e-3
+    sweep_s = mg.time_kernel(2, 0, 20) * 1e-3
+    sweep64_s = mg.time_kernel(0, 0, 10) * 1e-3
     log(f"[bench] fp32+ir (gmres-ir): {np.mean(tgis)*1e3:.1f} ms, {sgi}")
 
     # What the timed solve leaves out when the operator changes (the
@@ -729,18 +744,8 @@ def main():
         sweeps = 24 if l == L - 1 else (2 + l) + (2 + l)
         dof_per_cycle += 3 * sweeps * m_ * n_ * o_
 
-    # Roofline of the dominant kernel, the fine-level FP32 fused smoother:
-    # one sweep is two launches (one per z parity), each updating N/2 points
-    # at 36 algorithmic bytes per point (read x, read r, write x, FP32 vec3),
-    # so 18 N bytes per launch.  Each launch of 20 back-to-back sweeps is
-    # timed with a HIP event pair on the solver's stream (the mean kernel
-    # duration; the sweep time beside it also holds the gaps between
-    # launches).
-    sweep_s = mg.time_kernel(2, 0, 20) * 1e-3
-    launch_s = mg.time_kernel(5, 0, 20) * 1e-3
     alg_launch = 18.0 * N
     achieved = alg_launch / launch_s / 1e9
-    sweep64_s = mg.time_kernel(0, 0, 10) * 1e-3
     achieved64 = 72.0 * N / sweep64_s / 1e9
 
     cpu = None

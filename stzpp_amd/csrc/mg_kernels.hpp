@@ -570,14 +570,16 @@ __global__ void k_soa_to_aos(const double *__restrict__ s, double *__restrict__ 
     a[3 * p + 2] = s[p + 2 * N];
 }
 
-// FP32 defect d = res/s (s = ||res||_inf scale from the host) and e = 0.
+// FP32 defect d = res * inv_s (the scale from the host) and, when e is
+// given, e = 0 (a V-cycle whose first sweep starts from zero without reading
+// x needs no zeros in memory).
 template <typename TI = double>
 __global__ void k_defect_f32(const TI *__restrict__ res, double inv_s, float *__restrict__ d,
                              float *__restrict__ e, long n) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     d[t] = (float)((double)res[t] * inv_s);
-    e[t] = 0.0f;
+    if (e) e[t] = 0.0f;
 }
 
 // x64 += s * e32.

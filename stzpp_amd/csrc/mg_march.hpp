@@ -238,13 +238,16 @@ constexpr int march_waves() {
 }
 
 // Grid: (m/TX, n/TY, ceil(planes of parity P / kchunk)).  Requires a PLANE
-// operator, x and y periodic, m % TX == 0, n % TY == 0, TX in {16, 32, 64} and
-// every sub-pass of the tile at most NT points.  pk: the packed table of the
-// level (k_pack_march), one row of march_stride(PAT) values per plane.
+// operator, x and y periodic, m % TX == 0, n % TY == 0, TX in {8, 16, 32, 64}
+// and every sub-pass of the tile at most NT points.  pk: the packed table of
+// the level (k_pack_march), one row of march_stride(PAT) values per plane.
+// zflags: bit 0 = the planes of xs are all zero, bit 1 = those of xo are
+// (a sweep that starts from x = 0): such planes are staged as zeros without
+// reading them, so the buffer's contents do not matter.
 template <typename T, int PAT, int TX, int TY, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<T, TX, TY, NT>()))) void k_gs_march(Dims d, const T *__restrict__ xs, const T *__restrict__ xo,
                                                  T *__restrict__ xd, const T *__restrict__ r,
-                                                 const T *__restrict__ pk, int P, int kchunk) {
+                                                 const T *__restrict__ pk, int P, int kchunk, int zflags) {
     // The window holds what the four sub-passes read: their points reach
     // at most 3 (x) beyond the tile and their stencils one more, i.e. x in
     // [-3, TX + 3] (origin HXL = 3 points left of the tile), and y in
@@ -391,8 +394,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
         Vec a[NA];
         T b[NB];
     };
-    auto fetch = [&](const T *__restrict__ src, int kg, Stage &v) {
-        bool valid = true;
+    const bool zs = (zflags & 1) != 0, zo = (zflags & 2) != 0;
+    auto fetch = [&](const T *__restrict__ src, int kg, Stage &v, bool zero) {
+        bool valid = !zero;
         int kw = kg;
         if (kg < 0 || kg >= d.o) {
             if (d.pz) kw = kg < 0 ? kg + d.o : kg - d.o;
@@ -433,11 +437,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
     {
         Stage v;
         const int k = P + 2 * kk0;
-        fetch(xo, k - 1, v);
+        fetch(xo, k - 1, v, zo);
         store(sm, v);
-        fetch(xs, k, v);
+        fetch(xs, k, v, zs);
         store(sc, v);
-        fetch(xo, k + 1, v);
+        fetch(xo, k + 1, v, zo);
         store(sp, v);
     }
     __syncthreads();
@@ -472,8 +476,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
         T rn[4][3];
         if (more) {
             load_r(k + 2, rn);
-            fetch(xs, k + 2, n1);
-            fetch(xo, k + 3, n2);
+            fetch(xs, k + 2, n1, zs);
+            fetch(xo, k + 3, n2, zo);
         }
         const T *A = pk + (long)k * march_stride(PAT);
 

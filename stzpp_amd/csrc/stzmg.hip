@@ -126,6 +126,7 @@ struct stzmg_s {
     int kcapz64 = 0;
     int kcap64 = 0, kcap32 = 0;  // restart lengths the two bases hold
     hipGraphExec_t g64 = nullptr, g32 = nullptr;
+    hipGraphExec_t g64z = nullptr, g32z = nullptr;  // V-cycles that start from x = 0
     double *scratch = nullptr;  // this device's scratch of stzmg_norm_noalloc
     int *pkmap = nullptr;       // k_pack_march source maps, kPackMax ints per pattern
     int *pkrmap = nullptr;      // and those of the residual tables
@@ -464,8 +465,10 @@ long fused_resident() {
     return r;
 }
 
+// zero_start: the sweep starts from x = 0 (its buffer need not hold the
+// zeros; k_gs_march only, see zero_start_ok).
 template <typename T, int PAT, int TX, int TY, int NT, bool MARCH = false>
-void launch_fused_tile(stzmg_t h, Level &L) {
+void launch_fused_tile(stzmg_t h, Level &L, bool zero_start = false) {
     const Dims &d = L.d;
     T *x = xref_of<T>(L), *xb = xb_of<T>(L);
     const long tiles = (long)(d.m / TX) * (d.n / TY);
@@ -486,7 +489,8 @@ void launch_fused_tile(stzmg_t h, Level &L) {
         if (rec) (void)hipEventRecord((*h->launch_ev)[h->launch_ev_next], h->stream);
         if constexpr (MARCH)
             hipLaunchKernelGGL((k_gs_march<T, PAT, TX, TY, NT>), g, dim3(NT), 0, h->stream, d, (const T *)x, xo, xb,
-                               (const T *)r_of<T>(L), (const T *)pk_of<T>(L), P, kchunk);
+                               (const T *)r_of<T>(L), (const T *)pk_of<T>(L), P, kchunk,
+                               zero_start ? (P == 0 ? 3 : 1) : 0);
         else
             hipLaunchKernelGGL((k_gs_planes<T, PAT, TX, TY, NT>), g, dim3(NT), 0, h->stream, d, (const T *)x, xo,
                                xb, (const T *)r_of<T>(L), ops_of<T>(L), inv_of<T>(L), P, kchunk);
@@ -499,15 +503,15 @@ void launch_fused_tile(stzmg_t h, Level &L) {
 }
 
 template <typename T, int PAT>
-void launch_fused_pat(stzmg_t h, Level &L) {
+void launch_fused_pat(stzmg_t h, Level &L, bool zero_start) {
     const int ft = sizeof(T) == 4 ? L.ftile : L.ftile64;
     if (ft == 1) {
         constexpr int TX = FusedTile<T>::TX, TY = FusedTile<T>::TY, NT = FusedTile<T>::NT;
         if constexpr (STZ_MARCH != 0) {
             if (L.sym)
-                launch_fused_tile<T, PAT + kMarchSym, TX, TY, NT, true>(h, L);
+                launch_fused_tile<T, PAT + kMarchSym, TX, TY, NT, true>(h, L, zero_start);
             else
-                launch_fused_tile<T, PAT, TX, TY, NT, true>(h, L);
+                launch_fused_tile<T, PAT, TX, TY, NT, true>(h, L, zero_start);
         } else {
             launch_fused_tile<T, PAT, TX, TY, NT, false>(h, L);
         }
@@ -515,9 +519,9 @@ void launch_fused_pat(stzmg_t h, Level &L) {
     else if (ft == 2) {
         if constexpr (STZ_MARCH != 0) {
             if (L.sym)
-                launch_fused_tile<T, PAT + kMarchSym, kSmallTile, kSmallTile, kSmallThreads, true>(h, L);
+                launch_fused_tile<T, PAT + kMarchSym, kSmallTile, kSmallTile, kSmallThreads, true>(h, L, zero_start);
             else
-                launch_fused_tile<T, PAT, kSmallTile, kSmallTile, kSmallThreads, true>(h, L);
+                launch_fused_tile<T, PAT, kSmallTile, kSmallTile, kSmallThreads, true>(h, L, zero_start);
         } else {
             launch_fused_tile<T, PAT, kSmallTile, kSmallTile, kSmallThreads>(h, L);
         }
@@ -525,9 +529,9 @@ void launch_fused_pat(stzmg_t h, Level &L) {
     else {
         if constexpr (STZ_MARCH != 0) {
             if (L.sym)
-                launch_fused_tile<T, PAT + kMarchSym, kTinyTile, kTinyTile, kTinyThreads, true>(h, L);
+                launch_fused_tile<T, PAT + kMarchSym, kTinyTile, kTinyTile, kTinyThreads, true>(h, L, zero_start);
             else
-                launch_fused_tile<T, PAT, kTinyTile, kTinyTile, kTinyThreads, true>(h, L);
+                launch_fused_tile<T, PAT, kTinyTile, kTinyTile, kTinyThreads, true>(h, L, zero_start);
         } else {
             launch_fused_tile<T, PAT, kTinyTile, kTinyTile, kTinyThreads>(h, L);
         }
@@ -537,19 +541,24 @@ void launch_fused_pat(stzmg_t h, Level &L) {
 template <typename T>
 void launch_gs_sweep_colours(stzmg_t h, Level &L);
 
+// Whether a sweep of level L can start from x = 0 without the zeros in
+// memory (the fused sweep k_gs_march skips the planes it knows are zero).
+bool zero_start_ok(const Level &L) { return STZ_MARCH != 0 && L.fused; }
+
 // One Gauss-Seidel sweep of a level: two fused launches where the level
-// allows, eight colour launches otherwise.
+// allows, eight colour launches otherwise.  zero_start (only where
+// zero_start_ok): the sweep starts from x = 0 whatever x holds.
 template <typename T>
-void launch_gs_sweep(stzmg_t h, Level &L) {
+void launch_gs_sweep(stzmg_t h, Level &L, bool zero_start = false) {
     if (!L.fused) {
         launch_gs_sweep_colours<T>(h, L);
         return;
     }
     switch (L.pat) {
-        case PAT_SHEAR: launch_fused_pat<T, PAT_SHEAR>(h, L); break;
-        case PAT_TRANS_XZ: launch_fused_pat<T, PAT_TRANS_XZ>(h, L); break;
-        case PAT_NOCORNER: launch_fused_pat<T, PAT_NOCORNER>(h, L); break;
-        default: launch_fused_pat<T, PAT_DENSE>(h, L); break;
+        case PAT_SHEAR: launch_fused_pat<T, PAT_SHEAR>(h, L, zero_start); break;
+        case PAT_TRANS_XZ: launch_fused_pat<T, PAT_TRANS_XZ>(h, L, zero_start); break;
+        case PAT_NOCORNER: launch_fused_pat<T, PAT_NOCORNER>(h, L, zero_start); break;
+        default: launch_fused_pat<T, PAT_DENSE>(h, L, zero_start); break;
     }
 }
 
@@ -690,8 +699,21 @@ void launch_coarse(stzmg_t h, int first) {
 // One V-cycle (multigrid::v_cycle, multigrid.cc:253-297) in type T.  In FP32
 // the coarsest level is solved in FP64: r32 -> r64, bottom sweeps, x64 -> x32.
 // Levels from first_coarse_level() down run in one workgroup (mg_coarse.hpp).
+// Whether a V-cycle's first operation on the fine level is a sweep that can
+// start from x = 0 without the zeros in memory (k_gs_march): the fine level
+// is smoothed on the way down, at least once, by the fused sweep.
+bool fine_zero_start(stzmg_t h) {
+    const int L = (int)h->lv.size();
+    const int Lc = first_coarse_level(h);
+    const int top = Lc < L - 1 ? Lc : L - 1;
+    return top > 0 && sweeps_down(h, 0) > 0 && zero_start_ok(h->lv[0]);
+}
+
+// zero_top: the fine level enters with x = 0, which need not be in memory
+// where fine_zero_start.  Every coarser level enters with x = 0 from the
+// restriction, so the first down-sweep of every level starts from zero.
 template <typename T>
-void enqueue_vcycle(stzmg_t h) {
+void enqueue_vcycle(stzmg_t h, bool zero_top = false) {
     const int L = (int)h->lv.size();
     const int Lc = first_coarse_level(h);
     const int top = Lc < L - 1 ? Lc : L - 1;
@@ -699,7 +721,9 @@ void enqueue_vcycle(stzmg_t h) {
     for (int i = 0; i < L; i++) home[i] = xref_of<T>(h->lv[i]);
     double *home_b64 = h->lv[L - 1].x64;
     for (int i = 0; i < top; i++) {
-        for (int q = 0; q < sweeps_down(h, i); q++) launch_gs_sweep<T>(h, h->lv[i]);
+        for (int q = 0; q < sweeps_down(h, i); q++)
+            launch_gs_sweep<T>(h, h->lv[i],
+                               q == 0 && (i > 0 ? zero_start_ok(h->lv[i]) : zero_top && fine_zero_start(h)));
         launch_restrict<T>(h, i + 1);
     }
     if (h->qmap_ok && L > 1 && Lc >= L - 1) {
@@ -924,6 +948,15 @@ int pack_march(stzmg_t h) {
     return 0;
 }
 
+// Forget the captured V-cycles (their kernels or arguments changed).
+void drop_graphs(stzmg_t h) {
+    for (hipGraphExec_t *g : {&h->g64, &h->g32, &h->g64z, &h->g32z})
+        if (*g) {
+            hipGraphExecDestroy(*g);
+            *g = nullptr;
+        }
+}
+
 // Galerkin hierarchy and inverse tables after the fine operator changed.
 int build_hierarchy_ops(stzmg_t h) {
     if (int rc = coarse_kernel_attrs(h)) return rc;
@@ -993,8 +1026,7 @@ int build_hierarchy_ops(stzmg_t h) {
         L.tiled = h->tiled_resid && tiled_ok(L);
     }
     if (int rc = pack_march(h)) return rc;
-    if (h->g64) { hipGraphExecDestroy(h->g64); h->g64 = nullptr; }
-    if (h->g32) { hipGraphExecDestroy(h->g32); h->g32 = nullptr; }
+    drop_graphs(h);
     h->ops_set = true;
     h->qmap_ok = false;  // tabulated at the first V-cycle with this operator
     if (flag) return fail(h, STZMG_ERR_SINGULAR, "singular or non-finite centre block on some level");
@@ -1002,11 +1034,11 @@ int build_hierarchy_ops(stzmg_t h) {
 }
 
 template <typename T>
-int capture_vcycle(stzmg_t h, hipGraphExec_t *exec) {
+int capture_vcycle(stzmg_t h, hipGraphExec_t *exec, bool zero_top) {
     if (*exec) return 0;
     hipGraph_t g = nullptr;
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_vcycle<T>(h);
+    enqueue_vcycle<T>(h, zero_top);
     hipError_t e = hipStreamEndCapture(h->stream, &g);
     if (e != hipSuccess) return fail(h, STZMG_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
     e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
@@ -1015,18 +1047,23 @@ int capture_vcycle(stzmg_t h, hipGraphExec_t *exec) {
     return 0;
 }
 
+// n V-cycles; zero_first: the first starts from x = 0 on the fine level
+// (which then need not hold the zeros where zero_start_ok).
 template <typename T>
-int run_vcycles(stzmg_t h, int n, bool graph) {
+int run_vcycles(stzmg_t h, int n, bool graph, bool zero_first = false) {
     // Q of the current operator, before any capture that would use it
     if (h->bottom_map && !h->qmap_ok)
         if (int rc = build_bottom_map(h)) return rc;
     if (graph) {
         hipGraphExec_t *ex = sizeof(T) == 8 ? &h->g64 : &h->g32;
-        int rc = capture_vcycle<T>(h, ex);
-        if (rc) return rc;
-        for (int q = 0; q < n; q++) HIPCHK(h, hipGraphLaunch(*ex, h->stream));
+        hipGraphExec_t *exz = sizeof(T) == 8 ? &h->g64z : &h->g32z;
+        for (int q = 0; q < n; q++) {
+            hipGraphExec_t *e = q == 0 && zero_first ? exz : ex;
+            if (int rc = capture_vcycle<T>(h, e, e == exz)) return rc;
+            HIPCHK(h, hipGraphLaunch(*e, h->stream));
+        }
     } else {
-        for (int q = 0; q < n; q++) enqueue_vcycle<T>(h);
+        for (int q = 0; q < n; q++) enqueue_vcycle<T>(h, q == 0 && zero_first);
         LAUNCHCHK(h);
     }
     return 0;
@@ -1140,8 +1177,7 @@ int stzmg_create(const stzmg_desc *d, stzmg_t *out) {
 int stzmg_destroy(stzmg_t h) {
     if (!h) return STZMG_ERR_ARG;
     hipStreamSynchronize(h->stream);
-    if (h->g64) hipGraphExecDestroy(h->g64);
-    if (h->g32) hipGraphExecDestroy(h->g32);
+    drop_graphs(h);
     for (auto &L : h->lv) free_level(L);
     hipFree(h->part);
     hipFree(h->dres);
@@ -1202,8 +1238,7 @@ int stzmg_set_option(stzmg_t h, int option, int value) {
     } else {
         return fail(h, STZMG_ERR_ARG, "unknown option %d", option);
     }
-    if (h->g64) { hipGraphExecDestroy(h->g64); h->g64 = nullptr; }
-    if (h->g32) { hipGraphExecDestroy(h->g32); h->g32 = nullptr; }
+    drop_graphs(h);
     return 0;
 }
 
@@ -1557,8 +1592,8 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
             if (s.cycles >= opts->max_cycles) { rc = STZMG_ERR_NOT_CONVERGED; break; }
             const double scale = s.res / std::sqrt((double)n3);  // RMS of the defect
             hipLaunchKernelGGL((k_defect_f32<double>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream, (const double *)L.t64,
-                               1.0 / scale, L.r32, L.x32, n3);
-            if ((rc = run_vcycles<float>(h, inner, graph))) return rc;
+                               1.0 / scale, L.r32, fine_zero_start(h) ? nullptr : L.x32, n3);
+            if ((rc = run_vcycles<float>(h, inner, graph, true))) return rc;
             s.cycles += inner;
             hipLaunchKernelGGL(k_update_f64, dim3(nblk(n3)), dim3(kThreads), 0, h->stream, L.x64,
                                (const float *)L.x32, scale, n3);
@@ -1604,8 +1639,8 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
                 float *zj = h->kZ + (long)j * n3;
                 // z_j = M^-1 (sigma v_j)
                 hipLaunchKernelGGL((k_defect_f32<double>), dim3(nb3), dim3(kThreads), 0, h->stream, (const double *)vj, sigma,
-                                   L.r32, L.x32, n3);
-                if ((rc = run_vcycles<float>(h, inner, graph))) return rc;
+                                   L.r32, fine_zero_start(h) ? nullptr : L.x32, n3);
+                if ((rc = run_vcycles<float>(h, inner, graph, true))) return rc;
                 s.cycles += inner;
                 HIPCHK(h, hipMemcpyAsync(zj, L.x32, sizeof(float) * n3, hipMemcpyDeviceToDevice, h->stream));
                 // w' = A z_j (the true w is w' / sigma)
@@ -1716,8 +1751,8 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
                 float *zj = h->kZ + (long)j * n3;
                 // z_j = M^-1 (sigma v_j) = M^-1 (sigma / nrm_j u_j)
                 hipLaunchKernelGGL((k_defect_f32<float>), dim3(nb3), dim3(kThreads), 0, h->stream, uj,
-                                   sigma / nrm[j], L.r32, L.x32, n3);
-                if ((rc = run_vcycles<float>(h, inner, graph))) return rc;
+                                   sigma / nrm[j], L.r32, fine_zero_start(h) ? nullptr : L.x32, n3);
+                if ((rc = run_vcycles<float>(h, inner, graph, true))) return rc;
                 s.cycles += inner;
                 // w' = A z_j in FP32 (the true w is w' / sigma); the same
                 // pass keeps z_j (x32 is overwritten by the next V-cycles)
@@ -1810,8 +1845,9 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
                 double *zj = h->kZ64 + (long)j * n3;
                 // z_j = M^-1 v_j
                 HIPCHK(h, hipMemcpyAsync(L.r64, vj, sizeof(double) * n3, hipMemcpyDeviceToDevice, h->stream));
-                hipLaunchKernelGGL((k_fill<double>), dim3(nb3), dim3(kThreads), 0, h->stream, L.x64, 0.0, n3);
-                if ((rc = run_vcycles<double>(h, inner, graph))) return rc;
+                if (!fine_zero_start(h))
+                    hipLaunchKernelGGL((k_fill<double>), dim3(nb3), dim3(kThreads), 0, h->stream, L.x64, 0.0, n3);
+                if ((rc = run_vcycles<double>(h, inner, graph, true))) return rc;
                 s.cycles += inner;
                 // w = A z_j; the same pass keeps z_j
                 launch_apply<double>(h, L, (const double *)L.x64, w, zj);
