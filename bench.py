@@ -174,8 +174,9 @@ def cpu_baseline(n, st, b, gpu_fp64_cycles):
 def step_roofline(sim, n_points):
     """Roofline of the step's dominant kernel, the fine-level FP64 fused
     Gauss-Seidel sweep of its projection solver (two launches, one per z
-    parity, 72 algorithmic bytes per updated point), each launch timed with a
-    HIP event pair on the solver's stream."""
+    parity, 72 algorithmic bytes per updated point): after five unrecorded
+    sweeps, 20 back-to-back sweeps under one HIP event pair on the solver's
+    stream, and an event pair around every launch beside it."""
     import ctypes
 
     from stzpp_amd import _lib
@@ -183,16 +184,20 @@ def step_roofline(sim, n_points):
     L = _lib.load()
     h = L.stz_multigrid(sim.h)
     v = ctypes.c_double()
-    rc = L.stzmg_time_kernel(h, 4, 0, 20, ctypes.byref(v))
-    if rc:
+    if L.stzmg_time_kernel(h, 0, 0, 5, ctypes.byref(v)):
         return None
-    launch_s = v.value * 1e-3
+    if L.stzmg_time_kernel(h, 0, 0, 20, ctypes.byref(v)):
+        return None
+    launch_s = v.value * 1e-3 / 2
+    if L.stzmg_time_kernel(h, 4, 0, 20, ctypes.byref(v)):
+        return None
+    launch_ev_s = v.value * 1e-3
     alg = 36.0 * n_points
     achieved = alg / launch_s / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
             "kernel": "k_gs_march<double, ...> (fine-level fused GS of the projection, one z parity)",
-            "alg_bytes_per_launch": alg, "launch_us": launch_s * 1e6}
+            "alg_bytes_per_launch": alg, "launch_us": launch_s * 1e6, "launch_us_event_pairs": launch_ev_s * 1e6}
 
 
 def oracle_step_baseline(N, prd, params, state0, dt, family=None, Ut=0.0):
@@ -699,13 +704,15 @@ def main():
     # so 18 N bytes per launch.  Timed right after the timed solves, on the
     # level's fields (which it overwrites; the solution was copied out above),
     # after five unrecorded sweeps that bring the device back to the busy
-    # state of the solves: each launch of 20 back-to-back sweeps with a HIP
-    # event pair on the solver's stream (the mean kernel duration plus the
-    # dispatch inside the pair; the sweep time beside it also holds the gaps
-    # between launches).
+    # state of the solves.  The launch duration is that of 20 back-to-back
+    # sweeps (40 launches, as the V-cycles issue them) under one HIP event
+    # pair on the solver's stream, gaps included.  An event pair around
+    # every launch (launch_us_event_pairs) also holds each launch's dispatch
+    # and isolates it from the next, and reads a few microseconds longer.
     mg.time_kernel(2, 0, 5)
-    launch_s = mg.time_kernel(5, 0, 20) * 1e-3
     sweep_s = mg.time_kernel(2, 0, 20) * 1e-3
+    launch_s = sweep_s / 2
+    launch_ev_s = mg.time_kernel(5, 0, 20) * 1e-3
     sweep64_s = mg.time_kernel(0, 0, 10) * 1e-3
     log(f"[bench] fp32+ir (gmres-ir): {np.mean(tgis)*1e3:.1f} ms, {sgi}")
 
@@ -816,6 +823,7 @@ def main():
                       "one z parity)",
             "alg_bytes_per_launch": alg_launch,
             "launch_us": launch_s * 1e6,
+            "launch_us_event_pairs": launch_ev_s * 1e6,
         },
         "cpu_baseline": cpu,
     }
