@@ -570,16 +570,33 @@ __global__ void k_soa_to_aos(const double *__restrict__ s, double *__restrict__ 
     a[3 * p + 2] = s[p + 2 * N];
 }
 
+// Element-wise kernels take kEW elements per thread, kThreads apart (each
+// load instruction of a wave covers one contiguous segment, and a thread
+// has kEW loads in flight): launch nblk_ew(n) workgroups of kThreads.
+constexpr int kEW = 4;
+inline unsigned nblk_ew(long n) { return (unsigned)((n + (long)kThreads * kEW - 1) / ((long)kThreads * kEW)); }
+
 // FP32 defect d = res * inv_s (the scale from the host) and, when e is
 // given, e = 0 (a V-cycle whose first sweep starts from zero without reading
 // x needs no zeros in memory).
 template <typename TI = double>
-__global__ void k_defect_f32(const TI *__restrict__ res, double inv_s, float *__restrict__ d,
-                             float *__restrict__ e, long n) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    d[t] = (float)((double)res[t] * inv_s);
-    if (e) e[t] = 0.0f;
+__global__ __launch_bounds__(kThreads) void k_defect_f32(const TI *__restrict__ res, double inv_s,
+                                                         float *__restrict__ d, float *__restrict__ e, long n) {
+    const long b0 = (long)blockIdx.x * (kThreads * kEW) + threadIdx.x;
+    TI v[kEW];
+#pragma unroll
+    for (int u = 0; u < kEW; u++) {
+        const long i = b0 + (long)u * kThreads;
+        v[u] = i < n ? res[i] : TI(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kEW; u++) {
+        const long i = b0 + (long)u * kThreads;
+        if (i < n) {
+            d[i] = (float)((double)v[u] * inv_s);
+            if (e) e[i] = 0.0f;
+        }
+    }
 }
 
 // x64 += s * e32.
@@ -588,10 +605,21 @@ __global__ void k_update_f64(double *__restrict__ x, const float *__restrict__ e
     if (t < n) x[t] += s * (double)e[t];
 }
 
+// b = a converted (kEW elements per thread, grid nblk_ew(n)).
 template <typename A, typename B>
-__global__ void k_convert(const A *__restrict__ a, B *__restrict__ b, long n) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) b[t] = (B)a[t];
+__global__ __launch_bounds__(kThreads) void k_convert(const A *__restrict__ a, B *__restrict__ b, long n) {
+    const long b0 = (long)blockIdx.x * (kThreads * kEW) + threadIdx.x;
+    A v[kEW];
+#pragma unroll
+    for (int u = 0; u < kEW; u++) {
+        const long i = b0 + (long)u * kThreads;
+        v[u] = i < n ? a[i] : A(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kEW; u++) {
+        const long i = b0 + (long)u * kThreads;
+        if (i < n) b[i] = (B)v[u];
+    }
 }
 
 template <typename T>

@@ -115,12 +115,23 @@ __global__ __launch_bounds__(kThreads) void k_reduce_cols(const double *__restri
     if (threadIdx.x == 0) out[j] = scale ? red[0] * scale[j] : red[0];
 }
 
-// dst = a src (in FP64, stored as TD).
+// dst = a src (in FP64, stored as TD; kEW elements per thread, grid
+// nblk_ew(n3)).
 template <typename TD = double, typename TS = double>
 __global__ __launch_bounds__(kThreads) void k_scale(TD *__restrict__ dst, const TS *__restrict__ src, double a,
                                                     long n3) {
-    const long i = (long)blockIdx.x * kThreads + threadIdx.x;
-    if (i < n3) dst[i] = (TD)(a * (double)src[i]);
+    const long b0 = (long)blockIdx.x * (kThreads * kEW) + threadIdx.x;
+    TS v[kEW];
+#pragma unroll
+    for (int u = 0; u < kEW; u++) {
+        const long i = b0 + (long)u * kThreads;
+        v[u] = i < n3 ? src[i] : TS(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kEW; u++) {
+        const long i = b0 + (long)u * kThreads;
+        if (i < n3) dst[i] = (TD)(a * (double)v[u]);
+    }
 }
 
 // x += sum_j y[j] Z_j with FP32 Z (j < nv).

@@ -569,7 +569,7 @@ template <typename T>
 void restore_level(stzmg_t h, Level &L, T *home) {
     if (xref_of<T>(L) == home) return;
     const long n3 = 3 * L.d.N;
-    hipLaunchKernelGGL((k_convert<T, T>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream, (const T *)xref_of<T>(L),
+    hipLaunchKernelGGL((k_convert<T, T>), dim3(nblk_ew(n3)), dim3(kThreads), 0, h->stream, (const T *)xref_of<T>(L),
                        home, n3);
     xb_of<T>(L) = xref_of<T>(L);
     xref_of<T>(L) = home;
@@ -739,11 +739,11 @@ void enqueue_vcycle(stzmg_t h, bool zero_top = false) {
         Level &B = h->lv[L - 1];
         const long n3 = 3 * B.d.N;
         if (sizeof(T) == 4 && L > 1) {
-            hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream,
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk_ew(n3)), dim3(kThreads), 0, h->stream,
                                (const float *)B.r32, B.r64, n3);
             hipLaunchKernelGGL((k_fill<double>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream, B.x64, 0.0, n3);
             for (int q = 0; q < h->desc.bottom_sweeps; q++) launch_gs_sweep<double>(h, B);
-            hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream,
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk_ew(n3)), dim3(kThreads), 0, h->stream,
                                (const double *)B.x64, B.x32, n3);
         } else {
             for (int q = 0; q < h->desc.bottom_sweeps; q++) launch_gs_sweep<T>(h, B);
@@ -1366,9 +1366,9 @@ namespace {
 // Level fields rounded to FP32 (x32 <- x64, r32 <- r64).
 void level_to_fp32(stzmg_t h, Level &L) {
     const long n3 = 3 * L.d.N;
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream,
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk_ew(n3)), dim3(kThreads), 0, h->stream,
                        (const double *)L.x64, L.x32, n3);
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream,
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk_ew(n3)), dim3(kThreads), 0, h->stream,
                        (const double *)L.r64, L.r32, n3);
 }
 }  // namespace
@@ -1383,7 +1383,7 @@ int stzmg_gauss_seidel_fp32(stzmg_t h, int level, int nsweeps) {
     for (int q = 0; q < nsweeps; q++) launch_gs_sweep<float>(h, L);
     restore_level<float>(h, L, home);
     const long n3 = 3 * L.d.N;
-    hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream,
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk_ew(n3)), dim3(kThreads), 0, h->stream,
                        (const float *)L.x32, L.x64, n3);
     LAUNCHCHK(h);
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1398,7 +1398,7 @@ int stzmg_residual_fp32(stzmg_t h, int level, double *out) {
     const long N = L.d.N;
     level_to_fp32(h, L);
     launch_resid<float, false>(h, L, L.x32, L.r32, L.t32);
-    hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(3 * N)), dim3(kThreads), 0, h->stream,
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk_ew(3 * N)), dim3(kThreads), 0, h->stream,
                        (const float *)L.t32, L.t64, 3 * N);
     LAUNCHCHK(h);
     std::vector<double> soa((size_t)3 * N);
@@ -1591,7 +1591,7 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
         while (rc == 0 && s.res > target) {
             if (s.cycles >= opts->max_cycles) { rc = STZMG_ERR_NOT_CONVERGED; break; }
             const double scale = s.res / std::sqrt((double)n3);  // RMS of the defect
-            hipLaunchKernelGGL((k_defect_f32<double>), dim3(nblk(n3)), dim3(kThreads), 0, h->stream, (const double *)L.t64,
+            hipLaunchKernelGGL((k_defect_f32<double>), dim3(nblk_ew(n3)), dim3(kThreads), 0, h->stream, (const double *)L.t64,
                                1.0 / scale, L.r32, fine_zero_start(h) ? nullptr : L.x32, n3);
             if ((rc = run_vcycles<float>(h, inner, graph, true))) return rc;
             s.cycles += inner;
@@ -1628,7 +1628,7 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
         auto Hc = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
         const unsigned nb3 = nblk(n3);
         while (rc == 0 && s.res > target && beta > 0) {
-            hipLaunchKernelGGL((k_scale<double, double>), dim3(nb3), dim3(kThreads), 0, h->stream, h->kV, (const double *)L.t64,
+            hipLaunchKernelGGL((k_scale<double, double>), dim3(nblk_ew(n3)), dim3(kThreads), 0, h->stream, h->kV, (const double *)L.t64,
                                1.0 / beta, n3);
             std::fill(g.begin(), g.end(), 0.0);
             g[0] = beta;
@@ -1638,13 +1638,13 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
                 double *vj = h->kV + (long)j * n3, *w = h->kV + (long)(j + 1) * n3;
                 float *zj = h->kZ + (long)j * n3;
                 // z_j = M^-1 (sigma v_j)
-                hipLaunchKernelGGL((k_defect_f32<double>), dim3(nb3), dim3(kThreads), 0, h->stream, (const double *)vj, sigma,
+                hipLaunchKernelGGL((k_defect_f32<double>), dim3(nblk_ew(n3)), dim3(kThreads), 0, h->stream, (const double *)vj, sigma,
                                    L.r32, fine_zero_start(h) ? nullptr : L.x32, n3);
                 if ((rc = run_vcycles<float>(h, inner, graph, true))) return rc;
                 s.cycles += inner;
                 HIPCHK(h, hipMemcpyAsync(zj, L.x32, sizeof(float) * n3, hipMemcpyDeviceToDevice, h->stream));
                 // w' = A z_j (the true w is w' / sigma)
-                hipLaunchKernelGGL((k_convert<float, double>), dim3(nb3), dim3(kThreads), 0, h->stream,
+                hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk_ew(n3)), dim3(kThreads), 0, h->stream,
                                    (const float *)zj, h->ktmp, n3);
                 launch_apply<double>(h, L, (const double *)h->ktmp, w);
                 // d_i = (w', v_i); w' -= sum d_i v_i; |w'|^2
@@ -1664,7 +1664,7 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
                 Hc(j + 1, j) = wn / sigma;
                 if (!std::isfinite(wn)) { rc = STZMG_ERR_NONFINITE; break; }
                 if (wn > 0)
-                    hipLaunchKernelGGL((k_scale<double, double>), dim3(nb3), dim3(kThreads), 0, h->stream, w, (const double *)w,
+                    hipLaunchKernelGGL((k_scale<double, double>), dim3(nblk_ew(n3)), dim3(kThreads), 0, h->stream, w, (const double *)w,
                                        1.0 / wn, n3);
                 // Givens rotations
                 for (int i = 0; i < j; i++) {
@@ -1736,7 +1736,7 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
         const unsigned nb3 = nblk(n3);
         double *dscale = h->kdev + (kKrylovMax + 1);  // 1/nrm_i^2 on the device
         while (rc == 0 && s.res > target && beta > 0) {
-            hipLaunchKernelGGL((k_scale<float, double>), dim3(nb3), dim3(kThreads), 0, h->stream, h->kV32,
+            hipLaunchKernelGGL((k_scale<float, double>), dim3(nblk_ew(n3)), dim3(kThreads), 0, h->stream, h->kV32,
                                (const double *)L.t64, 1.0 / beta, n3);
             std::fill(g.begin(), g.end(), 0.0);
             g[0] = beta;
@@ -1750,7 +1750,7 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
                 float *w = h->kV32 + (long)(j + 1) * n3;
                 float *zj = h->kZ + (long)j * n3;
                 // z_j = M^-1 (sigma v_j) = M^-1 (sigma / nrm_j u_j)
-                hipLaunchKernelGGL((k_defect_f32<float>), dim3(nb3), dim3(kThreads), 0, h->stream, uj,
+                hipLaunchKernelGGL((k_defect_f32<float>), dim3(nblk_ew(n3)), dim3(kThreads), 0, h->stream, uj,
                                    sigma / nrm[j], L.r32, fine_zero_start(h) ? nullptr : L.x32, n3);
                 if ((rc = run_vcycles<float>(h, inner, graph, true))) return rc;
                 s.cycles += inner;
@@ -1834,7 +1834,7 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
         auto Hc = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
         const unsigned nb3 = nblk(n3);
         while (rc == 0 && s.res > target && beta > 0) {
-            hipLaunchKernelGGL((k_scale<double, double>), dim3(nb3), dim3(kThreads), 0, h->stream, h->kV,
+            hipLaunchKernelGGL((k_scale<double, double>), dim3(nblk_ew(n3)), dim3(kThreads), 0, h->stream, h->kV,
                                (const double *)L.t64, 1.0 / beta, n3);
             std::fill(g.begin(), g.end(), 0.0);
             g[0] = beta;
@@ -1867,7 +1867,7 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
                 Hc(j + 1, j) = wn;
                 if (!std::isfinite(wn)) { rc = STZMG_ERR_NONFINITE; break; }
                 if (wn > 0)
-                    hipLaunchKernelGGL((k_scale<double, double>), dim3(nb3), dim3(kThreads), 0, h->stream, w,
+                    hipLaunchKernelGGL((k_scale<double, double>), dim3(nblk_ew(n3)), dim3(kThreads), 0, h->stream, w,
                                        (const double *)w, 1.0 / wn, n3);
                 for (int i = 0; i < j; i++) {
                     const double a = Hc(i, j), b = Hc(i + 1, j);
