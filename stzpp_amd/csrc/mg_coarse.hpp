@@ -10,6 +10,8 @@
 // are those of the multi-launch path, so the results are the same.
 #pragma once
 
+#include <type_traits>
+
 #include "mg_kernels.hpp"
 
 namespace stzmg {
@@ -80,13 +82,43 @@ __device__ void wg_gs_sweep(const Dims &d, T *x, const T *r, const T *ops, const
             }
 }
 
+// Sum over a row of 16 lanes, left in every lane: DPP row permutes (quad
+// swaps, then the half-row and row mirrors) instead of LDS-routed shuffles.
+// All 64 lanes of the wave must be active.
+__device__ __forceinline__ double row16_sum(double v) {
+    auto step = [](double x, auto ctrl) {
+        constexpr int C = decltype(ctrl)::value;
+        const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), C, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), C, 0xF, 0xF, false);
+        return x + __hiloint2double(hi, lo);
+    };
+    v = step(v, std::integral_constant<int, 0xB1>{});   // quad_perm [1, 0, 3, 2]
+    v = step(v, std::integral_constant<int, 0x4E>{});   // quad_perm [2, 3, 0, 1]
+    v = step(v, std::integral_constant<int, 0x141>{});  // row_half_mirror
+    v = step(v, std::integral_constant<int, 0x140>{});  // row_mirror
+    return v;
+}
+__device__ __forceinline__ float row16_sum(float v) {
+    auto step = [](float x, auto ctrl) {
+        constexpr int C = decltype(ctrl)::value;
+        return x + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), C, 0xF, 0xF, false));
+    };
+    v = step(v, std::integral_constant<int, 0xB1>{});
+    v = step(v, std::integral_constant<int, 0x4E>{});
+    v = step(v, std::integral_constant<int, 0x141>{});
+    v = step(v, std::integral_constant<int, 0x140>{});
+    return v;
+}
+
 // Bottom solve: nsweeps colour sweeps of one level with x and r held in
 // LDS (xs, rs: 3 N values each) and 16 lanes per point.  Each lane sums up to
-// two of the 26 off-centre blocks, the 16 partial sums are added with
-// shuffles, and the first lane of the group applies the inverse centre block.
-// Colour order and per-point update are those of wg_gs_sweep; only the
-// summation order inside a point differs.  On exit x holds the result.
+// two of the 26 off-centre blocks, the 16 partial sums are added across the
+// lane row (row16_sum), and the first lane of the group applies the inverse
+// centre block.  Waves without a point in a pass skip it.  Colour order and
+// per-point update are those of wg_gs_sweep; only the summation order inside
+// a point differs.  On exit x holds the result.
 constexpr int kBottomLanes = 16;
+static_assert(kBottomLanes == 16, "row16_sum reduces rows of 16 lanes");
 template <typename T, int MODE>
 __device__ void wg_bottom_sweeps(const Dims &d, const T *ops, const T *inv, int nsweeps, T *xs, const T *rs) {
     const long N = d.N;
@@ -102,6 +134,9 @@ __device__ void wg_bottom_sweeps(const Dims &d, const T *ops, const T *inv, int 
                     const int tot = cntx * cnty * cntz;
                     for (int base = 0; base < tot; base += ngrp) {
                         const int t = base + grp;
+                        // first group of this wave (uniform): a wave past the
+                        // pass's points has nothing to sum
+                        if (base + (int)(threadIdx.x / 64) * (64 / kBottomLanes) >= tot) continue;
                         T a0 = 0, a1 = 0, a2 = 0;
                         long p = 0, c = 0;
                         if (t < tot) {
@@ -122,12 +157,9 @@ __device__ void wg_bottom_sweeps(const Dims &d, const T *ops, const T *inv, int 
                                 mv(A + 9 * b, xs[q], xs[q + N], xs[q + 2 * N], a0, a1, a2);
                             }
                         }
-#pragma unroll
-                        for (int off = kBottomLanes / 2; off > 0; off >>= 1) {
-                            a0 += __shfl_xor(a0, off, kBottomLanes);
-                            a1 += __shfl_xor(a1, off, kBottomLanes);
-                            a2 += __shfl_xor(a2, off, kBottomLanes);
-                        }
+                        a0 = row16_sum(a0);
+                        a1 = row16_sum(a1);
+                        a2 = row16_sum(a2);
                         if (t < tot && sl == 0) {
                             const T e0 = rs[p] - a0, e1 = rs[p + N] - a1, e2 = rs[p + 2 * N] - a2;
                             const T *B = inv + c * 9;
