@@ -433,19 +433,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
             if (EB % NT == 0 || tid + NT * j < EB) sl[lb_off[j]] = v.b[j];
     };
 
-    int sm = 0, sc = 1, sp = 2;  // slots of planes k-1, k, k+1
-    {
-        Stage v;
-        const int k = P + 2 * kk0;
-        fetch(xo, k - 1, v, zo);
-        store(sm, v);
-        fetch(xs, k, v, zs);
-        store(sc, v);
-        fetch(xo, k + 1, v, zo);
-        store(sp, v);
-    }
-    __syncthreads();
-
     // right-hand side of the four points of plane kr (two pairs)
     auto load_r = [&](int kr, T (&v)[4][3]) {
         const T *rk = r + mn * kr;
@@ -458,8 +445,24 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
             }
         });
     };
+
+    // The first three planes and the first right-hand side are all in
+    // flight before any of them is stored (one memory round trip, not
+    // three in a row).
+    int sm = 0, sc = 1, sp = 2;  // slots of planes k-1, k, k+1
     T rr[4][3];
-    load_r(P + 2 * kk0, rr);
+    {
+        Stage v0, v1, v2;
+        const int k = P + 2 * kk0;
+        fetch(xo, k - 1, v0, zo);
+        fetch(xs, k, v1, zs);
+        fetch(xo, k + 1, v2, zo);
+        load_r(k, rr);
+        store(sm, v0);
+        store(sc, v1);
+        store(sp, v2);
+    }
+    __syncthreads();
     // complete here: the wait the compiler places before the first use of rr
     // inside the loop would otherwise count the prefetches issued there too
 #pragma unroll

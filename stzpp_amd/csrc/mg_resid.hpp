@@ -133,13 +133,14 @@ __global__ __launch_bounds__(NT) void k_resid_march(Dims d, const T *__restrict_
     double q4 = 0, q2 = 0;
     int sm = 0, sc = 1, sp = 2;
     if (k0 < k1) {
-        Stage v;
-        fetch(k0 - 1, v);
-        store(sm, v);
-        fetch(k0, v);
-        store(sc, v);
-        fetch(k0 + 1, v);
-        store(sp, v);
+        // the three planes in flight before any is stored
+        Stage v0, v1, v2;
+        fetch(k0 - 1, v0);
+        fetch(k0, v1);
+        fetch(k0 + 1, v2);
+        store(sm, v0);
+        store(sc, v1);
+        store(sp, v2);
     }
     __syncthreads();
     for (int k = k0; k < k1; k++) {
