@@ -41,14 +41,14 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernel (fine-level FP32 fused
 # smoother, one z parity of the 256^3 workload), from two rocprofv3 --pmc
 # runs of `python3 bench.py --steps 1 --warmup 0 --no-cpu --no-graph`:
-# FETCH_SIZE 175.47 MiB x 2.0 + WRITE_SIZE 96.04 MiB x 1.0 per dispatch
+# FETCH_SIZE 176.49 MiB x 2.0 + WRITE_SIZE 96.05 MiB x 1.0 per dispatch
 # (average of the 722 dispatches that stage every plane; the 134 first
 # sweeps of V-cycles that start from x = 0 skip the zero planes and read
-# 83.7 MiB on average).  The factors come from the plain copy kernel
+# 86.9 MiB on average).  The factors come from the plain copy kernel
 # k_convert<double,double> of the same run (3 x 256^3 doubles = 384 MiB each
 # way), which reads FETCH_SIZE 192.0 MiB and WRITE_SIZE 384.0 MiB.  Summary:
 # profiles/mg_solve_256_trans_kernels.txt.
-TRAFFIC_256_BYTES_PER_LAUNCH = (175.47 * 2.0 + 96.04) * 2 ** 20
+TRAFFIC_256_BYTES_PER_LAUNCH = (176.49 * 2.0 + 96.05) * 2 ** 20
 
 
 def log(msg):
