@@ -21,8 +21,12 @@
 //   once per workgroup, so a staged value costs one load and one LDS store.
 // * The global loads of the next plane's right-hand side and of the next two
 //   staged planes are issued before the arithmetic of the current plane and
-//   stored to LDS after it; the new values of the tile's own points go from
-//   registers to x_new in the sub-pass that computes them.
+//   stored to LDS after it.  A thread owns a horizontal pair of points (one
+//   of sub-pass 0 and one of 1, likewise 2 and 3): the pair's right-hand
+//   side is one two-value load per component, and its new values go from
+//   registers to x_new as one two-value store after the second sub-pass.
+// * A sweep that starts from x = 0 stages the planes it knows are zero
+//   without reading them (zflags).
 #pragma once
 
 #include <type_traits>

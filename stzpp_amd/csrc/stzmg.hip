@@ -426,9 +426,9 @@ int fused_tile(const Level &L) {
     return best;
 }
 
-// The large tile class sweeps with the z-marching kernel k_gs_march
-// (mg_march.hpp); STZ_MARCH=0 builds the staged-broadcast k_gs_planes for it
-// instead (for comparison).  The small tile classes use k_gs_planes.
+// Every tile class sweeps with the z-marching kernel k_gs_march
+// (mg_march.hpp); STZ_MARCH=0 builds the staged-broadcast one-plane kernel
+// k_gs_planes instead (for comparison; without the zero-start sweeps).
 #ifndef STZ_MARCH
 #define STZ_MARCH 1
 #endif
