@@ -235,4 +235,173 @@ __global__ __launch_bounds__(NT) void k_resid_march(Dims d, const T *__restrict_
         block_sum2(q4, q2, part, (long)blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z));
 }
 
+// The restriction of k_restrict_march (r_c = (1/8) P^T res, x_c = 0; the
+// reference's region::restriction, region.cc:1767-1812) with the fine
+// residual staged in LDS, for levels periodic in x and y.  A workgroup owns
+// kRestrictTX x kRestrictTY coarse columns and walks a chunk of coarse
+// planes through the fine planes 2 K0 - 1 .. 2 K1 - 1 in order: each fine
+// plane's window ((2 kRestrictTX + 1) x (2 kRestrictTY + 1) points from
+// fine (2 I0 - 1, 2 J0 - 1)) is loaded once, its tile columns with 16-byte
+// loads and its halo column per element, the next plane in flight while
+// the current one is summed, two LDS slots used in turn.  Each coarse sum
+// is formed as in k_restrict_march: the carried partial of the shared odd
+// plane, then the terms of planes 2K and 2K + 1 in restrict_point's order,
+// the latter also into the next carried partial -- the results are the
+// same.  Grid: (mc / kRestrictTX, nc / kRestrictTY, ceil(oc / kchunk)).
+constexpr int kRestrictTX = 32, kRestrictTY = kThreads / kRestrictTX;
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_restrict_tile(Dims F, Dims C, const T *__restrict__ res,
+                                                            T *__restrict__ rc, T *__restrict__ xc, int kchunk) {
+    constexpr int RX = kRestrictTX, RY = kRestrictTY;
+    constexpr int WX = 2 * RX + 1, WY = 2 * RY + 1, CS = WX * WY, PS = 3 * CS;
+    constexpr int VW = 16 / (int)sizeof(T);  // values per 16-byte load
+    constexpr int LPR = 2 * RX / VW;         // lanes per window row
+    constexpr int RPI = 64 / LPR;            // window rows per wave instruction
+    constexpr int NW = kThreads / 64;
+    static_assert((2 * RX) % VW == 0 && 64 % LPR == 0, "tile width");
+    constexpr int NA = (3 * WY + NW * RPI - 1) / (NW * RPI);
+    constexpr int EB = 3 * WY, NB = (EB + kThreads - 1) / kThreads;
+    struct alignas(16) Vec {
+        T v[VW];
+    };
+    __shared__ T lds[2 * PS];
+
+    const int I0 = blockIdx.x * RX, J0 = blockIdx.y * RY;
+    const int K0 = blockIdx.z * kchunk;
+    const int K1 = K0 + kchunk < C.o ? K0 + kchunk : C.o;
+    if (K0 >= K1) return;  // the whole workgroup
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const long mn = (long)F.m * F.n;
+
+    // staging offsets, once per workgroup: window row = component-major,
+    // columns 1 .. 2 RX (fine x = 2 I0 ..) in chunk A, column 0 (2 I0 - 1,
+    // wrapped) in chunk B
+    unsigned ga[NA];
+    int la[NA];
+    const int col = lane % LPR;
+#pragma unroll
+    for (int i = 0; i < NA; i++) {
+        const int row0 = (wv + NW * i) * RPI + lane / LPR;
+        const int row = row0 < 3 * WY ? row0 : 3 * WY - 1;
+        const int c = row / WY, yy = row - c * WY;
+        ga[i] = (unsigned)(sizeof(T) * (c * F.N + (long)F.m * wrap1(2 * J0 - 1 + yy, F.n) + 2 * I0 + col * VW));
+        la[i] = c * CS + yy * WX + 1 + col * VW;
+    }
+    unsigned gb[NB];
+    int lb[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+        const int e = tid + kThreads * j < EB ? tid + kThreads * j : EB - 1;
+        const int c = e / WY, yy = e - c * WY;
+        gb[j] = (unsigned)(sizeof(T) * (c * F.N + (long)F.m * wrap1(2 * J0 - 1 + yy, F.n) +
+                                         wrap1(2 * I0 - 1, F.m)));
+        lb[j] = c * CS + yy * WX;
+    }
+    struct Stage {
+        Vec a[NA];
+        T b[NB];
+    };
+    auto fetch = [&](int fk, Stage &v) {
+        const T *base = res + mn * fk;
+#pragma unroll
+        for (int i = 0; i < NA; i++) v.a[i] = at_bytes(reinterpret_cast<const Vec *>(base), ga[i]);
+#pragma unroll
+        for (int j = 0; j < NB; j++) v.b[j] = at_bytes(base, gb[j]);
+    };
+    auto store = [&](int slot, const Stage &v) {
+        T *sl = lds + slot * PS;
+#pragma unroll
+        for (int i = 0; i < NA; i++)
+            if ((3 * WY) % (NW * RPI) == 0 || (wv + NW * i) * RPI + lane / LPR < 3 * WY) {
+#pragma unroll
+                for (int e = 0; e < VW; e++) sl[la[i] + e] = v.a[i].v[e];
+            }
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+            if (EB % kThreads == 0 || tid + kThreads * j < EB) sl[lb[j]] = v.b[j];
+    };
+    // fine plane f of the walk, wrapped where z is periodic; false where it
+    // lies outside a walled z range (its terms are left out, as in
+    // k_restrict_march)
+    auto fine_plane = [&](int f, int &fw) {
+        fw = f;
+        if (f < 0 || f >= F.o) {
+            if (!F.pz) return false;
+            fw = f < 0 ? f + F.o : f - F.o;
+        }
+        return true;
+    };
+
+    // the terms of the fine plane in slot sl with z weight wk, in
+    // restrict_point's order (fine rows -1, 0, +1, then columns)
+    const int tx = tid % RX, ty = tid / RX;
+    auto terms = [&](int slot, T wk, T &s0, T &s1, T &s2, bool part, T &q0, T &q1, T &q2) {
+        const T *sl = lds + slot * PS;
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            const T wj = b == 1 ? wk : T(0.5) * wk;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const T w = T(0.125) * (a == 1 ? wj : T(0.5) * wj);
+                const int q = (2 * ty + b) * WX + 2 * tx + a;
+                const T v0 = sl[q], v1 = sl[CS + q], v2 = sl[2 * CS + q];
+                s0 += w * v0;
+                s1 += w * v1;
+                s2 += w * v2;
+                if (part) {
+                    q0 += w * v0;
+                    q1 += w * v1;
+                    q2 += w * v2;
+                }
+            }
+        }
+    };
+
+    const int fa = 2 * K0 - 1, fb = 2 * K1 - 1;
+    T c0 = 0, c1 = 0, c2 = 0;  // carried partial of the shared odd plane
+    T s0 = 0, s1 = 0, s2 = 0;
+    T z0 = 0, z1 = 0, z2 = 0;  // unused second partial of the plane sums
+    Stage cur, nxt;
+    int fw;
+    bool vcur = fine_plane(fa, fw);
+    if (vcur) fetch(fw, cur);
+    for (int f = fa; f <= fb; f++) {
+        int fw2 = 0;
+        const bool vnext = f + 1 <= fb && fine_plane(f + 1, fw2);
+        if (vnext) fetch(fw2, nxt);
+        const int slot = f & 1;
+        // the slot's previous plane (f - 2) was summed before the barrier
+        // of the last iteration
+        if (vcur) store(slot, cur);
+        __syncthreads();
+        if (f == fa) {
+            if (vcur) terms(slot, T(0.5), c0, c1, c2, false, z0, z1, z2);
+        } else if (((f - fa) & 1) == 1) {
+            // f = 2K (always inside the fine range)
+            s0 = c0;
+            s1 = c1;
+            s2 = c2;
+            terms(slot, T(1), s0, s1, s2, false, z0, z1, z2);
+        } else {
+            // f = 2K + 1: completes coarse plane K
+            T q0 = 0, q1 = 0, q2 = 0;
+            if (vcur) terms(slot, T(0.5), s0, s1, s2, true, q0, q1, q2);
+            c0 = q0;
+            c1 = q1;
+            c2 = q2;
+            const int K = (f - 1) / 2;
+            const long p = (I0 + tx) + (long)C.m * ((J0 + ty) + (long)C.n * K);
+            rc[p] = s0;
+            rc[p + C.N] = s1;
+            rc[p + 2 * C.N] = s2;
+            xc[p] = 0;
+            xc[p + C.N] = 0;
+            xc[p + 2 * C.N] = 0;
+        }
+        cur = nxt;
+        vcur = vnext;
+    }
+}
+
 }  // namespace stzmg

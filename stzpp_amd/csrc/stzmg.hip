@@ -602,6 +602,18 @@ template <typename T>
 void launch_restrict(stzmg_t h, int l) {
     Level &F = h->lv[l - 1], &C = h->lv[l];
     launch_resid<T, false>(h, F, x_of<T>(F), r_of<T>(F), t_of<T>(F));
+    if (F.d.px && F.d.py && C.d.m % kRestrictTX == 0 && C.d.n % kRestrictTY == 0) {
+        // LDS-staged tiles; coarse planes per workgroup: as many as keep
+        // about 1024 workgroups
+        const long tiles = (long)(C.d.m / kRestrictTX) * (C.d.n / kRestrictTY);
+        int kc = 1;
+        while (kc < 16 && tiles * ((C.d.o + 2 * kc - 1) / (2 * kc)) >= 1024) kc *= 2;
+        if (h->restrict_planes > 0) kc = h->restrict_planes;
+        dim3 g(C.d.m / kRestrictTX, C.d.n / kRestrictTY, (C.d.o + kc - 1) / kc);
+        hipLaunchKernelGGL((k_restrict_tile<T>), g, dim3(kThreads), 0, h->stream, F.d, C.d, (const T *)t_of<T>(F),
+                           r_of<T>(C), x_of<T>(C), kc);
+        return;
+    }
     // coarse planes per thread: as many as keep about 1024 workgroups
     const unsigned bx = nblk((long)C.d.m * C.d.n);
     int kc = 1;
