@@ -43,6 +43,7 @@ struct Level {
     float *x32b = nullptr;
     bool fused = false;   // fused two-launch sweeps on this level
     bool tiled = false;   // LDS plane-marching residual on this level
+    bool tiled_small = false;  // the same with the small tiles (no norm partials)
     int ftile = 0;        // fused-smoother tile class of FP32 sweeps (fused_tile)
     int ftile64 = 0;      // and of FP64 sweeps
     int pat = 0;          // operator zero pattern (PAT_*)
@@ -234,6 +235,16 @@ bool tiled_ok(const Level &L) {
            L.d.n % ResidTile<double>::TY == 0;
 }
 
+// Smaller tiles of the marching residual (16 x 16, one point per thread) for
+// plane levels periodic in x and y that the tiles above do not divide (the
+// 96^2 and 48^2 levels of a 192^2 grid, 32^3 and 16^3 below 256^3); used for
+// the V-cycle's residuals, which take no norm partials.
+constexpr int kResidSmall = 16;
+bool tiled_small_ok(const Level &L) {
+    return L.mode == STZMG_OP_PLANE && L.d.px && L.d.py && !tiled_ok(L) && L.d.m % kResidSmall == 0 &&
+           L.d.n % kResidSmall == 0;
+}
+
 // Workgroups of the plane-marching residual in type T resident on the
 // device at once (CUs x the occupancy its LDS and registers allow; every
 // variant stages the same planes), queried once.
@@ -291,6 +302,22 @@ long resid_parts(stzmg_t h, const Level &L) {
     return (long)nblk((long)L.d.m * L.d.n) * L.d.o;
 }
 
+template <typename T, bool APPLY, int PAT>
+void launch_resid_small(stzmg_t h, Level &L, const T *x, const T *r, T *out, T *xcopy) {
+    // z chunks: about 1024 workgroups
+    const long tiles = (long)(L.d.m / kResidSmall) * (L.d.n / kResidSmall);
+    const long nz = std::min<long>(L.d.o, std::max<long>(1, 1024 / tiles));
+    const int kc = (int)((L.d.o + nz - 1) / nz);
+    dim3 g(L.d.m / kResidSmall, L.d.n / kResidSmall, (L.d.o + kc - 1) / kc);
+    constexpr int S = kResidSmall;
+    if (L.sym)
+        hipLaunchKernelGGL((k_resid_march<T, PAT + kMarchSym, false, APPLY, S, S, kThreads>), g, dim3(kThreads), 0,
+                           h->stream, L.d, x, r, (const T *)pkr_of<T>(L), out, (double *)nullptr, kc, xcopy);
+    else
+        hipLaunchKernelGGL((k_resid_march<T, PAT, false, APPLY, S, S, kThreads>), g, dim3(kThreads), 0, h->stream,
+                           L.d, x, r, (const T *)pkr_of<T>(L), out, (double *)nullptr, kc, xcopy);
+}
+
 template <typename T, bool NORM, bool APPLY, int PAT>
 void launch_resid_tiled(stzmg_t h, Level &L, const T *x, const T *r, T *out, T *xcopy, double *part) {
     int kc;
@@ -317,6 +344,15 @@ void launch_resid_any(stzmg_t h, Level &L, const T *x, const T *r, T *out, T *xc
             case PAT_TRANS_XZ: launch_resid_tiled<T, NORM, APPLY, PAT_TRANS_XZ>(h, L, x, r, out, xcopy, part); break;
             case PAT_NOCORNER: launch_resid_tiled<T, NORM, APPLY, PAT_NOCORNER>(h, L, x, r, out, xcopy, part); break;
             default: launch_resid_tiled<T, NORM, APPLY, PAT_DENSE>(h, L, x, r, out, xcopy, part); break;
+        }
+        return;
+    }
+    if (!NORM && L.tiled_small) {
+        switch (L.pat) {
+            case PAT_SHEAR: launch_resid_small<T, APPLY, PAT_SHEAR>(h, L, x, r, out, xcopy); break;
+            case PAT_TRANS_XZ: launch_resid_small<T, APPLY, PAT_TRANS_XZ>(h, L, x, r, out, xcopy); break;
+            case PAT_NOCORNER: launch_resid_small<T, APPLY, PAT_NOCORNER>(h, L, x, r, out, xcopy); break;
+            default: launch_resid_small<T, APPLY, PAT_DENSE>(h, L, x, r, out, xcopy); break;
         }
         return;
     }
@@ -928,7 +964,7 @@ int pack_march(stzmg_t h) {
         const long n = L.ncls * stride;
         // the marching residual's tables, wherever it may run (the tiled
         // residual can be switched on after the operator is set)
-        if (tiled_ok(L)) {
+        if (tiled_ok(L) || tiled_small_ok(L)) {
             const int *mapr = h->pkrmap + (long)key * kPackMax;
             if (!L.pkr32) HIPCHK(h, alloc(&L.pkr32, L.ncls * kPackMax));
             if (!L.pkr64) HIPCHK(h, alloc(&L.pkr64, L.ncls * kPackMax));
@@ -1004,6 +1040,7 @@ int build_hierarchy_ops(stzmg_t h) {
     for (auto &L : h->lv) {
         L.fused = false;
         L.tiled = false;
+        L.tiled_small = false;
         if (L.mode != STZMG_OP_PLANE) continue;
         std::vector<double> tab((size_t)L.ncls * kBlk);
         HIPCHK(h, hipMemcpy(tab.data(), L.ops64, sizeof(double) * tab.size(), hipMemcpyDeviceToHost));
@@ -1036,6 +1073,7 @@ int build_hierarchy_ops(stzmg_t h) {
         L.ftile64 = fused_tile<double>(L);
         L.fused = h->fused_smoother && L.ftile && L.ftile64 && L.x64b;
         L.tiled = h->tiled_resid && tiled_ok(L);
+        L.tiled_small = h->tiled_resid && tiled_small_ok(L);
     }
     if (int rc = pack_march(h)) return rc;
     drop_graphs(h);
@@ -1237,7 +1275,10 @@ int stzmg_set_option(stzmg_t h, int option, int value) {
         }
     } else if (option == STZMG_OPT_TILED_RESIDUAL) {
         h->tiled_resid = value != 0;
-        for (auto &L : h->lv) L.tiled = h->ops_set && h->tiled_resid && tiled_ok(L);
+        for (auto &L : h->lv) {
+            L.tiled = h->ops_set && h->tiled_resid && tiled_ok(L);
+            L.tiled_small = h->ops_set && h->tiled_resid && tiled_small_ok(L);
+        }
     } else if (option == STZMG_OPT_BOTTOM_MAP) {
         h->bottom_map = value != 0;
         h->qmap_ok = false;  // (re)built at the next V-cycle when on
