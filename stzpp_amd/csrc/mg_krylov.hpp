@@ -14,6 +14,13 @@ namespace stzmg {
 constexpr int kKrylovMax = 16;       // largest restart length
 constexpr int kDotBlocks = 1024;     // blocks of the grid-stride dot kernels
 
+// Grid-stride steps per pass of the multi-vector kernels (FP32 vectors: 4,
+// FP64: 2; bounded by the registers of the 17-vector instances).
+template <typename TV>
+struct KrylovUnroll {
+    static constexpr int U = sizeof(TV) == 4 ? 4 : 2;
+};
+
 // part[b * NV + j] = sum over this block's share of w . V_j, j < NV.
 template <int NV, typename TV = double>
 __global__ __launch_bounds__(kThreads) void k_mdot(const TV *__restrict__ w, const TV *__restrict__ V, long n3,
@@ -22,7 +29,26 @@ __global__ __launch_bounds__(kThreads) void k_mdot(const TV *__restrict__ w, con
     double acc[NV];
 #pragma unroll
     for (int j = 0; j < NV; j++) acc[j] = 0;
-    for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n3; i += (long)gridDim.x * kThreads) {
+    // U stride steps per pass with every load issued first (more loads in
+    // flight per wave); the terms are added in the order of the plain
+    // grid-stride loop, so the sums are the same
+    constexpr int U = KrylovUnroll<TV>::U;
+    const long S = (long)gridDim.x * kThreads;
+    long i = (long)blockIdx.x * kThreads + threadIdx.x;
+    for (; i + (U - 1) * S < n3; i += U * S) {
+        TV wv[U], vv[U][NV];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            wv[u] = w[i + u * S];
+#pragma unroll
+            for (int j = 0; j < NV; j++) vv[u][j] = V[(long)j * n3 + i + u * S];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int j = 0; j < NV; j++) acc[j] += (double)wv[u] * (double)vv[u][j];
+    }
+    for (; i < n3; i += S) {
         const double wi = (double)w[i];
 #pragma unroll
         for (int j = 0; j < NV; j++) acc[j] += wi * (double)V[(long)j * n3 + i];
@@ -54,7 +80,29 @@ __global__ __launch_bounds__(kThreads) void k_maxpy_norm(TV *__restrict__ w, con
 #pragma unroll
     for (int j = 0; j < NV; j++) c[j] = coef[j];
     double acc = 0;
-    for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n3; i += (long)gridDim.x * kThreads) {
+    // U stride steps per pass, loads first, terms in the plain loop's order
+    constexpr int U = KrylovUnroll<TV>::U;
+    const long S = (long)gridDim.x * kThreads;
+    long i = (long)blockIdx.x * kThreads + threadIdx.x;
+    for (; i + (U - 1) * S < n3; i += U * S) {
+        TV wv[U], vv[U][NV];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            wv[u] = w[i + u * S];
+#pragma unroll
+            for (int j = 0; j < NV; j++) vv[u][j] = V[(long)j * n3 + i + u * S];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            double wi = (double)wv[u];
+#pragma unroll
+            for (int j = 0; j < NV; j++) wi -= c[j] * (double)vv[u][j];
+            const TV ws = (TV)wi;
+            w[i + u * S] = ws;
+            acc += (double)ws * (double)ws;
+        }
+    }
+    for (; i < n3; i += S) {
         double wi = (double)w[i];
 #pragma unroll
         for (int j = 0; j < NV; j++) wi -= c[j] * (double)V[(long)j * n3 + i];
@@ -134,15 +182,36 @@ __global__ __launch_bounds__(kThreads) void k_scale(TD *__restrict__ dst, const 
     }
 }
 
-// x += sum_j y[j] Z_j with FP32 Z (j < nv).
+// x += sum_j y[j] Z_j (j < nv; kEW elements per thread, grid nblk_ew(n3)).
+// The loads of one basis vector for all kEW elements are issued together;
+// each element's terms are added in the order j = 0, 1, ...
 template <typename TZ = float>
 __global__ __launch_bounds__(kThreads) void k_xupdate(double *__restrict__ x, const TZ *__restrict__ Z,
                                                       const double *__restrict__ y, int nv, long n3) {
-    const long i = (long)blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n3) return;
-    double s = x[i];
-    for (int j = 0; j < nv; j++) s += y[j] * (double)Z[(long)j * n3 + i];
-    x[i] = s;
+    const long b0 = (long)blockIdx.x * (kThreads * kEW) + threadIdx.x;
+    double s[kEW];
+#pragma unroll
+    for (int u = 0; u < kEW; u++) {
+        const long i = b0 + (long)u * kThreads;
+        s[u] = i < n3 ? x[i] : 0.0;
+    }
+    for (int j = 0; j < nv; j++) {
+        const double yj = y[j];
+        const TZ *Zj = Z + (long)j * n3;
+        TZ z[kEW];
+#pragma unroll
+        for (int u = 0; u < kEW; u++) {
+            const long i = b0 + (long)u * kThreads;
+            z[u] = i < n3 ? Zj[i] : TZ(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kEW; u++) s[u] += yj * (double)z[u];
+    }
+#pragma unroll
+    for (int u = 0; u < kEW; u++) {
+        const long i = b0 + (long)u * kThreads;
+        if (i < n3) x[i] = s[u];
+    }
 }
 
 }  // namespace stzmg

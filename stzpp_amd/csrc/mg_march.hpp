@@ -488,7 +488,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
         }
         const T *A = pk + (long)k * march_stride(PAT);
 
-        // phase Z: planes k-1 and k+1
+        // phase Z: planes k-1 and k+1.  The two points of a pair are
+        // addressed from the first one's window index (the second sits a
+        // compile-time distance away), so the cells their neighbourhoods
+        // share are read from LDS once (not in the 8-wide class, whose one
+        // wave would then hold more values than its registers).
+        constexpr bool SHARE = TX != 8;
         T acc[4][3];
         static_for<4>([&](auto s) { acc[s][0] = acc[s][1] = acc[s][2] = 0; });
         {
@@ -503,20 +508,27 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(march_waves<
                 if constexpr (NG > 0) {
                     T c[NG];
                     march_load<T, PAT, 0, GR>(A, c);
-                    static_for<4>([&](auto s) {
-                        // waves without a point of this sub-pass skip it
-                        // (a wave-uniform branch)
-                        if (wv * 64 >= march_lanes<TX, TY, NT>(decltype(s)::value & 2)) return;
-                        auto ld = [&](int cc, int di, int dj, int dk) -> T {
-                            const int b0 = dk < 0 ? bm[s] : bp[s];
-                            const int xp = (SP::xf(decltype(s)::value) + HXL) & 1;
-                            const int off = dj * WX + (((xp + di) & 1) - xp) * HX + ((xp + di) >> 1);
-                            return lds[b0 + cc * CSZ + off];
-                        };
-                        static_for<march_len(PAT, 0)>([&](auto i) {
-                            if constexpr (march_group(PAT, 0, G, decltype(i)::value) == GR)
-                                march_block_acc<T, PAT, 0, decltype(i)::value>(c, ld, acc[s][0], acc[s][1],
-                                                                                acc[s][2]);
+                    static_for<2>([&](auto pg) {
+                        constexpr int PG = decltype(pg)::value;
+                        // waves without a point of this pair of sub-passes
+                        // skip it (a wave-uniform branch)
+                        if (wv * 64 >= march_lanes<TX, TY, NT>(2 * PG)) return;
+                        static_for<2>([&](auto h_) {
+                            constexpr int S = 2 * PG + decltype(h_)::value;
+                            constexpr int xp = (SP::xf(S) + HXL) & 1;
+                            // pb[S] - pb[2 PG]: x + 1 of a parity-split row
+                            constexpr int dl = !SHARE || S == 2 * PG ? 0 : (xp == 0 ? 1 - HX : HX);
+                            constexpr int SB = SHARE ? 2 * PG : S;
+                            auto ld = [&](int cc, int di, int dj, int dk) -> T {
+                                const int b0 = dk < 0 ? bm[SB] : bp[SB];
+                                const int off = dj * WX + (((xp + di) & 1) - xp) * HX + ((xp + di) >> 1);
+                                return lds[b0 + dl + cc * CSZ + off];
+                            };
+                            static_for<march_len(PAT, 0)>([&](auto i) {
+                                if constexpr (march_group(PAT, 0, G, decltype(i)::value) == GR)
+                                    march_block_acc<T, PAT, 0, decltype(i)::value>(c, ld, acc[S][0], acc[S][1],
+                                                                                    acc[S][2]);
+                            });
                         });
                     });
                 }

@@ -1744,7 +1744,7 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
             }
             for (int i = 0; i < nj; i++) hc[i] = y[i] / sigma;
             HIPCHK(h, hipMemcpyAsync(h->kdev, hc.data(), sizeof(double) * nj, hipMemcpyHostToDevice, h->stream));
-            hipLaunchKernelGGL(k_xupdate<float>, dim3(nb3), dim3(kThreads), 0, h->stream, L.x64, (const float *)h->kZ,
+            hipLaunchKernelGGL(k_xupdate<float>, dim3(nblk_ew(n3)), dim3(kThreads), 0, h->stream, L.x64, (const float *)h->kZ,
                                (const double *)h->kdev, nj, n3);
             LAUNCHCHK(h);
             s.outer++;
@@ -1852,7 +1852,7 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
             }
             for (int i = 0; i < nj; i++) hc[i] = y[i] / sigma;
             HIPCHK(h, hipMemcpyAsync(h->kdev, hc.data(), sizeof(double) * nj, hipMemcpyHostToDevice, h->stream));
-            hipLaunchKernelGGL(k_xupdate<float>, dim3(nb3), dim3(kThreads), 0, h->stream, L.x64, (const float *)h->kZ,
+            hipLaunchKernelGGL(k_xupdate<float>, dim3(nblk_ew(n3)), dim3(kThreads), 0, h->stream, L.x64, (const float *)h->kZ,
                                (const double *)h->kdev, nj, n3);
             LAUNCHCHK(h);
             s.outer++;
@@ -1944,7 +1944,7 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
                 y[i] = t / Hc(i, i);
             }
             HIPCHK(h, hipMemcpyAsync(h->kdev, y.data(), sizeof(double) * nj, hipMemcpyHostToDevice, h->stream));
-            hipLaunchKernelGGL(k_xupdate<double>, dim3(nb3), dim3(kThreads), 0, h->stream, h->kX64,
+            hipLaunchKernelGGL(k_xupdate<double>, dim3(nblk_ew(n3)), dim3(kThreads), 0, h->stream, h->kX64,
                                (const double *)h->kZ64, (const double *)h->kdev, nj, n3);
             LAUNCHCHK(h);
             s.outer++;
