@@ -183,7 +183,8 @@ void stzmg_solve_opts_default(stzmg_solve_opts *o);
  *                            the built-in threshold, 0 launches every pass
  *                            separately. */
 enum { STZMG_OPT_COARSE_KERNEL = 0, STZMG_OPT_FUSED_SMOOTHER = 1, STZMG_OPT_TILED_RESIDUAL = 2,
-       STZMG_OPT_BOTTOM_MAP = 3, STZMG_OPT_CHUNK_CAP = 4, STZMG_OPT_RESTRICT_PLANES = 5 };
+       STZMG_OPT_BOTTOM_MAP = 3, STZMG_OPT_CHUNK_CAP = 4, STZMG_OPT_RESTRICT_PLANES = 5,
+       STZMG_OPT_CENTRO_GALERKIN = 6 };
 /*   STZMG_OPT_FUSED_SMOOTHER 1 (default): per-plane levels with periodic x, y
  *                            (x and y sizes multiples of 8, more than 1024
  *                            points) sweep with two fused launches (one per z
@@ -213,7 +214,18 @@ enum { STZMG_OPT_COARSE_KERNEL = 0, STZMG_OPT_FUSED_SMOOTHER = 1, STZMG_OPT_TILE
  *                            differ in the last bits).
  *   STZMG_OPT_RESTRICT_PLANES 0 (default): coarse planes per restriction
  *                            thread chosen from the grid; value > 0 fixes it
- *                            (results unchanged). */
+ *                            (results unchanged).
+ *   STZMG_OPT_CENTRO_GALERKIN 1 (default): a per-plane Galerkin level whose
+ *                            mirrored blocks (offset o and -o) agree in every
+ *                            entry to 1e-13 of the level's largest entry --
+ *                            the product of a centrosymmetric operator, apart
+ *                            by the rounding of the product's summation
+ *                            order -- takes each pair's mean, so it is
+ *                            centrosymmetric bit for bit and its sweeps add
+ *                            opposite neighbours before one multiply.
+ *                            Entries change in the last bits; levels whose
+ *                            blocks differ by more (walls) are unchanged.
+ *                            0: the product as summed. */
 int stzmg_set_option(stzmg_t h, int option, int value);
 
 /* The HIP stream (hipStream_t) the handle launches on. */

@@ -499,6 +499,48 @@ __global__ void k_rat(Dims F, Dims C, const double *__restrict__ opsF, double *_
     opsC[rep * kBlk + DE] = acc;
 }
 
+// Rounding-level asymmetry of a Galerkin level, removed: when every mirrored
+// entry pair (block o, block -o) of every plane differs by at most rtol x
+// the level's largest entry, both entries become their mean, so the level
+// is centrosymmetric bit for bit and its sweeps add each pair of opposite
+// neighbours before one multiply.  The product of a centrosymmetric fine
+// operator with the symmetric transfers is centrosymmetric; k_rat's fixed
+// summation order leaves mirrored entries apart by rounding only (relative
+// to the largest entry: 2e-16 to 1.1e-15 on the transformed operator's
+// Galerkin levels, 64^3 periodic), while the walls of a walled level make
+// them differ by percents, and such a level is left as it is.  One
+// workgroup; ncls tables of kBlk values.
+constexpr int kSymThreads = 1024;
+constexpr double kCentroTol = 1e-13;  // rtol of k_centro_sym as the library uses it
+__global__ __launch_bounds__(kSymThreads) void k_centro_sym(double *__restrict__ ops, long ncls, double rtol) {
+    __shared__ double red[kSymThreads / 64];
+    const long ne = ncls * kBlk, np = ncls * 117;  // 13 blocks x 9 entries per plane
+    double m = 0;
+    for (long q = threadIdx.x; q < ne; q += kSymThreads) m = fmax(m, fabs(ops[q]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    double scale = 0;
+#pragma unroll
+    for (int w = 0; w < kSymThreads / 64; w++) scale = fmax(scale, red[w]);
+    const double lim = rtol * scale;
+    int bad = 0;
+    for (long q = threadIdx.x; q < np; q += kSymThreads) {
+        const long c = q / 117, r = q - c * 117, b = r / 9, e = r - b * 9;
+        const double a = ops[c * kBlk + 9 * b + e], z = ops[c * kBlk + 9 * (26 - b) + e];
+        if (!(fabs(a - z) <= lim)) bad = 1;  // a NaN also leaves the level as it is
+    }
+    if (__syncthreads_or(bad)) return;
+    for (long q = threadIdx.x; q < np; q += kSymThreads) {
+        const long c = q / 117, r = q - c * 117, b = r / 9, e = r - b * 9;
+        double &a = ops[c * kBlk + 9 * b + e], &z = ops[c * kBlk + 9 * (26 - b) + e];
+        const double v = 0.5 * (a + z);
+        a = v;
+        z = v;
+    }
+}
+
 // Centre-block inverse by cofactors over the determinant (src/mat3.hh:177-183)
 // and FP32 copies of the table.  flag[0] is set when a determinant is zero
 // or not finite.

@@ -104,6 +104,7 @@ struct stzmg_s {
     long coarse_max = kCoarseMaxPoints;  // levels this small run in one workgroup (0: off)
     int fused_smoother = 1;   // two-launch fused sweeps on eligible levels
     int tiled_resid = 1;      // plane-marching residual on eligible levels
+    int centro_galerkin = 1;  // k_centro_sym after each per-plane Galerkin product
     int bottom_map = 1;       // coarsest solve as x = Q r (mg_coarse.hpp)
     long chunk_cap = 0;       // cap on the workgroups the z-chunked launches assume resident (0: none)
     // stzmg_time_kernel kinds 4, 5: event pairs recorded around each fused
@@ -1014,10 +1015,13 @@ int build_hierarchy_ops(stzmg_t h) {
         if (l > 0) {
             Level &F = h->lv[l - 1];
             const long nthr = L.ncls * kBlk;
-            if (F.mode == STZMG_OP_PLANE && L.mode == STZMG_OP_PLANE)
+            if (F.mode == STZMG_OP_PLANE && L.mode == STZMG_OP_PLANE) {
                 hipLaunchKernelGGL((k_rat<0, 0>), dim3(nblk(nthr, 64)), dim3(64), 0, h->stream, F.d, L.d,
                                    (const double *)F.ops64, L.ops64, L.ncls);
-            else if (F.mode == STZMG_OP_POINT && L.mode == STZMG_OP_POINT)
+                if (h->centro_galerkin)
+                    hipLaunchKernelGGL(k_centro_sym, dim3(1), dim3(kSymThreads), 0, h->stream, L.ops64, L.ncls,
+                                       kCentroTol);
+            } else if (F.mode == STZMG_OP_POINT && L.mode == STZMG_OP_POINT)
                 hipLaunchKernelGGL((k_rat<1, 1>), dim3(nblk(nthr, 64)), dim3(64), 0, h->stream, F.d, L.d,
                                    (const double *)F.ops64, L.ops64, L.ncls);
             else
@@ -1272,6 +1276,12 @@ int stzmg_set_option(stzmg_t h, int option, int value) {
                 int rc = build_hierarchy_ops(h);
                 if (rc) return rc;
             }
+        }
+    } else if (option == STZMG_OPT_CENTRO_GALERKIN) {
+        h->centro_galerkin = value != 0;
+        if (h->ops_set) {
+            for (auto &L : h->lv) restore_level<double>(h, L, L.x64);  // pointers are home between calls
+            if (int rc = build_hierarchy_ops(h)) return rc;
         }
     } else if (option == STZMG_OPT_TILED_RESIDUAL) {
         h->tiled_resid = value != 0;

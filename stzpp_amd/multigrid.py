@@ -20,7 +20,7 @@ PLANE, POINT = 0, 1
 NORM_QUARTIC, NORM_L2SQ = 0, 1
 SOLVE_REF, SOLVE_RELRES, SOLVE_FP32_IR, SOLVE_FP32_GMRES, SOLVE_FP32_GMRES_IR, SOLVE_GMRES64 = 0, 1, 2, 3, 4, 5
 OPT_COARSE_KERNEL, OPT_FUSED_SMOOTHER, OPT_TILED_RESIDUAL, OPT_BOTTOM_MAP = 0, 1, 2, 3
-OPT_CHUNK_CAP, OPT_RESTRICT_PLANES = 4, 5
+OPT_CHUNK_CAP, OPT_RESTRICT_PLANES, OPT_CENTRO_GALERKIN = 4, 5, 6
 
 
 class MultigridError(RuntimeError):

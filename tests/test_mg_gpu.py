@@ -70,6 +70,55 @@ def test_gpu_level_ops_match_reference(golden, case, mode):
     assert relerr(mg.x(0) - x1, g["interp_x0"] - x1) < 1e-12
 
 
+def galerkin_pair(n, o, prd, planes):
+    """Level operators with STZMG_OPT_CENTRO_GALERKIN on (default) and off."""
+    out = []
+    for on in (1, 0):
+        mg = MG.Multigrid(n, n, o, prd)
+        mg.set_option(MG.OPT_CENTRO_GALERKIN, on)
+        mg.setup_matrices_planes(planes)
+        out.append([mg.level_operator(l).reshape(-1, 27, 9) for l in range(1, mg.nlevels)])
+    return out
+
+
+def test_gpu_centro_galerkin_on_a_periodic_transform():
+    # pure shear T = diag(e^{Ut t}, 1, e^{-Ut t}) at t = 0.005, 64^3 periodic:
+    # the Galerkin levels of this centrosymmetric operator come out of the
+    # summed product with mirrored entries apart by rounding only; with the
+    # option on they are centrosymmetric bit for bit, within 1e-13 of the
+    # product as summed and of the oracle's product (region.cc:1912-1966)
+    n, t = 64, 0.005
+    h = 1.0 / n
+    T = np.diag([np.exp(0.5 * t), 1.0, np.exp(-0.5 * t)])
+    planes = O.plane_operator(n, True, MG.trans_stencil(T, 1.0, 1.5, h, h, h, t), None)
+    sym, raw = galerkin_pair(n, n, (1, 1, 1), planes)
+    orc = O.OracleMG(n, n, n, (1, 1, 1), mode=O.PLANE)
+    orc.set_ops(0, planes)
+    orc.rat()
+    asym = 0.0
+    for l, (a, b) in enumerate(zip(sym, raw), start=1):
+        assert np.array_equal(a[:, :13], a[:, 26:13:-1]), l
+        asym = max(asym, relerr(b[:, :13], b[:, 26:13:-1]))
+        assert relerr(a, b) < REL
+        assert relerr(a, orc.ops(l).reshape(-1, 27, 9)) < REL
+    # the product as summed is not centrosymmetric bit for bit (else this
+    # case would not exercise the option), only to rounding
+    assert 0 < asym < 1e-14
+
+
+def test_gpu_centro_galerkin_leaves_walled_levels():
+    # the walls make the Galerkin levels' mirrored blocks differ by percents:
+    # the option leaves them as summed, bit for bit
+    n = 32
+    h = 1.0 / n
+    st, fix = MG.shear_stencil(1.0, 1.5, h, h, h, 1.0)
+    planes = O.plane_operator(n + 1, False, st, fix)
+    sym, raw = galerkin_pair(n, n + 1, (1, 1, 0), planes)
+    for l, (a, b) in enumerate(zip(sym, raw), start=1):
+        assert np.array_equal(a, b), l
+        assert relerr(a[:, :13], a[:, 26:13:-1]) > 1e-3, l
+
+
 @pytest.mark.parametrize("case", CASES)
 def test_gpu_colour_gs_matches_oracle(golden, case):
     g = golden(case)
