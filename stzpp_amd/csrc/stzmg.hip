@@ -14,6 +14,7 @@
 #include <cstring>
 #include <string>
 #include <algorithm>
+#include <tuple>
 #include <vector>
 
 #include "../../include/stzmg.h"
@@ -1038,6 +1039,15 @@ int build_hierarchy_ops(stzmg_t h) {
     int flag = 0;
     HIPCHK(h, hipMemcpyAsync(&flag, h->dflag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    // The captured V-cycles hold each level's kernel instances and buffer
+    // pointers, not its tables: they stay valid while every level keeps its
+    // kernel configuration (an operator that changes every step, as in the
+    // transformed step, then keeps its graphs).
+    auto config = [](const Level &L) {
+        return std::make_tuple(L.fused, L.tiled, L.tiled_small, L.ftile, L.ftile64, L.pat, L.sym);
+    };
+    std::vector<decltype(config(h->lv[0]))> before;
+    for (auto &L : h->lv) before.push_back(config(L));
     // Zero pattern of every per-plane level: the tightest compiled pattern
     // that holds all nonzero entries selects the fused smoother and tiled
     // residual instances.
@@ -1080,7 +1090,9 @@ int build_hierarchy_ops(stzmg_t h) {
         L.tiled_small = h->tiled_resid && tiled_small_ok(L);
     }
     if (int rc = pack_march(h)) return rc;
-    drop_graphs(h);
+    bool same = h->ops_set;
+    for (size_t l = 0; l < h->lv.size() && same; l++) same = config(h->lv[l]) == before[l];
+    if (!same) drop_graphs(h);
     h->ops_set = true;
     h->qmap_ok = false;  // tabulated at the first V-cycle with this operator
     if (flag) return fail(h, STZMG_ERR_SINGULAR, "singular or non-finite centre block on some level");

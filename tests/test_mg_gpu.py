@@ -337,6 +337,36 @@ def test_gpu_bottom_map_of_a_changed_operator():
     assert np.array_equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("mode", [MG.SOLVE_RELRES, MG.SOLVE_FP32_GMRES_IR])
+def test_gpu_graphs_kept_across_operator_change(mode):
+    # Two transformed operators (pure shear at two times) with the same kernel
+    # configuration on every level: the handle keeps the V-cycle graphs it
+    # captured for the first, and its solve with the second operator equals,
+    # bit for bit, that of a handle that only saw the second.
+    n = 32
+    h = 1.0 / n
+    ops = []
+    for t in (0.005, 0.01):
+        T = np.diag([np.exp(0.5 * t), 1.0, np.exp(-0.5 * t)])
+        ops.append(O.plane_operator(n, True, MG.trans_stencil(T, 1.0, 1.5, h, h, h, 0.005), None))
+    N = n * n * n
+    r = O.det_field(N, 35)
+    r -= r.mean(axis=0)
+    out = []
+    for first in (True, False):
+        mg = MG.Multigrid(n, n, n, (1, 1, 1))
+        if first:
+            mg.setup_matrices_planes(ops[0])
+            mg.setup_fields(np.zeros((N, 3)), r)
+            mg.solve(mode=mode, rtol=1e-8, use_graph=True)
+        mg.setup_matrices_planes(ops[1])
+        mg.setup_fields(np.zeros((N, 3)), r)
+        s = mg.solve(mode=mode, rtol=1e-8, use_graph=True)
+        assert s["res"] <= 1e-8 * s["res0"]
+        out.append(mg.x(0))
+    assert np.array_equal(out[0], out[1])
+
+
 def test_gpu_vcycle_converges_like_reference(golden):
     # The 8-colour V-cycle and the reference's lexicographic one reach the
     # same solution; per-cycle contraction is of the same order.
