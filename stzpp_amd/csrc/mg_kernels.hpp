@@ -541,6 +541,33 @@ __global__ __launch_bounds__(kSymThreads) void k_centro_sym(double *__restrict__
     }
 }
 
+// Zero pattern and centrosymmetry of a per-plane level, so that the host
+// reads 28 words instead of the tables: out[b] (b < 27) = OR over the
+// planes of the nonzero entries of block b (bit e for entry e, a NaN
+// counting as nonzero), out[27] = 1 when block(o) equals block(-o) bit for
+// bit in every plane.  One workgroup of kThreads.
+constexpr int kCfgWords = 28;
+__global__ __launch_bounds__(kThreads) void k_level_config(const double *__restrict__ ops, long ncls,
+                                                           unsigned *__restrict__ out) {
+    __shared__ unsigned used[27];
+    if (threadIdx.x < 27) used[threadIdx.x] = 0;
+    __syncthreads();
+    const long ne = ncls * kBlk, np = ncls * 117;  // 13 blocks x 9 entries per plane
+    for (long q = threadIdx.x; q < ne; q += kThreads)
+        if (ops[q] != 0.0) {
+            const int r = (int)(q % kBlk);
+            atomicOr(&used[r / 9], 1u << (r % 9));
+        }
+    int bad = 0;
+    for (long q = threadIdx.x; q < np; q += kThreads) {
+        const long c = q / 117, r = q - c * 117, b = r / 9, e = r - b * 9;
+        if (ops[c * kBlk + 9 * b + e] != ops[c * kBlk + 9 * (26 - b) + e]) bad = 1;
+    }
+    bad = __syncthreads_or(bad);  // (a barrier: the LDS masks are complete)
+    if (threadIdx.x < 27) out[threadIdx.x] = used[threadIdx.x];
+    if (threadIdx.x == 0) out[27] = bad ? 0u : 1u;
+}
+
 // Centre-block inverse by cofactors over the determinant (src/mat3.hh:177-183)
 // and FP32 copies of the table.  flag[0] is set when a determinant is zero
 // or not finite.

@@ -100,6 +100,7 @@ struct stzmg_s {
     long part_cap = 0;
     double *dres = nullptr;   // reduced norms (2 doubles)
     int *dflag = nullptr;     // singular-block flag
+    unsigned *dcfg = nullptr; // k_level_config words of every level
     bool ops_set = false;
     int std_iters = 16;       // per-handle copy of the reference's static heuristic
     long coarse_max = kCoarseMaxPoints;  // levels this small run in one workgroup (0: off)
@@ -1036,6 +1037,17 @@ int build_hierarchy_ops(stzmg_t h) {
                            L.ncls, L.inv64, L.inv32, h->dflag);
         LAUNCHCHK(h);
     }
+    // zero pattern and centrosymmetry of the per-plane levels, read back with
+    // the singular-block flag in one synchronization
+    const size_t ncfg = h->lv.size() * kCfgWords;
+    if (!h->dcfg) HIPCHK(h, alloc(&h->dcfg, (long)ncfg));
+    for (size_t l = 0; l < h->lv.size(); l++)
+        if (h->lv[l].mode == STZMG_OP_PLANE)
+            hipLaunchKernelGGL(k_level_config, dim3(1), dim3(kThreads), 0, h->stream, (const double *)h->lv[l].ops64,
+                               h->lv[l].ncls, h->dcfg + l * kCfgWords);
+    LAUNCHCHK(h);
+    std::vector<unsigned> cfg(ncfg, 0);
+    HIPCHK(h, hipMemcpyAsync(cfg.data(), h->dcfg, sizeof(unsigned) * ncfg, hipMemcpyDeviceToHost, h->stream));
     int flag = 0;
     HIPCHK(h, hipMemcpyAsync(&flag, h->dflag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1051,18 +1063,13 @@ int build_hierarchy_ops(stzmg_t h) {
     // Zero pattern of every per-plane level: the tightest compiled pattern
     // that holds all nonzero entries selects the fused smoother and tiled
     // residual instances.
-    for (auto &L : h->lv) {
+    for (size_t l = 0; l < h->lv.size(); l++) {
+        Level &L = h->lv[l];
         L.fused = false;
         L.tiled = false;
         L.tiled_small = false;
         if (L.mode != STZMG_OP_PLANE) continue;
-        std::vector<double> tab((size_t)L.ncls * kBlk);
-        HIPCHK(h, hipMemcpy(tab.data(), L.ops64, sizeof(double) * tab.size(), hipMemcpyDeviceToHost));
-        unsigned used[27] = {0};
-        for (long c = 0; c < L.ncls; c++)
-            for (int b = 0; b < 27; b++)
-                for (int e = 0; e < 9; e++)
-                    if (tab[(size_t)c * kBlk + 9 * b + e] != 0.0) used[b] |= 1u << e;
+        const unsigned *used = cfg.data() + l * kCfgWords;  // per block: entries nonzero on some plane
         int pat = PAT_DENSE;
         for (int cand : {PAT_SHEAR, PAT_TRANS_XZ, PAT_NOCORNER}) {
             bool fits = true;
@@ -1075,14 +1082,7 @@ int build_hierarchy_ops(stzmg_t h) {
         }
         L.pat = pat;
         // centrosymmetric: block(-o) equal to block(o) in every plane
-        L.sym = true;
-        for (long c = 0; c < L.ncls && L.sym; c++)
-            for (int b = 0; b < 13 && L.sym; b++)
-                for (int e = 0; e < 9; e++)
-                    if (tab[(size_t)c * kBlk + 9 * b + e] != tab[(size_t)c * kBlk + 9 * (26 - b) + e]) {
-                        L.sym = false;
-                        break;
-                    }
+        L.sym = used[27] != 0;
         L.ftile = fused_tile<float>(L);
         L.ftile64 = fused_tile<double>(L);
         L.fused = h->fused_smoother && L.ftile && L.ftile64 && L.x64b;
@@ -1248,6 +1248,7 @@ int stzmg_destroy(stzmg_t h) {
     hipFree(h->part);
     hipFree(h->dres);
     hipFree(h->dflag);
+    hipFree(h->dcfg);
     hipFree(h->kV); hipFree(h->kZ); hipFree(h->ktmp); hipFree(h->kpart); hipFree(h->kdev);
     hipFree(h->kV32);
     hipFree(h->kZ64); hipFree(h->kX64); hipFree(h->kB64);
