@@ -101,6 +101,8 @@ struct stzmg_s {
     double *dres = nullptr;   // reduced norms (2 doubles)
     int *dflag = nullptr;     // singular-block flag
     unsigned *dcfg = nullptr; // k_level_config words of every level
+    double *hres = nullptr;   // pinned host copy of dres (reference loop)
+    hipEvent_t evres = nullptr;  // recorded after that copy
     bool ops_set = false;
     int std_iters = 16;       // per-handle copy of the reference's static heuristic
     long coarse_max = kCoarseMaxPoints;  // levels this small run in one workgroup (0: off)
@@ -832,6 +834,24 @@ int norms64(stzmg_t h, int l, double out[2]) {
     return 0;
 }
 
+// The same norms, enqueued: the reduced pair is copied to the pinned h->hres
+// and h->evres recorded after the copy; the caller may enqueue more work
+// and then wait on the event.
+int norms64_enqueue(stzmg_t h, int l) {
+    Level &L = h->lv[l];
+    const long nb = resid_parts<double>(h, L);
+    if (ensure_part(h, nb)) return STZMG_ERR_NOMEM;
+    if (!h->hres) HIPCHK(h, hipHostMalloc((void **)&h->hres, 2 * sizeof(double), hipHostMallocDefault));
+    if (!h->evres) HIPCHK(h, hipEventCreateWithFlags(&h->evres, hipEventDisableTiming));
+    launch_resid<double, true>(h, L, L.x64, L.r64, L.t64);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, h->stream, (const double *)h->part, nb,
+                       h->dres);
+    LAUNCHCHK(h);
+    HIPCHK(h, hipMemcpyAsync(h->hres, h->dres, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipEventRecord(h->evres, h->stream));
+    return 0;
+}
+
 // Norms of r - A x on level l for explicit fields x, r (the residual left in
 // t64 of that level).
 int norms64_of(stzmg_t h, int l, const double *x, const double *r, double out[2]) {
@@ -1249,6 +1269,8 @@ int stzmg_destroy(stzmg_t h) {
     hipFree(h->dres);
     hipFree(h->dflag);
     hipFree(h->dcfg);
+    if (h->hres) hipHostFree(h->hres);
+    if (h->evres) hipEventDestroy(h->evres);
     hipFree(h->kV); hipFree(h->kZ); hipFree(h->ktmp); hipFree(h->kpart); hipFree(h->kdev);
     hipFree(h->kV32);
     hipFree(h->kZ64); hipFree(h->kX64); hipFree(h->kB64);
@@ -1611,15 +1633,30 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
         s.res0 = v[0];
         s.norms++;
         if (!std::isfinite(v[0])) rc = STZMG_ERR_NONFINITE;
+        // When extra cycles follow convergence, the V-cycle after a norm is
+        // enqueued before the host waits for that norm: if the loop goes on,
+        // it is the next iteration's cycle, if not, the first extra cycle --
+        // the same cycles in the same order either way, without the GPU
+        // idling while the host reads the norm.
+        int ahead = 0;
         while (rc == 0 && err > opts->tol) {
-            if (s.cycles >= opts->max_cycles) { rc = STZMG_ERR_NOT_CONVERGED; break; }
-            if ((rc = run_vcycles<double>(h, 1, graph))) return rc;
-            s.cycles++;
+            if (!ahead) {
+                if (s.cycles >= opts->max_cycles) { rc = STZMG_ERR_NOT_CONVERGED; break; }
+                if ((rc = run_vcycles<double>(h, 1, graph))) return rc;
+                s.cycles++;
+            }
+            ahead = 0;
             const bool eval = opts->ref_norm_schedule == 0 || num_iters >= h->std_iters / 16;
             if (eval) {
-                if ((rc = norms64(h, 0, v))) return rc;
+                if ((rc = norms64_enqueue(h, 0))) return rc;
+                if (opts->extra_cycles > 0 && s.cycles < opts->max_cycles) {
+                    if ((rc = run_vcycles<double>(h, 1, graph))) return rc;
+                    s.cycles++;
+                    ahead = 1;
+                }
+                HIPCHK(h, hipEventSynchronize(h->evres));
                 s.norms++;
-                err = v[0];
+                err = h->hres[0];
                 if (!std::isfinite(err)) { rc = STZMG_ERR_NONFINITE; break; }
             }
             num_iters++;
@@ -1627,8 +1664,8 @@ int stzmg_solve(stzmg_t h, const stzmg_solve_opts *opts, stzmg_solve_stats *st) 
         if (num_iters >= h->std_iters / 16) h->std_iters = num_iters * 16;
         h->std_iters--;
         if (rc == 0) {
-            if ((rc = run_vcycles<double>(h, opts->extra_cycles, graph))) return rc;
-            s.cycles += opts->extra_cycles;
+            if ((rc = run_vcycles<double>(h, opts->extra_cycles - ahead, graph))) return rc;
+            s.cycles += opts->extra_cycles - ahead;
         }
         HIPCHK(h, hipStreamSynchronize(h->stream));
         if (rc == 0 || rc == STZMG_ERR_NOT_CONVERGED) {
